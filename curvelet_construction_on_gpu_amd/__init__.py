@@ -1,0 +1,452 @@
+"""MI355X curvelet construction: Python mirror of cpu_curvelet's in-process API.
+
+The compute runs in libcurvelet_gpu.so (hand-written gfx950 HIP kernels behind a
+C ABI, include/curvelet_gpu.h).  This module only binds that library with
+ctypes and mirrors the reference's names so callers read like the reference:
+
+    CurveletParams                 cpu_curvelet/param_settings.hpp:9-24
+    read_TO_edges_from_file        cpu_curvelet/data_io.hpp:20-78
+    NeighborGraph                  CPUNeighborGraph<T>, preprocess.hpp:17-31
+    build_neighbor_csr_graph       preprocess.hpp:186-274
+    CurveletGPU                    CurveletCPU<T>, cpu_curvelet.hpp:22-123
+      .form_pairwise_bundles()     cpu_curvelet.hpp:305-361
+      .build_curvelets_greedy()    cpu_curvelet.hpp:363-634
+      .num_curvelets(), .chain_width(), .edge_chain_final, .curvelet_info
+    run_curvelet                   cpu_curvelet/main.cpp:19-96 (timed span)
+
+There is no CPU fallback: importing works without a GPU, but every call that
+needs the device raises CurveletError when the library or the device is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+__all__ = [
+    "CurveletError", "CurveletParams", "NeighborGraph", "CurveletGPU", "Context", "lib_path", "load_library",
+    "read_TO_edges_from_file", "build_neighbor_csr_graph", "run_curvelet", "synth_edges", "synth_preset",
+    "math_eval", "write_chain_file", "write_info_file", "chain_to_info_filename", "dt_radians",
+]
+
+_PKG = Path(__file__).resolve().parent
+_LIB_PATH = _PKG / "lib" / "libcurvelet_gpu.so"
+_lib = None
+
+
+class CurveletError(RuntimeError):
+    def __init__(self, msg: str, status: int | None = None):
+        super().__init__(msg)
+        self.status = status
+
+
+class _Params(C.Structure):
+    _fields_ = [("nrad", C.c_double), ("gap", C.c_double), ("dx", C.c_double), ("dt_deg", C.c_double),
+                ("max_k", C.c_double), ("curvelet_style", C.c_uint), ("group_max_sz", C.c_uint),
+                ("out_type", C.c_uint), ("sx", C.c_double), ("st", C.c_double), ("edge_data_sz", C.c_int),
+                ("max_candidates", C.c_uint)]
+
+
+class _GraphInfo(C.Structure):
+    _fields_ = [("num_edges", C.c_int), ("total_neighbor_pairs", C.c_int), ("max_num_of_neighbors", C.c_uint),
+                ("truncated_anchors", C.c_uint), ("order_ambiguous_anchors", C.c_uint)]
+
+
+class _RunStats(C.Structure):
+    _fields_ = [("valid_pairs", C.c_uint), ("num_curvelets", C.c_uint), ("chain_width", C.c_uint),
+                ("total_neighbor_pairs", C.c_int), ("max_num_of_neighbors", C.c_uint), ("ms_graph", C.c_double),
+                ("ms_curvelets", C.c_double), ("ms_total", C.c_double)]
+
+
+class _SynthConfig(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("num_curves", C.c_int), ("radius_min", C.c_double),
+                ("radius_max", C.c_double), ("radius_log_uniform", C.c_int), ("segment_min", C.c_double),
+                ("segment_max", C.c_double), ("bezier_extent", C.c_double), ("arc_max_length", C.c_double),
+                ("clutter_fraction", C.c_double), ("junction_patches", C.c_int), ("curves_per_patch", C.c_int),
+                ("patch_size", C.c_double), ("pos_sigma", C.c_double), ("theta_sigma_deg", C.c_double)]
+
+
+_P = C.POINTER
+_dp = _P(C.c_double)
+_ip = _P(C.c_int)
+_up = _P(C.c_uint)
+_ucp = _P(C.c_ubyte)
+
+_SIGNATURES = {
+    "cvg_params_default": (None, [_P(_Params)]),
+    "cvg_status_string": (C.c_char_p, [C.c_int]),
+    "cvg_create": (C.c_int, [C.c_int, _P(C.c_void_p)]),
+    "cvg_destroy": (None, [C.c_void_p]),
+    "cvg_last_error": (C.c_char_p, [C.c_void_p]),
+    "cvg_read_edge_file": (C.c_int, [C.c_char_p, C.c_int, _P(_dp), _ip, C.c_char_p, C.c_size_t]),
+    "cvg_free": (None, [C.c_void_p]),
+    "cvg_write_chain_file": (C.c_int, [C.c_char_p, _up, C.c_uint, C.c_uint]),
+    "cvg_write_info_file": (C.c_int, [C.c_char_p, _dp, C.c_uint, C.c_uint]),
+    "cvg_build_neighbor_csr_graph": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, C.c_double, C.c_uint, C.c_uint]),
+    "cvg_get_graph_info": (C.c_int, [C.c_void_p, _P(_GraphInfo)]),
+    "cvg_get_csr": (C.c_int, [C.c_void_p, _ip, _ip, _dp]),
+    "cvg_form_pairwise_bundles": (C.c_int, [C.c_void_p, _P(_Params), _up]),
+    "cvg_build_curvelets_greedy": (C.c_int, [C.c_void_p, _P(_Params), _up]),
+    "cvg_curvelet_shape": (C.c_int, [C.c_void_p, _up, _up]),
+    "cvg_get_curvelets": (C.c_int, [C.c_void_p, _up, _dp]),
+    "cvg_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _P(_Params), _P(_RunStats)]),
+    "cvg_run_device": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, _P(_Params), _P(_RunStats)]),
+    "cvg_upload_edges": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _P(C.c_void_p)]),
+    "cvg_device_curvelets": (C.c_int, [C.c_void_p, _P(C.c_void_p), _P(C.c_void_p)]),
+    "cvg_get_stream": (C.c_int, [C.c_void_p, _P(C.c_void_p)]),
+    "cvg_synchronize": (C.c_int, [C.c_void_p]),
+    "cvg_synth_preset": (C.c_int, [C.c_char_p, _P(_SynthConfig)]),
+    "cvg_synth_generate": (C.c_int, [_P(_SynthConfig), C.c_ulonglong, _dp, C.c_int, _ip]),
+    "cvg_debug_anchor_bundles": (C.c_int, [C.c_void_p, _P(_Params), C.c_int, _ucp, _ucp, _dp, _dp, _ip, _ip]),
+    "cvg_math_eval": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]),
+}
+
+EXPORTED_SYMBOLS = tuple(_SIGNATURES)
+
+
+def lib_path() -> Path:
+    return Path(os.environ.get("CVG_LIBRARY", _LIB_PATH))
+
+
+def load_library():
+    """Load libcurvelet_gpu.so (built in-tree by build.py) and declare signatures."""
+    global _lib
+    if _lib is None:
+        p = lib_path()
+        if not p.exists():
+            raise CurveletError(f"{p} is missing: build it with `python -m curvelet_construction_on_gpu_amd.build`")
+        lib = C.CDLL(str(p))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(_P(ctype))
+
+
+def dt_radians(dt_deg: float) -> float:
+    """main.cpp:24: dt = T(dt_deg / 180.0) * T(M_PI)."""
+    return (dt_deg / 180.0) * 3.14159265358979323846
+
+
+@dataclass
+class CurveletParams:
+    """CurveletParams (param_settings.hpp:9-24), same defaults."""
+    nrad: float = 3.5
+    gap: float = 1.5
+    dx: float = 0.4
+    dt_deg: float = 15.0
+    max_k: float = 0.3
+    curvelet_style: int = 2
+    group_max_sz: int = 4
+    out_type: int = 0
+    sx: float = 0.1
+    st: float = 0.08
+    edge_file: str = "eth3d_cables2.txt"
+    edge_data_sz: int = 4
+    max_candidates: int = 128
+
+    def _c(self) -> _Params:
+        return _Params(self.nrad, self.gap, self.dx, self.dt_deg, self.max_k, self.curvelet_style,
+                       self.group_max_sz, self.out_type, self.sx, self.st, self.edge_data_sz, self.max_candidates)
+
+
+@dataclass
+class NeighborGraph:
+    """CPUNeighborGraph<T> (preprocess.hpp:17-31)."""
+    num_edges: int = 0
+    total_neighbor_pairs: int = 0
+    max_num_of_neighbors: int = 0
+    edges: np.ndarray = field(default_factory=lambda: np.zeros((0, 4)))
+    neighbor_offsets: np.ndarray = field(default_factory=lambda: np.zeros(1, np.int32))
+    neighbor_ids: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    neighbor_dist2: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    truncated_anchors: int = 0
+    order_ambiguous_anchors: int = 0
+    _ctx: "Context | None" = None
+
+
+class Context:
+    """One HIP device context (own stream, resident device pools)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load_library()
+        h = C.c_void_p()
+        rc = self._lib.cvg_create(device, C.byref(h))
+        if rc != 0:
+            raise CurveletError(f"cvg_create(device={device}) failed: "
+                                f"{self._lib.cvg_status_string(rc).decode()}", rc)
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            msg = self._lib.cvg_last_error(self._h).decode()
+            raise CurveletError(f"{what}: {self._lib.cvg_status_string(rc).decode()}: {msg}", rc)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.cvg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- stage-wise API ----------------------------------------------------
+    def build_graph(self, edges: np.ndarray, rad: float, neighbor_radius: int = 3,
+                    max_candidates: int = 128) -> NeighborGraph:
+        e = np.ascontiguousarray(edges, dtype=np.float64)
+        n, sz = e.shape
+        self.check(self._lib.cvg_build_neighbor_csr_graph(self._h, n, sz, _ptr(e, C.c_double), rad,
+                                                          neighbor_radius, max_candidates),
+                   "build_neighbor_csr_graph")
+        return self.graph(edges=e)
+
+    def graph(self, edges: np.ndarray | None = None) -> NeighborGraph:
+        gi = _GraphInfo()
+        self.check(self._lib.cvg_get_graph_info(self._h, C.byref(gi)), "get_graph_info")
+        off = np.zeros(gi.num_edges + 1, np.int32)
+        ids = np.zeros(gi.total_neighbor_pairs, np.int32)
+        d2 = np.zeros(gi.total_neighbor_pairs, np.float64)
+        self.check(self._lib.cvg_get_csr(self._h, _ptr(off, C.c_int), _ptr(ids, C.c_int), _ptr(d2, C.c_double)),
+                   "get_csr")
+        return NeighborGraph(gi.num_edges, gi.total_neighbor_pairs, gi.max_num_of_neighbors,
+                             edges if edges is not None else np.zeros((0, 4)), off, ids, d2,
+                             gi.truncated_anchors, gi.order_ambiguous_anchors, self)
+
+    def form_pairwise_bundles(self, params: CurveletParams) -> int:
+        v = C.c_uint()
+        p = params._c()
+        self.check(self._lib.cvg_form_pairwise_bundles(self._h, C.byref(p), C.byref(v)), "form_pairwise_bundles")
+        return v.value
+
+    def build_curvelets_greedy(self, params: CurveletParams) -> int:
+        v = C.c_uint()
+        p = params._c()
+        self.check(self._lib.cvg_build_curvelets_greedy(self._h, C.byref(p), C.byref(v)), "build_curvelets_greedy")
+        return v.value
+
+    def curvelets(self) -> tuple[np.ndarray, np.ndarray]:
+        nc, w = C.c_uint(), C.c_uint()
+        self.check(self._lib.cvg_curvelet_shape(self._h, C.byref(nc), C.byref(w)), "curvelet_shape")
+        chains = np.zeros((nc.value, w.value), np.uint32)
+        info = np.zeros((nc.value, 10), np.float64)
+        self.check(self._lib.cvg_get_curvelets(self._h, _ptr(chains, C.c_uint), _ptr(info, C.c_double)),
+                   "get_curvelets")
+        return chains, info
+
+    # -- whole path ----------------------------------------------------------
+    def run(self, edges: np.ndarray, params: CurveletParams) -> dict:
+        e = np.ascontiguousarray(edges, dtype=np.float64)
+        n, sz = e.shape
+        st = _RunStats()
+        p = params._c()
+        self.check(self._lib.cvg_run(self._h, n, sz, _ptr(e, C.c_double), C.byref(p), C.byref(st)), "run")
+        return {k: getattr(st, k) for k, _ in _RunStats._fields_}
+
+    def upload(self, edges: np.ndarray) -> int:
+        e = np.ascontiguousarray(edges, dtype=np.float64)
+        n, sz = e.shape
+        d = C.c_void_p()
+        self.check(self._lib.cvg_upload_edges(self._h, n, sz, _ptr(e, C.c_double), C.byref(d)), "upload_edges")
+        return d.value
+
+    def run_device(self, dptr: int, n: int, sz: int, params: CurveletParams) -> dict:
+        st = _RunStats()
+        p = params._c()
+        self.check(self._lib.cvg_run_device(self._h, n, sz, C.c_void_p(dptr), C.byref(p), C.byref(st)),
+                   "run_device")
+        return {k: getattr(st, k) for k, _ in _RunStats._fields_}
+
+    def synchronize(self) -> None:
+        self.check(self._lib.cvg_synchronize(self._h), "synchronize")
+
+    def anchor_bundles(self, params: CurveletParams, anchor: int, max_deg: int):
+        """Per-neighbour dir class, hypothesis flag and bundle cells of one anchor (device)."""
+        K = 256
+        dir_ = np.zeros(max(max_deg, 1), np.uint8)
+        hyp = np.zeros(max(max_deg, 1), np.uint8)
+        bmin = np.zeros(max(max_deg, 1) * K)
+        bmax = np.zeros(max(max_deg, 1) * K)
+        deg, cells = C.c_int(), C.c_int()
+        p = params._c()
+        self.check(self._lib.cvg_debug_anchor_bundles(self._h, C.byref(p), anchor, _ptr(dir_, C.c_ubyte),
+                                                      _ptr(hyp, C.c_ubyte), _ptr(bmin, C.c_double),
+                                                      _ptr(bmax, C.c_double), C.byref(deg), C.byref(cells)),
+                   "debug_anchor_bundles")
+        d, k = deg.value, cells.value
+        return dir_[:d], hyp[:d], bmin[:d * k].reshape(d, k), bmax[:d * k].reshape(d, k)
+
+
+_default_ctx: Context | None = None
+
+
+def _ctx() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(int(os.environ.get("CVG_DEVICE", "0")))
+    return _default_ctx
+
+
+def build_neighbor_csr_graph(num_edges: int, sz_edge_data: int, edges: np.ndarray, rad: float,
+                             neighbor_radius: int, max_candidates: int, nthreads: int,
+                             graph: NeighborGraph, ctx: Context | None = None) -> bool:
+    """preprocess.hpp:186-195; nthreads is accepted for parity and unused."""
+    del nthreads
+    e = np.ascontiguousarray(edges, dtype=np.float64).reshape(num_edges, sz_edge_data)
+    c = ctx or _ctx()
+    try:
+        g = c.build_graph(e, rad, neighbor_radius, max_candidates)
+    except CurveletError as exc:
+        if exc.status in (1, 2):
+            return False
+        raise
+    graph.__dict__.update(g.__dict__)
+    return True
+
+
+class CurveletGPU:
+    """CurveletCPU<T> (cpu_curvelet.hpp:22-123) on the GPU; takes the graph built
+    by build_neighbor_csr_graph on the same context."""
+
+    def __init__(self, num_edges, sz_edge_data, graph: NeighborGraph, dx, dt, sx, st, max_k, group_max_sz,
+                 nthreads, to_edges, nrad, dt_deg: float | None = None):
+        del nthreads, to_edges
+        self._ctx = graph._ctx or _ctx()
+        # the C ABI takes dt in degrees (as CurveletParams); recover it when only radians are given
+        if dt_deg is None:
+            dt_deg = dt * 180.0 / 3.14159265358979323846
+            if dt_radians(dt_deg) != dt:
+                raise CurveletError("pass dt_deg: dt in radians does not round-trip to degrees exactly")
+        self.params = CurveletParams(nrad=nrad, dx=dx, dt_deg=dt_deg, max_k=max_k, sx=sx, st=st,
+                                     group_max_sz=group_max_sz, edge_data_sz=sz_edge_data)
+        self._n = num_edges
+        self.edge_chain_final = np.zeros((0, group_max_sz + 1), np.uint32)
+        self.curvelet_info = np.zeros((0, 10))
+
+    def form_pairwise_bundles(self) -> int:
+        return self._ctx.form_pairwise_bundles(self.params)
+
+    def build_curvelets_greedy(self) -> None:
+        self._ctx.build_curvelets_greedy(self.params)
+        self.edge_chain_final, self.curvelet_info = self._ctx.curvelets()
+
+    def num_curvelets(self) -> int:
+        return int(self.edge_chain_final.shape[0])
+
+    def chain_width(self) -> int:
+        return int(self.params.group_max_sz + 1)
+
+
+def run_curvelet(edges: np.ndarray, params: CurveletParams | None = None, ctx: Context | None = None):
+    """Edges (N x sz) -> (chains (C x G+1, 1-based), info (C x 10), stats) on the GPU."""
+    params = params or CurveletParams()
+    c = ctx or _ctx()
+    stats = c.run(edges, params)
+    chains, info = c.curvelets()
+    return chains, info, stats
+
+
+def read_TO_edges_from_file(path: str | os.PathLike, values_per_edge: int = 4) -> np.ndarray:
+    """data_io.hpp:20-78 semantics on an explicit path (no ../test_files prefix)."""
+    lib = load_library()
+    buf = _dp()
+    n = C.c_int()
+    err = C.create_string_buffer(1024)
+    rc = lib.cvg_read_edge_file(os.fsencode(str(path)), values_per_edge, C.byref(buf), C.byref(n), err, 1024)
+    if rc != 0:
+        raise CurveletError(err.value.decode(), rc)
+    try:
+        out = np.ctypeslib.as_array(buf, shape=(n.value * values_per_edge,)).copy()
+    finally:
+        lib.cvg_free(buf)
+    return out.reshape(n.value, values_per_edge)
+
+
+def write_chain_file(path, chains: np.ndarray) -> None:
+    lib = load_library()
+    c = np.ascontiguousarray(chains, dtype=np.uint32)
+    rows, w = c.shape
+    rc = lib.cvg_write_chain_file(os.fsencode(str(path)), _ptr(c, C.c_uint), rows, w)
+    if rc != 0:
+        raise CurveletError(f"write_chain_file({path}) failed", rc)
+
+
+def write_info_file(path, info: np.ndarray) -> None:
+    lib = load_library()
+    d = np.ascontiguousarray(info, dtype=np.float64)
+    rows, w = d.shape
+    rc = lib.cvg_write_info_file(os.fsencode(str(path)), _ptr(d, C.c_double), rows, w)
+    if rc != 0:
+        raise CurveletError(f"write_info_file({path}) failed", rc)
+
+
+def chain_to_info_filename(chain_file: str) -> str:
+    """data_io.hpp:135-145."""
+    pos = chain_file.find("chain")
+    if pos >= 0:
+        return chain_file[:pos] + "info" + chain_file[pos + 5:]
+    return "info_" + chain_file
+
+
+def synth_preset(name: str) -> _SynthConfig:
+    lib = load_library()
+    cfg = _SynthConfig()
+    rc = lib.cvg_synth_preset(name.encode(), C.byref(cfg))
+    if rc != 0:
+        raise CurveletError(f"unknown synthetic preset {name!r}", rc)
+    return cfg
+
+
+def synth_edges(preset: str | _SynthConfig, seed: int, **overrides) -> np.ndarray:
+    """Seeded synthetic edge map (N x 4: x, y, theta, strength)."""
+    lib = load_library()
+    cfg = synth_preset(preset) if isinstance(preset, str) else preset
+    for k, v in overrides.items():
+        setattr(cfg, k, v)
+    n = C.c_int()
+    rc = lib.cvg_synth_generate(C.byref(cfg), seed, None, 0, C.byref(n))
+    if rc != 0:
+        raise CurveletError("synth_generate failed", rc)
+    out = np.zeros((n.value, 4))
+    rc = lib.cvg_synth_generate(C.byref(cfg), seed, _ptr(out, C.c_double), n.value, C.byref(n))
+    if rc != 0:
+        raise CurveletError("synth_generate failed", rc)
+    return out
+
+
+MATH_OPS = {"sin": 0, "cos": 1, "atan2": 2, "sqrt": 3, "to2pi": 4, "fmod": 5}
+
+
+def math_eval(op: str, x: np.ndarray, y: np.ndarray | None = None, ctx: Context | None = None,
+              device: bool = True) -> np.ndarray:
+    """The library's math on the device (or, with device=False, its host build)."""
+    lib = load_library()
+    xa = np.ascontiguousarray(x, dtype=np.float64)
+    ya = None if y is None else np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros_like(xa)
+    h = (ctx or _ctx()).handle if device else None
+    rc = lib.cvg_math_eval(h, MATH_OPS[op], xa.size, _ptr(xa, C.c_double),
+                           None if ya is None else _ptr(ya, C.c_double), _ptr(out, C.c_double))
+    if rc != 0:
+        raise CurveletError(f"math_eval({op}) failed", rc)
+    return out

@@ -1,0 +1,1847 @@
+// MI355X (gfx950) curvelet-construction pipeline and its C ABI.
+//
+// Stages and kernels (one map per call; everything stays in HBM between
+// kernels, one HIP stream per context):
+//
+//   graph   k_prep_edges      row-major edges -> SoA x/y/theta, pixel cells, bbox
+//           k_cell_count      counting sort of edges into a dense pixel-cell grid
+//           scan              exclusive prefix sum over the grid (cell starts)
+//           k_cell_scatter    edges into cell order
+//           k_cell_order      within-cell order by edge id, coalesced SoA copies
+//           k_discover_count  per anchor: in-radius candidates in the 7x7 window
+//           scan              CSR offsets
+//           k_fill_csr        per anchor: gather in the reference's scan order,
+//                             distance sort (tie-exact std::sort replay), CSR
+//   curvelets k_edge_trig     per edge: sin/cos of theta, theta+-dt, theta+grid
+//           k_pair_class      per pair: direction-filter class
+//           k_bucket          anchors binned by their largest run (LDS capacity)
+//           k_curvelets       one 4-wave workgroup per anchor: bundles into LDS,
+//                             wave64 chain growth with ballot validity, dedup,
+//                             staging rows
+//           scan              per-anchor accepted counts -> output row offsets
+//           k_record          chain rows (1-based) and the 10 info columns
+//
+// The reference path this replaces is cpu_curvelet (preprocess.hpp,
+// cpu_curvelet.hpp); each kernel cites the lines it follows.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/curvelet_gpu.h"
+#include "cvg_device.hpp"
+#include "cvg_math.hpp"
+#include "cvg_stlsort.hpp"
+
+using cvg::Cand;
+using cvg::PairRec;
+
+__constant__ double c_sincos_tab[4 * CVG_SIN_ROWS] = CVG_SINCOS_TABLE_INIT;
+__constant__ double c_atan_tab[2 * CVG_ATAN_ROWS] = CVG_ATAN_TABLE_INIT;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = 4;  // waves per curvelet workgroup
+constexpr int kInfoWidth = 10;
+constexpr long long kMaxGridCells = 1LL << 28;
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct cvg_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::string err;
+    size_t lds_limit = 65536;
+
+    unsigned *mailbox = nullptr;  // pinned host scalars read back after syncs
+
+    // graph
+    int n = 0;
+    int sz = 4;
+    double rad = 0;
+    unsigned nbr_radius = 3;
+    unsigned max_cand = 128;
+    bool graph_ready = false;
+    int pairs = 0;
+    unsigned max_deg = 0;
+    unsigned truncated = 0;
+    unsigned ambiguous = 0;
+    int gx0 = 0, gy0 = 0, gw = 0, gh = 0;
+    DevBuf d_edges_up, d_x, d_y, d_th, d_cx, d_cy, d_scal, d_grid, d_tmp_ids, d_sorted, d_sx, d_sy, d_deg,
+        d_off, d_tiles, d_cand, d_ids, d_d2, d_pair_anchor;
+
+    // curvelets
+    bool curvelets_ready = false;
+    cvg_params cur_params{};
+    unsigned valid_pairs = 0;
+    unsigned num_curvelets = 0;
+    unsigned width = 0;
+    DevBuf d_trig, d_cls, d_cnt3, d_lists, d_tables, d_st_ids, d_st_meta, d_st_k, d_acc, d_rowoff, d_gbundle,
+        d_chains, d_info;
+
+    // diagnostics
+    DevBuf d_dbg;
+};
+
+namespace {
+
+int fail(cvg_context *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define CVG_HIP(ctx, call)                                                                               \
+    do {                                                                                                 \
+        hipError_t e_ = (call);                                                                          \
+        if (e_ != hipSuccess)                                                                            \
+            return fail((ctx), CVG_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+#define CVG_TRY(expr)              \
+    do {                           \
+        int rc_ = (expr);          \
+        if (rc_ != CVG_OK) return rc_; \
+    } while (0)
+
+int ensure(cvg_context *ctx, DevBuf &b, size_t bytes)
+{
+    if (bytes == 0) bytes = 8;
+    if (b.bytes >= bytes) return CVG_OK;
+    if (b.ptr) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(b.ptr);
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    const size_t want = bytes + bytes / 4;
+    hipError_t e = hipMalloc(&b.ptr, want);
+    if (e != hipSuccess) {
+        b.ptr = nullptr;
+        return fail(ctx, CVG_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(want) + " bytes failed");
+    }
+    b.bytes = want;
+    return CVG_OK;
+}
+
+void release(DevBuf &b)
+{
+    if (b.ptr) (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+}
+
+template <typename T>
+T *as(DevBuf &b)
+{
+    return static_cast<T *>(b.ptr);
+}
+
+inline int blocks_for(long long n, int block = kBlock)
+{
+    return static_cast<int>((n + block - 1) / block);
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive prefix sum over uint32 (in place allowed), out[n] = total.
+// ---------------------------------------------------------------------------
+
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__device__ unsigned block_exclusive_scan(unsigned v, unsigned *lds_waves, unsigned *total)
+{
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    unsigned x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) lds_waves[wid] = x;
+    __syncthreads();
+    unsigned pre = 0, tot = 0;
+    for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) {
+        const unsigned t = lds_waves[w];
+        if (w < wid) pre += t;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + x - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(const unsigned *__restrict__ in, long long n,
+                                                       unsigned *__restrict__ tile_sums)
+{
+    __shared__ unsigned lw[kBlock / 64];
+    const long long base = static_cast<long long>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
+    unsigned s = 0;
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < n) s += in[base + k];
+    unsigned tot;
+    block_exclusive_scan(s, lw, &tot);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+// Single workgroup: exclusive scan of the tile sums in place, total at [ntiles].
+__global__ __launch_bounds__(1024) void k_scan_tile_sums(unsigned *tile_sums, int ntiles)
+{
+    __shared__ unsigned lw[1024 / 64];
+    unsigned carry = 0;
+    for (int base = 0; base < ntiles; base += 1024) {
+        const int i = base + threadIdx.x;
+        const unsigned v = i < ntiles ? tile_sums[i] : 0u;
+        unsigned tot;
+        const unsigned ex = block_exclusive_scan(v, lw, &tot);
+        if (i < ntiles) tile_sums[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) tile_sums[ntiles] = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_apply(const unsigned *in, long long n, const unsigned *tile_prefix,
+                                                       int ntiles, unsigned *out)
+{
+    __shared__ unsigned lw[kBlock / 64];
+    const long long base = static_cast<long long>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
+    unsigned v[kScanItems];
+    unsigned s = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0u;
+        s += v[k];
+    }
+    unsigned tot;
+    unsigned run = block_exclusive_scan(s, lw, &tot) + tile_prefix[blockIdx.x];
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = tile_prefix[ntiles];
+}
+
+// out has n + 1 entries; tiles must hold ceil(n / kScanTile) + 1 entries.
+int exclusive_scan(cvg_context *ctx, const unsigned *in, long long n, unsigned *out, DevBuf &tiles)
+{
+    const int ntiles = static_cast<int>((n + kScanTile - 1) / kScanTile);
+    CVG_TRY(ensure(ctx, tiles, static_cast<size_t>(ntiles + 1) * sizeof(unsigned)));
+    unsigned *t = as<unsigned>(tiles);
+    if (ntiles > 0) hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, ctx->stream, in, n, t);
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(1024), 0, ctx->stream, t, ntiles);
+    if (ntiles > 0)
+        hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(kBlock), 0, ctx->stream, in, n, t, ntiles, out);
+    else
+        CVG_HIP(ctx, hipMemcpyAsync(out, t, sizeof(unsigned), hipMemcpyDeviceToDevice, ctx->stream));
+    CVG_HIP(ctx, hipGetLastError());
+    return CVG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Graph stage: build_neighbor_csr_graph (preprocess.hpp:62-274)
+// ---------------------------------------------------------------------------
+
+struct GridDesc {
+    int x0, y0, w, h;
+};
+
+// Scalars block layout (d_scal, uint32): [0..3] bbox (minx, maxx, miny, maxy as
+// int), [4] bad-input flag, [5] max degree, [6] truncated anchors,
+// [7] order-ambiguous anchors, [8] valid pairs, [9] anchors needing the
+// std::sort replay, [10..23] bucket counts.
+enum {
+    S_MINX = 0,
+    S_MAXX = 1,
+    S_MINY = 2,
+    S_MAXY = 3,
+    S_BAD = 4,
+    S_MAXDEG = 5,
+    S_TRUNC = 6,
+    S_AMBIG = 7,
+    S_VALID = 8,
+    S_TIES = 9,
+    S_BUCKET = 10,
+    S_COUNT = 24
+};
+constexpr int kMaxBuckets = S_COUNT - S_BUCKET;
+
+__global__ void k_init_scalars(unsigned *s)
+{
+    const int t = threadIdx.x;
+    if (t < S_COUNT) {
+        int v = 0;
+        if (t == S_MINX || t == S_MINY) v = INT_MAX;
+        if (t == S_MAXX || t == S_MAXY) v = INT_MIN;
+        s[t] = static_cast<unsigned>(v);
+    }
+}
+
+// Cells are (lrint(x), lrint(y)) (preprocess.hpp:85-86).  Coordinates must be
+// finite with |x|, |y| < 2^30 and |theta| < 2^20 (beyond that the reference's
+// int casts and the trig reduction leave their domain); others raise S_BAD.
+__global__ __launch_bounds__(kBlock) void k_prep_edges(int n, int sz, const double *__restrict__ e,
+                                                       double *__restrict__ x, double *__restrict__ y,
+                                                       double *__restrict__ th, int *__restrict__ cx,
+                                                       int *__restrict__ cy, unsigned *__restrict__ scal)
+{
+    __shared__ int smin_x, smax_x, smin_y, smax_y, sbad;
+    if (threadIdx.x == 0) {
+        smin_x = INT_MAX;
+        smax_x = INT_MIN;
+        smin_y = INT_MAX;
+        smax_y = INT_MIN;
+        sbad = 0;
+    }
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double xv = e[static_cast<long long>(i) * sz];
+        const double yv = e[static_cast<long long>(i) * sz + 1];
+        const double tv = e[static_cast<long long>(i) * sz + 2];
+        const bool ok = fabs(xv) < 0x1p30 && fabs(yv) < 0x1p30 && fabs(tv) < 0x1p20;
+        if (!ok) atomicOr(&sbad, 1);
+        const int ix = ok ? static_cast<int>(rint(xv)) : 0;
+        const int iy = ok ? static_cast<int>(rint(yv)) : 0;
+        x[i] = xv;
+        y[i] = yv;
+        th[i] = tv;
+        cx[i] = ix;
+        cy[i] = iy;
+        atomicMin(&smin_x, ix);
+        atomicMax(&smax_x, ix);
+        atomicMin(&smin_y, iy);
+        atomicMax(&smax_y, iy);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicMin(reinterpret_cast<int *>(&scal[S_MINX]), smin_x);
+        atomicMax(reinterpret_cast<int *>(&scal[S_MAXX]), smax_x);
+        atomicMin(reinterpret_cast<int *>(&scal[S_MINY]), smin_y);
+        atomicMax(reinterpret_cast<int *>(&scal[S_MAXY]), smax_y);
+        if (sbad) atomicOr(&scal[S_BAD], 1u);
+    }
+}
+
+__device__ inline int cell_index(const GridDesc &g, int cx, int cy) { return (cy - g.y0) * g.w + (cx - g.x0); }
+
+__global__ __launch_bounds__(kBlock) void k_cell_count(int n, const int *__restrict__ cx, const int *__restrict__ cy,
+                                                       GridDesc g, unsigned *__restrict__ cnt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[cell_index(g, cx[i], cy[i])], 1u);
+}
+
+// cnt holds the per-cell counts on entry and is decremented to zero.
+__global__ __launch_bounds__(kBlock) void k_cell_scatter(int n, const int *__restrict__ cx, const int *__restrict__ cy,
+                                                         GridDesc g, const unsigned *__restrict__ start,
+                                                         unsigned *__restrict__ cnt, int *__restrict__ tmp)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const int c = cell_index(g, cx[i], cy[i]);
+        const unsigned pos = start[c] + atomicSub(&cnt[c], 1u) - 1u;
+        tmp[pos] = i;
+    }
+}
+
+// Within-cell order: ascending edge id (the reference's order is whatever its
+// unstable std::sort of the index permutation yields, preprocess.hpp:90-92;
+// the two agree for up to 16 edges and only matter for the cases that
+// k_fill_csr counts as order-ambiguous).
+__global__ __launch_bounds__(kBlock) void k_cell_order(int n, const int *__restrict__ cx, const int *__restrict__ cy,
+                                                       GridDesc g, const unsigned *__restrict__ start,
+                                                       const int *__restrict__ tmp, const double *__restrict__ x,
+                                                       const double *__restrict__ y, int *__restrict__ sorted,
+                                                       double *__restrict__ sx, double *__restrict__ sy)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = cell_index(g, cx[i], cy[i]);
+    const unsigned s = start[c], e = start[c + 1];
+    unsigned r = 0;
+    for (unsigned k = s; k < e; ++k) r += (tmp[k] < i) ? 1u : 0u;
+    const unsigned p = s + r;
+    sorted[p] = i;
+    sx[p] = x[i];
+    sy[p] = y[i];
+}
+
+// discover_neighbors_csr counting pass (preprocess.hpp:114-184): the window is
+// rows py = cy-R..cy+R, and in each row the cells cx-R..cx+R are one
+// contiguous range of the cell-sorted arrays.  Accept d2 <= rad^2 in the
+// reference's form `!(d2 > rad2)`; keep at most max_candidates.
+__global__ __launch_bounds__(kBlock) void k_discover_count(int n, const double *__restrict__ x,
+                                                           const double *__restrict__ y, const int *__restrict__ cx,
+                                                           const int *__restrict__ cy, GridDesc g, int R, double rad2,
+                                                           unsigned max_cand, const unsigned *__restrict__ start,
+                                                           const int *__restrict__ sorted,
+                                                           const double *__restrict__ sx,
+                                                           const double *__restrict__ sy, unsigned *__restrict__ deg,
+                                                           unsigned *__restrict__ scal)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    const double ax = x[a], ay = y[a];
+    const int ix = cx[a] - g.x0, iy = cy[a] - g.y0;
+    unsigned cnt = 0;
+    for (int dy = -R; dy <= R; ++dy) {
+        const int row = (iy + dy) * g.w;
+        const unsigned s = start[row + ix - R], e = start[row + ix + R + 1];
+        for (unsigned k = s; k < e; ++k) {
+            if (sorted[k] == a) continue;
+            const double d = (ax - sx[k]) * (ax - sx[k]) + (ay - sy[k]) * (ay - sy[k]);
+            if (!(d > rad2)) ++cnt;
+        }
+    }
+    const unsigned d = cnt < max_cand ? cnt : max_cand;
+    deg[a] = d;
+    atomicMax(&scal[S_MAXDEG], d);
+    if (d >= max_cand) atomicAdd(&scal[S_TRUNC], 1u);  // preprocess.hpp:224, a >= test
+}
+
+// discover_neighbors_csr gathering pass + the distance sort (preprocess.hpp:141-183)
+// + the CSR fill (:240-262).  Candidates are gathered in the reference's scan
+// order (py outer, px inner, then within-cell order), cut at max_candidates,
+// then ordered by d2.  Distinct distances have one order; with ties the
+// result is libstdc++'s: up to 16 entries std::sort is an insertion sort
+// (stable), beyond that the introsort replay decides.
+__global__ __launch_bounds__(kBlock) void k_fill_csr(int n, const double *__restrict__ x, const double *__restrict__ y,
+                                                     const int *__restrict__ cx, const int *__restrict__ cy, GridDesc g,
+                                                     int R, double rad2, const unsigned *__restrict__ start,
+                                                     const int *__restrict__ sorted, const double *__restrict__ sx,
+                                                     const double *__restrict__ sy, const unsigned *__restrict__ off,
+                                                     Cand *__restrict__ cand, int *__restrict__ ids,
+                                                     double *__restrict__ d2, int *__restrict__ pair_anchor,
+                                                     int *__restrict__ tie_list, unsigned *__restrict__ scal)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    const double ax = x[a], ay = y[a];
+    const int ix = cx[a] - g.x0, iy = cy[a] - g.y0;
+    const unsigned o = off[a];
+    const unsigned dg = off[a + 1] - o;
+    unsigned kept = 0;
+    int last_cell = -1;
+    bool ambiguous = false;
+    bool cut_checked = false;
+    for (int dy = -R; dy <= R && !cut_checked; ++dy) {
+        const int row = (iy + dy) * g.w;
+        for (int dx = -R; dx <= R && !cut_checked; ++dx) {
+            const int c = row + ix + dx;
+            const unsigned cs = start[c], ce = start[c + 1];
+            const unsigned cell_first = kept;
+            for (unsigned k = cs; k < ce; ++k) {
+                const int b = sorted[k];
+                if (b == a) continue;
+                const double d = (ax - sx[k]) * (ax - sx[k]) + (ay - sy[k]) * (ay - sy[k]);
+                if (d > rad2) continue;
+                if (kept < dg) {
+                    // exact distance tie with an earlier candidate of the same cell
+                    for (unsigned q = cell_first; q < kept; ++q)
+                        if (cand[o + q].d == d) ambiguous = true;
+                    cand[o + kept] = Cand{d, b};
+                    ++kept;
+                    last_cell = c;
+                } else {
+                    // first candidate beyond the cap: the cut splits a cell?
+                    if (c == last_cell) ambiguous = true;
+                    cut_checked = true;
+                    break;
+                }
+            }
+        }
+    }
+    // stable rank order (the insertion-sort result) and tie detection
+    bool tie = false;
+    for (unsigned i = 0; i < dg; ++i) {
+        const Cand ci = cand[o + i];
+        unsigned r = 0;
+        for (unsigned j = 0; j < dg; ++j) {
+            const double dj = cand[o + j].d;
+            r += (dj < ci.d || (dj == ci.d && j < i)) ? 1u : 0u;
+            tie = tie || (dj == ci.d && j != i);
+        }
+        ids[o + r] = ci.id;
+        d2[o + r] = ci.d;
+    }
+    if (tie && dg > 16) tie_list[atomicAdd(&scal[S_TIES], 1u)] = a;
+    for (unsigned i = 0; i < dg; ++i) pair_anchor[o + i] = a;
+    if (ambiguous && n > 16) atomicAdd(&scal[S_AMBIG], 1u);
+}
+
+// Anchors whose (more than 16) candidates contain exact distance ties: replay
+// libstdc++'s introsort on the scan-order list k_fill_csr left in cand.
+// Grid-stride over the device-side count, so no host round trip is needed.
+__global__ __launch_bounds__(kBlock) void k_tie_sort(const int *__restrict__ tie_list,
+                                                     const unsigned *__restrict__ scal,
+                                                     const unsigned *__restrict__ off, Cand *__restrict__ cand,
+                                                     int *__restrict__ ids, double *__restrict__ d2)
+{
+    const unsigned count = scal[S_TIES];
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
+        const int a = tie_list[t];
+        const unsigned o = off[a];
+        const int dg = static_cast<int>(off[a + 1] - o);
+        cvg::stl_sort(cand + o, dg);
+        for (int i = 0; i < dg; ++i) {
+            ids[o + i] = cand[o + i].id;
+            d2[o + i] = cand[o + i].d;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Curvelet stage
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_edge_trig(int n, const double *__restrict__ th, double dt, int kt,
+                                                      const double *__restrict__ off_t, double *__restrict__ trig)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double t = th[i];
+    double s, c;
+    cvg::cr_sincos(t, c_sincos_tab, &s, &c);
+    trig[cvg::TRIG_COS * n + i] = c;
+    trig[cvg::TRIG_SIN * n + i] = s;
+    cvg::cr_sincos(t + dt, c_sincos_tab, &s, &c);  // T0_p_dt, cpu_curvelet.hpp:655
+    trig[cvg::TRIG_SIN_P * n + i] = s;
+    trig[cvg::TRIG_COS_P * n + i] = c;
+    cvg::cr_sincos(t - dt, c_sincos_tab, &s, &c);  // T0_m_dt, :656
+    trig[cvg::TRIG_SIN_M * n + i] = s;
+    trig[cvg::TRIG_COS_M * n + i] = c;
+    for (int j = 0; j < kt; ++j) {
+        cvg::cr_sincos(t + off_t[j], c_sincos_tab, &s, &c);  // T2t = T2 + delta_t2, :702
+        trig[static_cast<long long>(cvg::TRIG_GRID + 2 * j) * n + i] = s;
+        trig[static_cast<long long>(cvg::TRIG_GRID + 2 * j + 1) * n + i] = c;
+    }
+}
+
+// Direction filter per CSR pair (cpu_curvelet.hpp:459-461) and per-anchor
+// class counts cnt3[3a + {0: class 1, 1: class 2, 2: class 0}].
+__global__ __launch_bounds__(kBlock) void k_pair_class(int pairs, int n, const int *__restrict__ pair_anchor,
+                                                       const int *__restrict__ ids, const double *__restrict__ x,
+                                                       const double *__restrict__ y, const double *__restrict__ trig,
+                                                       unsigned char *__restrict__ cls, unsigned *__restrict__ cnt3)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= pairs) return;
+    const int a = pair_anchor[q];
+    const int b = ids[q];
+    const int k = cvg::direction_class(x[a], y[a], trig[cvg::TRIG_COS * n + a], trig[cvg::TRIG_SIN * n + a], x[b],
+                                       y[b], c_sincos_tab, c_atan_tab);
+    cls[q] = static_cast<unsigned char>(k);
+    atomicAdd(&cnt3[3 * a + (k == 1 ? 0 : (k == 2 ? 1 : 2))], 1u);
+}
+
+// Anchors in spatial (cell) order, binned by the largest of their three class
+// counts, which sizes the LDS slots the fused kernel needs.
+__global__ __launch_bounds__(kBlock) void k_bucket(int n, const int *__restrict__ sorted,
+                                                   const unsigned *__restrict__ cnt3, int nb, const int *caps,
+                                                   int *__restrict__ lists, unsigned *__restrict__ scal)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int a = sorted[p];
+    const unsigned m = max(cnt3[3 * a], max(cnt3[3 * a + 1], cnt3[3 * a + 2]));
+    int b = 0;
+    while (b < nb - 1 && m > static_cast<unsigned>(caps[b])) ++b;
+    const unsigned pos = atomicAdd(&scal[S_BUCKET + b], 1u);
+    lists[static_cast<long long>(b) * n + pos] = a;
+}
+
+// Host-computed per-parameter tables, laid out in one device block.  Lanes of
+// the fused kernel own bundle cells in "rank" order: rank = position of cell
+// c = i*Kt + j when cells are sorted by (ox^2 + ot^2, c), so the cell that
+// fill_curvelet_info picks (smallest distance, first in row-major order,
+// cpu_curvelet.hpp:202-219) is the lowest lane whose cell is valid.
+struct Tables {
+    double *rank_ox;       // [K]
+    int *rank_j;           // [K]
+    int *rank_cell;        // [K]
+    int *rank_elig;        // [K] d < 100 (the initial best distance)
+    double *off_x;         // [Kx]
+    double *off_t;         // [Kt]
+    int *caps;             // [kMaxBuckets]
+};
+
+struct FusedArgs {
+    int n;
+    const double *x, *y, *th, *trig;
+    const unsigned *off;
+    const int *ids;
+    const unsigned char *cls;
+    const int *list;
+    int list_len;
+    int nb_cap;
+    int chain_stride;  // LDS chain slots per seed: max(G, 2)
+    int st_stride;     // staging ids per row: max(G, 1)
+    double dt, dx, dx2, close_thr, max_k;
+    int kt, K, G;
+    const double *rank_ox;
+    const int *rank_j, *rank_cell, *rank_elig;
+    int rank_of_cell0;
+    const double *off_t;
+    int *st_ids;
+    unsigned *st_meta;
+    double *st_k;
+    unsigned *acc_count;
+    unsigned *valid_pairs;
+    double *g_bundle;  // non-null: bundles live in global memory, nb_cap*K*2 doubles per list entry
+};
+
+// Order-preserving compaction of [0, n) by pred into out (LDS); returns the count.
+template <typename Pred>
+__device__ int block_compact(int n, Pred pred, int *out, int *wsum)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    int total = 0;
+    for (int base = 0; base < n; base += 64 * kWaves) {
+        const int i = base + static_cast<int>(threadIdx.x);
+        const bool f = i < n && pred(i);
+        const unsigned long long m = __ballot(f);
+        const int below = __popcll(m & ((1ULL << lane) - 1ULL));
+        if (lane == 0) wsum[wave] = __popcll(m);
+        __syncthreads();
+        int pre = 0, chunk = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            const int c = wsum[w];
+            if (w < wave) pre += c;
+            chunk += c;
+        }
+        if (f) out[total + pre + below] = i;
+        total += chunk;
+        __syncthreads();
+    }
+    return total;
+}
+
+// LDS carve-up for nb slots; returns the byte size.
+struct Carve {
+    size_t rec, t0, sinm, sinp, blo, bhi, skmax, skmin;  // double arrays (offsets in doubles)
+    size_t nid, nflag, lrun, hlist, slen, sbest, sacc, chains, misc;  // int arrays (offsets in ints from int base)
+    size_t int_base_bytes, bytes;
+};
+
+__host__ __device__ inline Carve carve(int nb, int kt, int K, int chain_stride, bool lds_bundles)
+{
+    Carve c{};
+    size_t d = 0;
+    c.rec = d;
+    d += static_cast<size_t>(nb) * 8;
+    c.t0 = d;
+    d += static_cast<size_t>(nb) * 2;
+    c.sinm = d;
+    d += static_cast<size_t>(nb) * kt;
+    c.sinp = d;
+    d += static_cast<size_t>(nb) * kt;
+    c.blo = d;
+    if (lds_bundles) d += static_cast<size_t>(nb) * K;
+    c.bhi = d;
+    if (lds_bundles) d += static_cast<size_t>(nb) * K;
+    c.skmax = d;
+    d += nb;
+    c.skmin = d;
+    d += nb;
+    c.int_base_bytes = d * sizeof(double);
+    size_t i = 0;
+    c.nid = i;
+    i += nb;
+    c.nflag = i;
+    i += nb;
+    c.lrun = i;
+    i += nb;
+    c.hlist = i;
+    i += nb;
+    c.slen = i;
+    i += nb;
+    c.sbest = i;
+    i += nb;
+    c.sacc = i;
+    i += nb;
+    c.chains = i;
+    i += static_cast<size_t>(nb) * chain_stride;
+    c.misc = i;
+    i += 16;
+    c.bytes = c.int_base_bytes + i * sizeof(int);
+    return c;
+}
+
+enum { M_ACC = 0, M_VP = 1, M_NL = 2, M_NH = 3, M_WSUM = 8 };
+
+// build_curvelets_greedy for one anchor per workgroup (cpu_curvelet.hpp:437-591),
+// plus the form_pairwise_bundles valid-pair count (:305-361), in one pass.
+// Runs: 0 = forward neighbours (class 1), 1 = backward (class 2), 2 = neither
+// (class 0, validity only for the count).  Lanes own bundle cells in rank
+// order, CPL cells per lane.
+template <int CPL>
+__global__ __launch_bounds__(64 * kWaves) void k_curvelets(FusedArgs A)
+{
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    constexpr int kThreads = 64 * kWaves;
+
+    // XCD-aware order: the grid is a multiple of 8 and block b runs on XCD b % 8;
+    // give each XCD a contiguous slice of the spatially ordered anchor list.
+    const int per = static_cast<int>(gridDim.x) / 8;
+    const int L = static_cast<int>(blockIdx.x % 8) * per + static_cast<int>(blockIdx.x / 8);
+    if (L >= A.list_len) return;
+    const int a = A.list[L];
+
+    const bool lds_bundles = (A.g_bundle == nullptr);
+    const Carve cv = carve(A.nb_cap, A.kt, A.K, A.chain_stride, lds_bundles);
+    double *rec = lds + cv.rec;
+    double *t0 = lds + cv.t0;
+    double *sinm = lds + cv.sinm;
+    double *sinp = lds + cv.sinp;
+    double *blo = lds_bundles ? lds + cv.blo : A.g_bundle + static_cast<size_t>(L) * A.nb_cap * A.K * 2;
+    double *bhi = lds_bundles ? lds + cv.bhi : blo + static_cast<size_t>(A.nb_cap) * A.K;
+    double *skmax = lds + cv.skmax;
+    double *skmin = lds + cv.skmin;
+    int *ib = reinterpret_cast<int *>(reinterpret_cast<char *>(lds) + cv.int_base_bytes);
+    int *nid = ib + cv.nid;
+    int *nflag = ib + cv.nflag;
+    int *lrun = ib + cv.lrun;
+    int *hlist = ib + cv.hlist;
+    int *slen = ib + cv.slen;
+    int *sbest = ib + cv.sbest;
+    int *sacc = ib + cv.sacc;
+    int *chains = ib + cv.chains;
+    int *misc = ib + cv.misc;
+
+    const int n = A.n;
+    const double ax = A.x[a], ay = A.y[a], at = A.th[a];
+    const double cos_at = A.trig[cvg::TRIG_COS * n + a];
+    const double sin_at = A.trig[cvg::TRIG_SIN * n + a];
+    const unsigned base = A.off[a];
+    const int deg = static_cast<int>(A.off[a + 1] - base);
+    const int K = A.K;
+    const int kt = A.kt;
+    const int G = A.G;
+    const int CS = A.chain_stride;
+
+    // per-lane cell constants
+    double ox[CPL], sT2t[CPL], cT2t[CPL];
+    int cj[CPL], elig[CPL];
+    bool live[CPL];
+#pragma unroll
+    for (int m = 0; m < CPL; ++m) {
+        const int rho = lane + 64 * m;
+        live[m] = rho < K;
+        const int r = live[m] ? rho : 0;
+        ox[m] = A.rank_ox[r];
+        cj[m] = A.rank_j[r];
+        elig[m] = live[m] ? A.rank_elig[r] : 0;
+        sT2t[m] = A.trig[static_cast<long long>(cvg::TRIG_GRID + 2 * cj[m]) * n + a];
+        cT2t[m] = A.trig[static_cast<long long>(cvg::TRIG_GRID + 2 * cj[m] + 1) * n + a];
+    }
+    if (tid == 0) {
+        misc[M_ACC] = 0;
+        misc[M_VP] = 0;
+    }
+    __syncthreads();
+
+    for (int run = 0; run < 3; ++run) {
+        const unsigned char want = run == 0 ? 1 : (run == 1 ? 2 : 0);
+        const int nL = block_compact(
+            deg, [&](int r) { return A.cls[base + r] == want; }, lrun, misc + M_WSUM);
+        if (nL == 0) continue;
+
+        // neighbour records (compute_curve_bundle's per-pair constants, :646-656, 682, 690, 730-733)
+        for (int k = tid; k < nL; k += kThreads) {
+            const int r = lrun[k];
+            const int b = A.ids[base + r];
+            const double nx = A.x[b], ny = A.y[b], nt = A.th[b];
+            const PairRec p = cvg::pair_record(ax, ay, nx, ny, A.trig[cvg::TRIG_SIN_P * n + b],
+                                               A.trig[cvg::TRIG_COS_P * n + b], A.trig[cvg::TRIG_SIN_M * n + b],
+                                               A.trig[cvg::TRIG_COS_M * n + b]);
+            double *rr = rec + static_cast<size_t>(k) * 8;
+            rr[0] = p.DX;
+            rr[1] = p.DY;
+            rr[2] = p.DX2;
+            rr[3] = p.DY2;
+            rr[4] = p.cT0m_DX;
+            rr[5] = p.sT0m_DY;
+            rr[6] = p.cT0p_DX;
+            rr[7] = p.sT0p_DY;
+            t0[2 * k] = nt - A.dt;
+            t0[2 * k + 1] = nt + A.dt;
+            const bool close = p.DX2 + p.DY2 < A.close_thr;
+            const bool branch_a = (p.DX * cos_at + p.DY * sin_at) < 0;
+            nid[k] = b;
+            nflag[k] = (close ? 1 : 0) | (branch_a ? 2 : 0);
+        }
+        __syncthreads();
+
+        // per-pair sines sin(T2t - T0_m_dt), sin(T2t - T0_p_dt) (:728-729)
+        const int ntrig = nL * kt * 2;
+        for (int q = tid; q < ntrig; q += kThreads) {
+            const int k = q / (2 * kt);
+            const int rem = q - k * 2 * kt;
+            const int j = rem >> 1;
+            const int pm = rem & 1;
+            if (!(nflag[k] & 1)) {
+                const double T2t = at + A.off_t[j];
+                const double s = cvg::cr_sin(T2t - t0[2 * k + pm], c_sincos_tab);
+                (pm ? sinp : sinm)[k * kt + j] = s;
+            }
+        }
+        __syncthreads();
+
+        // bundle cells, one neighbour per wave at a time
+        for (int k = wave; k < nL; k += kWaves) {
+            const int fl = nflag[k];
+            const bool close = fl & 1;
+            const double *rr = rec + static_cast<size_t>(k) * 8;
+            PairRec p;
+            p.DX = rr[0];
+            p.DY = rr[1];
+            p.DX2 = rr[2];
+            p.DY2 = rr[3];
+            p.cT0m_DX = rr[4];
+            p.sT0m_DY = rr[5];
+            p.cT0p_DX = rr[6];
+            p.sT0p_DY = rr[7];
+            bool any = false;
+#pragma unroll
+            for (int m = 0; m < CPL; ++m) {
+                if (!live[m]) continue;
+                const int rho = lane + 64 * m;
+                double lo = -A.max_k, hi = A.max_k;
+                if (!close)
+                    cvg::bundle_cell(p, (fl & 2) != 0, ox[m], sT2t[m], cT2t[m], sinm[k * kt + cj[m]],
+                                     sinp[k * kt + cj[m]], A.dx, A.dx2, A.max_k, lo, hi);
+                blo[static_cast<size_t>(k) * K + rho] = lo;
+                bhi[static_cast<size_t>(k) * K + rho] = hi;
+                any = any || (hi > lo);
+            }
+            const bool hyp = close || (__ballot(any) != 0ULL);  // bundle_valid_check, :821-830
+            if (lane == 0) nflag[k] = fl | (hyp ? 4 : 0);
+        }
+        __syncthreads();
+
+        const int nH = block_compact(
+            nL, [&](int k) { return (nflag[k] & 4) != 0; }, hlist, misc + M_WSUM);
+        if (tid == 0) misc[M_VP] += nH;
+        if (run == 2 || nH == 0) {
+            __syncthreads();
+            continue;
+        }
+
+        // chain growth by bundle intersection (:479-536); seeds split over waves
+        for (int si = wave; si < nH; si += kWaves) {
+            const int s = hlist[si];
+            double lo[CPL], hi[CPL];
+#pragma unroll
+            for (int m = 0; m < CPL; ++m) {
+                const int rho = live[m] ? lane + 64 * m : 0;
+                lo[m] = blo[static_cast<size_t>(s) * K + rho];
+                hi[m] = bhi[static_cast<size_t>(s) * K + rho];
+            }
+            int len = 1;
+            int *ch = chains + static_cast<size_t>(si) * CS;
+            if (lane == 0) ch[0] = a;
+            for (int ri = 0; ri < nH; ++ri) {
+                const int r = hlist[ri];
+                if (ri == si) {  // the seed joins at its own distance rank (:496-510)
+                    if (lane == 0) ch[len] = nid[r];
+                    ++len;
+                    if (len >= G) break;
+                    continue;
+                }
+                double nlo[CPL], nhi[CPL];
+                bool ok = false;
+#pragma unroll
+                for (int m = 0; m < CPL; ++m) {
+                    const int rho = live[m] ? lane + 64 * m : 0;
+                    const double bl = blo[static_cast<size_t>(r) * K + rho];
+                    const double bh = bhi[static_cast<size_t>(r) * K + rho];
+                    nlo[m] = (lo[m] < bl) ? bl : lo[m];  // bundle_intersection, :838-841
+                    nhi[m] = (hi[m] > bh) ? bh : hi[m];
+                    ok = ok || (live[m] && nhi[m] > nlo[m]);
+                }
+                if (__ballot(ok) != 0ULL) {
+                    if (lane == 0) ch[len] = nid[r];
+                    ++len;
+#pragma unroll
+                    for (int m = 0; m < CPL; ++m) {
+                        lo[m] = nlo[m];
+                        hi[m] = nhi[m];
+                    }
+                }
+                if (len >= G) break;
+            }
+            // best cell of the running bundle (fill_curvelet_info, :202-226)
+            int best = -1;
+#pragma unroll
+            for (int m = 0; m < CPL; ++m) {
+                const unsigned long long mask = __ballot(live[m] && elig[m] && hi[m] > lo[m]);
+                if (best < 0 && mask != 0ULL) best = 64 * m + __ffsll(static_cast<long long>(mask)) - 1;
+            }
+            if (best < 0) best = A.rank_of_cell0;
+            const int bm = best >> 6;
+            const int bl = best & 63;
+            double kmax = 0.0, kmin = 0.0;
+#pragma unroll
+            for (int m = 0; m < CPL; ++m) {
+                const double h = __shfl(hi[m], bl, 64);
+                const double l = __shfl(lo[m], bl, 64);
+                if (m == bm) {
+                    kmax = h;
+                    kmin = l;
+                }
+            }
+            if (lane == 0) {
+                slen[si] = len;
+                sbest[si] = A.rank_cell[best];
+                skmax[si] = kmax;
+                skmin[si] = kmin;
+            }
+        }
+        __syncthreads();
+
+        // accept (len > 2, :549) unless identical to an earlier accepted chain of
+        // this (anchor, run) (check_curvelet_exist, :876-916); record in seed order
+        if (wave == 0) {
+            int acc = misc[M_ACC];
+            for (int si = 0; si < nH; ++si) {
+                const int len = slen[si];
+                bool accept = len > 2;
+                if (accept) {
+                    const int *mine = chains + static_cast<size_t>(si) * CS;
+                    for (int kb = 0; kb < si && accept; kb += 64) {
+                        const int k = kb + lane;
+                        bool same = false;
+                        if (k < si && sacc[k] && slen[k] == len) {
+                            const int *other = chains + static_cast<size_t>(k) * CS;
+                            same = true;
+                            for (int l = 0; l < len; ++l) same = same && (other[l] == mine[l]);
+                        }
+                        if (__ballot(same) != 0ULL) accept = false;
+                    }
+                }
+                if (lane == 0) sacc[si] = accept ? 1 : 0;
+                if (accept) {
+                    const long long q = static_cast<long long>(base) + acc;
+                    const int *mine = chains + static_cast<size_t>(si) * CS;
+                    for (int l = lane; l < len; l += 64) A.st_ids[q * A.st_stride + l] = mine[l];
+                    if (lane == 0) {
+                        A.st_meta[q] = static_cast<unsigned>(len) | (run == 0 ? 0x100u : 0u) |
+                                       (static_cast<unsigned>(sbest[si]) << 16);
+                        A.st_k[2 * q] = skmax[si];
+                        A.st_k[2 * q + 1] = skmin[si];
+                    }
+                    ++acc;
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+            if (lane == 0) misc[M_ACC] = acc;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        A.acc_count[a] = static_cast<unsigned>(misc[M_ACC]);
+        atomicAdd(A.valid_pairs, static_cast<unsigned>(misc[M_VP]));
+    }
+}
+
+struct RecordArgs {
+    int n;
+    const double *x, *y, *th;
+    const unsigned *off, *acc_count, *row_off;
+    const int *st_ids;
+    const unsigned *st_meta;
+    const double *st_k;
+    int st_stride;
+    int G, kt;
+    double nrad;
+    const double *off_x, *off_t;
+    unsigned *chains;
+    double *info;
+};
+
+// record_curvelet_chain + fill_curvelet_info (cpu_curvelet.hpp:197-280) for
+// every accepted staging row, written at its compacted position
+// (compact_curvelet_output, :282-303).
+__global__ __launch_bounds__(kBlock) void k_record(RecordArgs R)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= R.n) return;
+    const unsigned cnt = R.acc_count[a];
+    if (cnt == 0) return;
+    const unsigned base = R.off[a];
+    const unsigned o = R.row_off[a];
+    const double ax = R.x[a], ay = R.y[a], at = R.th[a];
+    const int W = R.G + 1;
+    for (unsigned k = 0; k < cnt; ++k) {
+        const long long q = static_cast<long long>(base) + k;
+        const unsigned meta = R.st_meta[q];
+        const int len = static_cast<int>(meta & 0xffu);
+        const bool fwd = (meta >> 8) & 1u;
+        const int c = static_cast<int>(meta >> 16);
+        const int *chain = R.st_ids + q * R.st_stride;
+        unsigned *row = R.chains + static_cast<long long>(o + k) * W;
+        row[0] = static_cast<unsigned>(a) + 1;
+        for (int l = 0; l < R.G; ++l) row[l + 1] = l < len ? static_cast<unsigned>(chain[l]) + 1 : 0u;
+
+        const double ox = R.off_x[c / R.kt];
+        const double ot = R.off_t[c % R.kt];
+        const double k_max = R.st_k[2 * q];
+        const double k_min = R.st_k[2 * q + 1];
+        const double kk = (k_max + k_min) / 2.0;
+        const double theta = cvg::wrap_two_pi(at + ot);
+        double s, cth;
+        cvg::cr_sincos(theta, c_sincos_tab, &s, &cth);
+        const double px = ax - ox * s;
+        const double py = ay + ox * cth;
+        double length = 0.0;
+        for (int l = 0; l + 1 < len; ++l) {
+            const int e1 = chain[l], e2 = chain[l + 1];
+            const double x1 = R.x[e1], y1 = R.y[e1], x2 = R.x[e2], y2 = R.y[e2];
+            length += sqrt((x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2));
+        }
+        const double quality = (length > 0.0 && len > 0) ? 2.0 / (R.nrad / length + length / double(len)) : 0.0;
+        double *inf = R.info + static_cast<long long>(o + k) * kInfoWidth;
+        inf[0] = fwd ? 1.0 : 0.0;
+        inf[1] = k_max;
+        inf[2] = k_min;
+        inf[3] = at;
+        inf[4] = px;
+        inf[5] = py;
+        inf[6] = theta;
+        inf[7] = kk;
+        inf[8] = length;
+        inf[9] = quality;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics kernels
+// ---------------------------------------------------------------------------
+
+// One anchor: per neighbour r and cell c (reference cell order) the bundle,
+// with every per-pair quantity computed from scratch by the shared device
+// functions.
+__global__ void k_debug_anchor(int a, int n, const double *__restrict__ x, const double *__restrict__ y,
+                               const double *__restrict__ th, const unsigned *__restrict__ off,
+                               const int *__restrict__ ids, double dt, double dx, double dx2, double close_thr,
+                               double max_k, int kx, int kt, const double *__restrict__ off_x,
+                               const double *__restrict__ off_t, double *__restrict__ out /* deg*K*2 + deg*2 */)
+{
+    const int K = kx * kt;
+    const unsigned base = off[a];
+    const int deg = static_cast<int>(off[a + 1] - base);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= deg * K) return;
+    const int r = t / K, c = t % K;
+    const int i = c / kt, j = c % kt;
+    const int b = ids[base + r];
+    const double ax = x[a], ay = y[a], at = th[a];
+    const double nx = x[b], ny = y[b], nt = th[b];
+    double sa, ca;
+    cvg::cr_sincos(at, c_sincos_tab, &sa, &ca);
+    double sp, cp, sm, cm;
+    cvg::cr_sincos(nt + dt, c_sincos_tab, &sp, &cp);
+    cvg::cr_sincos(nt - dt, c_sincos_tab, &sm, &cm);
+    const PairRec p = cvg::pair_record(ax, ay, nx, ny, sp, cp, sm, cm);
+    const bool close = p.DX2 + p.DY2 < close_thr;
+    const bool branch_a = (p.DX * ca + p.DY * sa) < 0;
+    const double T2t = at + off_t[j];
+    double s2, c2;
+    cvg::cr_sincos(T2t, c_sincos_tab, &s2, &c2);
+    const double sin_m = cvg::cr_sin(T2t - (nt - dt), c_sincos_tab);
+    const double sin_p = cvg::cr_sin(T2t - (nt + dt), c_sincos_tab);
+    double lo = -max_k, hi = max_k;
+    if (!close) cvg::bundle_cell(p, branch_a, off_x[i], s2, c2, sin_m, sin_p, dx, dx2, max_k, lo, hi);
+    out[static_cast<size_t>(r) * K + c] = lo;
+    out[static_cast<size_t>(deg) * K + static_cast<size_t>(r) * K + c] = hi;
+    if (c == 0) {
+        out[static_cast<size_t>(2) * deg * K + r] =
+            static_cast<double>(cvg::direction_class(ax, ay, ca, sa, nx, ny, c_sincos_tab, c_atan_tab));
+        out[static_cast<size_t>(2) * deg * K + deg + r] = close ? 1.0 : 0.0;
+    }
+}
+
+__host__ __device__ inline double math_op(int op, double x, double y, const double *sc, const double *atb)
+{
+    switch (op) {
+    case 0: return cvg::cr_sin(x, sc);
+    case 1: return cvg::cr_cos(x, sc);
+    case 2: return cvg::cr_atan2(y, x, atb);
+    case 3: return sqrt(x);
+    case 4: return cvg::wrap_two_pi(x);
+    case 5: return fmod(x, y);
+    default: return 0.0;
+    }
+}
+
+__global__ void k_math_eval(int op, int n, const double *__restrict__ x, const double *__restrict__ y,
+                            double *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = math_op(op, x[i], y ? y[i] : 0.0, c_sincos_tab, c_atan_tab);
+}
+
+// ---------------------------------------------------------------------------
+// Host orchestration
+// ---------------------------------------------------------------------------
+
+int read_scalars(cvg_context *ctx)
+{
+    CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox, ctx->d_scal.ptr, S_COUNT * sizeof(unsigned), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return CVG_OK;
+}
+
+int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double rad, unsigned R, unsigned max_cand)
+{
+    ctx->graph_ready = false;
+    ctx->curvelets_ready = false;
+    if (n <= 0 || sz < 4 || d_edges == nullptr)  // preprocess.hpp:197
+        return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "build_neighbor_csr_graph: need num_edges > 0, sz_edge_data >= 4");
+    if (R > 4096) return fail(ctx, CVG_ERR_UNSUPPORTED, "neighbor_radius above 4096 cells");
+    if (max_cand == 0) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "max_candidates must be positive");
+    ctx->n = n;
+    ctx->sz = sz;
+    ctx->rad = rad;
+    ctx->nbr_radius = R;
+    ctx->max_cand = max_cand;
+    hipStream_t st = ctx->stream;
+    const size_t nd = static_cast<size_t>(n) * sizeof(double), ni = static_cast<size_t>(n) * sizeof(int);
+    CVG_TRY(ensure(ctx, ctx->d_x, nd));
+    CVG_TRY(ensure(ctx, ctx->d_y, nd));
+    CVG_TRY(ensure(ctx, ctx->d_th, nd));
+    CVG_TRY(ensure(ctx, ctx->d_cx, ni));
+    CVG_TRY(ensure(ctx, ctx->d_cy, ni));
+    CVG_TRY(ensure(ctx, ctx->d_scal, S_COUNT * sizeof(unsigned)));
+    unsigned *scal = as<unsigned>(ctx->d_scal);
+    hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, st, scal);
+    hipLaunchKernelGGL(k_prep_edges, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, sz, d_edges, as<double>(ctx->d_x),
+                       as<double>(ctx->d_y), as<double>(ctx->d_th), as<int>(ctx->d_cx), as<int>(ctx->d_cy), scal);
+    CVG_HIP(ctx, hipGetLastError());
+    CVG_TRY(read_scalars(ctx));
+    const unsigned *mb = ctx->mailbox;
+    if (mb[S_BAD])
+        return fail(ctx, CVG_ERR_BAD_INPUT,
+                    "edge values out of range: need finite x, y with |x|,|y| < 2^30 and |theta| < 2^20");
+    const int minx = static_cast<int>(mb[S_MINX]), maxx = static_cast<int>(mb[S_MAXX]);
+    const int miny = static_cast<int>(mb[S_MINY]), maxy = static_cast<int>(mb[S_MAXY]);
+    GridDesc g;
+    g.x0 = minx - static_cast<int>(R);
+    g.y0 = miny - static_cast<int>(R);
+    const long long w = static_cast<long long>(maxx) - minx + 1 + 2LL * R;
+    const long long h = static_cast<long long>(maxy) - miny + 1 + 2LL * R;
+    if (w * h > kMaxGridCells)
+        return fail(ctx, CVG_ERR_UNSUPPORTED,
+                    "edge coordinates span " + std::to_string(w) + " x " + std::to_string(h) +
+                        " pixel cells; the dense cell grid supports up to 2^28 cells");
+    g.w = static_cast<int>(w);
+    g.h = static_cast<int>(h);
+    ctx->gx0 = g.x0;
+    ctx->gy0 = g.y0;
+    ctx->gw = g.w;
+    ctx->gh = g.h;
+    const long long cells = w * h;
+
+    // counting sort into cells
+    CVG_TRY(ensure(ctx, ctx->d_grid, static_cast<size_t>(2 * cells + 1) * sizeof(unsigned)));
+    unsigned *cnt = as<unsigned>(ctx->d_grid);
+    unsigned *start = cnt + cells;
+    CVG_HIP(ctx, hipMemsetAsync(cnt, 0, static_cast<size_t>(cells) * sizeof(unsigned), st));
+    hipLaunchKernelGGL(k_cell_count, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
+                       as<int>(ctx->d_cy), g, cnt);
+    CVG_TRY(exclusive_scan(ctx, cnt, cells, start, ctx->d_tiles));
+    CVG_TRY(ensure(ctx, ctx->d_tmp_ids, ni));
+    CVG_TRY(ensure(ctx, ctx->d_sorted, ni));
+    CVG_TRY(ensure(ctx, ctx->d_sx, nd));
+    CVG_TRY(ensure(ctx, ctx->d_sy, nd));
+    hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
+                       as<int>(ctx->d_cy), g, start, cnt, as<int>(ctx->d_tmp_ids));
+    hipLaunchKernelGGL(k_cell_order, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
+                       as<int>(ctx->d_cy), g, start, as<int>(ctx->d_tmp_ids), as<double>(ctx->d_x),
+                       as<double>(ctx->d_y), as<int>(ctx->d_sorted), as<double>(ctx->d_sx), as<double>(ctx->d_sy));
+    CVG_HIP(ctx, hipGetLastError());
+
+    // neighbour counts and CSR offsets
+    CVG_TRY(ensure(ctx, ctx->d_deg, ni));
+    CVG_TRY(ensure(ctx, ctx->d_off, static_cast<size_t>(n + 1) * sizeof(unsigned)));
+    const double rad2 = rad * rad;  // preprocess.hpp:221
+    hipLaunchKernelGGL(k_discover_count, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<double>(ctx->d_x),
+                       as<double>(ctx->d_y), as<int>(ctx->d_cx), as<int>(ctx->d_cy), g, static_cast<int>(R), rad2,
+                       max_cand, start, as<int>(ctx->d_sorted), as<double>(ctx->d_sx), as<double>(ctx->d_sy),
+                       as<unsigned>(ctx->d_deg), scal);
+    CVG_HIP(ctx, hipGetLastError());
+    CVG_TRY(exclusive_scan(ctx, as<unsigned>(ctx->d_deg), n, as<unsigned>(ctx->d_off), ctx->d_tiles));
+    unsigned pairs_u = 0;
+    CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox + S_COUNT, as<unsigned>(ctx->d_off) + n, sizeof(unsigned),
+                                hipMemcpyDeviceToHost, st));
+    CVG_TRY(read_scalars(ctx));
+    pairs_u = ctx->mailbox[S_COUNT];
+    if (pairs_u > static_cast<unsigned>(INT_MAX))
+        return fail(ctx, CVG_ERR_UNSUPPORTED, "more than 2^31-1 neighbour pairs");
+    const int pairs = static_cast<int>(pairs_u);
+    ctx->pairs = pairs;
+    ctx->max_deg = ctx->mailbox[S_MAXDEG];
+    if (ctx->max_deg >= (1u << 23))
+        return fail(ctx, CVG_ERR_UNSUPPORTED, "an anchor has 2^23 or more neighbours");
+    ctx->truncated = ctx->mailbox[S_TRUNC];
+
+    // gather, sort, fill
+    const size_t np = static_cast<size_t>(pairs);
+    CVG_TRY(ensure(ctx, ctx->d_cand, np * sizeof(Cand)));
+    CVG_TRY(ensure(ctx, ctx->d_ids, np * sizeof(int)));
+    CVG_TRY(ensure(ctx, ctx->d_d2, np * sizeof(double)));
+    CVG_TRY(ensure(ctx, ctx->d_pair_anchor, np * sizeof(int)));
+    hipLaunchKernelGGL(k_fill_csr, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<double>(ctx->d_x),
+                       as<double>(ctx->d_y), as<int>(ctx->d_cx), as<int>(ctx->d_cy), g, static_cast<int>(R), rad2,
+                       start, as<int>(ctx->d_sorted), as<double>(ctx->d_sx), as<double>(ctx->d_sy),
+                       as<unsigned>(ctx->d_off), as<Cand>(ctx->d_cand), as<int>(ctx->d_ids), as<double>(ctx->d_d2),
+                       as<int>(ctx->d_pair_anchor), as<int>(ctx->d_tmp_ids), scal);
+    hipLaunchKernelGGL(k_tie_sort, dim3(std::min(blocks_for(n), 64)), dim3(kBlock), 0, st, as<int>(ctx->d_tmp_ids),
+                       scal, as<unsigned>(ctx->d_off), as<Cand>(ctx->d_cand), as<int>(ctx->d_ids),
+                       as<double>(ctx->d_d2));
+    CVG_HIP(ctx, hipGetLastError());
+    ctx->graph_ready = true;
+    return CVG_OK;
+}
+
+struct Derived {
+    double dt, dx, dx2, close_thr, max_k, nrad;
+    unsigned kx, kt, K, G;
+    int cpl;
+    std::vector<double> off_x, off_t, rank_ox;
+    std::vector<int> rank_j, rank_cell, rank_elig;
+    int rank_of_cell0;
+};
+
+int derive(cvg_context *ctx, const cvg_params &p, Derived &d)
+{
+    if (!(p.sx > 0) || !(p.st > 0))
+        return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "sx and st must be positive");
+    d.dt = (p.dt_deg / 180.0) * 3.14159265358979323846;  // main.cpp:24
+    d.dx = p.dx;
+    d.dx2 = p.dx * p.dx;                                   // cpu_curvelet.hpp:654
+    d.close_thr = 4 * p.dx * p.dx;                         // :682, (4*dx)*dx
+    d.max_k = p.max_k;
+    d.nrad = p.nrad;
+    const double fx = 2 * std::floor(p.dx / p.sx + 0.5) + 1;  // :73-74
+    const double ft = 2 * std::floor(d.dt / p.st + 0.5) + 1;
+    if (!(fx >= 1 && ft >= 1) || fx * ft > 256)
+        return fail(ctx, CVG_ERR_UNSUPPORTED, "bundle grid must have between 1 and 256 cells (dx/sx, dt/st)");
+    d.kx = static_cast<unsigned>(fx);
+    d.kt = static_cast<unsigned>(ft);
+    d.K = d.kx * d.kt;
+    d.cpl = d.K <= 64 ? 1 : (d.K <= 128 ? 2 : 4);
+    if (p.group_max_sz > 255) return fail(ctx, CVG_ERR_UNSUPPORTED, "group_max_sz above 255");
+    d.G = p.group_max_sz;
+    d.off_x.resize(d.kx);
+    d.off_t.resize(d.kt);
+    for (unsigned i = 0; i < d.kx; ++i) d.off_x[i] = p.sx * ((double)i - (double(d.kx) - 1.0) / 2.0);  // :697
+    for (unsigned j = 0; j < d.kt; ++j) d.off_t[j] = p.st * ((double)j - (double(d.kt) - 1.0) / 2.0);  // :698
+    std::vector<int> order(d.K);
+    std::vector<double> dist(d.K);
+    for (unsigned c = 0; c < d.K; ++c) {
+        const double ox = d.off_x[c / d.kt], ot = d.off_t[c % d.kt];
+        dist[c] = ox * ox + ot * ot;  // fill_curvelet_info's d, :211
+        order[c] = static_cast<int>(c);
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int u, int v) { return dist[u] < dist[v]; });
+    d.rank_ox.resize(d.K);
+    d.rank_j.resize(d.K);
+    d.rank_cell.resize(d.K);
+    d.rank_elig.resize(d.K);
+    for (unsigned r = 0; r < d.K; ++r) {
+        const int c = order[r];
+        d.rank_cell[r] = c;
+        d.rank_ox[r] = d.off_x[c / d.kt];
+        d.rank_j[r] = c % d.kt;
+        d.rank_elig[r] = dist[c] < 100.0 ? 1 : 0;
+        if (c == 0) d.rank_of_cell0 = static_cast<int>(r);
+    }
+    return CVG_OK;
+}
+
+template <typename T>
+size_t put(std::vector<char> &blob, const std::vector<T> &v)
+{
+    const size_t align = 16;
+    size_t off = (blob.size() + align - 1) / align * align;
+    blob.resize(off + v.size() * sizeof(T));
+    if (!v.empty()) std::memcpy(blob.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+}
+
+int build_curvelets(cvg_context *ctx, const cvg_params &p)
+{
+    if (!ctx->graph_ready) return fail(ctx, CVG_ERR_STATE, "build the neighbour graph first");
+    ctx->curvelets_ready = false;
+    Derived d;
+    CVG_TRY(derive(ctx, p, d));
+    hipStream_t st = ctx->stream;
+    const int n = ctx->n;
+    const int pairs = ctx->pairs;
+
+    // bucket capacities (neighbour slots per workgroup)
+    std::vector<int> caps = {8, 16, 24, 32, 48, 64};
+    const int maxcap = static_cast<int>(std::max<unsigned>(ctx->max_deg, 1u));
+    while (!caps.empty() && caps.back() >= maxcap) caps.pop_back();
+    caps.push_back(maxcap);
+    const int nb = static_cast<int>(caps.size());
+    if (nb > kMaxBuckets) return fail(ctx, CVG_ERR_STATE, "internal: too many buckets");
+
+    // parameter tables
+    std::vector<char> blob;
+    const size_t o_rank_ox = put(blob, d.rank_ox);
+    const size_t o_rank_j = put(blob, d.rank_j);
+    const size_t o_rank_cell = put(blob, d.rank_cell);
+    const size_t o_rank_elig = put(blob, d.rank_elig);
+    const size_t o_off_x = put(blob, d.off_x);
+    const size_t o_off_t = put(blob, d.off_t);
+    const size_t o_caps = put(blob, caps);
+    CVG_TRY(ensure(ctx, ctx->d_tables, blob.size()));
+    CVG_HIP(ctx, hipMemcpyAsync(ctx->d_tables.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, st));
+    char *tb = static_cast<char *>(ctx->d_tables.ptr);
+    Tables T;
+    T.rank_ox = reinterpret_cast<double *>(tb + o_rank_ox);
+    T.rank_j = reinterpret_cast<int *>(tb + o_rank_j);
+    T.rank_cell = reinterpret_cast<int *>(tb + o_rank_cell);
+    T.rank_elig = reinterpret_cast<int *>(tb + o_rank_elig);
+    T.off_x = reinterpret_cast<double *>(tb + o_off_x);
+    T.off_t = reinterpret_cast<double *>(tb + o_off_t);
+    T.caps = reinterpret_cast<int *>(tb + o_caps);
+
+    // per-edge trig, per-pair classes, buckets
+    const int ntrig = cvg::TRIG_GRID + 2 * static_cast<int>(d.kt);
+    CVG_TRY(ensure(ctx, ctx->d_trig, static_cast<size_t>(ntrig) * n * sizeof(double)));
+    CVG_TRY(ensure(ctx, ctx->d_cls, static_cast<size_t>(pairs) + 1));
+    CVG_TRY(ensure(ctx, ctx->d_cnt3, static_cast<size_t>(3) * n * sizeof(unsigned)));
+    CVG_TRY(ensure(ctx, ctx->d_lists, static_cast<size_t>(nb) * n * sizeof(int)));
+    unsigned *scal = as<unsigned>(ctx->d_scal);
+    hipLaunchKernelGGL(k_edge_trig, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<double>(ctx->d_th), d.dt,
+                       static_cast<int>(d.kt), T.off_t, as<double>(ctx->d_trig));
+    CVG_HIP(ctx, hipMemsetAsync(ctx->d_cnt3.ptr, 0, static_cast<size_t>(3) * n * sizeof(unsigned), st));
+    CVG_HIP(ctx, hipMemsetAsync(scal + S_VALID, 0, (S_COUNT - S_VALID) * sizeof(unsigned), st));
+    if (pairs > 0)
+        hipLaunchKernelGGL(k_pair_class, dim3(blocks_for(pairs)), dim3(kBlock), 0, st, pairs, n,
+                           as<int>(ctx->d_pair_anchor), as<int>(ctx->d_ids), as<double>(ctx->d_x),
+                           as<double>(ctx->d_y), as<double>(ctx->d_trig), as<unsigned char>(ctx->d_cls),
+                           as<unsigned>(ctx->d_cnt3));
+    hipLaunchKernelGGL(k_bucket, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_sorted),
+                       as<unsigned>(ctx->d_cnt3), nb, T.caps, as<int>(ctx->d_lists), scal);
+    CVG_HIP(ctx, hipGetLastError());
+    CVG_TRY(read_scalars(ctx));
+    std::vector<unsigned> bucket_count(nb);
+    for (int b = 0; b < nb; ++b) bucket_count[b] = ctx->mailbox[S_BUCKET + b];
+    ctx->ambiguous = ctx->mailbox[S_AMBIG];
+
+    // staging
+    const int st_stride = static_cast<int>(std::max(d.G, 1u));
+    const int chain_stride = static_cast<int>(std::max(d.G, 2u));
+    const size_t np = static_cast<size_t>(std::max(pairs, 1));
+    CVG_TRY(ensure(ctx, ctx->d_st_ids, np * st_stride * sizeof(int)));
+    CVG_TRY(ensure(ctx, ctx->d_st_meta, np * sizeof(unsigned)));
+    CVG_TRY(ensure(ctx, ctx->d_st_k, np * 2 * sizeof(double)));
+    CVG_TRY(ensure(ctx, ctx->d_acc, static_cast<size_t>(n) * sizeof(unsigned)));
+    CVG_TRY(ensure(ctx, ctx->d_rowoff, static_cast<size_t>(n + 1) * sizeof(unsigned)));
+
+    FusedArgs A{};
+    A.n = n;
+    A.x = as<double>(ctx->d_x);
+    A.y = as<double>(ctx->d_y);
+    A.th = as<double>(ctx->d_th);
+    A.trig = as<double>(ctx->d_trig);
+    A.off = as<unsigned>(ctx->d_off);
+    A.ids = as<int>(ctx->d_ids);
+    A.cls = as<unsigned char>(ctx->d_cls);
+    A.chain_stride = chain_stride;
+    A.st_stride = st_stride;
+    A.dt = d.dt;
+    A.dx = d.dx;
+    A.dx2 = d.dx2;
+    A.close_thr = d.close_thr;
+    A.max_k = d.max_k;
+    A.kt = static_cast<int>(d.kt);
+    A.K = static_cast<int>(d.K);
+    A.G = static_cast<int>(d.G);
+    A.rank_ox = T.rank_ox;
+    A.rank_j = T.rank_j;
+    A.rank_cell = T.rank_cell;
+    A.rank_elig = T.rank_elig;
+    A.rank_of_cell0 = d.rank_of_cell0;
+    A.off_t = T.off_t;
+    A.st_ids = as<int>(ctx->d_st_ids);
+    A.st_meta = as<unsigned>(ctx->d_st_meta);
+    A.st_k = as<double>(ctx->d_st_k);
+    A.acc_count = as<unsigned>(ctx->d_acc);
+    A.valid_pairs = scal + S_VALID;
+
+    for (int b = 0; b < nb; ++b) {
+        const int len = static_cast<int>(bucket_count[b]);
+        if (len == 0) continue;
+        A.list = as<int>(ctx->d_lists) + static_cast<size_t>(b) * n;
+        A.list_len = len;
+        A.nb_cap = caps[b];
+        bool lds_bundles = true;
+        Carve cv = carve(A.nb_cap, A.kt, A.K, chain_stride, true);
+        if (cv.bytes > ctx->lds_limit) {
+            lds_bundles = false;
+            cv = carve(A.nb_cap, A.kt, A.K, chain_stride, false);
+            if (cv.bytes > ctx->lds_limit)
+                return fail(ctx, CVG_ERR_UNSUPPORTED, "neighbour lists too long for the workgroup's LDS");
+            CVG_TRY(ensure(ctx, ctx->d_gbundle, static_cast<size_t>(len) * A.nb_cap * A.K * 2 * sizeof(double)));
+            A.g_bundle = as<double>(ctx->d_gbundle);
+        } else {
+            A.g_bundle = nullptr;
+        }
+        const int grid = (len + 7) / 8 * 8;
+        const unsigned shmem = static_cast<unsigned>(cv.bytes);
+        if (d.cpl == 1) {
+            CVG_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(k_curvelets<1>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
+            hipLaunchKernelGGL(k_curvelets<1>, dim3(grid), dim3(64 * kWaves), shmem, st, A);
+        } else if (d.cpl == 2) {
+            CVG_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(k_curvelets<2>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
+            hipLaunchKernelGGL(k_curvelets<2>, dim3(grid), dim3(64 * kWaves), shmem, st, A);
+        } else {
+            CVG_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(k_curvelets<4>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
+            hipLaunchKernelGGL(k_curvelets<4>, dim3(grid), dim3(64 * kWaves), shmem, st, A);
+        }
+        CVG_HIP(ctx, hipGetLastError());
+        (void)lds_bundles;
+    }
+    // anchors with an empty bucket list never ran: their counts must be zero
+    // (every anchor is in exactly one bucket, so all acc_count entries are written)
+
+    CVG_TRY(exclusive_scan(ctx, as<unsigned>(ctx->d_acc), n, as<unsigned>(ctx->d_rowoff), ctx->d_tiles));
+    CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox + S_COUNT, as<unsigned>(ctx->d_rowoff) + n, sizeof(unsigned),
+                                hipMemcpyDeviceToHost, st));
+    CVG_TRY(read_scalars(ctx));
+    const unsigned C = ctx->mailbox[S_COUNT];
+    ctx->valid_pairs = ctx->mailbox[S_VALID];
+    ctx->num_curvelets = C;
+    ctx->width = d.G + 1;
+
+    CVG_TRY(ensure(ctx, ctx->d_chains, static_cast<size_t>(C) * ctx->width * sizeof(unsigned)));
+    CVG_TRY(ensure(ctx, ctx->d_info, static_cast<size_t>(C) * kInfoWidth * sizeof(double)));
+    RecordArgs R{};
+    R.n = n;
+    R.x = A.x;
+    R.y = A.y;
+    R.th = A.th;
+    R.off = A.off;
+    R.acc_count = A.acc_count;
+    R.row_off = as<unsigned>(ctx->d_rowoff);
+    R.st_ids = A.st_ids;
+    R.st_meta = A.st_meta;
+    R.st_k = A.st_k;
+    R.st_stride = st_stride;
+    R.G = static_cast<int>(d.G);
+    R.kt = static_cast<int>(d.kt);
+    R.nrad = d.nrad;
+    R.off_x = T.off_x;
+    R.off_t = T.off_t;
+    R.chains = as<unsigned>(ctx->d_chains);
+    R.info = as<double>(ctx->d_info);
+    if (C > 0) hipLaunchKernelGGL(k_record, dim3(blocks_for(n)), dim3(kBlock), 0, st, R);
+    CVG_HIP(ctx, hipGetLastError());
+    ctx->cur_params = p;
+    ctx->curvelets_ready = true;
+    return CVG_OK;
+}
+
+bool same_params(const cvg_params &a, const cvg_params &b)
+{
+    return a.nrad == b.nrad && a.dx == b.dx && a.dt_deg == b.dt_deg && a.max_k == b.max_k &&
+           a.group_max_sz == b.group_max_sz && a.sx == b.sx && a.st == b.st;
+}
+
+int check_ctx(cvg_context *ctx)
+{
+    if (!ctx) return CVG_ERR_INVALID_ARGUMENT;
+    CVG_HIP(ctx, hipSetDevice(ctx->device));
+    return CVG_OK;
+}
+
+int run_common(cvg_context *ctx, int n, int sz, const double *d_edges, const cvg_params *p, cvg_run_stats *stats)
+{
+    CVG_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    CVG_TRY(build_graph(ctx, n, sz, d_edges, p->nrad, 3u, p->max_candidates));  // main.cpp:48 passes 3
+    CVG_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    CVG_TRY(build_curvelets(ctx, *p));
+    CVG_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+    CVG_HIP(ctx, hipEventSynchronize(ctx->ev[2]));
+    if (stats) {
+        float t01 = 0, t12 = 0, t02 = 0;
+        (void)hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]);
+        (void)hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]);
+        (void)hipEventElapsedTime(&t02, ctx->ev[0], ctx->ev[2]);
+        stats->valid_pairs = ctx->valid_pairs;
+        stats->num_curvelets = ctx->num_curvelets;
+        stats->chain_width = ctx->width;
+        stats->total_neighbor_pairs = ctx->pairs;
+        stats->max_num_of_neighbors = ctx->max_deg;
+        stats->ms_graph = t01;
+        stats->ms_curvelets = t12;
+        stats->ms_total = t02;
+    }
+    return CVG_OK;
+}
+
+int upload(cvg_context *ctx, int n, int sz, const double *host, const double **dptr)
+{
+    if (!host || n <= 0 || sz < 1) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "upload: bad edge array");
+    const size_t bytes = static_cast<size_t>(n) * sz * sizeof(double);
+    CVG_TRY(ensure(ctx, ctx->d_edges_up, bytes));
+    CVG_HIP(ctx, hipMemcpyAsync(ctx->d_edges_up.ptr, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    *dptr = as<double>(ctx->d_edges_up);
+    return CVG_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+void cvg_params_default(cvg_params *p)
+{
+    if (!p) return;
+    p->nrad = 3.5;
+    p->gap = 1.5;
+    p->dx = 0.4;
+    p->dt_deg = 15.0;
+    p->max_k = 0.3;
+    p->curvelet_style = 2;
+    p->group_max_sz = 4;
+    p->out_type = 0;
+    p->sx = 0.1;
+    p->st = 0.08;
+    p->edge_data_sz = 4;
+    p->max_candidates = 128;
+}
+
+const char *cvg_status_string(int status)
+{
+    switch (status) {
+    case CVG_OK: return "ok";
+    case CVG_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case CVG_ERR_BAD_INPUT: return "bad input";
+    case CVG_ERR_IO: return "i/o error";
+    case CVG_ERR_OUT_OF_MEMORY: return "out of memory";
+    case CVG_ERR_DEVICE: return "device error";
+    case CVG_ERR_UNSUPPORTED: return "unsupported";
+    case CVG_ERR_STATE: return "bad call order";
+    default: return "unknown status";
+    }
+}
+
+int cvg_create(int device, cvg_context **out_ctx)
+{
+    if (!out_ctx) return CVG_ERR_INVALID_ARGUMENT;
+    *out_ctx = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return CVG_ERR_DEVICE;
+    if (device < 0 || device >= count) return CVG_ERR_INVALID_ARGUMENT;
+    cvg_context *ctx = new (std::nothrow) cvg_context();
+    if (!ctx) return CVG_ERR_OUT_OF_MEMORY;
+    ctx->device = device;
+    int rc = CVG_OK;
+    do {
+        if (hipSetDevice(device) != hipSuccess) { rc = CVG_ERR_DEVICE; break; }
+        if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { rc = CVG_ERR_DEVICE; break; }
+        for (auto &e : ctx->ev)
+            if (hipEventCreate(&e) != hipSuccess) rc = CVG_ERR_DEVICE;
+        if (rc != CVG_OK) break;
+        if (hipHostMalloc(reinterpret_cast<void **>(&ctx->mailbox), 64 * sizeof(unsigned), hipHostMallocDefault) !=
+            hipSuccess) {
+            rc = CVG_ERR_OUT_OF_MEMORY;
+            break;
+        }
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+            size_t lim = prop.sharedMemPerBlock;
+            if (prop.maxSharedMemoryPerMultiProcessor > 0)
+                lim = std::min(lim, static_cast<size_t>(prop.maxSharedMemoryPerMultiProcessor));
+            if (lim > 0) ctx->lds_limit = lim;
+        }
+    } while (false);
+    if (rc != CVG_OK) {
+        cvg_destroy(ctx);
+        return rc;
+    }
+    *out_ctx = ctx;
+    return CVG_OK;
+}
+
+void cvg_destroy(cvg_context *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    DevBuf *bufs[] = {&ctx->d_edges_up, &ctx->d_x,     &ctx->d_y,       &ctx->d_th,      &ctx->d_cx,
+                      &ctx->d_cy,       &ctx->d_scal,  &ctx->d_grid,    &ctx->d_tmp_ids, &ctx->d_sorted,
+                      &ctx->d_sx,       &ctx->d_sy,    &ctx->d_deg,     &ctx->d_off,     &ctx->d_tiles,
+                      &ctx->d_cand,     &ctx->d_ids,   &ctx->d_d2,      &ctx->d_pair_anchor, &ctx->d_trig,
+                      &ctx->d_cls,      &ctx->d_cnt3,  &ctx->d_lists,   &ctx->d_tables,  &ctx->d_st_ids,
+                      &ctx->d_st_meta,  &ctx->d_st_k,  &ctx->d_acc,     &ctx->d_rowoff,  &ctx->d_gbundle,
+                      &ctx->d_chains,   &ctx->d_info,  &ctx->d_dbg};
+    for (DevBuf *b : bufs) release(*b);
+    for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->mailbox) (void)hipHostFree(ctx->mailbox);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *cvg_last_error(const cvg_context *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int cvg_build_neighbor_csr_graph(cvg_context *ctx, int num_edges, int sz_edge_data, const double *edges,
+                                 double rad, unsigned neighbor_radius, unsigned max_candidates)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!edges || num_edges <= 0 || sz_edge_data < 4)
+        return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "build_neighbor_csr_graph: need num_edges > 0, sz_edge_data >= 4");
+    const double *d = nullptr;
+    CVG_TRY(upload(ctx, num_edges, sz_edge_data, edges, &d));
+    CVG_TRY(build_graph(ctx, num_edges, sz_edge_data, d, rad, neighbor_radius, max_candidates));
+    CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return CVG_OK;
+}
+
+int cvg_get_graph_info(cvg_context *ctx, cvg_graph_info *info)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!info) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "null info");
+    if (!ctx->graph_ready) return fail(ctx, CVG_ERR_STATE, "no neighbour graph");
+    info->num_edges = ctx->n;
+    info->total_neighbor_pairs = ctx->pairs;
+    info->max_num_of_neighbors = ctx->max_deg;
+    info->truncated_anchors = ctx->truncated;
+    // the ambiguity count is produced by the fill kernel; read it now
+    CVG_TRY(read_scalars(ctx));
+    ctx->ambiguous = ctx->mailbox[S_AMBIG];
+    info->order_ambiguous_anchors = ctx->ambiguous;
+    return CVG_OK;
+}
+
+int cvg_get_csr(cvg_context *ctx, int *offsets, int *ids, double *dist2)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!ctx->graph_ready) return fail(ctx, CVG_ERR_STATE, "no neighbour graph");
+    hipStream_t st = ctx->stream;
+    if (offsets)
+        CVG_HIP(ctx, hipMemcpyAsync(offsets, ctx->d_off.ptr, static_cast<size_t>(ctx->n + 1) * sizeof(int),
+                                    hipMemcpyDeviceToHost, st));
+    if (ids && ctx->pairs > 0)
+        CVG_HIP(ctx, hipMemcpyAsync(ids, ctx->d_ids.ptr, static_cast<size_t>(ctx->pairs) * sizeof(int),
+                                    hipMemcpyDeviceToHost, st));
+    if (dist2 && ctx->pairs > 0)
+        CVG_HIP(ctx, hipMemcpyAsync(dist2, ctx->d_d2.ptr, static_cast<size_t>(ctx->pairs) * sizeof(double),
+                                    hipMemcpyDeviceToHost, st));
+    CVG_HIP(ctx, hipStreamSynchronize(st));
+    return CVG_OK;
+}
+
+int cvg_form_pairwise_bundles(cvg_context *ctx, const cvg_params *params, unsigned *valid_pairs)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!params) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "null params");
+    CVG_TRY(build_curvelets(ctx, *params));
+    CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (valid_pairs) *valid_pairs = ctx->valid_pairs;
+    return CVG_OK;
+}
+
+int cvg_build_curvelets_greedy(cvg_context *ctx, const cvg_params *params, unsigned *num_curvelets)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!params) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "null params");
+    if (!(ctx->curvelets_ready && same_params(ctx->cur_params, *params))) {
+        CVG_TRY(build_curvelets(ctx, *params));
+        CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (num_curvelets) *num_curvelets = ctx->num_curvelets;
+    return CVG_OK;
+}
+
+int cvg_curvelet_shape(cvg_context *ctx, unsigned *num_curvelets, unsigned *chain_width)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!ctx->curvelets_ready) return fail(ctx, CVG_ERR_STATE, "no curvelets");
+    if (num_curvelets) *num_curvelets = ctx->num_curvelets;
+    if (chain_width) *chain_width = ctx->width;
+    return CVG_OK;
+}
+
+int cvg_get_curvelets(cvg_context *ctx, unsigned *chains, double *info)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!ctx->curvelets_ready) return fail(ctx, CVG_ERR_STATE, "no curvelets");
+    const size_t C = ctx->num_curvelets;
+    if (chains && C > 0)
+        CVG_HIP(ctx, hipMemcpyAsync(chains, ctx->d_chains.ptr, C * ctx->width * sizeof(unsigned),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+    if (info && C > 0)
+        CVG_HIP(ctx, hipMemcpyAsync(info, ctx->d_info.ptr, C * kInfoWidth * sizeof(double), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+    CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return CVG_OK;
+}
+
+int cvg_run(cvg_context *ctx, int num_edges, int sz_edge_data, const double *edges, const cvg_params *params,
+            cvg_run_stats *stats)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!params || !edges || num_edges <= 0 || sz_edge_data < 4)
+        return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "cvg_run: bad arguments");
+    const double *d = nullptr;
+    CVG_TRY(upload(ctx, num_edges, sz_edge_data, edges, &d));
+    return run_common(ctx, num_edges, sz_edge_data, d, params, stats);
+}
+
+int cvg_run_device(cvg_context *ctx, int num_edges, int sz_edge_data, const double *device_edges,
+                   const cvg_params *params, cvg_run_stats *stats)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!params || !device_edges || num_edges <= 0 || sz_edge_data < 4)
+        return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "cvg_run_device: bad arguments");
+    return run_common(ctx, num_edges, sz_edge_data, device_edges, params, stats);
+}
+
+int cvg_upload_edges(cvg_context *ctx, int num_edges, int sz_edge_data, const double *edges,
+                     const double **device_edges)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!device_edges) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "null output pointer");
+    CVG_TRY(upload(ctx, num_edges, sz_edge_data, edges, device_edges));
+    CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return CVG_OK;
+}
+
+int cvg_device_curvelets(cvg_context *ctx, const unsigned **device_chains, const double **device_info)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!ctx->curvelets_ready) return fail(ctx, CVG_ERR_STATE, "no curvelets");
+    if (device_chains) *device_chains = as<unsigned>(ctx->d_chains);
+    if (device_info) *device_info = as<double>(ctx->d_info);
+    return CVG_OK;
+}
+
+int cvg_get_stream(cvg_context *ctx, void **hip_stream)
+{
+    if (!ctx || !hip_stream) return CVG_ERR_INVALID_ARGUMENT;
+    *hip_stream = ctx->stream;
+    return CVG_OK;
+}
+
+int cvg_synchronize(cvg_context *ctx)
+{
+    CVG_TRY(check_ctx(ctx));
+    CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return CVG_OK;
+}
+
+int cvg_debug_anchor_bundles(cvg_context *ctx, const cvg_params *params, int anchor, unsigned char *dir,
+                             unsigned char *hyp, double *bmin, double *bmax, int *deg_out, int *cells_out)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!params) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "null params");
+    if (!ctx->graph_ready) return fail(ctx, CVG_ERR_STATE, "no neighbour graph");
+    if (anchor < 0 || anchor >= ctx->n) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "anchor out of range");
+    Derived d;
+    CVG_TRY(derive(ctx, *params, d));
+    unsigned off2[2];
+    CVG_HIP(ctx, hipMemcpyAsync(off2, as<unsigned>(ctx->d_off) + anchor, 2 * sizeof(unsigned), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const int deg = static_cast<int>(off2[1] - off2[0]);
+    const int K = static_cast<int>(d.K);
+    if (deg_out) *deg_out = deg;
+    if (cells_out) *cells_out = K;
+    if (deg == 0) return CVG_OK;
+    std::vector<char> blob;
+    const size_t o_x = put(blob, d.off_x);
+    const size_t o_t = put(blob, d.off_t);
+    const size_t o_out = (blob.size() + 15) / 16 * 16;
+    const size_t out_doubles = static_cast<size_t>(2) * deg * K + 2 * deg;
+    CVG_TRY(ensure(ctx, ctx->d_dbg, o_out + out_doubles * sizeof(double)));
+    char *base = static_cast<char *>(ctx->d_dbg.ptr);
+    CVG_HIP(ctx, hipMemcpyAsync(base, blob.data(), blob.size(), hipMemcpyHostToDevice, ctx->stream));
+    double *out = reinterpret_cast<double *>(base + o_out);
+    hipLaunchKernelGGL(k_debug_anchor, dim3(blocks_for(static_cast<long long>(deg) * K)), dim3(kBlock), 0, ctx->stream,
+                       anchor, ctx->n, as<double>(ctx->d_x), as<double>(ctx->d_y), as<double>(ctx->d_th),
+                       as<unsigned>(ctx->d_off), as<int>(ctx->d_ids), d.dt, d.dx, d.dx2, d.close_thr, d.max_k,
+                       static_cast<int>(d.kx), static_cast<int>(d.kt), reinterpret_cast<double *>(base + o_x),
+                       reinterpret_cast<double *>(base + o_t), out);
+    CVG_HIP(ctx, hipGetLastError());
+    std::vector<double> h(out_doubles);
+    CVG_HIP(ctx, hipMemcpyAsync(h.data(), out, out_doubles * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int r = 0; r < deg; ++r) {
+        bool valid = false;
+        for (int c = 0; c < K; ++c) {
+            const double lo = h[static_cast<size_t>(r) * K + c];
+            const double hi = h[static_cast<size_t>(deg) * K + static_cast<size_t>(r) * K + c];
+            if (bmin) bmin[static_cast<size_t>(r) * K + c] = lo;
+            if (bmax) bmax[static_cast<size_t>(r) * K + c] = hi;
+            valid = valid || (hi > lo);
+        }
+        const bool close = h[static_cast<size_t>(2) * deg * K + deg + r] != 0.0;
+        if (dir) dir[r] = static_cast<unsigned char>(h[static_cast<size_t>(2) * deg * K + r]);
+        if (hyp) hyp[r] = (close || valid) ? 1 : 0;
+    }
+    return CVG_OK;
+}
+
+int cvg_math_eval(cvg_context *ctx, int op, int n, const double *x, const double *y, double *out)
+{
+    if (!x || !out || n < 0 || op < 0 || op > 5) return CVG_ERR_INVALID_ARGUMENT;
+    if ((op == 2 || op == 5) && !y) return CVG_ERR_INVALID_ARGUMENT;
+    if (!ctx) {
+        for (int i = 0; i < n; ++i)
+            out[i] = math_op(op, x[i], y ? y[i] : 0.0, cvg::kSinCosTableHost, cvg::kAtanTableHost);
+        return CVG_OK;
+    }
+    CVG_TRY(check_ctx(ctx));
+    if (n == 0) return CVG_OK;
+    const size_t bytes = static_cast<size_t>(n) * sizeof(double);
+    CVG_TRY(ensure(ctx, ctx->d_dbg, 3 * bytes));
+    double *dx = as<double>(ctx->d_dbg);
+    double *dy = dx + n;
+    double *dout = dy + n;
+    CVG_HIP(ctx, hipMemcpyAsync(dx, x, bytes, hipMemcpyHostToDevice, ctx->stream));
+    if (y) CVG_HIP(ctx, hipMemcpyAsync(dy, y, bytes, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_math_eval, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, op, n, dx, y ? dy : nullptr,
+                       dout);
+    CVG_HIP(ctx, hipGetLastError());
+    CVG_HIP(ctx, hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return CVG_OK;
+}
+
+}  // extern "C"

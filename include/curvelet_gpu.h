@@ -1,0 +1,212 @@
+/* curvelet_gpu.h - C ABI of the MI355X curvelet-construction library
+ * (libcurvelet_gpu.so).
+ *
+ * The reference (cpu_curvelet) has no FFI: its boundary is the CURVELET command
+ * line plus the in-process C++ API that run_curvelet<T> calls
+ * (cpu_curvelet/main.cpp:19-96).  Each entry point below replaces one piece of
+ * that API; the comment names the reference interface it stands in for.
+ * Every function returns a cvg_status (0 = success), never aborts and never
+ * lets a C++ exception cross the boundary; cvg_last_error() describes the last
+ * failure on a context.  Plain pointers and sizes only.
+ *
+ * Threading: one context per host thread.  A context owns one HIP stream and
+ * device-resident pools (grow-only) for the spatial hash, the neighbour graph,
+ * the curvelet staging rows and the outputs.  Results never depend on stream,
+ * batch composition or launch shape.
+ */
+#ifndef CURVELET_GPU_H
+#define CURVELET_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference returns bool and prints to std::cerr
+ * (data_io.hpp:24-75, preprocess.hpp:197, main.cpp:38-59); these split that
+ * false into causes. */
+typedef enum cvg_status {
+    CVG_OK = 0,
+    CVG_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad size, bad parameter */
+    CVG_ERR_BAD_INPUT = 2,        /* malformed edge file or non-finite / out-of-range edge values */
+    CVG_ERR_IO = 3,               /* file missing or unwritable */
+    CVG_ERR_OUT_OF_MEMORY = 4,    /* host or device allocation failed */
+    CVG_ERR_DEVICE = 5,           /* HIP runtime error */
+    CVG_ERR_UNSUPPORTED = 6,      /* parameters outside what the kernels are built for */
+    CVG_ERR_STATE = 7             /* call order: e.g. curvelets before the neighbour graph */
+} cvg_status;
+
+/* CurveletParams, cpu_curvelet/param_settings.hpp:9-24, same fields and
+ * defaults.  dt_deg stays in degrees and is converted exactly as
+ * main.cpp:24 does: dt = (dt_deg / 180) * pi.  gap, curvelet_style and
+ * out_type are accepted and ignored, as in the reference. */
+typedef struct cvg_params {
+    double nrad;
+    double gap;
+    double dx;
+    double dt_deg;
+    double max_k;
+    unsigned curvelet_style;
+    unsigned group_max_sz;
+    unsigned out_type;
+    double sx;
+    double st;
+    int edge_data_sz;
+    unsigned max_candidates;
+} cvg_params;
+
+/* Scalar members of CPUNeighborGraph<T> (preprocess.hpp:17-31) plus the
+ * library's own diagnostics. */
+typedef struct cvg_graph_info {
+    int num_edges;
+    int total_neighbor_pairs;
+    unsigned max_num_of_neighbors;
+    /* anchors whose list reached max_candidates (the reference's warning count, preprocess.hpp:224,264-268) */
+    unsigned truncated_anchors;
+    /* anchors whose neighbour order depends on the within-cell order that
+     * libstdc++'s unstable std::sort gives the reference's spatial index
+     * (exact distance ties inside one pixel cell, or a max_candidates cut
+     * inside one cell).  The library orders such cells by edge id. */
+    unsigned order_ambiguous_anchors;
+} cvg_graph_info;
+
+/* Timings are device time measured with HIP events on the context's stream. */
+typedef struct cvg_run_stats {
+    unsigned valid_pairs;      /* form_pairwise_bundles count (main.cpp:61) */
+    unsigned num_curvelets;    /* out_h (main.cpp:68) */
+    unsigned chain_width;      /* out_w = group_max_sz + 1 (main.cpp:69) */
+    int total_neighbor_pairs;
+    unsigned max_num_of_neighbors;
+    double ms_graph;           /* spatial hash + CSR neighbour graph */
+    double ms_curvelets;       /* bundles + chain growth + dedup + record */
+    double ms_total;
+} cvg_run_stats;
+
+typedef struct cvg_context cvg_context;
+
+/* ---- library / context ------------------------------------------------ */
+
+/* CurveletParams{} default member initialisers (param_settings.hpp:10-23). */
+void cvg_params_default(cvg_params *params);
+const char *cvg_status_string(int status);
+/* Bind a context to HIP device `device` with its own stream. */
+int cvg_create(int device, cvg_context **out_ctx);
+void cvg_destroy(cvg_context *ctx);
+const char *cvg_last_error(const cvg_context *ctx);
+
+/* ---- edge-file codec (data_io.hpp) ------------------------------------ */
+
+/* read_TO_edges_from_file<double> (data_io.hpp:20-78) minus the
+ * "../test_files/" prefix (edge_input_path, data_io.hpp:10-13), which the CLI
+ * adds: exactly values_per_edge numbers per line, blank / short / long lines
+ * and empty files rejected with the reference's messages (written to err).
+ * *edges is allocated by the library (release with cvg_free). */
+int cvg_read_edge_file(const char *path, int values_per_edge, double **edges, int *num_edges, char *err,
+                       size_t err_len);
+void cvg_free(void *ptr);
+/* write_int_array_to_file / write_double_array_to_file (data_io.hpp:80-133):
+ * tab after every value, newline per row, doubles in default ostream format
+ * (6 significant digits).  Writes exactly `path`; the CLI adds "../outputs/". */
+int cvg_write_chain_file(const char *path, const unsigned *chains, unsigned rows, unsigned width);
+int cvg_write_info_file(const char *path, const double *info, unsigned rows, unsigned width);
+
+/* ---- stage-wise mirror of the reference's in-process API -------------- */
+
+/* build_neighbor_csr_graph(num_edges, sz_edge_data, edges, rad, neighbor_radius,
+ * max_candidates, nthreads, graph) (preprocess.hpp:186-274).  `edges` is a
+ * host array, row-major num_edges x sz_edge_data (sz_edge_data >= 4). */
+int cvg_build_neighbor_csr_graph(cvg_context *ctx, int num_edges, int sz_edge_data, const double *edges,
+                                 double rad, unsigned neighbor_radius, unsigned max_candidates);
+int cvg_get_graph_info(cvg_context *ctx, cvg_graph_info *info);
+/* neighbor_offsets (num_edges + 1), neighbor_ids and neighbor_dist2
+ * (total_neighbor_pairs each) of CPUNeighborGraph<T>. */
+int cvg_get_csr(cvg_context *ctx, int *offsets, int *ids, double *dist2);
+
+/* CurveletCPU<T>::form_pairwise_bundles (cpu_curvelet.hpp:305-361): the
+ * valid-pair count.  Runs the fused bundle + growth pass and keeps its
+ * results, so a following cvg_build_curvelets_greedy with the same params
+ * does not recompute. */
+int cvg_form_pairwise_bundles(cvg_context *ctx, const cvg_params *params, unsigned *valid_pairs);
+/* CurveletCPU<T>::build_curvelets_greedy (cpu_curvelet.hpp:363-634) with
+ * compact_curvelet_output (:282-303). */
+int cvg_build_curvelets_greedy(cvg_context *ctx, const cvg_params *params, unsigned *num_curvelets);
+/* num_curvelets() and chain_width() (cpu_curvelet.hpp:95-96). */
+int cvg_curvelet_shape(cvg_context *ctx, unsigned *num_curvelets, unsigned *chain_width);
+/* _edge_chain_final (num_curvelets x chain_width, 1-based ids, row-major) and
+ * _curvelet_info transposed to row-major num_curvelets x 10 as
+ * main.cpp:81-88 does. */
+int cvg_get_curvelets(cvg_context *ctx, unsigned *chains, double *info);
+
+/* ---- whole path --------------------------------------------------------- */
+
+/* The timed span of run_curvelet<double> (main.cpp:43-66): edges in host
+ * memory -> graph -> bundles -> chains and info on the device.  Fetch the
+ * results with cvg_get_curvelets. */
+int cvg_run(cvg_context *ctx, int num_edges, int sz_edge_data, const double *edges, const cvg_params *params,
+            cvg_run_stats *stats);
+/* Same, with the edge array already in device memory (for instance from
+ * cvg_upload_edges).  No host<->device copy of edges or results. */
+int cvg_run_device(cvg_context *ctx, int num_edges, int sz_edge_data, const double *device_edges,
+                   const cvg_params *params, cvg_run_stats *stats);
+/* Copy a host edge array into a context-owned device buffer (valid until the
+ * next upload or cvg_destroy). */
+int cvg_upload_edges(cvg_context *ctx, int num_edges, int sz_edge_data, const double *edges,
+                     const double **device_edges);
+/* Device pointers to the last run's outputs (valid until the next run). */
+int cvg_device_curvelets(cvg_context *ctx, const unsigned **device_chains, const double **device_info);
+/* The context's HIP stream (hipStream_t), for callers that order their own work after it. */
+int cvg_get_stream(cvg_context *ctx, void **hip_stream);
+int cvg_synchronize(cvg_context *ctx);
+
+/* ---- synthetic third-order edge maps ------------------------------------ */
+
+/* Seeded generator of edge maps in the test_files format (x, y, theta,
+ * strength).  No counterpart in the reference; used for benchmarks and tests. */
+typedef struct cvg_synth_config {
+    int width;
+    int height;
+    int num_curves;
+    double radius_min;        /* circles / arcs */
+    double radius_max;
+    int radius_log_uniform;   /* 1: radius = 1/curvature with curvature log-uniform */
+    double segment_min;       /* line segments */
+    double segment_max;
+    double bezier_extent;     /* control points within this box (<= 0: the whole image) */
+    double arc_max_length;    /* cap on one arc's sampled length (<= 0: none) */
+    double clutter_fraction;  /* share of all edges that are clutter */
+    int junction_patches;     /* T-junction patches */
+    int curves_per_patch;
+    double patch_size;
+    double pos_sigma;         /* px */
+    double theta_sigma_deg;
+} cvg_synth_config;
+
+/* Named presets: "C1" (320x240, ~5k edges), "C2" (480x320, ~20k),
+ * "C3" (1920x1080, ~200k, T-junction patches). */
+int cvg_synth_preset(const char *name, cvg_synth_config *config);
+/* Writes up to `capacity` edges (4 doubles each) to `edges` (may be NULL to
+ * only count) and the total count to *num_edges.  Deterministic in (config, seed). */
+int cvg_synth_generate(const cvg_synth_config *config, unsigned long long seed, double *edges, int capacity,
+                       int *num_edges);
+
+/* ---- diagnostics ------------------------------------------------------- */
+
+/* Per-neighbour detail of one anchor of the current graph, computed on the
+ * device by the same device functions the fused kernel uses: dir[r] = 1
+ * (forward run), 2 (backward run), 0 (neither); hyp[r] = bundle valid or
+ * close; bmin/bmax = deg x K cells in the reference's cell order
+ * (i * Kt + j).  Mirrors compute_curve_bundle (cpu_curvelet.hpp:636-819) and
+ * the direction filter (:459-461). */
+int cvg_debug_anchor_bundles(cvg_context *ctx, const cvg_params *params, int anchor, unsigned char *dir,
+                             unsigned char *hyp, double *bmin, double *bmax, int *deg, int *cells);
+/* Evaluate the library's correctly rounded math on n arguments.
+ * op: 0 sin(x), 1 cos(x), 2 atan2(y, x), 3 sqrt(x), 4 To2Pi(x), 5 fmod(x, y).
+ * ctx == NULL evaluates the host build of the same code; otherwise the device. */
+int cvg_math_eval(cvg_context *ctx, int op, int n, const double *x, const double *y, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
