@@ -1,0 +1,145 @@
+#!/usr/bin/env python3
+"""Benchmark: the curvelet-construction path on one MI355X.
+
+Workload: one synthetic 1920x1080 third-order edge map (~207k edges; 1500
+curves, 40 T-junction patches, 30% clutter; seed 1003), group_max_sz = 7,
+other parameters at the reference defaults.  A step is one pass of the whole
+path over that map: spatial hash + CSR neighbour graph + pairwise bundles +
+chain growth + dedup + record, edges already resident in HBM, chains and info
+left in HBM.
+
+Prints one JSON line.  `value` is edges/s over the timed steps (host clock,
+synchronised on both sides).  `roofline` is for the dominant kernel
+(k_curvelets, the fused bundle + growth kernel), timed with HIP events on the
+context's stream; its algorithmic FP64 work is the bundle transport (1,900
+FLOP per neighbour pair) plus the growth intersections (3 FLOP per cell per
+intersection).  `cpu_baseline` is the reference's own cpu_curvelet (built in
+place under oracle/_ref) on this host's cores, timed on the same map with
+span (ii): CSR build + build_curvelets_greedy.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+WORKLOAD = {"preset": "C3", "seed": 1003, "group_max_sz": 7}
+F_PAIR = 1900.0           # FP64 FLOP per bundle pair (transport of 63 cells incl. 14 sines)
+PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector peak
+PEAK_HBM_GBS = 8000.0
+
+
+def host_cores() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    lim = os.environ.get("OMP_NUM_THREADS")
+    if lim and lim.isdigit():
+        n = min(n, int(lim))
+    return max(1, n)
+
+
+def cpu_baseline(edges, params):
+    """The reference binary's code (oracle/_ref) on host cores, span (ii)."""
+    try:
+        from oracle import Ref
+        if not Ref.available():
+            return None
+        ref = Ref()
+        cores = host_cores()
+        r = ref.run(edges, params, nthreads=cores, valid_pairs=False)
+        t = r["times"]["csr"] + r["times"]["greedy"]
+        return {"value": len(edges) / t, "unit": "edges/s", "cores": cores, "kind": "reference",
+                "sample": f"the full {len(edges)}-edge map once, CSR + build_curvelets_greedy "
+                          f"({t:.3f} s on {cores} threads)",
+                "seconds": t, "num_curvelets": int(len(r["chains"]))}
+    except Exception as exc:  # the baseline is reported, never required
+        return {"error": repr(exc)}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); maps shard across GPUs "
+              f"without communication, so per-GPU throughput is the figure that matters.", file=sys.stderr)
+        return 2
+
+    from curvelet_construction_on_gpu_amd import Context, CurveletParams, synth_edges
+
+    edges = synth_edges(WORKLOAD["preset"], WORKLOAD["seed"])
+    params = CurveletParams(group_max_sz=WORKLOAD["group_max_sz"])
+    n, sz = edges.shape
+
+    ctx = Context(0)
+    dptr = ctx.upload(edges)  # inputs resident in HBM before the timed region
+    for _ in range(args.warmup):
+        st = ctx.run_device(dptr, n, sz, params)
+    ctx.synchronize()
+    per_step = []
+    fused = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        st = ctx.run_device(dptr, n, sz, params)
+        per_step.append(time.perf_counter() - s0)
+        fused.append(st["ms_fused_kernel"])
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    pairs = st["total_neighbor_pairs"]
+    ms_step = elapsed / args.steps * 1e3
+    fused_ms = statistics.mean(fused)
+    flops = pairs * F_PAIR + st["intersections"] * 63 * 3
+    achieved = flops / (fused_ms * 1e-3) / 1e12
+    line = {
+        "metric": "edges/s per map at 1080p (~200k edges)",
+        "value": n * args.steps / elapsed,
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "1920x1080 synthetic third-order edge map, ~207k edges, group_max_sz=7 "
+                               "(preset C3, seed 1003)",
+                   "edges": n, "neighbor_pairs": pairs, "valid_pairs": st["valid_pairs"],
+                   "curvelets": st["num_curvelets"], "group_max_sz": WORKLOAD["group_max_sz"]},
+        "bundle_pairs_per_s": pairs * args.steps / elapsed,
+        "ms_per_step_median": statistics.median(per_step) * 1e3,
+        "stage_ms": {"graph": st["ms_graph"], "curvelets": st["ms_curvelets"], "fused_kernel": fused_ms},
+        "roofline": {"bound": "mfma", "pipe": "fp64 vector ALU (no MFMA in this path)",
+                     "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "k_curvelets", "kernel_ms": fused_ms,
+                     "flop_per_launch": flops},
+    }
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(edges, params)
+        if cb is not None:
+            line["cpu_baseline"] = cb
+            if "value" in cb:
+                line["speedup_vs_cpu_baseline"] = line["value"] / cb["value"]
+    ctx.close()
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

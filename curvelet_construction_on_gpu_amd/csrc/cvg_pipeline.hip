@@ -67,7 +67,7 @@ struct DevBuf {
 struct cvg_context {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     std::string err;
     size_t lds_limit = 65536;
 
@@ -92,6 +92,7 @@ struct cvg_context {
     bool curvelets_ready = false;
     cvg_params cur_params{};
     unsigned valid_pairs = 0;
+    unsigned long long intersections = 0;
     unsigned num_curvelets = 0;
     unsigned width = 0;
     DevBuf d_trig, d_cls, d_cnt3, d_lists, d_tables, d_st_ids, d_st_meta, d_st_k, d_acc, d_rowoff, d_gbundle,
@@ -282,9 +283,13 @@ enum {
 };
 constexpr int kMaxBuckets = S_COUNT - S_BUCKET;
 
+// A 64-bit counter of bundle intersections follows the S_COUNT scalars.
+constexpr int kScalarWords = S_COUNT + 2;
+
 __global__ void k_init_scalars(unsigned *s)
 {
     const int t = threadIdx.x;
+    if (t >= S_COUNT && t < kScalarWords) s[t] = 0u;
     if (t < S_COUNT) {
         int v = 0;
         if (t == S_MINX || t == S_MINY) v = INT_MAX;
@@ -604,6 +609,7 @@ struct FusedArgs {
     double *st_k;
     unsigned *acc_count;
     unsigned *valid_pairs;
+    unsigned long long *intersections;
     double *g_bundle;  // non-null: bundles live in global memory, nb_cap*K*2 doubles per list entry
 };
 
@@ -685,7 +691,7 @@ __host__ __device__ inline Carve carve(int nb, int kt, int K, int chain_stride, 
     return c;
 }
 
-enum { M_ACC = 0, M_VP = 1, M_NL = 2, M_NH = 3, M_WSUM = 8 };
+enum { M_ACC = 0, M_VP = 1, M_NL = 2, M_NH = 3, M_ISECT = 4, M_WSUM = 8 };
 
 // build_curvelets_greedy for one anchor per workgroup (cpu_curvelet.hpp:437-591),
 // plus the form_pairwise_bundles valid-pair count (:305-361), in one pass.
@@ -759,6 +765,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_curvelets(FusedArgs A)
         misc[M_ACC] = 0;
         misc[M_VP] = 0;
     }
+    int isect = 0;  // intersections tested by this wave (lane 0 counts)
     __syncthreads();
 
     for (int run = 0; run < 3; ++run) {
@@ -869,6 +876,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_curvelets(FusedArgs A)
                     if (len >= G) break;
                     continue;
                 }
+                ++isect;
                 double nlo[CPL], nhi[CPL];
                 bool ok = false;
 #pragma unroll
@@ -959,6 +967,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_curvelets(FusedArgs A)
         }
         __syncthreads();
     }
+    if (lane == 0 && isect > 0) atomicAdd(A.intersections, static_cast<unsigned long long>(isect));
     if (tid == 0) {
         A.acc_count[a] = static_cast<unsigned>(misc[M_ACC]);
         atomicAdd(A.valid_pairs, static_cast<unsigned>(misc[M_VP]));
@@ -1108,7 +1117,7 @@ __global__ void k_math_eval(int op, int n, const double *__restrict__ x, const d
 
 int read_scalars(cvg_context *ctx)
 {
-    CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox, ctx->d_scal.ptr, S_COUNT * sizeof(unsigned), hipMemcpyDeviceToHost,
+    CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox, ctx->d_scal.ptr, kScalarWords * sizeof(unsigned), hipMemcpyDeviceToHost,
                                 ctx->stream));
     CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return CVG_OK;
@@ -1134,7 +1143,7 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     CVG_TRY(ensure(ctx, ctx->d_th, nd));
     CVG_TRY(ensure(ctx, ctx->d_cx, ni));
     CVG_TRY(ensure(ctx, ctx->d_cy, ni));
-    CVG_TRY(ensure(ctx, ctx->d_scal, S_COUNT * sizeof(unsigned)));
+    CVG_TRY(ensure(ctx, ctx->d_scal, kScalarWords * sizeof(unsigned)));
     unsigned *scal = as<unsigned>(ctx->d_scal);
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, st, scal);
     hipLaunchKernelGGL(k_prep_edges, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, sz, d_edges, as<double>(ctx->d_x),
@@ -1194,10 +1203,10 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     CVG_HIP(ctx, hipGetLastError());
     CVG_TRY(exclusive_scan(ctx, as<unsigned>(ctx->d_deg), n, as<unsigned>(ctx->d_off), ctx->d_tiles));
     unsigned pairs_u = 0;
-    CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox + S_COUNT, as<unsigned>(ctx->d_off) + n, sizeof(unsigned),
+    CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox + 32, as<unsigned>(ctx->d_off) + n, sizeof(unsigned),
                                 hipMemcpyDeviceToHost, st));
     CVG_TRY(read_scalars(ctx));
-    pairs_u = ctx->mailbox[S_COUNT];
+    pairs_u = ctx->mailbox[32];
     if (pairs_u > static_cast<unsigned>(INT_MAX))
         return fail(ctx, CVG_ERR_UNSUPPORTED, "more than 2^31-1 neighbour pairs");
     const int pairs = static_cast<int>(pairs_u);
@@ -1341,7 +1350,7 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     hipLaunchKernelGGL(k_edge_trig, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<double>(ctx->d_th), d.dt,
                        static_cast<int>(d.kt), T.off_t, as<double>(ctx->d_trig));
     CVG_HIP(ctx, hipMemsetAsync(ctx->d_cnt3.ptr, 0, static_cast<size_t>(3) * n * sizeof(unsigned), st));
-    CVG_HIP(ctx, hipMemsetAsync(scal + S_VALID, 0, (S_COUNT - S_VALID) * sizeof(unsigned), st));
+    CVG_HIP(ctx, hipMemsetAsync(scal + S_VALID, 0, (kScalarWords - S_VALID) * sizeof(unsigned), st));
     if (pairs > 0)
         hipLaunchKernelGGL(k_pair_class, dim3(blocks_for(pairs)), dim3(kBlock), 0, st, pairs, n,
                            as<int>(ctx->d_pair_anchor), as<int>(ctx->d_ids), as<double>(ctx->d_x),
@@ -1395,7 +1404,9 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     A.st_k = as<double>(ctx->d_st_k);
     A.acc_count = as<unsigned>(ctx->d_acc);
     A.valid_pairs = scal + S_VALID;
+    A.intersections = reinterpret_cast<unsigned long long *>(scal + S_COUNT);
 
+    CVG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
     for (int b = 0; b < nb; ++b) {
         const int len = static_cast<int>(bucket_count[b]);
         if (len == 0) continue;
@@ -1432,15 +1443,17 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
         CVG_HIP(ctx, hipGetLastError());
         (void)lds_bundles;
     }
-    // anchors with an empty bucket list never ran: their counts must be zero
-    // (every anchor is in exactly one bucket, so all acc_count entries are written)
+    CVG_HIP(ctx, hipEventRecord(ctx->ev[4], st));
+    // every anchor is in exactly one bucket, so every acc_count entry is written
 
     CVG_TRY(exclusive_scan(ctx, as<unsigned>(ctx->d_acc), n, as<unsigned>(ctx->d_rowoff), ctx->d_tiles));
-    CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox + S_COUNT, as<unsigned>(ctx->d_rowoff) + n, sizeof(unsigned),
+    CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox + 32, as<unsigned>(ctx->d_rowoff) + n, sizeof(unsigned),
                                 hipMemcpyDeviceToHost, st));
     CVG_TRY(read_scalars(ctx));
-    const unsigned C = ctx->mailbox[S_COUNT];
+    const unsigned C = ctx->mailbox[32];
     ctx->valid_pairs = ctx->mailbox[S_VALID];
+    ctx->intersections = static_cast<unsigned long long>(ctx->mailbox[S_COUNT]) |
+                         (static_cast<unsigned long long>(ctx->mailbox[S_COUNT + 1]) << 32);
     ctx->num_curvelets = C;
     ctx->width = d.G + 1;
 
@@ -1506,6 +1519,11 @@ int run_common(cvg_context *ctx, int n, int sz, const double *d_edges, const cvg
         stats->ms_graph = t01;
         stats->ms_curvelets = t12;
         stats->ms_total = t02;
+        float tf = 0;
+        (void)hipEventElapsedTime(&tf, ctx->ev[3], ctx->ev[4]);
+        stats->ms_fused_kernel = tf;
+        stats->intersections = ctx->intersections;
+        stats->order_ambiguous_anchors = ctx->ambiguous;
     }
     return CVG_OK;
 }

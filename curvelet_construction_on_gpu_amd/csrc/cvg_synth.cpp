@@ -191,7 +191,8 @@ int cvg_synth_preset(const char *name, cvg_synth_config *config)
     if (!name || !config) return CVG_ERR_INVALID_ARGUMENT;
     set_defaults(config);
     if (std::strcmp(name, "C1") == 0) {
-        // 320x240, 40 curves, radius U[10,150], 10% clutter
+        // 320x240, 40 curves, radius U[10,150], 10% clutter (~6k edges)
+        config->arc_max_length = 150;
         return CVG_OK;
     }
     if (std::strcmp(name, "C2") == 0) {
@@ -215,10 +216,11 @@ int cvg_synth_preset(const char *name, cvg_synth_config *config)
         config->radius_min = 3;
         config->radius_max = 500;
         config->radius_log_uniform = 1;
-        config->bezier_extent = 240;
-        config->arc_max_length = 260;
+        config->segment_max = 100;
+        config->bezier_extent = 100;
+        config->arc_max_length = 100;
         config->clutter_fraction = 0.3;
-        config->junction_patches = 60;
+        config->junction_patches = 40;
         config->curves_per_patch = 20;
         config->patch_size = 50;
         return CVG_OK;

@@ -21,6 +21,7 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 ORACLE_LIB = HERE / "liboracle.so"
+ORACLE_CR_LIB = HERE / "liboracle_cr.so"
 REF_LIB = HERE / "_ref" / "libcurvelet_ref.so"
 REF_CLI = HERE / "_ref" / "CURVELET"
 REFERENCE_SRC = Path("/root/reference/cpu_curvelet")
@@ -33,7 +34,7 @@ _up = _P(C.c_uint)
 
 def build(ref: bool | None = None) -> None:
     """Compile the oracle (and the in-place reference build when the sources exist)."""
-    subprocess.run(["make", "-s", "-C", str(HERE), "liboracle.so"], check=True)
+    subprocess.run(["make", "-s", "-C", str(HERE), "all"], check=True)
     if ref is None:
         ref = REFERENCE_SRC.exists()
     if ref:
@@ -57,12 +58,14 @@ def _f(a, t):
 
 
 class Oracle:
-    """The CPU restatement (liboracle.so)."""
+    """The CPU restatement: liboracle.so (glibc sin/cos/atan2, as the reference),
+    or with cr=True liboracle_cr.so (the GPU path's correctly rounded math)."""
 
-    def __init__(self):
-        if not ORACLE_LIB.exists():
+    def __init__(self, cr: bool = False):
+        path = ORACLE_CR_LIB if cr else ORACLE_LIB
+        if not path.exists():
             build(ref=False)
-        lib = C.CDLL(str(ORACLE_LIB))
+        lib = C.CDLL(str(path))
         sig = {
             "orc_dt_from_deg": (C.c_double, [C.c_double]),
             "orc_create": (C.c_void_p, []),

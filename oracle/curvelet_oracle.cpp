@@ -28,6 +28,10 @@
 
 #include <omp.h>
 
+#ifdef ORC_CR_MATH
+#include "../curvelet_construction_on_gpu_amd/csrc/cvg_math.hpp"
+#endif
+
 namespace {
 
 // curvelet_utils.hpp:10 defines M_PI with this literal.
@@ -54,17 +58,29 @@ double dist2(double x1, double y1, double x2, double y2)
     return (x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2);
 }
 
+#ifndef ORC_CR_MATH
 // Opaque libm entry points: calls through volatile pointers are never merged.
 double (*volatile libm_sin)(double) = ::sin;
 double (*volatile libm_cos)(double) = ::cos;
 void (*volatile libm_sincos)(double, double *, double *) = ::sincos;
+double (*volatile libm_atan2)(double, double) = ::atan2;
+#else
+// Second build (liboracle_cr.so): the same restatement with every sin, cos,
+// sincos and atan2 replaced by the GPU path's correctly rounded functions.
+// It separates the arithmetic (which must then match the GPU bit for bit)
+// from the libm difference (glibc vs correct rounding).
+double libm_sin(double x) { return cvg::cr_sin(x, cvg::kSinCosTableHost); }
+double libm_cos(double x) { return cvg::cr_cos(x, cvg::kSinCosTableHost); }
+void libm_sincos(double x, double *s, double *c) { cvg::cr_sincos(x, cvg::kSinCosTableHost, s, c); }
+double libm_atan2(double y, double x) { return cvg::cr_atan2(y, x, cvg::kAtanTableHost); }
+#endif
 
 // dot(angle_from_pt_to_pt(a, n), theta_a), curvelet_utils.hpp:58-69 as used at
 // cpu_curvelet.hpp:459-461, with the calls of the forward (run 0) or backward
 // (run 1) specialisation.
 double direction_dot(int run, double ax, double ay, double at, double nx, double ny)
 {
-    const double dir = wrap_two_pi(std::atan2(ny - ay, nx - ax));
+    const double dir = wrap_two_pi(libm_atan2(ny - ay, nx - ax));
     if (run == 0) {
         const double cd = libm_cos(dir), ca = libm_cos(at), sd = libm_sin(dir), sa = libm_sin(at);
         return cd * ca + sd * sa;

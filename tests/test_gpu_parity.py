@@ -1,0 +1,123 @@
+"""GPU parity: the HIP path against the oracle on the same inputs.
+
+Two checkers:
+  oracle     - the restatement with glibc sin/cos/atan2, bit-exact with the
+               reference build.  Bar (the reference's): neighbour lists,
+               valid-pair counts and curvelet chains bit-exact; curvelet info
+               within relative 1e-9 (absolute 1e-12 near zero).
+  oracle_cr  - the same restatement with the device's correctly rounded
+               math.  Bar: everything bit-exact (CSR, direction classes,
+               hypothesis flags, every bundle cell, chains, info), which pins
+               the device arithmetic (operand order, rounding points,
+               compare-and-assign semantics) independently of libm.
+glibc misrounds about 0.1-0.2% of sin/cos/atan2 arguments; that difference is
+the only reason the first bar is a tolerance on bundles and info.
+"""
+import numpy as np
+import pytest
+
+from curvelet_construction_on_gpu_amd import CurveletParams, math_eval
+from tests._inputs import edges_for
+
+pytestmark = pytest.mark.gpu
+
+INFO_RTOL = 1e-9
+INFO_ATOL = 1e-12
+
+CASES = [
+    "file:TO_edges_digit1",
+    "file:TO_edges_ABC_0006_thresh1",
+    "file:TO_edges",
+    "synth:C1:1001",
+    "synth:C2:1002",
+]
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def test_device_math_matches_host_build(ctx):
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(-8, 8, 20000), rng.uniform(-1e4, 1e4, 2000),
+                        np.ldexp(rng.uniform(-1, 1, 2000), -rng.integers(0, 60, 2000)),
+                        [0.0, -0.0, 1.5707963267948966, 3.141592653589793, 6.283185307179586]])
+    y = rng.uniform(-8, 8, x.size)
+    for op in ("sin", "cos", "atan2", "to2pi"):
+        d = math_eval(op, x, y, ctx=ctx)
+        h = math_eval(op, x, y, device=False)
+        assert np.array_equal(bits(d), bits(h)), op
+
+
+def test_device_sqrt_fmod_ieee(ctx):
+    rng = np.random.default_rng(4)
+    x = np.concatenate([rng.uniform(0, 50, 20000), np.ldexp(rng.uniform(0, 1, 2000), -rng.integers(0, 900, 2000))])
+    assert np.array_equal(math_eval("sqrt", x, ctx=ctx), np.sqrt(x))
+    a = rng.uniform(-1e6, 1e6, 20000)
+    b = np.full_like(a, 2.0 * 3.141592653589793)
+    assert np.array_equal(math_eval("fmod", a, b, ctx=ctx), np.fmod(a, b))
+
+
+@pytest.mark.parametrize("key", CASES)
+def test_csr_bit_exact(ctx, oracle, key):
+    e = edges_for(key)
+    p = CurveletParams()
+    want = oracle.build_csr(e, p, nthreads=0)
+    g = ctx.build_graph(e, p.nrad, 3, p.max_candidates)
+    assert g.total_neighbor_pairs == want["pairs"]
+    assert g.max_num_of_neighbors == want["max_deg"]
+    assert np.array_equal(g.neighbor_offsets, want["offsets"])
+    assert np.array_equal(g.neighbor_ids, want["ids"])
+    assert np.array_equal(bits(g.neighbor_dist2), bits(want["dist2"]))
+    assert g.truncated_anchors == want["truncated"]
+    assert g.order_ambiguous_anchors == 0
+
+
+@pytest.mark.parametrize("key", ["file:TO_edges_digit1", "file:TO_edges_ABC_0006_thresh1", "synth:C1:1001"])
+def test_anchor_bundles(ctx, oracle, oracle_cr, key):
+    e = edges_for(key)
+    p = CurveletParams()
+    want = oracle.build_csr(e, p)
+    oracle_cr.build_csr(e, p)
+    ctx.build_graph(e, p.nrad, 3, p.max_candidates)
+    off = want["offsets"]
+    rng = np.random.default_rng(5)
+    anchors = rng.choice(len(e), size=min(len(e), 150), replace=False)
+    cells = exact = 0
+    for a in anchors:
+        deg = int(off[a + 1] - off[a])
+        if deg == 0:
+            continue
+        gd, gh, glo, ghi = ctx.anchor_bundles(p, int(a), deg)
+        cd, ch, clo, chi = oracle_cr.anchor_detail(p, int(a), deg)
+        od, oh, olo, ohi = oracle.anchor_detail(p, int(a), deg)
+        # correctly rounded math on both sides: bit-exact
+        assert np.array_equal(gd == 1, (cd & 1) != 0) and np.array_equal(gd == 2, (cd & 2) != 0), a
+        assert np.array_equal(gh, ch), a
+        assert np.array_equal(bits(glo), bits(clo)) and np.array_equal(bits(ghi), bits(chi)), a
+        # glibc math: same decisions, values within tolerance
+        assert np.array_equal(gd == 1, (od & 1) != 0) and np.array_equal(gd == 2, (od & 2) != 0), a
+        assert np.array_equal(gh, oh), a
+        np.testing.assert_allclose(glo, olo, rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose(ghi, ohi, rtol=1e-9, atol=1e-12)
+        cells += glo.size + ghi.size
+        exact += int((bits(glo) == bits(olo)).sum() + (bits(ghi) == bits(ohi)).sum())
+    assert exact / cells > 0.98, exact / cells
+
+
+@pytest.mark.parametrize("G", [4, 7])
+@pytest.mark.parametrize("key", CASES)
+def test_curvelets_parity(ctx, oracle, oracle_cr, key, G):
+    e = edges_for(key)
+    p = CurveletParams(group_max_sz=G)
+    stats = ctx.run(e, p)
+    chains, info = ctx.curvelets()
+    want = oracle.run(e, p, nthreads=0)
+    assert stats["total_neighbor_pairs"] == want["csr"]["pairs"]
+    assert stats["valid_pairs"] == want["valid_pairs"]
+    assert chains.shape == want["chains"].shape
+    assert np.array_equal(chains, want["chains"])
+    np.testing.assert_allclose(info, want["info"], rtol=INFO_RTOL, atol=INFO_ATOL)
+    exact = oracle_cr.run(e, p, nthreads=0)
+    assert np.array_equal(chains, exact["chains"])
+    assert np.array_equal(bits(info), bits(exact["info"]))
