@@ -124,6 +124,7 @@ def main(argv=None) -> int:
         "bundle_pairs_per_s": pairs * args.steps / elapsed,
         "ms_per_step_median": statistics.median(per_step) * 1e3,
         "stage_ms": {"graph": st["ms_graph"], "curvelets": st["ms_curvelets"], "fused_kernel": fused_ms},
+        "lds_bytes_per_workgroup": st["lds_bytes_per_workgroup"],
         "roofline": {"bound": "mfma", "pipe": "fp64 vector ALU (no MFMA in this path)",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,

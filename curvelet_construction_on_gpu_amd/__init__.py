@@ -59,7 +59,8 @@ class _RunStats(C.Structure):
     _fields_ = [("valid_pairs", C.c_uint), ("num_curvelets", C.c_uint), ("chain_width", C.c_uint),
                 ("total_neighbor_pairs", C.c_int), ("max_num_of_neighbors", C.c_uint), ("ms_graph", C.c_double),
                 ("ms_curvelets", C.c_double), ("ms_total", C.c_double), ("ms_fused_kernel", C.c_double),
-                ("intersections", C.c_ulonglong), ("order_ambiguous_anchors", C.c_uint)]
+                ("intersections", C.c_ulonglong), ("order_ambiguous_anchors", C.c_uint),
+                ("lds_bytes_per_workgroup", C.c_uint)]
 
 
 class _SynthConfig(C.Structure):

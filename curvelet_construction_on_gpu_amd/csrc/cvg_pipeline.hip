@@ -49,7 +49,6 @@ __constant__ double c_atan_tab[2 * CVG_ATAN_ROWS] = CVG_ATAN_TABLE_INIT;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kWaves = 4;  // waves per curvelet workgroup
 constexpr int kInfoWidth = 10;
 constexpr long long kMaxGridCells = 1LL << 28;
 
@@ -67,6 +66,9 @@ struct DevBuf {
 struct cvg_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    // extra streams so the bucket launches of the fused kernel run concurrently
+    hipStream_t aux[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev_join[3] = {nullptr, nullptr, nullptr};
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     std::string err;
     size_t lds_limit = 65536;
@@ -95,7 +97,7 @@ struct cvg_context {
     unsigned long long intersections = 0;
     unsigned num_curvelets = 0;
     unsigned width = 0;
-    DevBuf d_trig, d_cls, d_cnt3, d_lists, d_tables, d_st_ids, d_st_meta, d_st_k, d_acc, d_rowoff, d_gbundle,
+    DevBuf d_trig, d_cls, d_psin, d_cnt3, d_lists, d_tables, d_st_ids, d_st_meta, d_st_k, d_acc, d_rowoff, d_gbundle,
         d_chains, d_info;
 
     // diagnostics
@@ -421,74 +423,174 @@ __global__ __launch_bounds__(kBlock) void k_discover_count(int n, const double *
     if (d >= max_cand) atomicAdd(&scal[S_TRUNC], 1u);  // preprocess.hpp:224, a >= test
 }
 
-// discover_neighbors_csr gathering pass + the distance sort (preprocess.hpp:141-183)
-// + the CSR fill (:240-262).  Candidates are gathered in the reference's scan
-// order (py outer, px inner, then within-cell order), cut at max_candidates,
-// then ordered by d2.  Distinct distances have one order; with ties the
-// result is libstdc++'s: up to 16 entries std::sort is an insertion sort
-// (stable), beyond that the introsort replay decides.
-__global__ __launch_bounds__(kBlock) void k_fill_csr(int n, const double *__restrict__ x, const double *__restrict__ y,
-                                                     const int *__restrict__ cx, const int *__restrict__ cy, GridDesc g,
-                                                     int R, double rad2, const unsigned *__restrict__ start,
-                                                     const int *__restrict__ sorted, const double *__restrict__ sx,
-                                                     const double *__restrict__ sy, const unsigned *__restrict__ off,
-                                                     Cand *__restrict__ cand, int *__restrict__ ids,
-                                                     double *__restrict__ d2, int *__restrict__ pair_anchor,
-                                                     int *__restrict__ tie_list, unsigned *__restrict__ scal)
+// LDS or global writes by some lanes made visible to the other lanes of the
+// same wave.
+__device__ inline void wave_sync()
 {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Broadcast a lane's value to the whole wave (lane index uniform).
+__device__ inline double bcast(double v, int lane)
+{
+    const unsigned long long u = __double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane(static_cast<unsigned>(u), lane);
+    const unsigned hi = __builtin_amdgcn_readlane(static_cast<unsigned>(u >> 32), lane);
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+__device__ inline int bcast(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+
+constexpr int kFillWaves = 4;  // anchors (waves) per workgroup in k_fill_csr
+
+// discover_neighbors_csr gathering pass + the distance sort (preprocess.hpp:141-183)
+// + the CSR fill (:240-262), one wave per anchor.  The lanes walk the window's
+// candidates in the reference's scan order (py outer, px inner, then
+// within-cell order): the 2R+1 rows are contiguous ranges of the cell-sorted
+// arrays, flattened and taken 64 at a time; a ballot prefix keeps the first
+// max_candidates in-radius ones.  Then every lane ranks its candidates by d2
+// against the whole list (ties by scan position, which is the insertion-sort
+// order std::sort yields up to 16 entries); lists with ties and more than 16
+// entries are queued for the std::sort replay (k_tie_sort).
+__global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const double *__restrict__ x,
+                                                              const double *__restrict__ y, const int *__restrict__ cx,
+                                                              const int *__restrict__ cy, GridDesc g, int R, double rad2,
+                                                              const unsigned *__restrict__ start,
+                                                              const int *__restrict__ sorted,
+                                                              const double *__restrict__ sx,
+                                                              const double *__restrict__ sy,
+                                                              const unsigned *__restrict__ off, Cand *__restrict__ cand,
+                                                              int *__restrict__ ids, double *__restrict__ d2,
+                                                              int *__restrict__ pair_anchor, int *__restrict__ tie_list,
+                                                              unsigned max_cand, int cap,
+                                                              unsigned *__restrict__ scal)
+{
+    // per wave: cap candidates (d2, id, cell) in LDS for the ranking
+    extern __shared__ double fill_lds[];
+    const int lane = threadIdx.x & 63;
+    const int w = static_cast<int>(threadIdx.x >> 6);
+    double *ld = fill_lds + static_cast<size_t>(w) * cap * 2;
+    int *lid = reinterpret_cast<int *>(ld + cap);
+    int *lcell = lid + cap;
+    const int a = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x >> 6) + w;
     if (a >= n) return;
+    const unsigned long long lt_mask = (1ULL << lane) - 1ULL;
     const double ax = x[a], ay = y[a];
     const int ix = cx[a] - g.x0, iy = cy[a] - g.y0;
     const unsigned o = off[a];
-    const unsigned dg = off[a + 1] - o;
-    unsigned kept = 0;
-    int last_cell = -1;
+    const int dg = static_cast<int>(off[a + 1] - o);
+    // scan: rows dy = -R..R, each [rs, re) in the cell-sorted arrays.  Lane t
+    // of a 64-window finds its row by walking the (uniform) row lengths.
+    int kept = 0;
+    int last_cell = -1;        // cell of the last kept candidate
     bool ambiguous = false;
-    bool cut_checked = false;
-    for (int dy = -R; dy <= R && !cut_checked; ++dy) {
-        const int row = (iy + dy) * g.w;
-        for (int dx = -R; dx <= R && !cut_checked; ++dx) {
-            const int c = row + ix + dx;
-            const unsigned cs = start[c], ce = start[c + 1];
-            const unsigned cell_first = kept;
-            for (unsigned k = cs; k < ce; ++k) {
-                const int b = sorted[k];
-                if (b == a) continue;
-                const double d = (ax - sx[k]) * (ax - sx[k]) + (ay - sy[k]) * (ay - sy[k]);
-                if (d > rad2) continue;
-                if (kept < dg) {
-                    // exact distance tie with an earlier candidate of the same cell
-                    for (unsigned q = cell_first; q < kept; ++q)
-                        if (cand[o + q].d == d) ambiguous = true;
-                    cand[o + kept] = Cand{d, b};
-                    ++kept;
-                    last_cell = c;
-                } else {
-                    // first candidate beyond the cap: the cut splits a cell?
-                    if (c == last_cell) ambiguous = true;
-                    cut_checked = true;
-                    break;
-                }
+    bool done = false;
+    int row = 0;               // first row not yet exhausted, and its consumed count
+    int row_used = 0;
+    while (!done && row <= 2 * R) {
+        // the 64 next candidates in scan order
+        int r = row, used = row_used, t = lane, k = -1;
+        while (r <= 2 * R) {
+            const int rr = (iy - R + r) * g.w;
+            const int rs = static_cast<int>(start[rr + ix - R]);
+            const int rlen = static_cast<int>(start[rr + ix + R + 1]) - rs;
+            if (t < rlen - used) {
+                k = rs + used + t;
+                break;
+            }
+            t -= rlen - used;
+            used = 0;
+            ++r;
+        }
+        bool pass = false;
+        double d = 0.0;
+        int b = -1, cell = -1;
+        if (k >= 0) {
+            b = sorted[k];
+            d = (ax - sx[k]) * (ax - sx[k]) + (ay - sy[k]) * (ay - sy[k]);
+            pass = b != a && !(d > rad2);
+            if (pass) cell = (cy[b] - g.y0) * g.w + (cx[b] - g.x0);
+        }
+        const unsigned long long m = __ballot(pass);
+        const int pos = kept + __popcll(m & lt_mask);
+        if (pass && pos < dg) {
+            cand[o + pos] = Cand{d, b};  // scan order, for the std::sort replay
+            ld[pos] = d;
+            lid[pos] = b;
+            lcell[pos] = cell;
+        }
+        // a max_candidates cut inside one cell makes the kept set depend on
+        // the within-cell order
+        const bool beyond = pass && pos == dg;
+        const unsigned long long bm = __ballot(beyond);
+        if (bm != 0ULL) {
+            const int bl = __ffsll(static_cast<long long>(bm)) - 1;
+            const int bcell = bcast(cell, bl);
+            int prev = last_cell;
+            const unsigned long long pm = __ballot(pass && pos == dg - 1);
+            if (pm != 0ULL) prev = bcast(cell, __ffsll(static_cast<long long>(pm)) - 1);
+            if (dg > 0 && prev == bcell) ambiguous = true;
+            done = true;
+        } else if (m != 0ULL) {
+            const int hl = 63 - __clzll(static_cast<long long>(m));  // last passing lane
+            if (kept + __popcll(m) <= dg) last_cell = bcast(cell, hl);
+        }
+        kept += __popcll(m);
+        // without a cut (dg < max_candidates) nothing passes after the dg-th
+        if (kept >= dg && static_cast<unsigned>(dg) < max_cand) done = true;
+        // advance the row cursor past the 64 candidates just taken
+        const unsigned long long valid = __ballot(k >= 0);
+        if (valid != ~0ULL) break;  // fewer than 64 left: the window is exhausted
+        int adv = 64;
+        while (row <= 2 * R && adv > 0) {
+            const int rr = (iy - R + row) * g.w;
+            const int rlen = static_cast<int>(start[rr + ix + R + 1]) - static_cast<int>(start[rr + ix - R]);
+            const int left = rlen - row_used;
+            if (adv < left) {
+                row_used += adv;
+                adv = 0;
+            } else {
+                adv -= left;
+                row_used = 0;
+                ++row;
             }
         }
     }
-    // stable rank order (the insertion-sort result) and tie detection
+    wave_sync();
+    // rank sort by d2 with ties by scan position; detect ties and same-cell ties
     bool tie = false;
-    for (unsigned i = 0; i < dg; ++i) {
-        const Cand ci = cand[o + i];
-        unsigned r = 0;
-        for (unsigned j = 0; j < dg; ++j) {
-            const double dj = cand[o + j].d;
-            r += (dj < ci.d || (dj == ci.d && j < i)) ? 1u : 0u;
-            tie = tie || (dj == ci.d && j != i);
+    for (int i0 = 0; i0 < dg; i0 += 64) {
+        const int i = i0 + lane;
+        const bool active = i < dg;
+        Cand ci{0.0, 0};
+        int celli = -1;
+        if (active) {
+            ci = Cand{ld[i], lid[i]};
+            celli = lcell[i];
         }
-        ids[o + r] = ci.id;
-        d2[o + r] = ci.d;
+        int rank = 0;
+        for (int j = 0; j < dg; ++j) {
+            const double dj = ld[j];
+            const int cellj = lcell[j];
+            rank += (dj < ci.d || (dj == ci.d && j < i)) ? 1 : 0;
+            const bool eq = active && dj == ci.d && j != i;
+            tie = tie || eq;
+            ambiguous = ambiguous || (eq && cellj == celli);
+        }
+        if (active) {
+            ids[o + rank] = ci.id;
+            d2[o + rank] = ci.d;
+        }
     }
-    if (tie && dg > 16) tie_list[atomicAdd(&scal[S_TIES], 1u)] = a;
-    for (unsigned i = 0; i < dg; ++i) pair_anchor[o + i] = a;
-    if (ambiguous && n > 16) atomicAdd(&scal[S_AMBIG], 1u);
+    const bool any_tie = __ballot(tie) != 0ULL;
+    const bool any_amb = __ballot(ambiguous) != 0ULL;
+    for (int i = lane; i < dg; i += 64) pair_anchor[o + i] = a;
+    if (lane == 0) {
+        if (any_tie && dg > 16) tie_list[atomicAdd(&scal[S_TIES], 1u)] = a;
+        if (any_amb && n > 16) atomicAdd(&scal[S_AMBIG], 1u);
+    }
 }
 
 // Anchors whose (more than 16) candidates contain exact distance ties: replay
@@ -563,13 +665,59 @@ __global__ __launch_bounds__(kBlock) void k_bucket(int n, const int *__restrict_
                                                    int *__restrict__ lists, unsigned *__restrict__ scal)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const int a = sorted[p];
-    const unsigned m = max(cnt3[3 * a], max(cnt3[3 * a + 1], cnt3[3 * a + 2]));
-    int b = 0;
-    while (b < nb - 1 && m > static_cast<unsigned>(caps[b])) ++b;
-    const unsigned pos = atomicAdd(&scal[S_BUCKET + b], 1u);
-    lists[static_cast<long long>(b) * n + pos] = a;
+    const int lane = threadIdx.x & 63;
+    int a = -1, b = -1;
+    if (p < n) {
+        a = sorted[p];
+        const unsigned m = max(cnt3[3 * a], max(cnt3[3 * a + 1], cnt3[3 * a + 2]));
+        b = 0;
+        while (b < nb - 1 && m > static_cast<unsigned>(caps[b])) ++b;
+    }
+    // one atomic per wave and bucket: the lowest lane of each bucket group
+    // reserves the group's slots, the others take consecutive ones
+    for (int k = 0; k < nb; ++k) {
+        const unsigned long long grp = __ballot(b == k);
+        if (grp == 0ULL) continue;
+        const int leader = __ffsll(static_cast<long long>(grp)) - 1;
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(&scal[S_BUCKET + k], static_cast<unsigned>(__popcll(grp)));
+        base = __shfl(base, leader, 64);
+        if (b == k) {
+            const unsigned pos = base + static_cast<unsigned>(__popcll(grp & ((1ULL << lane) - 1ULL)));
+            lists[static_cast<long long>(k) * n + pos] = a;
+        }
+    }
+}
+
+// Per-pair sines of the transport (compute_curve_bundle, cpu_curvelet.hpp:728-729):
+// psin[q*2Kt + 2j] = sin(T2t_j - (theta_n - dt)), psin[q*2Kt + 2j + 1] =
+// sin(T2t_j - (theta_n + dt)) with T2t_j = theta_a + delta_t2[j], for every
+// pair that is not "close" (close pairs keep the default bundle and never read
+// them).  One thread per (pair, j); the sin/cos table lives in LDS.
+__global__ __launch_bounds__(kBlock) void k_pair_trig(int pairs, int kt, const int *__restrict__ pair_anchor,
+                                                      const int *__restrict__ ids, const double *__restrict__ x,
+                                                      const double *__restrict__ y, const double *__restrict__ th,
+                                                      const double *__restrict__ off_t, double dt, double close_thr,
+                                                      double *__restrict__ psin)
+{
+    __shared__ double tab[4 * CVG_SIN_ROWS];
+    for (int i = threadIdx.x; i < 4 * CVG_SIN_ROWS; i += blockDim.x) tab[i] = c_sincos_tab[i];
+    __syncthreads();
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<long long>(pairs) * kt) return;
+    const int q = static_cast<int>(t / kt);
+    const int j = static_cast<int>(t - static_cast<long long>(q) * kt);
+    const int a = pair_anchor[q];
+    const int b = ids[q];
+    const double DX = x[a] - x[b];
+    const double DY = y[a] - y[b];
+    if (DX * DX + DY * DY < close_thr) return;  // :682
+    const double T2t = th[a] + off_t[j];
+    const double nt = th[b];
+    double2 v;
+    v.x = cvg::cr_sin(T2t - (nt - dt), tab);
+    v.y = cvg::cr_sin(T2t - (nt + dt), tab);
+    reinterpret_cast<double2 *>(psin)[static_cast<long long>(q) * kt + j] = v;
 }
 
 // Host-computed per-parameter tables, laid out in one device block.  Lanes of
@@ -589,21 +737,20 @@ struct Tables {
 
 struct FusedArgs {
     int n;
-    const double *x, *y, *th, *trig;
+    const double *x, *y, *th, *trig, *psin;
     const unsigned *off;
     const int *ids;
     const unsigned char *cls;
     const int *list;
     int list_len;
-    int nb_cap;
-    int chain_stride;  // LDS chain slots per seed: max(G, 2)
+    int nb_cap;        // neighbour slots per wave (the bucket's largest run)
+    int chain_stride;  // LDS chain slots: max(G, 2)
     int st_stride;     // staging ids per row: max(G, 1)
-    double dt, dx, dx2, close_thr, max_k;
+    double dx, dx2, close_thr, max_k;
     int kt, K, G;
     const double *rank_ox;
     const int *rank_j, *rank_cell, *rank_elig;
     int rank_of_cell0;
-    const double *off_t;
     int *st_ids;
     unsigned *st_meta;
     double *st_k;
@@ -613,130 +760,84 @@ struct FusedArgs {
     double *g_bundle;  // non-null: bundles live in global memory, nb_cap*K*2 doubles per list entry
 };
 
-// Order-preserving compaction of [0, n) by pred into out (LDS); returns the count.
-template <typename Pred>
-__device__ int block_compact(int n, Pred pred, int *out, int *wsum)
-{
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    int total = 0;
-    for (int base = 0; base < n; base += 64 * kWaves) {
-        const int i = base + static_cast<int>(threadIdx.x);
-        const bool f = i < n && pred(i);
-        const unsigned long long m = __ballot(f);
-        const int below = __popcll(m & ((1ULL << lane) - 1ULL));
-        if (lane == 0) wsum[wave] = __popcll(m);
-        __syncthreads();
-        int pre = 0, chunk = 0;
-        for (int w = 0; w < kWaves; ++w) {
-            const int c = wsum[w];
-            if (w < wave) pre += c;
-            chunk += c;
-        }
-        if (f) out[total + pre + below] = i;
-        total += chunk;
-        __syncthreads();
-    }
-    return total;
-}
+constexpr int kFusedWaves = 4;  // independent anchors (waves) per workgroup
 
-// LDS carve-up for nb slots; returns the byte size.
-struct Carve {
-    size_t rec, t0, sinm, sinp, blo, bhi, skmax, skmin;  // double arrays (offsets in doubles)
-    size_t nid, nflag, lrun, hlist, slen, sbest, sacc, chains, misc;  // int arrays (offsets in ints from int base)
+// Per-wave LDS slice for nb slots.
+struct WaveCarve {
+    size_t blo, bhi, sines;                            // doubles
+    size_t lrun, hlist, nid, acc_len, acc_chain, cur;  // ints, from int_base
     size_t int_base_bytes, bytes;
 };
 
-__host__ __device__ inline Carve carve(int nb, int kt, int K, int chain_stride, bool lds_bundles)
+__host__ __device__ inline WaveCarve wave_carve(int nb, int K, int kt, int chain_stride, bool lds_bundles)
 {
-    Carve c{};
+    WaveCarve c{};
     size_t d = 0;
-    c.rec = d;
-    d += static_cast<size_t>(nb) * 8;
-    c.t0 = d;
-    d += static_cast<size_t>(nb) * 2;
-    c.sinm = d;
-    d += static_cast<size_t>(nb) * kt;
-    c.sinp = d;
-    d += static_cast<size_t>(nb) * kt;
+    c.sines = d;  // one chunk (<= 64 slots) of per-pair sines, 2*kt per slot
+    d += static_cast<size_t>(nb < 64 ? nb : 64) * kt * 2;
     c.blo = d;
     if (lds_bundles) d += static_cast<size_t>(nb) * K;
     c.bhi = d;
     if (lds_bundles) d += static_cast<size_t>(nb) * K;
-    c.skmax = d;
-    d += nb;
-    c.skmin = d;
-    d += nb;
     c.int_base_bytes = d * sizeof(double);
     size_t i = 0;
-    c.nid = i;
-    i += nb;
-    c.nflag = i;
-    i += nb;
     c.lrun = i;
     i += nb;
     c.hlist = i;
     i += nb;
-    c.slen = i;
+    c.nid = i;
     i += nb;
-    c.sbest = i;
+    c.acc_len = i;
     i += nb;
-    c.sacc = i;
-    i += nb;
-    c.chains = i;
+    c.acc_chain = i;
     i += static_cast<size_t>(nb) * chain_stride;
-    c.misc = i;
-    i += 16;
+    c.cur = i;
+    i += chain_stride;
+    i = (i + 1) & ~static_cast<size_t>(1);  // keep slices 8-byte aligned
     c.bytes = c.int_base_bytes + i * sizeof(int);
     return c;
 }
 
-enum { M_ACC = 0, M_VP = 1, M_NL = 2, M_NH = 3, M_ISECT = 4, M_WSUM = 8 };
-
-// build_curvelets_greedy for one anchor per workgroup (cpu_curvelet.hpp:437-591),
+// build_curvelets_greedy for one anchor per wavefront (cpu_curvelet.hpp:437-591)
 // plus the form_pairwise_bundles valid-pair count (:305-361), in one pass.
 // Runs: 0 = forward neighbours (class 1), 1 = backward (class 2), 2 = neither
-// (class 0, validity only for the count).  Lanes own bundle cells in rank
-// order, CPL cells per lane.
+// (class 0: validity only, for the count).  For each run the neighbours' bundles
+// go to the wave's LDS slice (lane = bundle cell in rank order, CPL cells per
+// lane); then every hypothesis seeds a chain that grows by intersecting bundles
+// in CSR order, validity by one ballot; accepted chains are deduplicated
+// against the run's earlier ones and written to the anchor's staging rows.
 template <int CPL>
-__global__ __launch_bounds__(64 * kWaves) void k_curvelets(FusedArgs A)
+__global__ __launch_bounds__(64 * kFusedWaves) void k_curvelets(FusedArgs A)
 {
     extern __shared__ double lds[];
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
-    constexpr int kThreads = 64 * kWaves;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
 
     // XCD-aware order: the grid is a multiple of 8 and block b runs on XCD b % 8;
-    // give each XCD a contiguous slice of the spatially ordered anchor list.
+    // each XCD gets a contiguous slice of the spatially ordered anchor list.
     const int per = static_cast<int>(gridDim.x) / 8;
-    const int L = static_cast<int>(blockIdx.x % 8) * per + static_cast<int>(blockIdx.x / 8);
-    if (L >= A.list_len) return;
+    const int blk = static_cast<int>(blockIdx.x % 8) * per + static_cast<int>(blockIdx.x / 8);
+    const int L = blk * kFusedWaves + wave;
+    if (L >= A.list_len) return;  // whole wave; no workgroup barriers below
     const int a = A.list[L];
 
     const bool lds_bundles = (A.g_bundle == nullptr);
-    const Carve cv = carve(A.nb_cap, A.kt, A.K, A.chain_stride, lds_bundles);
-    double *rec = lds + cv.rec;
-    double *t0 = lds + cv.t0;
-    double *sinm = lds + cv.sinm;
-    double *sinp = lds + cv.sinp;
-    double *blo = lds_bundles ? lds + cv.blo : A.g_bundle + static_cast<size_t>(L) * A.nb_cap * A.K * 2;
-    double *bhi = lds_bundles ? lds + cv.bhi : blo + static_cast<size_t>(A.nb_cap) * A.K;
-    double *skmax = lds + cv.skmax;
-    double *skmin = lds + cv.skmin;
-    int *ib = reinterpret_cast<int *>(reinterpret_cast<char *>(lds) + cv.int_base_bytes);
-    int *nid = ib + cv.nid;
-    int *nflag = ib + cv.nflag;
-    int *lrun = ib + cv.lrun;
-    int *hlist = ib + cv.hlist;
-    int *slen = ib + cv.slen;
-    int *sbest = ib + cv.sbest;
-    int *sacc = ib + cv.sacc;
-    int *chains = ib + cv.chains;
-    int *misc = ib + cv.misc;
+    const WaveCarve wc = wave_carve(A.nb_cap, A.K, A.kt, A.chain_stride, lds_bundles);
+    char *slice = reinterpret_cast<char *>(lds) + static_cast<size_t>(wave) * wc.bytes;
+    double *blo = lds_bundles ? reinterpret_cast<double *>(slice) + wc.blo
+                              : A.g_bundle + static_cast<size_t>(L) * A.nb_cap * A.K * 2;
+    double *bhi = lds_bundles ? reinterpret_cast<double *>(slice) + wc.bhi : blo + static_cast<size_t>(A.nb_cap) * A.K;
+    double2 *sines = reinterpret_cast<double2 *>(reinterpret_cast<double *>(slice) + wc.sines);
+    int *ib = reinterpret_cast<int *>(slice + wc.int_base_bytes);
+    int *lrun = ib + wc.lrun;
+    int *hlist = ib + wc.hlist;
+    int *nid = ib + wc.nid;
+    int *acc_len = ib + wc.acc_len;
+    int *acc_chain = ib + wc.acc_chain;
+    int *cur = ib + wc.cur;
 
     const int n = A.n;
-    const double ax = A.x[a], ay = A.y[a], at = A.th[a];
+    const double ax = A.x[a], ay = A.y[a];
     const double cos_at = A.trig[cvg::TRIG_COS * n + a];
     const double sin_at = A.trig[cvg::TRIG_SIN * n + a];
     const unsigned base = A.off[a];
@@ -745,6 +846,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_curvelets(FusedArgs A)
     const int kt = A.kt;
     const int G = A.G;
     const int CS = A.chain_stride;
+    const unsigned long long lt_mask = (1ULL << lane) - 1ULL;
 
     // per-lane cell constants
     double ox[CPL], sT2t[CPL], cT2t[CPL];
@@ -761,102 +863,89 @@ __global__ __launch_bounds__(64 * kWaves) void k_curvelets(FusedArgs A)
         sT2t[m] = A.trig[static_cast<long long>(cvg::TRIG_GRID + 2 * cj[m]) * n + a];
         cT2t[m] = A.trig[static_cast<long long>(cvg::TRIG_GRID + 2 * cj[m] + 1) * n + a];
     }
-    if (tid == 0) {
-        misc[M_ACC] = 0;
-        misc[M_VP] = 0;
-    }
-    int isect = 0;  // intersections tested by this wave (lane 0 counts)
-    __syncthreads();
 
+    int acc = 0;       // accepted rows of this anchor (both runs)
+    int vp = 0;        // valid or close pairs (all three classes)
+    int isect = 0;     // intersections tested
     for (int run = 0; run < 3; ++run) {
         const unsigned char want = run == 0 ? 1 : (run == 1 ? 2 : 0);
-        const int nL = block_compact(
-            deg, [&](int r) { return A.cls[base + r] == want; }, lrun, misc + M_WSUM);
+        // the run's neighbours, in CSR order
+        int nL = 0;
+        for (int r0 = 0; r0 < deg; r0 += 64) {
+            const int r = r0 + lane;
+            const bool f = r < deg && A.cls[base + r] == want;
+            const unsigned long long m = __ballot(f);
+            if (f) lrun[nL + __popcll(m & lt_mask)] = r;
+            nL += __popcll(m);
+        }
         if (nL == 0) continue;
+        wave_sync();
 
-        // neighbour records (compute_curve_bundle's per-pair constants, :646-656, 682, 690, 730-733)
-        for (int k = tid; k < nL; k += kThreads) {
-            const int r = lrun[k];
-            const int b = A.ids[base + r];
-            const double nx = A.x[b], ny = A.y[b], nt = A.th[b];
-            const PairRec p = cvg::pair_record(ax, ay, nx, ny, A.trig[cvg::TRIG_SIN_P * n + b],
+        // bundles, 64 neighbours at a time: lane k holds slot k's record, every
+        // lane then computes its cells for each slot in turn
+        int nH = 0;
+        for (int c0 = 0; c0 < nL; c0 += 64) {
+            const int k = c0 + lane;
+            const bool has = k < nL;
+            const int r = has ? lrun[k] : 0;
+            const int b = has ? A.ids[base + r] : a;
+            const PairRec p = cvg::pair_record(ax, ay, A.x[b], A.y[b], A.trig[cvg::TRIG_SIN_P * n + b],
                                                A.trig[cvg::TRIG_COS_P * n + b], A.trig[cvg::TRIG_SIN_M * n + b],
                                                A.trig[cvg::TRIG_COS_M * n + b]);
-            double *rr = rec + static_cast<size_t>(k) * 8;
-            rr[0] = p.DX;
-            rr[1] = p.DY;
-            rr[2] = p.DX2;
-            rr[3] = p.DY2;
-            rr[4] = p.cT0m_DX;
-            rr[5] = p.sT0m_DY;
-            rr[6] = p.cT0p_DX;
-            rr[7] = p.sT0p_DY;
-            t0[2 * k] = nt - A.dt;
-            t0[2 * k + 1] = nt + A.dt;
-            const bool close = p.DX2 + p.DY2 < A.close_thr;
-            const bool branch_a = (p.DX * cos_at + p.DY * sin_at) < 0;
-            nid[k] = b;
-            nflag[k] = (close ? 1 : 0) | (branch_a ? 2 : 0);
-        }
-        __syncthreads();
-
-        // per-pair sines sin(T2t - T0_m_dt), sin(T2t - T0_p_dt) (:728-729)
-        const int ntrig = nL * kt * 2;
-        for (int q = tid; q < ntrig; q += kThreads) {
-            const int k = q / (2 * kt);
-            const int rem = q - k * 2 * kt;
-            const int j = rem >> 1;
-            const int pm = rem & 1;
-            if (!(nflag[k] & 1)) {
-                const double T2t = at + A.off_t[j];
-                const double s = cvg::cr_sin(T2t - t0[2 * k + pm], c_sincos_tab);
-                (pm ? sinp : sinm)[k * kt + j] = s;
+            const int close = (p.DX2 + p.DY2 < A.close_thr) ? 1 : 0;              // :682
+            const int branch_a = ((p.DX * cos_at + p.DY * sin_at) < 0) ? 1 : 0;  // :690
+            if (has) nid[k] = b;
+            const int cnt = min(64, nL - c0);
+            // this chunk's per-pair sines, fetched in parallel into LDS
+            for (int t = lane; t < cnt * kt; t += 64) {
+                const int slot = t / kt;
+                const int jj = t - slot * kt;
+                const long long pq = static_cast<long long>(base) + lrun[c0 + slot];
+                sines[t] = reinterpret_cast<const double2 *>(A.psin)[pq * kt + jj];
             }
-        }
-        __syncthreads();
-
-        // bundle cells, one neighbour per wave at a time
-        for (int k = wave; k < nL; k += kWaves) {
-            const int fl = nflag[k];
-            const bool close = fl & 1;
-            const double *rr = rec + static_cast<size_t>(k) * 8;
-            PairRec p;
-            p.DX = rr[0];
-            p.DY = rr[1];
-            p.DX2 = rr[2];
-            p.DY2 = rr[3];
-            p.cT0m_DX = rr[4];
-            p.sT0m_DY = rr[5];
-            p.cT0p_DX = rr[6];
-            p.sT0p_DY = rr[7];
-            bool any = false;
+            wave_sync();
+            unsigned long long hyp_mask = 0ULL;
+            for (int kk = 0; kk < cnt; ++kk) {
+                PairRec q;
+                q.DX = bcast(p.DX, kk);
+                q.DY = bcast(p.DY, kk);
+                q.DX2 = bcast(p.DX2, kk);
+                q.DY2 = bcast(p.DY2, kk);
+                q.cT0m_DX = bcast(p.cT0m_DX, kk);
+                q.sT0m_DY = bcast(p.sT0m_DY, kk);
+                q.cT0p_DX = bcast(p.cT0p_DX, kk);
+                q.sT0p_DY = bcast(p.sT0p_DY, kk);
+                const bool qclose = bcast(close, kk) != 0;
+                const bool qbr = bcast(branch_a, kk) != 0;
+                const double2 *ps = sines + kk * kt;
+                bool any = false;
 #pragma unroll
-            for (int m = 0; m < CPL; ++m) {
-                if (!live[m]) continue;
-                const int rho = lane + 64 * m;
-                double lo = -A.max_k, hi = A.max_k;
-                if (!close)
-                    cvg::bundle_cell(p, (fl & 2) != 0, ox[m], sT2t[m], cT2t[m], sinm[k * kt + cj[m]],
-                                     sinp[k * kt + cj[m]], A.dx, A.dx2, A.max_k, lo, hi);
-                blo[static_cast<size_t>(k) * K + rho] = lo;
-                bhi[static_cast<size_t>(k) * K + rho] = hi;
-                any = any || (hi > lo);
+                for (int m = 0; m < CPL; ++m) {
+                    if (!live[m]) continue;
+                    const int rho = lane + 64 * m;
+                    double lo = -A.max_k, hi = A.max_k;
+                    if (!qclose) {
+                        const double2 s = ps[cj[m]];
+                        cvg::bundle_cell(q, qbr, ox[m], sT2t[m], cT2t[m], s.x, s.y, A.dx, A.dx2, A.max_k, lo, hi);
+                    }
+                    blo[static_cast<size_t>(c0 + kk) * K + rho] = lo;
+                    bhi[static_cast<size_t>(c0 + kk) * K + rho] = hi;
+                    any = any || (hi > lo);
+                }
+                const bool hyp = qclose || (__ballot(any) != 0ULL);  // bundle_valid_check, :821-830
+                if (hyp) hyp_mask |= 1ULL << kk;
             }
-            const bool hyp = close || (__ballot(any) != 0ULL);  // bundle_valid_check, :821-830
-            if (lane == 0) nflag[k] = fl | (hyp ? 4 : 0);
+            // hypotheses of this chunk, in slot order
+            if ((hyp_mask >> lane) & 1ULL) hlist[nH + __popcll(hyp_mask & lt_mask)] = c0 + lane;
+            nH += __popcll(hyp_mask);
         }
-        __syncthreads();
+        vp += nH;
+        if (run == 2 || nH == 0) continue;
+        wave_sync();
 
-        const int nH = block_compact(
-            nL, [&](int k) { return (nflag[k] & 4) != 0; }, hlist, misc + M_WSUM);
-        if (tid == 0) misc[M_VP] += nH;
-        if (run == 2 || nH == 0) {
-            __syncthreads();
-            continue;
-        }
-
-        // chain growth by bundle intersection (:479-536); seeds split over waves
-        for (int si = wave; si < nH; si += kWaves) {
+        // growth (:479-536), dedup (:546-560, 876-916) and record, seed by seed
+        int nacc = 0;
+        for (int si = 0; si < nH; ++si) {
             const int s = hlist[si];
             double lo[CPL], hi[CPL];
 #pragma unroll
@@ -866,12 +955,37 @@ __global__ __launch_bounds__(64 * kWaves) void k_curvelets(FusedArgs A)
                 hi[m] = bhi[static_cast<size_t>(s) * K + rho];
             }
             int len = 1;
-            int *ch = chains + static_cast<size_t>(si) * CS;
-            if (lane == 0) ch[0] = a;
+            if (lane == 0) cur[0] = a;
+            // next candidate bundle, loaded one step ahead
+            double pbl[CPL], pbh[CPL];
+            {
+                const int r0 = hlist[0];
+#pragma unroll
+                for (int m = 0; m < CPL; ++m) {
+                    const int rho = live[m] ? lane + 64 * m : 0;
+                    pbl[m] = blo[static_cast<size_t>(r0) * K + rho];
+                    pbh[m] = bhi[static_cast<size_t>(r0) * K + rho];
+                }
+            }
             for (int ri = 0; ri < nH; ++ri) {
                 const int r = hlist[ri];
+                double bl[CPL], bh[CPL];
+#pragma unroll
+                for (int m = 0; m < CPL; ++m) {
+                    bl[m] = pbl[m];
+                    bh[m] = pbh[m];
+                }
+                if (ri + 1 < nH) {
+                    const int rn = hlist[ri + 1];
+#pragma unroll
+                    for (int m = 0; m < CPL; ++m) {
+                        const int rho = live[m] ? lane + 64 * m : 0;
+                        pbl[m] = blo[static_cast<size_t>(rn) * K + rho];
+                        pbh[m] = bhi[static_cast<size_t>(rn) * K + rho];
+                    }
+                }
                 if (ri == si) {  // the seed joins at its own distance rank (:496-510)
-                    if (lane == 0) ch[len] = nid[r];
+                    if (lane == 0) cur[len] = nid[r];
                     ++len;
                     if (len >= G) break;
                     continue;
@@ -881,15 +995,12 @@ __global__ __launch_bounds__(64 * kWaves) void k_curvelets(FusedArgs A)
                 bool ok = false;
 #pragma unroll
                 for (int m = 0; m < CPL; ++m) {
-                    const int rho = live[m] ? lane + 64 * m : 0;
-                    const double bl = blo[static_cast<size_t>(r) * K + rho];
-                    const double bh = bhi[static_cast<size_t>(r) * K + rho];
-                    nlo[m] = (lo[m] < bl) ? bl : lo[m];  // bundle_intersection, :838-841
-                    nhi[m] = (hi[m] > bh) ? bh : hi[m];
+                    nlo[m] = (lo[m] < bl[m]) ? bl[m] : lo[m];  // bundle_intersection, :838-841
+                    nhi[m] = (hi[m] > bh[m]) ? bh[m] : hi[m];
                     ok = ok || (live[m] && nhi[m] > nlo[m]);
                 }
                 if (__ballot(ok) != 0ULL) {
-                    if (lane == 0) ch[len] = nid[r];
+                    if (lane == 0) cur[len] = nid[r];
                     ++len;
 #pragma unroll
                     for (int m = 0; m < CPL; ++m) {
@@ -899,6 +1010,23 @@ __global__ __launch_bounds__(64 * kWaves) void k_curvelets(FusedArgs A)
                 }
                 if (len >= G) break;
             }
+            if (len <= 2) continue;  // :549
+            wave_sync();
+            // identical to an earlier accepted chain of this (anchor, run)?
+            bool dup = false;
+            for (int k0 = 0; k0 < nacc && !dup; k0 += 64) {
+                const int k = k0 + lane;
+                bool same = k < nacc && acc_len[k] == len;
+                if (same) {
+                    const int *other = acc_chain + static_cast<size_t>(k) * CS;
+                    for (int l = 0; l < len; ++l) same = same && (other[l] == cur[l]);
+                }
+                dup = __ballot(same) != 0ULL;
+            }
+            if (dup) continue;
+            for (int l = lane; l < len; l += 64) acc_chain[static_cast<size_t>(nacc) * CS + l] = cur[l];
+            if (lane == 0) acc_len[nacc] = len;
+            ++nacc;
             // best cell of the running bundle (fill_curvelet_info, :202-226)
             int best = -1;
 #pragma unroll
@@ -912,65 +1040,27 @@ __global__ __launch_bounds__(64 * kWaves) void k_curvelets(FusedArgs A)
             double kmax = 0.0, kmin = 0.0;
 #pragma unroll
             for (int m = 0; m < CPL; ++m) {
-                const double h = __shfl(hi[m], bl, 64);
-                const double l = __shfl(lo[m], bl, 64);
                 if (m == bm) {
-                    kmax = h;
-                    kmin = l;
+                    kmax = bcast(hi[m], bl);
+                    kmin = bcast(lo[m], bl);
                 }
             }
+            const long long q = static_cast<long long>(base) + acc;
+            for (int l = lane; l < len; l += 64) A.st_ids[q * A.st_stride + l] = cur[l];
             if (lane == 0) {
-                slen[si] = len;
-                sbest[si] = A.rank_cell[best];
-                skmax[si] = kmax;
-                skmin[si] = kmin;
+                A.st_meta[q] = static_cast<unsigned>(len) | (run == 0 ? 0x100u : 0u) |
+                               (static_cast<unsigned>(A.rank_cell[best]) << 16);
+                A.st_k[2 * q] = kmax;
+                A.st_k[2 * q + 1] = kmin;
             }
+            ++acc;
+            wave_sync();
         }
-        __syncthreads();
-
-        // accept (len > 2, :549) unless identical to an earlier accepted chain of
-        // this (anchor, run) (check_curvelet_exist, :876-916); record in seed order
-        if (wave == 0) {
-            int acc = misc[M_ACC];
-            for (int si = 0; si < nH; ++si) {
-                const int len = slen[si];
-                bool accept = len > 2;
-                if (accept) {
-                    const int *mine = chains + static_cast<size_t>(si) * CS;
-                    for (int kb = 0; kb < si && accept; kb += 64) {
-                        const int k = kb + lane;
-                        bool same = false;
-                        if (k < si && sacc[k] && slen[k] == len) {
-                            const int *other = chains + static_cast<size_t>(k) * CS;
-                            same = true;
-                            for (int l = 0; l < len; ++l) same = same && (other[l] == mine[l]);
-                        }
-                        if (__ballot(same) != 0ULL) accept = false;
-                    }
-                }
-                if (lane == 0) sacc[si] = accept ? 1 : 0;
-                if (accept) {
-                    const long long q = static_cast<long long>(base) + acc;
-                    const int *mine = chains + static_cast<size_t>(si) * CS;
-                    for (int l = lane; l < len; l += 64) A.st_ids[q * A.st_stride + l] = mine[l];
-                    if (lane == 0) {
-                        A.st_meta[q] = static_cast<unsigned>(len) | (run == 0 ? 0x100u : 0u) |
-                                       (static_cast<unsigned>(sbest[si]) << 16);
-                        A.st_k[2 * q] = skmax[si];
-                        A.st_k[2 * q + 1] = skmin[si];
-                    }
-                    ++acc;
-                }
-                __builtin_amdgcn_wave_barrier();
-            }
-            if (lane == 0) misc[M_ACC] = acc;
-        }
-        __syncthreads();
     }
-    if (lane == 0 && isect > 0) atomicAdd(A.intersections, static_cast<unsigned long long>(isect));
-    if (tid == 0) {
-        A.acc_count[a] = static_cast<unsigned>(misc[M_ACC]);
-        atomicAdd(A.valid_pairs, static_cast<unsigned>(misc[M_VP]));
+    if (lane == 0) {
+        A.acc_count[a] = static_cast<unsigned>(acc);
+        if (vp) atomicAdd(A.valid_pairs, static_cast<unsigned>(vp));
+        if (isect) atomicAdd(A.intersections, static_cast<unsigned long long>(isect));
     }
 }
 
@@ -1222,11 +1312,21 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     CVG_TRY(ensure(ctx, ctx->d_ids, np * sizeof(int)));
     CVG_TRY(ensure(ctx, ctx->d_d2, np * sizeof(double)));
     CVG_TRY(ensure(ctx, ctx->d_pair_anchor, np * sizeof(int)));
-    hipLaunchKernelGGL(k_fill_csr, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<double>(ctx->d_x),
+    // one wave per anchor; each wave keeps its candidates in LDS
+    int fill_waves = kFillWaves;
+    const int fill_cap = static_cast<int>(std::max(ctx->max_deg, 1u));
+    const size_t per_wave = static_cast<size_t>(fill_cap) * (2 * sizeof(double));
+    while (fill_waves > 1 && per_wave * fill_waves > ctx->lds_limit) fill_waves /= 2;
+    if (per_wave * fill_waves > ctx->lds_limit)
+        return fail(ctx, CVG_ERR_UNSUPPORTED, "an anchor has too many neighbours for one workgroup's LDS");
+    const unsigned fill_shmem = static_cast<unsigned>(per_wave * fill_waves);
+    CVG_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(k_fill_csr),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(fill_shmem)));
+    hipLaunchKernelGGL(k_fill_csr, dim3((n + fill_waves - 1) / fill_waves), dim3(64 * fill_waves), fill_shmem, st, n, as<double>(ctx->d_x),
                        as<double>(ctx->d_y), as<int>(ctx->d_cx), as<int>(ctx->d_cy), g, static_cast<int>(R), rad2,
                        start, as<int>(ctx->d_sorted), as<double>(ctx->d_sx), as<double>(ctx->d_sy),
                        as<unsigned>(ctx->d_off), as<Cand>(ctx->d_cand), as<int>(ctx->d_ids), as<double>(ctx->d_d2),
-                       as<int>(ctx->d_pair_anchor), as<int>(ctx->d_tmp_ids), scal);
+                       as<int>(ctx->d_pair_anchor), as<int>(ctx->d_tmp_ids), max_cand, fill_cap, scal);
     hipLaunchKernelGGL(k_tie_sort, dim3(std::min(blocks_for(n), 64)), dim3(kBlock), 0, st, as<int>(ctx->d_tmp_ids),
                        scal, as<unsigned>(ctx->d_off), as<Cand>(ctx->d_cand), as<int>(ctx->d_ids),
                        as<double>(ctx->d_d2));
@@ -1340,10 +1440,11 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     T.off_t = reinterpret_cast<double *>(tb + o_off_t);
     T.caps = reinterpret_cast<int *>(tb + o_caps);
 
-    // per-edge trig, per-pair classes, buckets
+    // per-edge trig, per-pair classes and sines, buckets
     const int ntrig = cvg::TRIG_GRID + 2 * static_cast<int>(d.kt);
     CVG_TRY(ensure(ctx, ctx->d_trig, static_cast<size_t>(ntrig) * n * sizeof(double)));
     CVG_TRY(ensure(ctx, ctx->d_cls, static_cast<size_t>(pairs) + 1));
+    CVG_TRY(ensure(ctx, ctx->d_psin, static_cast<size_t>(std::max(pairs, 1)) * d.kt * 2 * sizeof(double)));
     CVG_TRY(ensure(ctx, ctx->d_cnt3, static_cast<size_t>(3) * n * sizeof(unsigned)));
     CVG_TRY(ensure(ctx, ctx->d_lists, static_cast<size_t>(nb) * n * sizeof(int)));
     unsigned *scal = as<unsigned>(ctx->d_scal);
@@ -1351,11 +1452,16 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
                        static_cast<int>(d.kt), T.off_t, as<double>(ctx->d_trig));
     CVG_HIP(ctx, hipMemsetAsync(ctx->d_cnt3.ptr, 0, static_cast<size_t>(3) * n * sizeof(unsigned), st));
     CVG_HIP(ctx, hipMemsetAsync(scal + S_VALID, 0, (kScalarWords - S_VALID) * sizeof(unsigned), st));
-    if (pairs > 0)
+    if (pairs > 0) {
         hipLaunchKernelGGL(k_pair_class, dim3(blocks_for(pairs)), dim3(kBlock), 0, st, pairs, n,
                            as<int>(ctx->d_pair_anchor), as<int>(ctx->d_ids), as<double>(ctx->d_x),
                            as<double>(ctx->d_y), as<double>(ctx->d_trig), as<unsigned char>(ctx->d_cls),
                            as<unsigned>(ctx->d_cnt3));
+        hipLaunchKernelGGL(k_pair_trig, dim3(blocks_for(static_cast<long long>(pairs) * d.kt)), dim3(kBlock), 0, st,
+                           pairs, static_cast<int>(d.kt), as<int>(ctx->d_pair_anchor), as<int>(ctx->d_ids),
+                           as<double>(ctx->d_x), as<double>(ctx->d_y), as<double>(ctx->d_th), T.off_t, d.dt,
+                           d.close_thr, as<double>(ctx->d_psin));
+    }
     hipLaunchKernelGGL(k_bucket, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_sorted),
                        as<unsigned>(ctx->d_cnt3), nb, T.caps, as<int>(ctx->d_lists), scal);
     CVG_HIP(ctx, hipGetLastError());
@@ -1380,12 +1486,12 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     A.y = as<double>(ctx->d_y);
     A.th = as<double>(ctx->d_th);
     A.trig = as<double>(ctx->d_trig);
+    A.psin = as<double>(ctx->d_psin);
     A.off = as<unsigned>(ctx->d_off);
     A.ids = as<int>(ctx->d_ids);
     A.cls = as<unsigned char>(ctx->d_cls);
     A.chain_stride = chain_stride;
     A.st_stride = st_stride;
-    A.dt = d.dt;
     A.dx = d.dx;
     A.dx2 = d.dx2;
     A.close_thr = d.close_thr;
@@ -1398,7 +1504,6 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     A.rank_cell = T.rank_cell;
     A.rank_elig = T.rank_elig;
     A.rank_of_cell0 = d.rank_of_cell0;
-    A.off_t = T.off_t;
     A.st_ids = as<int>(ctx->d_st_ids);
     A.st_meta = as<unsigned>(ctx->d_st_meta);
     A.st_k = as<double>(ctx->d_st_k);
@@ -1406,42 +1511,61 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     A.valid_pairs = scal + S_VALID;
     A.intersections = reinterpret_cast<unsigned long long *>(scal + S_COUNT);
 
-    CVG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+    // bundles of buckets too large for LDS go to global memory, one region each
+    std::vector<size_t> gb_off(nb, 0);
+    size_t gb_total = 0;
     for (int b = 0; b < nb; ++b) {
+        if (bucket_count[b] == 0) continue;
+        if (wave_carve(caps[b], A.K, A.kt, chain_stride, true).bytes * kFusedWaves > ctx->lds_limit) {
+            gb_off[b] = gb_total;
+            gb_total += static_cast<size_t>(bucket_count[b]) * caps[b] * A.K * 2;
+        }
+    }
+    if (gb_total > 0) CVG_TRY(ensure(ctx, ctx->d_gbundle, gb_total * sizeof(double)));
+
+    // the smallest bucket (most anchors) on the main stream, the long-list
+    // buckets on the auxiliary streams so their tails overlap it
+    CVG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+    int aux_used = 0;
+    for (int b = nb - 1; b >= 0; --b) {
         const int len = static_cast<int>(bucket_count[b]);
         if (len == 0) continue;
+        hipStream_t ls = st;
+        if (b > 0) {
+            const int ai = aux_used < 3 ? aux_used : 2;
+            ls = ctx->aux[ai];
+            if (aux_used < 3) ++aux_used;
+        }
         A.list = as<int>(ctx->d_lists) + static_cast<size_t>(b) * n;
         A.list_len = len;
         A.nb_cap = caps[b];
-        bool lds_bundles = true;
-        Carve cv = carve(A.nb_cap, A.kt, A.K, chain_stride, true);
-        if (cv.bytes > ctx->lds_limit) {
-            lds_bundles = false;
-            cv = carve(A.nb_cap, A.kt, A.K, chain_stride, false);
-            if (cv.bytes > ctx->lds_limit)
+        WaveCarve wc = wave_carve(A.nb_cap, A.K, A.kt, chain_stride, true);
+        if (wc.bytes * kFusedWaves > ctx->lds_limit) {
+            wc = wave_carve(A.nb_cap, A.K, A.kt, chain_stride, false);
+            if (wc.bytes * kFusedWaves > ctx->lds_limit)
                 return fail(ctx, CVG_ERR_UNSUPPORTED, "neighbour lists too long for the workgroup's LDS");
-            CVG_TRY(ensure(ctx, ctx->d_gbundle, static_cast<size_t>(len) * A.nb_cap * A.K * 2 * sizeof(double)));
-            A.g_bundle = as<double>(ctx->d_gbundle);
+            A.g_bundle = as<double>(ctx->d_gbundle) + gb_off[b];
         } else {
             A.g_bundle = nullptr;
         }
-        const int grid = (len + 7) / 8 * 8;
-        const unsigned shmem = static_cast<unsigned>(cv.bytes);
-        if (d.cpl == 1) {
-            CVG_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(k_curvelets<1>),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
-            hipLaunchKernelGGL(k_curvelets<1>, dim3(grid), dim3(64 * kWaves), shmem, st, A);
-        } else if (d.cpl == 2) {
-            CVG_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(k_curvelets<2>),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
-            hipLaunchKernelGGL(k_curvelets<2>, dim3(grid), dim3(64 * kWaves), shmem, st, A);
-        } else {
-            CVG_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(k_curvelets<4>),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
-            hipLaunchKernelGGL(k_curvelets<4>, dim3(grid), dim3(64 * kWaves), shmem, st, A);
-        }
+        const int blocks = (len + kFusedWaves - 1) / kFusedWaves;
+        const int grid = (blocks + 7) / 8 * 8;
+        const unsigned shmem = static_cast<unsigned>(wc.bytes * kFusedWaves);
+        const void *fn = d.cpl == 1 ? reinterpret_cast<const void *>(k_curvelets<1>)
+                                    : (d.cpl == 2 ? reinterpret_cast<const void *>(k_curvelets<2>)
+                                                  : reinterpret_cast<const void *>(k_curvelets<4>));
+        CVG_HIP(ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
+        if (d.cpl == 1)
+            hipLaunchKernelGGL(k_curvelets<1>, dim3(grid), dim3(64 * kFusedWaves), shmem, ls, A);
+        else if (d.cpl == 2)
+            hipLaunchKernelGGL(k_curvelets<2>, dim3(grid), dim3(64 * kFusedWaves), shmem, ls, A);
+        else
+            hipLaunchKernelGGL(k_curvelets<4>, dim3(grid), dim3(64 * kFusedWaves), shmem, ls, A);
         CVG_HIP(ctx, hipGetLastError());
-        (void)lds_bundles;
+    }
+    for (int i = 0; i < aux_used; ++i) {
+        CVG_HIP(ctx, hipEventRecord(ctx->ev_join[i], ctx->aux[i]));
+        CVG_HIP(ctx, hipStreamWaitEvent(st, ctx->ev_join[i], 0));
     }
     CVG_HIP(ctx, hipEventRecord(ctx->ev[4], st));
     // every anchor is in exactly one bucket, so every acc_count entry is written
@@ -1524,6 +1648,7 @@ int run_common(cvg_context *ctx, int n, int sz, const double *d_edges, const cvg
         stats->ms_fused_kernel = tf;
         stats->intersections = ctx->intersections;
         stats->order_ambiguous_anchors = ctx->ambiguous;
+        stats->lds_bytes_per_workgroup = static_cast<unsigned>(ctx->lds_limit);
     }
     return CVG_OK;
 }
@@ -1592,6 +1717,12 @@ int cvg_create(int device, cvg_context **out_ctx)
     do {
         if (hipSetDevice(device) != hipSuccess) { rc = CVG_ERR_DEVICE; break; }
         if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { rc = CVG_ERR_DEVICE; break; }
+        for (int i = 0; i < 3 && rc == CVG_OK; ++i) {
+            if (hipStreamCreateWithFlags(&ctx->aux[i], hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&ctx->ev_join[i], hipEventDisableTiming) != hipSuccess)
+                rc = CVG_ERR_DEVICE;
+        }
+        if (rc != CVG_OK) break;
         for (auto &e : ctx->ev)
             if (hipEventCreate(&e) != hipSuccess) rc = CVG_ERR_DEVICE;
         if (rc != CVG_OK) break;
@@ -1625,13 +1756,18 @@ void cvg_destroy(cvg_context *ctx)
                       &ctx->d_cy,       &ctx->d_scal,  &ctx->d_grid,    &ctx->d_tmp_ids, &ctx->d_sorted,
                       &ctx->d_sx,       &ctx->d_sy,    &ctx->d_deg,     &ctx->d_off,     &ctx->d_tiles,
                       &ctx->d_cand,     &ctx->d_ids,   &ctx->d_d2,      &ctx->d_pair_anchor, &ctx->d_trig,
-                      &ctx->d_cls,      &ctx->d_cnt3,  &ctx->d_lists,   &ctx->d_tables,  &ctx->d_st_ids,
+                      &ctx->d_cls,      &ctx->d_psin,  &ctx->d_cnt3,  &ctx->d_lists,   &ctx->d_tables,  &ctx->d_st_ids,
                       &ctx->d_st_meta,  &ctx->d_st_k,  &ctx->d_acc,     &ctx->d_rowoff,  &ctx->d_gbundle,
                       &ctx->d_chains,   &ctx->d_info,  &ctx->d_dbg};
     for (DevBuf *b : bufs) release(*b);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->mailbox) (void)hipHostFree(ctx->mailbox);
+    for (int i = 0; i < 3; ++i) {
+        if (ctx->aux[i]) (void)hipStreamSynchronize(ctx->aux[i]);
+        if (ctx->ev_join[i]) (void)hipEventDestroy(ctx->ev_join[i]);
+        if (ctx->aux[i]) (void)hipStreamDestroy(ctx->aux[i]);
+    }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -84,6 +84,7 @@ typedef struct cvg_run_stats {
     double ms_fused_kernel;    /* the bundle + growth kernel launches alone */
     unsigned long long intersections; /* bundle intersections tested during growth */
     unsigned order_ambiguous_anchors;
+    unsigned lds_bytes_per_workgroup; /* the device limit the kernels were sized for */
 } cvg_run_stats;
 
 typedef struct cvg_context cvg_context;

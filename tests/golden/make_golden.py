@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Regenerate the golden fixtures from the reference build (oracle/_ref).
+
+Run in a checkout where /root/reference exists (`make -C oracle ref` first):
+
+    python tests/golden/make_golden.py
+
+Writes, under tests/golden/:
+  ref_cli/<case>.json          counts from the reference CLI's stdout
+                               (CSR pairs, max degree, valid pairs, out_h/out_w)
+                               and SHA-256 of its chain/info text files
+  ref_cli/<case>_{chain,info}.txt.gz
+                               the reference CLI's own output files (small cases)
+  full/<case>.npz              full-precision arrays from the reference's
+                               cpu_curvelet code via oracle/_ref/libcurvelet_ref.so:
+                               CSR (offsets, ids, dist2), valid pairs, chains, info
+  pairs/<case>.npz             per-pair bundles (compute_curve_bundle) and the
+                               forward/backward direction-filter values for
+                               sampled neighbour pairs and constructed geometries
+The inputs are the reference's shipped edge files (inputs/*.txt.gz) and
+synthetic maps from the package's generator (inputs/<case>.npy, written here).
+"""
+from __future__ import annotations
+
+import gzip
+import hashlib
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parents[1]
+sys.path.insert(0, str(ROOT))
+
+from oracle import REF_CLI, Ref  # noqa: E402
+from curvelet_construction_on_gpu_amd import CurveletParams, synth_edges  # noqa: E402
+from tests._inputs import shipped_edges, shipped_text  # noqa: E402
+
+# (case name, input, group_max_sz, keep full text, keep full-precision arrays)
+CASES = [
+    ("digit1_G4", "file:TO_edges_digit1", 4, True, True),
+    ("digit1_G7", "file:TO_edges_digit1", 7, True, True),
+    ("abc_G4", "file:TO_edges_ABC_0006_thresh1", 4, True, True),
+    ("abc_G7", "file:TO_edges_ABC_0006_thresh1", 7, False, False),
+    ("toedges_G4", "file:TO_edges", 4, False, False),
+    ("toedges_G7", "file:TO_edges", 7, False, False),
+    ("c1s1001_G4", "synth:C1:1001", 4, False, True),
+]
+
+
+def sha(b: bytes) -> str:
+    return hashlib.sha256(b).hexdigest()
+
+
+def edge_text(key: str) -> tuple[str, bytes]:
+    kind, _, rest = key.partition(":")
+    if kind == "file":
+        return rest + ".txt", shipped_text(rest)
+    preset, seed = rest.split(":")
+    e = synth_edges(preset, int(seed))
+    np.save(HERE / "inputs" / f"{preset}_{seed}.npy", e)
+    lines = "\n".join("\t".join(repr(float(v)) for v in row) for row in e) + "\n"
+    return f"{preset}_{seed}.txt", lines.encode()
+
+
+def edges_of(key: str) -> np.ndarray:
+    kind, _, rest = key.partition(":")
+    if kind == "file":
+        return shipped_edges(rest)
+    preset, seed = rest.split(":")
+    return np.load(HERE / "inputs" / f"{preset}_{seed}.npy")
+
+
+def run_cli(fname: str, text: bytes, G: int) -> tuple[dict, bytes, bytes]:
+    with tempfile.TemporaryDirectory() as td:
+        td = Path(td)
+        (td / "test_files").mkdir()
+        (td / "run").mkdir()
+        (td / "test_files" / fname).write_bytes(text)
+        out = subprocess.run([str(REF_CLI), "--edge-file", fname, "--group-max-sz", str(G), "--nthreads", "4"],
+                             cwd=td / "run", capture_output=True, text=True, check=True).stdout
+        chain = (td / "outputs" / "chain_cpu.txt").read_bytes()
+        info = (td / "outputs" / "info_cpu.txt").read_bytes()
+    m = re.search(r"CSR neighbor graph: (\d+) anchors, (\d+) total pairs, max number of neighbors per anchor: (\d+)",
+                  out)
+    v = re.search(r"Pairwise curve bundles formed: (\d+) valid pairs", out)
+    s = re.search(r"\(out_h, out_w\) = \((\d+), (\d+)\)", out)
+    counts = {"edges": int(m.group(1)), "pairs": int(m.group(2)), "max_deg": int(m.group(3)),
+              "valid_pairs": int(v.group(1)), "out_h": int(s.group(1)), "out_w": int(s.group(2))}
+    return counts, chain, info
+
+
+def pair_samples(e: np.ndarray, csr: dict, rng: np.random.Generator, count: int):
+    off, ids = csr["offsets"], csr["ids"]
+    anchors = rng.choice(len(e), size=min(count, len(e)), replace=False)
+    pairs = []
+    for a in anchors:
+        if off[a + 1] > off[a]:
+            pairs.append((int(a), int(ids[rng.integers(off[a], off[a + 1])])))
+    return pairs
+
+
+def constructed_geometries():
+    """Special configurations: close pairs, collinear, mirror-symmetric,
+    perpendicular neighbours, both transport branches."""
+    g = []
+    for d in (0.3, 0.79, 0.8, 0.81, 1.0, 2.0, 3.4):
+        g.append(((10.0, 10.0, 0.0), (10.0 + d, 10.0, 0.0)))          # collinear along x
+        g.append(((10.0, 10.0, np.pi / 2), (10.0, 10.0 + d, np.pi / 2)))
+        g.append(((10.0, 10.0, 0.3), (10.0 + d * np.cos(0.3), 10.0 + d * np.sin(0.3), 0.3)))
+    for th in (0.0, 0.25, 1.0, 2.5):
+        g.append(((5.0, 5.0, th), (7.0, 6.0, th + 0.2)))
+        g.append(((5.0, 5.0, th), (7.0, 4.0, th - 0.2)))                # mirror of the previous
+        g.append(((5.0, 5.0, th), (3.0, 6.0, th)))                      # behind: branch B/A swap
+    g.append(((0.0, 0.0, np.pi / 2), (2.0, 0.0, np.pi / 2)))            # perpendicular neighbour
+    g.append(((0.0, 0.0, 0.0), (0.0, 2.0, 0.0)))
+    return g
+
+
+def main() -> int:
+    if not REF_CLI.exists() or not Ref.available():
+        print("oracle/_ref is missing: run `make -C oracle ref` where the reference sources exist", file=sys.stderr)
+        return 1
+    for d in ("ref_cli", "full", "pairs", "inputs"):
+        (HERE / d).mkdir(exist_ok=True)
+    ref = Ref()
+    manifest = {}
+    for name, key, G, keep_text, keep_full in CASES:
+        fname, text = edge_text(key)
+        counts, chain, info = run_cli(fname, text, G)
+        rec = dict(counts, input=key, group_max_sz=G, chain_sha256=sha(chain), info_sha256=sha(info),
+                   input_sha256=sha(text))
+        if keep_text:
+            for kind, data in (("chain", chain), ("info", info)):
+                with gzip.GzipFile(HERE / "ref_cli" / f"{name}_{kind}.txt.gz", "wb", mtime=0) as f:
+                    f.write(data)
+        e = edges_of(key)
+        p = CurveletParams(group_max_sz=G)
+        r = ref.run(e, p, nthreads=4)
+        assert r["csr"]["pairs"] == counts["pairs"] and r["valid_pairs"] == counts["valid_pairs"]
+        assert len(r["chains"]) == counts["out_h"]
+        rec["full_chain_sha256"] = sha(np.ascontiguousarray(r["chains"]).tobytes())
+        rec["full_info_sha256"] = sha(np.ascontiguousarray(r["info"]).tobytes())
+        if keep_full:
+            np.savez_compressed(HERE / "full" / f"{name}.npz", offsets=r["csr"]["offsets"], ids=r["csr"]["ids"],
+                                dist2=r["csr"]["dist2"], valid_pairs=np.array(r["valid_pairs"]),
+                                chains=r["chains"], info=r["info"])
+        (HERE / "ref_cli" / f"{name}.json").write_text(json.dumps(rec, indent=1) + "\n")
+        manifest[name] = rec
+        print(name, counts)
+
+    # per-pair bundles and direction values
+    rng = np.random.default_rng(20261202)
+    p = CurveletParams()
+    for name, key, count in (("abc_pairs", "file:TO_edges_ABC_0006_thresh1", 400),
+                             ("c1_pairs", "synth:C1:1001", 200)):
+        e = edges_of(key)
+        r = ref.run(e, p, nthreads=4, valid_pairs=False)
+        prs = pair_samples(e, r["csr"], rng, count)
+        A = np.array([e[a, :3] for a, _ in prs])
+        N = np.array([e[b, :3] for _, b in prs])
+        geo = constructed_geometries()
+        A = np.concatenate([A, np.array([a for a, _ in geo])])
+        N = np.concatenate([N, np.array([b for _, b in geo])])
+        lo, hi, hyp, dot0, dot1 = [], [], [], [], []
+        for a, nb in zip(A, N):
+            h, l, u = ref.pair_bundle(p, a, nb)
+            lo.append(l)
+            hi.append(u)
+            hyp.append(h)
+            dot0.append(ref.dir_dot(a, nb))  # compiled to two sincos calls: the backward-run form
+        np.savez_compressed(HERE / "pairs" / f"{name}.npz", anchor=A, neighbour=N, lo=np.array(lo), hi=np.array(hi),
+                            hyp=np.array(hyp), dot_sincos=np.array(dot0))
+        print(name, len(A), "pairs")
+    (HERE / "manifest.json").write_text(json.dumps(manifest, indent=1, sort_keys=True) + "\n")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,212 @@
+"""GPU edge cases against the oracle (the reference-exact CPU restatement).
+
+Inputs the shipped files never exercise: exact distance ties (integer and
+half-integer lattices, so neighbour lists of 36 with many ties go through the
+std::sort replay), lrint's round-half-even cells, max_candidates cuts,
+duplicate points (same-cell ties the library must flag), tiny and empty maps,
+chain sizes below 3, bundle grids needing 2 and 4 cells per lane, invalid
+input, and the stage-wise mirror of the reference API.
+"""
+import numpy as np
+import pytest
+
+from curvelet_construction_on_gpu_amd import (CurveletError, CurveletGPU, CurveletParams, NeighborGraph,
+                                              build_neighbor_csr_graph, dt_radians)
+from tests._inputs import synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+def lattice(nx, ny, step=1.0, offset=0.0, seed=0):
+    rng = np.random.default_rng(seed)
+    xs, ys = np.meshgrid(np.arange(nx) * step + offset, np.arange(ny) * step + offset)
+    th = rng.uniform(0, np.pi, xs.size)
+    return np.stack([xs.ravel(), ys.ravel(), th, np.ones(xs.size)], axis=1)
+
+
+def check_same(ctx, oracle, edges, p, ambiguous_ok=False):
+    want = oracle.run(edges, p)
+    stats = ctx.run(edges, p)
+    g = ctx.graph()
+    if not ambiguous_ok:
+        assert g.order_ambiguous_anchors == 0
+        assert np.array_equal(g.neighbor_offsets, want["csr"]["offsets"])
+        assert np.array_equal(g.neighbor_ids, want["csr"]["ids"])
+        assert np.array_equal(g.neighbor_dist2, want["csr"]["dist2"])
+    chains, info = ctx.curvelets()
+    if not ambiguous_ok:
+        assert stats["valid_pairs"] == want["valid_pairs"]
+        assert np.array_equal(chains, want["chains"])
+        np.testing.assert_allclose(info, want["info"], rtol=1e-9, atol=1e-12)
+    return stats, g, want
+
+
+def test_integer_lattice_ties(ctx, oracle):
+    e = lattice(40, 30, seed=1)
+    stats, g, _ = check_same(ctx, oracle, e, CurveletParams(group_max_sz=7))
+    assert g.max_num_of_neighbors == 36  # d2 <= 12.25 inside the 7x7 window
+    d = g.neighbor_dist2
+    assert len(np.unique(d)) < 10  # heavy ties
+
+
+def test_half_integer_rows_round_half_even(ctx, oracle):
+    # x = k + 0.5: lrint sends 0.5 -> 0, 1.5 -> 2, 2.5 -> 2, 3.5 -> 4, so which
+    # in-radius points the +-3-cell window sees depends on round-half-even.
+    # Rows 8 px apart never pair; two points sharing a cell are never at equal
+    # distance from a half-integer anchor, so no anchor is order-ambiguous.
+    rng = np.random.default_rng(2)
+    xs = np.arange(40) + 0.5
+    e = np.array([[x, y, rng.uniform(0, np.pi), 1.0] for y in (0.5, 8.5, 16.5) for x in xs])
+    _, g, _ = check_same(ctx, oracle, e, CurveletParams(group_max_sz=5))
+    assert g.max_num_of_neighbors < 6  # the window misses in-radius points at |dx| = 3
+    e2 = lattice(20, 20, step=1.5, offset=-0.5, seed=3)
+    check_same(ctx, oracle, e2, CurveletParams(group_max_sz=5))
+
+
+def test_half_integer_lattice_flags_same_cell_ties(ctx, oracle):
+    # a 2-D half-integer lattice puts 2x2 points in one cell; anchors on their
+    # bisectors see exact same-cell ties, whose order in the reference comes
+    # from libstdc++'s unstable sort of the spatial index: flagged, and the
+    # neighbour sets still agree
+    e = lattice(25, 25, offset=0.5, seed=2)
+    p = CurveletParams()
+    want = oracle.build_csr(e, p)
+    g = ctx.build_graph(e, p.nrad, 3, p.max_candidates)
+    assert g.order_ambiguous_anchors > 0
+    off = want["offsets"]
+    assert np.array_equal(g.neighbor_offsets, off)
+    for a in range(len(e)):
+        s = slice(off[a], off[a + 1])
+        assert sorted(g.neighbor_ids[s]) == sorted(want["ids"][s])
+
+
+@pytest.mark.parametrize("max_cand", [1, 5, 17])
+def test_max_candidates_cut(ctx, oracle, max_cand):
+    e = lattice(30, 30, seed=4)
+    p = CurveletParams(max_candidates=max_cand)
+    _, g, want = check_same(ctx, oracle, e, p)
+    assert g.truncated_anchors == want["csr"]["truncated"] > 0
+
+
+def test_synthetic_cut_inside_cells(ctx, oracle):
+    # subpixel data with several edges per cell: a cut can fall inside a cell;
+    # the library flags those anchors, all others must match exactly
+    e = synthetic("C1", 1001)
+    p = CurveletParams(max_candidates=6)
+    want = oracle.build_csr(e, p)
+    g = ctx.build_graph(e, p.nrad, 3, p.max_candidates)
+    assert g.truncated_anchors == want["truncated"]
+    off = want["offsets"]
+    same = np.array([np.array_equal(g.neighbor_ids[off[a]:off[a + 1]], want["ids"][off[a]:off[a + 1]])
+                     for a in range(len(e))])
+    assert (~same).sum() <= g.order_ambiguous_anchors
+
+
+def test_duplicates_flagged(ctx, oracle):
+    e = synthetic("C1", 1001)[:800].copy()
+    e = np.concatenate([e, e[::7]])  # exact duplicates: same cell, equal distances
+    p = CurveletParams()
+    want = oracle.build_csr(e, p)
+    g = ctx.build_graph(e, p.nrad, 3, p.max_candidates)
+    assert g.order_ambiguous_anchors > 0
+    assert np.array_equal(g.neighbor_offsets, want["offsets"])
+    off = want["offsets"]
+    for a in range(len(e)):  # same neighbour sets, each sorted by distance
+        s = slice(off[a], off[a + 1])
+        assert sorted(g.neighbor_ids[s]) == sorted(want["ids"][s])
+        assert np.all(np.diff(g.neighbor_dist2[s]) >= 0)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 17])
+def test_tiny_maps(ctx, oracle, n):
+    e = synthetic("C1", 1001)[:n].copy()
+    check_same(ctx, oracle, e, CurveletParams())
+
+
+def test_isolated_edges(ctx, oracle):
+    e = lattice(10, 10, step=10.0, seed=5)
+    stats, g, _ = check_same(ctx, oracle, e, CurveletParams())
+    assert g.total_neighbor_pairs == 0 and stats["num_curvelets"] == 0
+
+
+@pytest.mark.parametrize("G", [0, 1, 2, 3, 5, 12])
+def test_group_sizes(ctx, oracle, G):
+    e = synthetic("C1", 1002)
+    stats, _, _ = check_same(ctx, oracle, e, CurveletParams(group_max_sz=G))
+    if G < 3:
+        assert stats["num_curvelets"] == 0
+    assert stats["chain_width"] == G + 1
+
+
+@pytest.mark.parametrize("sx,st", [(0.05, 0.08), (0.05, 0.04), (0.2, 0.3)])
+def test_bundle_grid_shapes(ctx, oracle, sx, st):
+    # K = 119 (2 cells per lane), 255 (4 per lane), 9 (one lane in seven)
+    e = synthetic("C1", 1001)[:2500].copy()
+    check_same(ctx, oracle, e, CurveletParams(sx=sx, st=st, group_max_sz=6))
+
+
+@pytest.mark.parametrize("nrad,dx,max_k", [(5.0, 0.4, 0.3), (2.0, 0.25, 0.5), (3.5, 0.8, 0.1)])
+def test_other_parameters(ctx, oracle, nrad, dx, max_k):
+    e = synthetic("C1", 1003)
+    check_same(ctx, oracle, e, CurveletParams(nrad=nrad, dx=dx, max_k=max_k))
+
+
+def test_wider_rows_and_negative_coordinates(ctx, oracle):
+    e = synthetic("C2", 1002)[:6000].copy()
+    e[:, 0] -= 250.0
+    e[:, 1] -= 100.0
+    e6 = np.concatenate([e, np.full((len(e), 2), 7.0)], axis=1)  # edge_data_sz = 6
+    p = CurveletParams(edge_data_sz=6)
+    want = oracle.run(e6, p)
+    ctx.run(e6, p)
+    chains, info = ctx.curvelets()
+    assert np.array_equal(chains, want["chains"])
+    np.testing.assert_allclose(info, want["info"], rtol=1e-9, atol=1e-12)
+
+
+def test_bad_input_rejected(ctx):
+    e = synthetic("C1", 1001)[:100].copy()
+    e[5, 1] = np.nan
+    with pytest.raises(CurveletError) as ei:
+        ctx.run(e, CurveletParams())
+    assert ei.value.status == 2
+    e[5, 1] = 3.0
+    e[7, 2] = 1e30
+    with pytest.raises(CurveletError):
+        ctx.run(e, CurveletParams())
+    with pytest.raises(CurveletError):
+        ctx.run(e[:, :3], CurveletParams())  # sz_edge_data < 4 (preprocess.hpp:197)
+
+
+def test_stagewise_api_matches_reference_shape(ctx, oracle):
+    """build_neighbor_csr_graph -> CurveletGPU -> form_pairwise_bundles ->
+    build_curvelets_greedy, the call sequence of main.cpp:47-64."""
+    e = synthetic("C1", 1004)
+    p = CurveletParams(group_max_sz=4)
+    graph = NeighborGraph()
+    assert build_neighbor_csr_graph(len(e), 4, e, p.nrad, 3, p.max_candidates, 8, graph, ctx=ctx)
+    cv = CurveletGPU(len(e), 4, graph, p.dx, dt_radians(p.dt_deg), p.sx, p.st, p.max_k, p.group_max_sz, 8, e,
+                     p.nrad, dt_deg=p.dt_deg)
+    vp = cv.form_pairwise_bundles()
+    cv.build_curvelets_greedy()
+    want = oracle.run(e, p)
+    assert vp == want["valid_pairs"]
+    assert cv.num_curvelets() == len(want["chains"]) and cv.chain_width() == 5
+    assert np.array_equal(cv.edge_chain_final, want["chains"])
+    assert np.array_equal(graph.neighbor_ids, want["csr"]["ids"])
+    bad = NeighborGraph()
+    assert not build_neighbor_csr_graph(10, 3, e[:10, :3], p.nrad, 3, 128, 1, bad, ctx=ctx)
+
+
+def test_results_independent_of_history(ctx):
+    """Pools are reused across calls: a map's results must not depend on what ran before."""
+    a = synthetic("C1", 1001)
+    b = synthetic("C2", 1002)
+    p = CurveletParams(group_max_sz=7)
+    ctx.run(a, p)
+    ca, ia = ctx.curvelets()
+    ctx.run(b, p)
+    ctx.run(a[:300], CurveletParams(group_max_sz=3))
+    ctx.run(a, p)
+    ca2, ia2 = ctx.curvelets()
+    assert np.array_equal(ca, ca2) and np.array_equal(ia.view(np.uint64), ia2.view(np.uint64))
