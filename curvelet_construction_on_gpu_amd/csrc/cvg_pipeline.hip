@@ -331,10 +331,21 @@ __global__ __launch_bounds__(kBlock) void k_prep_edges(int n, int sz, const doub
         th[i] = tv;
         cx[i] = ix;
         cy[i] = iy;
-        atomicMin(&smin_x, ix);
-        atomicMax(&smax_x, ix);
-        atomicMin(&smin_y, iy);
-        atomicMax(&smax_y, iy);
+    }
+    // wave min/max, then one LDS atomic per wave
+    int mnx = i < n ? cx[i] : INT_MAX, mxx = i < n ? cx[i] : INT_MIN;
+    int mny = i < n ? cy[i] : INT_MAX, mxy = i < n ? cy[i] : INT_MIN;
+    for (int o = 32; o > 0; o >>= 1) {
+        mnx = min(mnx, __shfl_xor(mnx, o, 64));
+        mxx = max(mxx, __shfl_xor(mxx, o, 64));
+        mny = min(mny, __shfl_xor(mny, o, 64));
+        mxy = max(mxy, __shfl_xor(mxy, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&smin_x, mnx);
+        atomicMax(&smax_x, mxx);
+        atomicMin(&smin_y, mny);
+        atomicMax(&smax_y, mxy);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -664,8 +675,12 @@ __global__ __launch_bounds__(kBlock) void k_bucket(int n, const int *__restrict_
                                                    const unsigned *__restrict__ cnt3, int nb, const int *caps,
                                                    int *__restrict__ lists, unsigned *__restrict__ scal)
 {
+    __shared__ unsigned s_count[kMaxBuckets];
+    __shared__ unsigned s_base[kMaxBuckets];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
+    if (threadIdx.x < kMaxBuckets) s_count[threadIdx.x] = 0;
+    __syncthreads();
     int a = -1, b = -1;
     if (p < n) {
         a = sorted[p];
@@ -673,20 +688,23 @@ __global__ __launch_bounds__(kBlock) void k_bucket(int n, const int *__restrict_
         b = 0;
         while (b < nb - 1 && m > static_cast<unsigned>(caps[b])) ++b;
     }
-    // one atomic per wave and bucket: the lowest lane of each bucket group
-    // reserves the group's slots, the others take consecutive ones
+    // per wave and bucket: the group's leader reserves slots in the block's
+    // count, then one global atomic per block and bucket
+    unsigned my_off = 0;
     for (int k = 0; k < nb; ++k) {
         const unsigned long long grp = __ballot(b == k);
         if (grp == 0ULL) continue;
         const int leader = __ffsll(static_cast<long long>(grp)) - 1;
-        unsigned base = 0;
-        if (lane == leader) base = atomicAdd(&scal[S_BUCKET + k], static_cast<unsigned>(__popcll(grp)));
-        base = __shfl(base, leader, 64);
-        if (b == k) {
-            const unsigned pos = base + static_cast<unsigned>(__popcll(grp & ((1ULL << lane) - 1ULL)));
-            lists[static_cast<long long>(k) * n + pos] = a;
-        }
+        unsigned off = 0;
+        if (lane == leader) off = atomicAdd(&s_count[k], static_cast<unsigned>(__popcll(grp)));
+        off = __shfl(off, leader, 64);
+        if (b == k) my_off = off + static_cast<unsigned>(__popcll(grp & ((1ULL << lane) - 1ULL)));
     }
+    __syncthreads();
+    if (threadIdx.x < nb && s_count[threadIdx.x] > 0)
+        s_base[threadIdx.x] = atomicAdd(&scal[S_BUCKET + threadIdx.x], s_count[threadIdx.x]);
+    __syncthreads();
+    if (b >= 0) lists[static_cast<long long>(b) * n + s_base[b] + my_off] = a;
 }
 
 // Per-pair sines of the transport (compute_curve_bundle, cpu_curvelet.hpp:728-729):
@@ -743,7 +761,8 @@ struct FusedArgs {
     const unsigned char *cls;
     const int *list;
     int list_len;
-    int nb_cap;        // neighbour slots per wave (the bucket's largest run)
+    int nb_cap;        // bundle slots per wave (the bucket's largest run)
+    int deg_cap;       // neighbour records per wave (bounds the bucket's degrees)
     int chain_stride;  // LDS chain slots: max(G, 2)
     int st_stride;     // staging ids per row: max(G, 1)
     double dx, dx2, close_thr, max_k;
@@ -764,28 +783,34 @@ constexpr int kFusedWaves = 4;  // independent anchors (waves) per workgroup
 
 // Per-wave LDS slice for nb slots.
 struct WaveCarve {
-    size_t blo, bhi, sines;                            // doubles
-    size_t lrun, hlist, nid, acc_len, acc_chain, cur;  // ints, from int_base
+    size_t rec, sines, blo, bhi;                                 // doubles
+    size_t nflag, nid, lrun, hlist, acc_len, acc_chain, cur;     // ints, from int_base
     size_t int_base_bytes, bytes;
 };
 
-__host__ __device__ inline WaveCarve wave_carve(int nb, int K, int kt, int chain_stride, bool lds_bundles)
+// Per-wave LDS slice: records and sines for up to dc neighbours (all classes),
+// bundles for up to nb neighbours of one run.
+__host__ __device__ inline WaveCarve wave_carve(int nb, int dc, int K, int kt, int chain_stride, bool lds_bundles)
 {
     WaveCarve c{};
     size_t d = 0;
-    c.sines = d;  // one chunk (<= 64 slots) of per-pair sines, 2*kt per slot
-    d += static_cast<size_t>(nb < 64 ? nb : 64) * kt * 2;
+    c.rec = d;
+    d += static_cast<size_t>(dc) * 8;
+    c.sines = d;
+    d += static_cast<size_t>(dc) * kt * 2;
     c.blo = d;
     if (lds_bundles) d += static_cast<size_t>(nb) * K;
     c.bhi = d;
     if (lds_bundles) d += static_cast<size_t>(nb) * K;
     c.int_base_bytes = d * sizeof(double);
     size_t i = 0;
+    c.nflag = i;
+    i += dc;
+    c.nid = i;
+    i += dc;
     c.lrun = i;
     i += nb;
     c.hlist = i;
-    i += nb;
-    c.nid = i;
     i += nb;
     c.acc_len = i;
     i += nb;
@@ -800,12 +825,16 @@ __host__ __device__ inline WaveCarve wave_carve(int nb, int K, int kt, int chain
 
 // build_curvelets_greedy for one anchor per wavefront (cpu_curvelet.hpp:437-591)
 // plus the form_pairwise_bundles valid-pair count (:305-361), in one pass.
+//
+// Prologue: the wave loads everything about the anchor's neighbours at once
+// (class, id, the pair record of compute_curve_bundle, its 2*Kt sines) into
+// its LDS slice, so the rest runs without global-memory round trips.
 // Runs: 0 = forward neighbours (class 1), 1 = backward (class 2), 2 = neither
-// (class 0: validity only, for the count).  For each run the neighbours' bundles
-// go to the wave's LDS slice (lane = bundle cell in rank order, CPL cells per
-// lane); then every hypothesis seeds a chain that grows by intersecting bundles
-// in CSR order, validity by one ballot; accepted chains are deduplicated
-// against the run's earlier ones and written to the anchor's staging rows.
+// (class 0: validity only, for the count).  For each run the neighbours'
+// bundles go to LDS (lane = bundle cell in rank order, CPL cells per lane);
+// then every hypothesis seeds a chain grown by intersecting bundles in CSR
+// order (validity = one ballot); accepted chains are deduplicated against the
+// run's earlier ones and written to the anchor's staging rows.
 template <int CPL>
 __global__ __launch_bounds__(64 * kFusedWaves) void k_curvelets(FusedArgs A)
 {
@@ -822,26 +851,28 @@ __global__ __launch_bounds__(64 * kFusedWaves) void k_curvelets(FusedArgs A)
     const int a = A.list[L];
 
     const bool lds_bundles = (A.g_bundle == nullptr);
-    const WaveCarve wc = wave_carve(A.nb_cap, A.K, A.kt, A.chain_stride, lds_bundles);
+    const WaveCarve wc = wave_carve(A.nb_cap, A.deg_cap, A.K, A.kt, A.chain_stride, lds_bundles);
     char *slice = reinterpret_cast<char *>(lds) + static_cast<size_t>(wave) * wc.bytes;
+    double *rec = reinterpret_cast<double *>(slice) + wc.rec;
+    double2 *sines = reinterpret_cast<double2 *>(reinterpret_cast<double *>(slice) + wc.sines);
     double *blo = lds_bundles ? reinterpret_cast<double *>(slice) + wc.blo
                               : A.g_bundle + static_cast<size_t>(L) * A.nb_cap * A.K * 2;
     double *bhi = lds_bundles ? reinterpret_cast<double *>(slice) + wc.bhi : blo + static_cast<size_t>(A.nb_cap) * A.K;
-    double2 *sines = reinterpret_cast<double2 *>(reinterpret_cast<double *>(slice) + wc.sines);
     int *ib = reinterpret_cast<int *>(slice + wc.int_base_bytes);
+    int *nflag = ib + wc.nflag;
+    int *nid = ib + wc.nid;
     int *lrun = ib + wc.lrun;
     int *hlist = ib + wc.hlist;
-    int *nid = ib + wc.nid;
     int *acc_len = ib + wc.acc_len;
     int *acc_chain = ib + wc.acc_chain;
     int *cur = ib + wc.cur;
 
     const int n = A.n;
+    const unsigned base = A.off[a];
+    const int deg = static_cast<int>(A.off[a + 1] - base);
     const double ax = A.x[a], ay = A.y[a];
     const double cos_at = A.trig[cvg::TRIG_COS * n + a];
     const double sin_at = A.trig[cvg::TRIG_SIN * n + a];
-    const unsigned base = A.off[a];
-    const int deg = static_cast<int>(A.off[a + 1] - base);
     const int K = A.K;
     const int kt = A.kt;
     const int G = A.G;
@@ -850,7 +881,7 @@ __global__ __launch_bounds__(64 * kFusedWaves) void k_curvelets(FusedArgs A)
 
     // per-lane cell constants
     double ox[CPL], sT2t[CPL], cT2t[CPL];
-    int cj[CPL], elig[CPL];
+    int cj[CPL], elig[CPL], ccell[CPL];
     bool live[CPL];
 #pragma unroll
     for (int m = 0; m < CPL; ++m) {
@@ -859,21 +890,50 @@ __global__ __launch_bounds__(64 * kFusedWaves) void k_curvelets(FusedArgs A)
         const int r = live[m] ? rho : 0;
         ox[m] = A.rank_ox[r];
         cj[m] = A.rank_j[r];
+        ccell[m] = A.rank_cell[r];
         elig[m] = live[m] ? A.rank_elig[r] : 0;
         sT2t[m] = A.trig[static_cast<long long>(cvg::TRIG_GRID + 2 * cj[m]) * n + a];
         cT2t[m] = A.trig[static_cast<long long>(cvg::TRIG_GRID + 2 * cj[m] + 1) * n + a];
     }
 
-    int acc = 0;       // accepted rows of this anchor (both runs)
-    int vp = 0;        // valid or close pairs (all three classes)
-    int isect = 0;     // intersections tested
+    // prologue: lane r stages neighbour r (compute_curve_bundle's per-pair
+    // constants, :646-656, 682, 690, 730-733, and the transport sines)
+    for (int r = lane; r < deg; r += 64) {
+        const int cls = A.cls[base + r];
+        const int b = A.ids[base + r];
+        const PairRec p = cvg::pair_record(ax, ay, A.x[b], A.y[b], A.trig[cvg::TRIG_SIN_P * n + b],
+                                           A.trig[cvg::TRIG_COS_P * n + b], A.trig[cvg::TRIG_SIN_M * n + b],
+                                           A.trig[cvg::TRIG_COS_M * n + b]);
+        const bool close = p.DX2 + p.DY2 < A.close_thr;
+        const bool branch_a = (p.DX * cos_at + p.DY * sin_at) < 0;
+        double *rr = rec + static_cast<size_t>(r) * 8;
+        rr[0] = p.DX;
+        rr[1] = p.DY;
+        rr[2] = p.DX2;
+        rr[3] = p.DY2;
+        rr[4] = p.cT0m_DX;
+        rr[5] = p.sT0m_DY;
+        rr[6] = p.cT0p_DX;
+        rr[7] = p.sT0p_DY;
+        nflag[r] = cls | (close ? 4 : 0) | (branch_a ? 8 : 0);
+        nid[r] = b;
+        if (!close) {
+            const double2 *ps = reinterpret_cast<const double2 *>(A.psin) + (static_cast<long long>(base) + r) * kt;
+            for (int j = 0; j < kt; ++j) sines[r * kt + j] = ps[j];
+        }
+    }
+    wave_sync();
+
+    int acc = 0;    // accepted rows of this anchor (both runs)
+    int vp = 0;     // valid or close pairs (all three classes)
+    int isect = 0;  // intersections tested
     for (int run = 0; run < 3; ++run) {
-        const unsigned char want = run == 0 ? 1 : (run == 1 ? 2 : 0);
+        const int want = run == 0 ? 1 : (run == 1 ? 2 : 0);
         // the run's neighbours, in CSR order
         int nL = 0;
         for (int r0 = 0; r0 < deg; r0 += 64) {
             const int r = r0 + lane;
-            const bool f = r < deg && A.cls[base + r] == want;
+            const bool f = r < deg && (nflag[r] & 3) == want;
             const unsigned long long m = __ballot(f);
             if (f) lrun[nL + __popcll(m & lt_mask)] = r;
             nL += __popcll(m);
@@ -881,63 +941,48 @@ __global__ __launch_bounds__(64 * kFusedWaves) void k_curvelets(FusedArgs A)
         if (nL == 0) continue;
         wave_sync();
 
-        // bundles, 64 neighbours at a time: lane k holds slot k's record, every
-        // lane then computes its cells for each slot in turn
+        // bundles: every lane computes its cells for each slot in turn
         int nH = 0;
-        for (int c0 = 0; c0 < nL; c0 += 64) {
-            const int k = c0 + lane;
-            const bool has = k < nL;
-            const int r = has ? lrun[k] : 0;
-            const int b = has ? A.ids[base + r] : a;
-            const PairRec p = cvg::pair_record(ax, ay, A.x[b], A.y[b], A.trig[cvg::TRIG_SIN_P * n + b],
-                                               A.trig[cvg::TRIG_COS_P * n + b], A.trig[cvg::TRIG_SIN_M * n + b],
-                                               A.trig[cvg::TRIG_COS_M * n + b]);
-            const int close = (p.DX2 + p.DY2 < A.close_thr) ? 1 : 0;              // :682
-            const int branch_a = ((p.DX * cos_at + p.DY * sin_at) < 0) ? 1 : 0;  // :690
-            if (has) nid[k] = b;
-            const int cnt = min(64, nL - c0);
-            // this chunk's per-pair sines, fetched in parallel into LDS
-            for (int t = lane; t < cnt * kt; t += 64) {
-                const int slot = t / kt;
-                const int jj = t - slot * kt;
-                const long long pq = static_cast<long long>(base) + lrun[c0 + slot];
-                sines[t] = reinterpret_cast<const double2 *>(A.psin)[pq * kt + jj];
-            }
-            wave_sync();
-            unsigned long long hyp_mask = 0ULL;
-            for (int kk = 0; kk < cnt; ++kk) {
-                PairRec q;
-                q.DX = bcast(p.DX, kk);
-                q.DY = bcast(p.DY, kk);
-                q.DX2 = bcast(p.DX2, kk);
-                q.DY2 = bcast(p.DY2, kk);
-                q.cT0m_DX = bcast(p.cT0m_DX, kk);
-                q.sT0m_DY = bcast(p.sT0m_DY, kk);
-                q.cT0p_DX = bcast(p.cT0p_DX, kk);
-                q.sT0p_DY = bcast(p.sT0p_DY, kk);
-                const bool qclose = bcast(close, kk) != 0;
-                const bool qbr = bcast(branch_a, kk) != 0;
-                const double2 *ps = sines + kk * kt;
-                bool any = false;
+        unsigned long long hyp_mask = 0ULL;
+        for (int k = 0; k < nL; ++k) {
+            const int r = __builtin_amdgcn_readfirstlane(lrun[k]);
+            const int fl = __builtin_amdgcn_readfirstlane(nflag[r]);
+            const double *rr = rec + static_cast<size_t>(r) * 8;
+            PairRec q;
+            q.DX = rr[0];
+            q.DY = rr[1];
+            q.DX2 = rr[2];
+            q.DY2 = rr[3];
+            q.cT0m_DX = rr[4];
+            q.sT0m_DY = rr[5];
+            q.cT0p_DX = rr[6];
+            q.sT0p_DY = rr[7];
+            const bool qclose = (fl & 4) != 0;
+            const bool qbr = (fl & 8) != 0;
+            const double2 *ps = sines + r * kt;
+            bool any = false;
 #pragma unroll
-                for (int m = 0; m < CPL; ++m) {
-                    if (!live[m]) continue;
-                    const int rho = lane + 64 * m;
-                    double lo = -A.max_k, hi = A.max_k;
-                    if (!qclose) {
-                        const double2 s = ps[cj[m]];
-                        cvg::bundle_cell(q, qbr, ox[m], sT2t[m], cT2t[m], s.x, s.y, A.dx, A.dx2, A.max_k, lo, hi);
-                    }
-                    blo[static_cast<size_t>(c0 + kk) * K + rho] = lo;
-                    bhi[static_cast<size_t>(c0 + kk) * K + rho] = hi;
-                    any = any || (hi > lo);
+            for (int m = 0; m < CPL; ++m) {
+                if (!live[m]) continue;
+                const int rho = lane + 64 * m;
+                double lo = -A.max_k, hi = A.max_k;
+                if (!qclose) {
+                    const double2 s = ps[cj[m]];
+                    cvg::bundle_cell(q, qbr, ox[m], sT2t[m], cT2t[m], s.x, s.y, A.dx, A.dx2, A.max_k, lo, hi);
                 }
-                const bool hyp = qclose || (__ballot(any) != 0ULL);  // bundle_valid_check, :821-830
-                if (hyp) hyp_mask |= 1ULL << kk;
+                blo[static_cast<size_t>(k) * K + rho] = lo;
+                bhi[static_cast<size_t>(k) * K + rho] = hi;
+                any = any || (hi > lo);
             }
-            // hypotheses of this chunk, in slot order
-            if ((hyp_mask >> lane) & 1ULL) hlist[nH + __popcll(hyp_mask & lt_mask)] = c0 + lane;
-            nH += __popcll(hyp_mask);
+            const bool hyp = qclose || (__ballot(any) != 0ULL);  // bundle_valid_check, :821-830
+            if (hyp) hyp_mask |= 1ULL << (k & 63);
+            // hypotheses in slot order, 64 slots at a time
+            if ((k & 63) == 63 || k == nL - 1) {
+                const int c0 = k & ~63;
+                if ((hyp_mask >> lane) & 1ULL) hlist[nH + __popcll(hyp_mask & lt_mask)] = c0 + lane;
+                nH += __popcll(hyp_mask);
+                hyp_mask = 0ULL;
+            }
         }
         vp += nH;
         if (run == 2 || nH == 0) continue;
@@ -985,7 +1030,7 @@ __global__ __launch_bounds__(64 * kFusedWaves) void k_curvelets(FusedArgs A)
                     }
                 }
                 if (ri == si) {  // the seed joins at its own distance rank (:496-510)
-                    if (lane == 0) cur[len] = nid[r];
+                    if (lane == 0) cur[len] = nid[lrun[r]];
                     ++len;
                     if (len >= G) break;
                     continue;
@@ -1000,7 +1045,7 @@ __global__ __launch_bounds__(64 * kFusedWaves) void k_curvelets(FusedArgs A)
                     ok = ok || (live[m] && nhi[m] > nlo[m]);
                 }
                 if (__ballot(ok) != 0ULL) {
-                    if (lane == 0) cur[len] = nid[r];
+                    if (lane == 0) cur[len] = nid[lrun[r]];
                     ++len;
 #pragma unroll
                     for (int m = 0; m < CPL; ++m) {
@@ -1038,18 +1083,20 @@ __global__ __launch_bounds__(64 * kFusedWaves) void k_curvelets(FusedArgs A)
             const int bm = best >> 6;
             const int bl = best & 63;
             double kmax = 0.0, kmin = 0.0;
+            int best_cell = 0;
 #pragma unroll
             for (int m = 0; m < CPL; ++m) {
                 if (m == bm) {
                     kmax = bcast(hi[m], bl);
                     kmin = bcast(lo[m], bl);
+                    best_cell = bcast(ccell[m], bl);
                 }
             }
             const long long q = static_cast<long long>(base) + acc;
             for (int l = lane; l < len; l += 64) A.st_ids[q * A.st_stride + l] = cur[l];
             if (lane == 0) {
                 A.st_meta[q] = static_cast<unsigned>(len) | (run == 0 ? 0x100u : 0u) |
-                               (static_cast<unsigned>(A.rank_cell[best]) << 16);
+                               (static_cast<unsigned>(best_cell) << 16);
                 A.st_k[2 * q] = kmax;
                 A.st_k[2 * q + 1] = kmin;
             }
@@ -1412,7 +1459,7 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     const int pairs = ctx->pairs;
 
     // bucket capacities (neighbour slots per workgroup)
-    std::vector<int> caps = {8, 16, 24, 32, 48, 64};
+    std::vector<int> caps = {4, 6, 8, 11, 16, 24, 32, 48, 64};
     const int maxcap = static_cast<int>(std::max<unsigned>(ctx->max_deg, 1u));
     while (!caps.empty() && caps.back() >= maxcap) caps.pop_back();
     caps.push_back(maxcap);
@@ -1514,34 +1561,37 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     // bundles of buckets too large for LDS go to global memory, one region each
     std::vector<size_t> gb_off(nb, 0);
     size_t gb_total = 0;
+    auto deg_cap_of = [&](int b) { return std::min(3 * caps[b], static_cast<int>(std::max(ctx->max_deg, 1u))); };
     for (int b = 0; b < nb; ++b) {
         if (bucket_count[b] == 0) continue;
-        if (wave_carve(caps[b], A.K, A.kt, chain_stride, true).bytes * kFusedWaves > ctx->lds_limit) {
+        if (wave_carve(caps[b], deg_cap_of(b), A.K, A.kt, chain_stride, true).bytes * kFusedWaves > ctx->lds_limit) {
             gb_off[b] = gb_total;
             gb_total += static_cast<size_t>(bucket_count[b]) * caps[b] * A.K * 2;
         }
     }
     if (gb_total > 0) CVG_TRY(ensure(ctx, ctx->d_gbundle, gb_total * sizeof(double)));
 
-    // the smallest bucket (most anchors) on the main stream, the long-list
-    // buckets on the auxiliary streams so their tails overlap it
+    // bucket launches: the short-list bucket 0 (most anchors) alone on the main
+    // stream, the long-list buckets round-robin over the auxiliary streams,
+    // longest lists first, so their long-latency waves start at once and
+    // overlap the bulk
     CVG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
-    int aux_used = 0;
+    for (int i = 0; i < 3; ++i) CVG_HIP(ctx, hipStreamWaitEvent(ctx->aux[i], ctx->ev[3], 0));
+    bool aux_used[3] = {false, false, false};
+    int rr = 0;
     for (int b = nb - 1; b >= 0; --b) {
         const int len = static_cast<int>(bucket_count[b]);
         if (len == 0) continue;
-        hipStream_t ls = st;
-        if (b > 0) {
-            const int ai = aux_used < 3 ? aux_used : 2;
-            ls = ctx->aux[ai];
-            if (aux_used < 3) ++aux_used;
-        }
+        const int slot = b == 0 ? 0 : 1 + (rr++ % 3);
+        hipStream_t ls = slot == 0 ? st : ctx->aux[slot - 1];
+        if (slot > 0) aux_used[slot - 1] = true;
         A.list = as<int>(ctx->d_lists) + static_cast<size_t>(b) * n;
         A.list_len = len;
         A.nb_cap = caps[b];
-        WaveCarve wc = wave_carve(A.nb_cap, A.K, A.kt, chain_stride, true);
+        A.deg_cap = deg_cap_of(b);
+        WaveCarve wc = wave_carve(A.nb_cap, A.deg_cap, A.K, A.kt, chain_stride, true);
         if (wc.bytes * kFusedWaves > ctx->lds_limit) {
-            wc = wave_carve(A.nb_cap, A.K, A.kt, chain_stride, false);
+            wc = wave_carve(A.nb_cap, A.deg_cap, A.K, A.kt, chain_stride, false);
             if (wc.bytes * kFusedWaves > ctx->lds_limit)
                 return fail(ctx, CVG_ERR_UNSUPPORTED, "neighbour lists too long for the workgroup's LDS");
             A.g_bundle = as<double>(ctx->d_gbundle) + gb_off[b];
@@ -1563,7 +1613,8 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
             hipLaunchKernelGGL(k_curvelets<4>, dim3(grid), dim3(64 * kFusedWaves), shmem, ls, A);
         CVG_HIP(ctx, hipGetLastError());
     }
-    for (int i = 0; i < aux_used; ++i) {
+    for (int i = 0; i < 3; ++i) {
+        if (!aux_used[i]) continue;
         CVG_HIP(ctx, hipEventRecord(ctx->ev_join[i], ctx->aux[i]));
         CVG_HIP(ctx, hipStreamWaitEvent(st, ctx->ev_join[i], 0));
     }
