@@ -1,0 +1,308 @@
+"""Pins of the oracle that do not come from the reference's outputs (CPU).
+
+Each check is chosen so that a plausible slip in the restatement (a dropped
+term, a flipped sign, a swapped operand, an off-by-one window) fails it:
+
+* neighbours: brute force over all pairs with the +-3-cell window of lrint
+  cells (round half to even) and d^2 <= nrad^2, sorted by distance;
+* bundles: "close" pairs give the default bundle; a mirror image (y -> -y,
+  theta -> -theta) maps cell (i, j) to (Kx-1-i, Kt-1-j) and [lo, hi] to
+  [-hi, -lo] exactly; the centre cell of a pair lying on a circle of
+  curvature k0 (tangent at the anchor) contains k0, in both branches;
+* direction filter: a neighbour straight ahead is forward, behind is backward;
+* chains and info: every row starts with its anchor twice, members follow in
+  CSR order, 3 <= length <= G, no duplicate rows per (anchor, direction);
+  info columns follow from the chain and the intersection of its members'
+  and its seed's bundles (length, quality, mid curvature, best cell,
+  orientation, point); the seed need not be a member: a chain that fills up
+  before the scan reaches the seed's own rank never lists it;
+* the library's math (the same code the GPU runs) against high-precision
+  decimal evaluation: sin, cos and atan2 correctly rounded, sqrt and fmod
+  exact.
+"""
+import math
+from decimal import Decimal, getcontext
+
+import numpy as np
+import pytest
+
+from curvelet_construction_on_gpu_amd import CurveletParams, math_eval
+from tests._inputs import edges_for
+
+P = CurveletParams()
+KX, KT = 9, 7
+
+
+# ----------------------------------------------------------------- neighbours
+
+def brute_force_csr(e, nrad):
+    x, y = e[:, 0], e[:, 1]
+    cx, cy = np.rint(x).astype(np.int64), np.rint(y).astype(np.int64)  # rint: half to even, as lrint
+    offs, ids, d2s = [0], [], []
+    for a in range(len(e)):
+        win = (np.abs(cx - cx[a]) <= 3) & (np.abs(cy - cy[a]) <= 3)
+        win[a] = False
+        b = np.nonzero(win)[0]
+        d2 = (x[a] - x[b]) * (x[a] - x[b]) + (y[a] - y[b]) * (y[a] - y[b])
+        keep = d2 <= nrad * nrad
+        b, d2 = b[keep], d2[keep]
+        o = np.argsort(d2, kind="stable")
+        ids.extend(b[o])
+        d2s.extend(d2[o])
+        offs.append(len(ids))
+    return np.array(offs), np.array(ids), np.array(d2s)
+
+
+@pytest.mark.parametrize("key,n", [("synth:C1:1001", 1500), ("file:TO_edges_ABC_0006_thresh1", 1200)])
+def test_neighbours_brute_force(oracle, key, n):
+    e = np.array(edges_for(key)[:n])
+    got = oracle.build_csr(e, P)
+    off, ids, d2 = brute_force_csr(e, P.nrad)
+    assert np.array_equal(got["offsets"], off)
+    # ties in d^2 are the only freedom; the maps above have none within an anchor
+    assert np.array_equal(got["dist2"], d2)
+    assert np.array_equal(got["ids"], ids)
+
+
+def test_window_misses_in_radius_points(oracle):
+    # anchor at x = 0.5 (cell 0), neighbour at x = 3.5 (cell 4): 3 px apart,
+    # inside nrad = 3.5 but four cells away
+    e = np.array([[0.5, 0.0, 0.0, 1.0], [3.5, 0.0, 0.0, 1.0], [2.5, 0.0, 0.0, 1.0]])
+    got = oracle.build_csr(e, P)
+    assert list(got["ids"][got["offsets"][0]:got["offsets"][1]]) == [2]
+    assert list(got["ids"][got["offsets"][1]:got["offsets"][2]]) == [2]  # symmetric: 0 is missed too
+    assert sorted(got["ids"][got["offsets"][2]:got["offsets"][3]]) == [0, 1]
+
+
+def test_max_candidates_truncates_in_scan_order(oracle):
+    # cells are scanned row by row (y outer, x inner); the first max_candidates
+    # in-radius points of that scan are kept, then sorted by distance
+    e = np.array([[5.0, 5.0, 0.0, 1.0],
+                  [5.0, 3.0, 0.0, 1.0],   # row y=3: scanned first, d^2 = 4
+                  [6.0, 5.0, 0.0, 1.0],   # row y=5, d^2 = 1
+                  [5.0, 6.0, 0.0, 1.0]])  # row y=6, d^2 = 1
+    got = oracle.build_csr(e, CurveletParams(max_candidates=2))
+    assert list(got["ids"][0:got["offsets"][1]]) == [2, 1]
+    assert got["truncated"] >= 1
+
+
+# -------------------------------------------------------------------- bundles
+
+def test_close_pair_default_bundle(oracle):
+    for d in (0.0, 0.3, 0.79):
+        hyp, lo, hi = oracle.pair_bundle(P, (4.0, 4.0, 0.7), (4.0 + d, 4.0, 2.0))
+        assert hyp
+        assert np.all(lo == -P.max_k) and np.all(hi == P.max_k)
+
+
+def test_mirror_symmetry(oracle):
+    rng = np.random.default_rng(11)
+    for _ in range(300):
+        ta = rng.uniform(0, np.pi)
+        r, phi = rng.uniform(0.85, 3.5), rng.uniform(-np.pi, np.pi)
+        n = (r * math.cos(phi), r * math.sin(phi), rng.uniform(0, np.pi))
+        h1, lo1, hi1 = oracle.pair_bundle(P, (0.0, 0.0, ta), n)
+        h2, lo2, hi2 = oracle.pair_bundle(P, (0.0, 0.0, -ta), (n[0], -n[1], -n[2]))
+        assert h1 == h2
+        lo1, hi1 = lo1.reshape(KX, KT), hi1.reshape(KX, KT)
+        lo2, hi2 = lo2.reshape(KX, KT)[::-1, ::-1], hi2.reshape(KX, KT)[::-1, ::-1]
+        assert np.array_equal(lo2, -hi1) and np.array_equal(hi2, -lo1)
+
+
+@pytest.mark.parametrize("k0", [-0.2, -0.05, 0.0, 0.05, 0.2])
+@pytest.mark.parametrize("s", [-2.5, -1.2, 1.2, 2.5])
+def test_circle_curvature_in_centre_cell(oracle, k0, s):
+    """A neighbour on the circle of curvature k0 through the anchor (tangent
+    theta_a, centre on the left normal) with the circle's tangent: the
+    unperturbed cell's interval holds k0 (s > 0: ahead, branch A; s < 0: behind,
+    branch B)."""
+    for ta in (0.1, 0.9, 2.0, 3.0):
+        if k0 == 0.0:
+            nx, ny = s * math.cos(ta), s * math.sin(ta)
+        else:
+            cx, cy = -math.sin(ta) / k0, math.cos(ta) / k0
+            nx, ny = cx + math.sin(ta + k0 * s) / k0, cy - math.cos(ta + k0 * s) / k0
+        hyp, lo, hi = oracle.pair_bundle(P, (0.0, 0.0, ta), (nx, ny, ta + k0 * s))
+        c = (KX // 2) * KT + KT // 2
+        assert hyp
+        assert lo[c] <= k0 <= hi[c], (ta, lo[c], hi[c])
+
+
+def test_direction_filter(oracle):
+    for ta in np.linspace(0.05, 3.1, 17):
+        ahead = (2.0 * math.cos(ta), 2.0 * math.sin(ta))
+        behind = (-ahead[0], -ahead[1])
+        for run in (0, 1):
+            assert oracle.dir_dot((0.0, 0.0, ta), ahead, run) > 0.99
+            assert oracle.dir_dot((0.0, 0.0, ta), behind, run) < -0.99
+
+
+# ------------------------------------------------------- chains and info pins
+
+def to2pi(a):
+    two_pi = 2.0 * math.pi
+    if a >= two_pi:
+        a = math.fmod(a, two_pi)
+    elif a < 0:
+        a = two_pi + math.fmod(a, two_pi)
+    return a if 0 <= a < two_pi else 0.0
+
+
+@pytest.mark.parametrize("key,G", [("file:TO_edges_digit1", 4), ("synth:C1:1001", 7), ("synth:C2:1002", 5)])
+def test_chain_and_info_invariants(oracle, key, G):
+    e = np.array(edges_for(key))
+    if len(e) > 4000:
+        e = e[:4000]
+    p = CurveletParams(group_max_sz=G)
+    r = oracle.run(e, p)
+    off, ids = r["csr"]["offsets"], r["csr"]["ids"]
+    chains, info = r["chains"], r["info"]
+    assert chains.shape[1] == G + 1 and len(chains) > 0
+    anchors = chains[:, 0].astype(np.int64) - 1
+    assert np.all(np.diff(anchors) >= 0)  # anchor-major
+    seen = set()
+    ox = P.sx * (np.arange(KX) - (KX - 1) / 2.0)
+    ot = P.st * (np.arange(KT) - (KT - 1) / 2.0)
+    dist = (ox[:, None] ** 2 + ot[None, :] ** 2).ravel()
+    for row, inf in zip(chains, info):
+        a = int(row[0]) - 1
+        assert row[1] == row[0]
+        members = [int(v) - 1 for v in row[2:] if v != 0]
+        L = 1 + len(members)
+        assert 3 <= L <= G
+        assert np.all(row[1 + L:] == 0)
+        nbrs = list(ids[off[a]:off[a + 1]])
+        pos = [nbrs.index(m) for m in members]
+        assert pos == sorted(pos) and len(set(pos)) == len(pos)  # CSR order, no repeats
+        fwd = inf[0]
+        assert fwd in (0.0, 1.0)
+        key_row = (a, fwd, tuple(members))
+        assert key_row not in seen
+        seen.add(key_row)
+        run = 0 if fwd == 1.0 else 1
+        for m in members:
+            d = oracle.dir_dot(e[a, :3], e[m, :2], run)
+            assert (d > 0) if fwd == 1.0 else (d < 0)
+        # the running bundle is the seed's bundle intersected with every
+        # member's; the seed is a member, or (when the chain filled up before
+        # the scan reached the seed's own rank) a neighbour after the last member
+        lo = np.full(KX * KT, -np.inf)
+        hi = np.full(KX * KT, np.inf)
+        for m in members:
+            _, l, h = oracle.pair_bundle(p, e[a, :3], e[m, :3])
+            lo, hi = np.maximum(lo, l), np.minimum(hi, h)
+        match = None
+        for s in members + nbrs[max(pos) + 1:]:
+            _, l, h = oracle.pair_bundle(p, e[a, :3], e[s, :3])
+            slo, shi = np.maximum(lo, l), np.minimum(hi, h)
+            valid = shi > slo
+            if not valid.any():
+                continue
+            best = int(np.flatnonzero(valid & (dist == dist[valid].min()))[0])
+            if inf[1] == shi[best] and inf[2] == slo[best]:
+                match = best
+                break
+        assert match is not None
+        i, j = divmod(match, KT)
+        assert inf[7] == (inf[1] + inf[2]) / 2.0
+        assert inf[3] == e[a, 2]
+        assert inf[6] == to2pi(e[a, 2] + ot[j])
+        np.testing.assert_allclose(inf[4], e[a, 0] - ox[i] * math.sin(inf[6]), rtol=0, atol=1e-12)
+        np.testing.assert_allclose(inf[5], e[a, 1] + ox[i] * math.cos(inf[6]), rtol=0, atol=1e-12)
+        pts = [a] + members
+        length = 0.0
+        for u, v in zip(pts[:-1], pts[1:]):
+            length += math.sqrt((e[u, 0] - e[v, 0]) * (e[u, 0] - e[v, 0]) + (e[u, 1] - e[v, 1]) * (e[u, 1] - e[v, 1]))
+        assert inf[8] == length
+        assert inf[9] == (2.0 / (P.nrad / length + length / L) if length > 0 else 0.0)
+
+
+# ------------------------------------------------------------------ host math
+
+getcontext().prec = 70
+
+
+def _pi():
+    # Machin: pi = 16 atan(1/5) - 4 atan(1/239)
+    def atan_inv(n):
+        x = Decimal(1) / n
+        x2 = x * x
+        s, t, k = Decimal(0), x, 1
+        while t > Decimal(10) ** -75:
+            s += t / k if (k // 2) % 2 == 0 else -t / k
+            t *= x2
+            k += 2
+        return s
+    return 16 * atan_inv(5) - 4 * atan_inv(239)
+
+
+PI = _pi()
+
+
+def d_sin(x: Decimal) -> Decimal:
+    x = x - 2 * PI * (x / (2 * PI)).to_integral_value()
+    s, t, k = Decimal(0), x, 1
+    while abs(t) > Decimal(10) ** -75:
+        s += t
+        t = -t * x * x / ((k + 1) * (k + 2))
+        k += 2
+    return s
+
+
+def d_cos(x: Decimal) -> Decimal:
+    return d_sin(x + PI / 2)
+
+
+def d_atan(z: Decimal) -> Decimal:
+    halvings = 0
+    while abs(z) > Decimal("0.05"):
+        z = z / (1 + (1 + z * z).sqrt())
+        halvings += 1
+    s, t, k = Decimal(0), z, 1
+    while abs(t) > Decimal(10) ** -75:
+        s += t / k
+        t = -t * z * z
+        k += 2
+    return s * (2 ** halvings)
+
+
+def d_atan2(y: float, x: float) -> Decimal:
+    Y, X = Decimal(y), Decimal(x)
+    if x > 0:
+        return d_atan(Y / X)
+    if x < 0:
+        return d_atan(Y / X) + (PI if y >= 0 else -PI)
+    return PI / 2 if y > 0 else -PI / 2
+
+
+def test_host_sin_cos_correctly_rounded():
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.uniform(-7.0, 7.0, 1500), rng.uniform(-1e-3, 1e-3, 100),
+                        np.array([0.5, 1.0, math.pi / 2, math.pi, 2 * math.pi, 3.0, -2.75, 1e-9])])
+    s = math_eval("sin", x, device=False)
+    c = math_eval("cos", x, device=False)
+    for xi, si, ci in zip(x, s, c):
+        assert si == float(d_sin(Decimal(xi))), xi
+        assert ci == float(d_cos(Decimal(xi))), xi
+    # glibc is within one unit in the last place of them
+    assert np.all(np.abs(s - np.sin(x)) <= np.spacing(np.abs(s)))
+
+
+def test_host_atan2_correctly_rounded():
+    rng = np.random.default_rng(6)
+    ys = np.concatenate([rng.uniform(-4, 4, 1200), [0.0, 1.0, -1.0, 3.0, 1e-12]])
+    xs = np.concatenate([rng.uniform(-4, 4, 1200), [1.0, 0.0, 0.0, -3.0, -1.0]])
+    got = math_eval("atan2", xs, ys, device=False)  # (x, y) -> atan2(y, x)
+    for yi, xi, gi in zip(ys, xs, got):
+        assert gi == float(d_atan2(float(yi), float(xi))), (yi, xi)
+
+
+def test_host_sqrt_fmod_exact():
+    rng = np.random.default_rng(7)
+    x = rng.uniform(0, 100, 2000)
+    assert np.array_equal(math_eval("sqrt", x, device=False), np.sqrt(x))
+    a = rng.uniform(-30, 30, 2000)
+    b = np.full_like(a, 2.0 * math.pi)
+    assert np.array_equal(math_eval("fmod", a, b, device=False), np.fmod(a, b))
+    t = math_eval("to2pi", a, device=False)
+    assert np.array_equal(t, np.array([to2pi(v) for v in a]))
