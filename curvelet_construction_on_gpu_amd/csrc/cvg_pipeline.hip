@@ -97,7 +97,7 @@ struct cvg_context {
     unsigned long long intersections = 0;
     unsigned num_curvelets = 0;
     unsigned width = 0;
-    DevBuf d_trig, d_cls, d_psin, d_cnt3, d_lists, d_tables, d_st_ids, d_st_meta, d_st_k, d_acc, d_rowoff, d_gbundle,
+    DevBuf d_trig, d_cls, d_prec, d_psin, d_cnt3, d_lists, d_tables, d_st_ids, d_st_meta, d_st_k, d_acc, d_rowoff, d_gbundle,
         d_chains, d_info;
 
     // diagnostics
@@ -652,20 +652,37 @@ __global__ __launch_bounds__(kBlock) void k_edge_trig(int n, const double *__res
     }
 }
 
-// Direction filter per CSR pair (cpu_curvelet.hpp:459-461) and per-anchor
-// class counts cnt3[3a + {0: class 1, 1: class 2, 2: class 0}].
-__global__ __launch_bounds__(kBlock) void k_pair_class(int pairs, int n, const int *__restrict__ pair_anchor,
-                                                       const int *__restrict__ ids, const double *__restrict__ x,
-                                                       const double *__restrict__ y, const double *__restrict__ trig,
-                                                       unsigned char *__restrict__ cls, unsigned *__restrict__ cnt3)
+// Per CSR pair q = (a, b): the direction filter (cpu_curvelet.hpp:459-461),
+// the cell-independent part of compute_curve_bundle (:646-656, 682, 690) and
+// per-anchor class counts cnt3[3a + {0: class 1, 1: class 2, 2: class 0}].
+//   pflag[q] = class | close << 2 | branch_a << 3
+//   prec[8q .. 8q+7] = DX, DY, DX2, DY2, DX cos(T0-dt), DY sin(T0-dt),
+//                      DX cos(T0+dt), DY sin(T0+dt)
+__global__ __launch_bounds__(kBlock) void k_pair_prep(int pairs, int n, const int *__restrict__ pair_anchor,
+                                                      const int *__restrict__ ids, const double *__restrict__ x,
+                                                      const double *__restrict__ y, const double *__restrict__ trig,
+                                                      double close_thr, unsigned char *__restrict__ pflag,
+                                                      double *__restrict__ prec, unsigned *__restrict__ cnt3)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= pairs) return;
     const int a = pair_anchor[q];
     const int b = ids[q];
-    const int k = cvg::direction_class(x[a], y[a], trig[cvg::TRIG_COS * n + a], trig[cvg::TRIG_SIN * n + a], x[b],
-                                       y[b], c_sincos_tab, c_atan_tab);
-    cls[q] = static_cast<unsigned char>(k);
+    const double ax = x[a], ay = y[a], bx = x[b], by = y[b];
+    const double cos_at = trig[cvg::TRIG_COS * n + a];
+    const double sin_at = trig[cvg::TRIG_SIN * n + a];
+    const int k = cvg::direction_class(ax, ay, cos_at, sin_at, bx, by, c_sincos_tab, c_atan_tab);
+    const cvg::PairRec p = cvg::pair_record(ax, ay, bx, by, trig[cvg::TRIG_SIN_P * n + b],
+                                            trig[cvg::TRIG_COS_P * n + b], trig[cvg::TRIG_SIN_M * n + b],
+                                            trig[cvg::TRIG_COS_M * n + b]);
+    const bool close = p.DX2 + p.DY2 < close_thr;
+    const bool branch_a = (p.DX * cos_at + p.DY * sin_at) < 0;
+    pflag[q] = static_cast<unsigned char>(k | (close ? 4 : 0) | (branch_a ? 8 : 0));
+    double2 *o = reinterpret_cast<double2 *>(prec + static_cast<long long>(q) * 8);
+    o[0] = make_double2(p.DX, p.DY);
+    o[1] = make_double2(p.DX2, p.DY2);
+    o[2] = make_double2(p.cT0m_DX, p.sT0m_DY);
+    o[3] = make_double2(p.cT0p_DX, p.sT0p_DY);
     atomicAdd(&cnt3[3 * a + (k == 1 ? 0 : (k == 2 ? 1 : 2))], 1u);
 }
 
@@ -755,17 +772,16 @@ struct Tables {
 
 struct FusedArgs {
     int n;
-    const double *x, *y, *th, *trig, *psin;
+    const double *trig, *psin, *prec;
     const unsigned *off;
     const int *ids;
-    const unsigned char *cls;
+    const unsigned char *pflag;
     const int *list;
     int list_len;
-    int nb_cap;        // bundle slots per wave (the bucket's largest run)
-    int deg_cap;       // neighbour records per wave (bounds the bucket's degrees)
-    int chain_stride;  // LDS chain slots: max(G, 2)
+    int nb_cap;        // bundle slots per wave: the bucket's longest run
+    int chain_stride;  // LDS ints per accepted chain: max(G, 2)
     int st_stride;     // staging ids per row: max(G, 1)
-    double dx, dx2, close_thr, max_k;
+    double dx, dx2, max_k;
     int kt, K, G;
     const double *rank_ox;
     const int *rank_j, *rank_cell, *rank_elig;
@@ -776,261 +792,266 @@ struct FusedArgs {
     unsigned *acc_count;
     unsigned *valid_pairs;
     unsigned long long *intersections;
-    double *g_bundle;  // non-null: bundles live in global memory, nb_cap*K*2 doubles per list entry
+    double *g_bundle;  // global-bundle kernels: nb_cap*K*2 doubles per list entry
 };
 
-constexpr int kFusedWaves = 4;  // independent anchors (waves) per workgroup
-
-// Per-wave LDS slice for nb slots.
-struct WaveCarve {
-    size_t rec, sines, blo, bhi;                                 // doubles
-    size_t nflag, nid, lrun, hlist, acc_len, acc_chain, cur;     // ints, from int_base
-    size_t int_base_bytes, bytes;
-};
-
-// Per-wave LDS slice: records and sines for up to dc neighbours (all classes),
-// bundles for up to nb neighbours of one run.
-__host__ __device__ inline WaveCarve wave_carve(int nb, int dc, int K, int kt, int chain_stride, bool lds_bundles)
+// LDS bytes per wave: the run's hypothesis bundles (lo then hi; none when they
+// live in global memory) and the run's accepted chains.
+__host__ __device__ inline size_t fused_slice_bytes(int nb_cap, int K, int chain_stride, bool lds_bundles)
 {
-    WaveCarve c{};
-    size_t d = 0;
-    c.rec = d;
-    d += static_cast<size_t>(dc) * 8;
-    c.sines = d;
-    d += static_cast<size_t>(dc) * kt * 2;
-    c.blo = d;
-    if (lds_bundles) d += static_cast<size_t>(nb) * K;
-    c.bhi = d;
-    if (lds_bundles) d += static_cast<size_t>(nb) * K;
-    c.int_base_bytes = d * sizeof(double);
-    size_t i = 0;
-    c.nflag = i;
-    i += dc;
-    c.nid = i;
-    i += dc;
-    c.lrun = i;
-    i += nb;
-    c.hlist = i;
-    i += nb;
-    c.acc_len = i;
-    i += nb;
-    c.acc_chain = i;
-    i += static_cast<size_t>(nb) * chain_stride;
-    c.cur = i;
-    i += chain_stride;
-    i = (i + 1) & ~static_cast<size_t>(1);  // keep slices 8-byte aligned
-    c.bytes = c.int_base_bytes + i * sizeof(int);
-    return c;
+    size_t b = lds_bundles ? static_cast<size_t>(nb_cap) * K * 2 * sizeof(double) : 0;
+    b += static_cast<size_t>(nb_cap) * chain_stride * sizeof(int);
+    return (b + 15) & ~static_cast<size_t>(15);
+}
+
+// Load from a wave-uniform address through the scalar data cache (read only).
+template <typename T>
+__device__ __forceinline__ T uload(const T *p)
+{
+    return *(const __attribute__((address_space(4))) T *)p;
+}
+
+// Lane-indexed arrays: entry k of a wave-wide list lives in lane k % 64 of
+// register v[k / 64]; k is wave-uniform.
+template <int NCH>
+__device__ __forceinline__ int lane_get(const int (&v)[NCH], int k)
+{
+    int r = 0;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        const int t = __builtin_amdgcn_readlane(v[c], k & 63);
+        if ((k >> 6) == c) r = t;
+    }
+    return r;
+}
+
+template <int NCH>
+__device__ __forceinline__ void lane_set(int (&v)[NCH], int k, int val, int lane)
+{
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+        if ((k >> 6) == c && lane == (k & 63)) v[c] = val;
+}
+
+// Lowest remaining index of a multi-word lane mask (uniform), -1 when empty.
+template <int NCH>
+__device__ __forceinline__ int pop_next(unsigned long long (&m)[NCH])
+{
+    int k = -1;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        const bool take = k < 0 && m[c] != 0ULL;
+        const int r = __builtin_ctzll(m[c] | (1ULL << 63));
+        k = take ? 64 * c + r : k;
+        m[c] = take ? (m[c] & (m[c] - 1ULL)) : m[c];
+    }
+    return k;
+}
+
+__device__ __forceinline__ unsigned chain_hash(unsigned h, int id)
+{
+    return (h ^ static_cast<unsigned>(id)) * 16777619u;
 }
 
 // build_curvelets_greedy for one anchor per wavefront (cpu_curvelet.hpp:437-591)
 // plus the form_pairwise_bundles valid-pair count (:305-361), in one pass.
 //
-// Prologue: the wave loads everything about the anchor's neighbours at once
-// (class, id, the pair record of compute_curve_bundle, its 2*Kt sines) into
-// its LDS slice, so the rest runs without global-memory round trips.
+// Control state is wave-uniform (scalar registers); per-neighbour flags and ids,
+// hypothesis ids and the accepted chains' hashes sit in lane-indexed registers
+// and are read with readlane, so no loop waits on memory for control.
 // Runs: 0 = forward neighbours (class 1), 1 = backward (class 2), 2 = neither
-// (class 0: validity only, for the count).  For each run the neighbours'
-// bundles go to LDS (lane = bundle cell in rank order, CPL cells per lane);
-// then every hypothesis seeds a chain grown by intersecting bundles in CSR
-// order (validity = one ballot); accepted chains are deduplicated against the
-// run's earlier ones and written to the anchor's staging rows.
-template <int CPL>
-__global__ __launch_bounds__(64 * kFusedWaves) void k_curvelets(FusedArgs A)
+// (class 0: validity only, for the count).  For each neighbour of the run, in
+// CSR order, the lanes compute the bundle's cells (lane = cell in rank order,
+// CPL cells per lane) from the pair record and sines that k_pair_prep and
+// k_pair_trig wrote, prefetched one neighbour ahead; hypotheses (valid or close
+// pairs) are kept, compactly, in the wave's LDS slice.  Then every hypothesis
+// seeds a chain grown by intersecting the bundles in CSR order (validity = one
+// ballot); a chain longer than 2 is accepted unless an earlier accepted chain
+// of this (anchor, run) is identical (hash match, then element check), and is
+// written to the anchor's staging rows with its best cell (fill_curvelet_info,
+// :202-226: the first valid cell in rank order).
+template <int CPL, int NCH, bool GB>
+__global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
 {
     extern __shared__ double lds[];
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    const int wpb = static_cast<int>(blockDim.x >> 6);
 
     // XCD-aware order: the grid is a multiple of 8 and block b runs on XCD b % 8;
     // each XCD gets a contiguous slice of the spatially ordered anchor list.
-    const int per = static_cast<int>(gridDim.x) / 8;
-    const int blk = static_cast<int>(blockIdx.x % 8) * per + static_cast<int>(blockIdx.x / 8);
-    const int L = blk * kFusedWaves + wave;
+    const int per = static_cast<int>(gridDim.x) >> 3;
+    const int blk = static_cast<int>(blockIdx.x & 7) * per + static_cast<int>(blockIdx.x >> 3);
+    const int L = blk * wpb + wave;
     if (L >= A.list_len) return;  // whole wave; no workgroup barriers below
-    const int a = A.list[L];
+    const int a = uload(A.list + L);
 
-    const bool lds_bundles = (A.g_bundle == nullptr);
-    const WaveCarve wc = wave_carve(A.nb_cap, A.deg_cap, A.K, A.kt, A.chain_stride, lds_bundles);
-    char *slice = reinterpret_cast<char *>(lds) + static_cast<size_t>(wave) * wc.bytes;
-    double *rec = reinterpret_cast<double *>(slice) + wc.rec;
-    double2 *sines = reinterpret_cast<double2 *>(reinterpret_cast<double *>(slice) + wc.sines);
-    double *blo = lds_bundles ? reinterpret_cast<double *>(slice) + wc.blo
-                              : A.g_bundle + static_cast<size_t>(L) * A.nb_cap * A.K * 2;
-    double *bhi = lds_bundles ? reinterpret_cast<double *>(slice) + wc.bhi : blo + static_cast<size_t>(A.nb_cap) * A.K;
-    int *ib = reinterpret_cast<int *>(slice + wc.int_base_bytes);
-    int *nflag = ib + wc.nflag;
-    int *nid = ib + wc.nid;
-    int *lrun = ib + wc.lrun;
-    int *hlist = ib + wc.hlist;
-    int *acc_len = ib + wc.acc_len;
-    int *acc_chain = ib + wc.acc_chain;
-    int *cur = ib + wc.cur;
-
-    const int n = A.n;
-    const unsigned base = A.off[a];
-    const int deg = static_cast<int>(A.off[a + 1] - base);
-    const double ax = A.x[a], ay = A.y[a];
-    const double cos_at = A.trig[cvg::TRIG_COS * n + a];
-    const double sin_at = A.trig[cvg::TRIG_SIN * n + a];
     const int K = A.K;
     const int kt = A.kt;
     const int G = A.G;
     const int CS = A.chain_stride;
-    const unsigned long long lt_mask = (1ULL << lane) - 1ULL;
+    const int NB = A.nb_cap;
+    char *sl = reinterpret_cast<char *>(lds) + static_cast<size_t>(wave) * fused_slice_bytes(NB, K, CS, !GB);
+    double *blo;
+    double *bhi;
+    int *accc;
+    if constexpr (GB) {
+        blo = A.g_bundle + static_cast<size_t>(L) * NB * K * 2;
+        bhi = blo + static_cast<size_t>(NB) * K;
+        accc = reinterpret_cast<int *>(sl);
+    } else {
+        blo = reinterpret_cast<double *>(sl);
+        bhi = blo + static_cast<size_t>(NB) * K;
+        accc = reinterpret_cast<int *>(bhi + static_cast<size_t>(NB) * K);
+    }
 
-    // per-lane cell constants
+    const int n = A.n;
+    const unsigned base = uload(A.off + a);
+    const int deg = static_cast<int>(uload(A.off + a + 1) - base);
+
+    // per-lane cells (rank order)
     double ox[CPL], sT2t[CPL], cT2t[CPL];
-    int cj[CPL], elig[CPL], ccell[CPL];
-    bool live[CPL];
+    int cj[CPL], ccell[CPL], coff[CPL];
+    bool live[CPL], elig[CPL];
 #pragma unroll
     for (int m = 0; m < CPL; ++m) {
         const int rho = lane + 64 * m;
         live[m] = rho < K;
         const int r = live[m] ? rho : 0;
+        coff[m] = r;
         ox[m] = A.rank_ox[r];
         cj[m] = A.rank_j[r];
         ccell[m] = A.rank_cell[r];
-        elig[m] = live[m] ? A.rank_elig[r] : 0;
+        elig[m] = live[m] && A.rank_elig[r] != 0;
         sT2t[m] = A.trig[static_cast<long long>(cvg::TRIG_GRID + 2 * cj[m]) * n + a];
         cT2t[m] = A.trig[static_cast<long long>(cvg::TRIG_GRID + 2 * cj[m] + 1) * n + a];
     }
 
-    // prologue: lane r stages neighbour r (compute_curve_bundle's per-pair
-    // constants, :646-656, 682, 690, 730-733, and the transport sines)
-    for (int r = lane; r < deg; r += 64) {
-        const int cls = A.cls[base + r];
-        const int b = A.ids[base + r];
-        const PairRec p = cvg::pair_record(ax, ay, A.x[b], A.y[b], A.trig[cvg::TRIG_SIN_P * n + b],
-                                           A.trig[cvg::TRIG_COS_P * n + b], A.trig[cvg::TRIG_SIN_M * n + b],
-                                           A.trig[cvg::TRIG_COS_M * n + b]);
-        const bool close = p.DX2 + p.DY2 < A.close_thr;
-        const bool branch_a = (p.DX * cos_at + p.DY * sin_at) < 0;
-        double *rr = rec + static_cast<size_t>(r) * 8;
-        rr[0] = p.DX;
-        rr[1] = p.DY;
-        rr[2] = p.DX2;
-        rr[3] = p.DY2;
-        rr[4] = p.cT0m_DX;
-        rr[5] = p.sT0m_DY;
-        rr[6] = p.cT0p_DX;
-        rr[7] = p.sT0p_DY;
-        nflag[r] = cls | (close ? 4 : 0) | (branch_a ? 8 : 0);
-        nid[r] = b;
-        if (!close) {
-            const double2 *ps = reinterpret_cast<const double2 *>(A.psin) + (static_cast<long long>(base) + r) * kt;
-            for (int j = 0; j < kt; ++j) sines[r * kt + j] = ps[j];
-        }
+    // the anchor's neighbours: flags (class | close | branch) and ids
+    int nfl[NCH], nid[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        const int r = 64 * c + lane;
+        nfl[c] = r < deg ? static_cast<int>(A.pflag[base + r]) : 3;
+        nid[c] = r < deg ? A.ids[base + r] : 0;
     }
-    wave_sync();
 
-    int acc = 0;    // accepted rows of this anchor (both runs)
-    int vp = 0;     // valid or close pairs (all three classes)
-    int isect = 0;  // intersections tested
+    const double2 *psin2 = reinterpret_cast<const double2 *>(A.psin);
+    auto load_rec = [&](int k) { return A.prec[(static_cast<long long>(base) + k) * 8 + (lane & 7)]; };
+    auto load_sines = [&](int k, double2 (&s)[CPL]) {
+        const double2 *ps = psin2 + (static_cast<long long>(base) + k) * kt;
+#pragma unroll
+        for (int m = 0; m < CPL; ++m) s[m] = ps[cj[m]];
+    };
+
+    int acc = 0;        // accepted rows of this anchor (both runs)
+    int vp = 0;         // valid or close pairs (all three classes)
+    unsigned isect = 0; // intersections tested
     for (int run = 0; run < 3; ++run) {
         const int want = run == 0 ? 1 : (run == 1 ? 2 : 0);
-        // the run's neighbours, in CSR order
-        int nL = 0;
-        for (int r0 = 0; r0 < deg; r0 += 64) {
-            const int r = r0 + lane;
-            const bool f = r < deg && (nflag[r] & 3) == want;
-            const unsigned long long m = __ballot(f);
-            if (f) lrun[nL + __popcll(m & lt_mask)] = r;
-            nL += __popcll(m);
-        }
-        if (nL == 0) continue;
-        wave_sync();
+        unsigned long long mk[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) mk[c] = __ballot((nfl[c] & 3) == want);
 
-        // bundles: every lane computes its cells for each slot in turn
+        // bundles of the run's neighbours (compute_curve_bundle, :636-819)
         int nH = 0;
-        unsigned long long hyp_mask = 0ULL;
-        for (int k = 0; k < nL; ++k) {
-            const int r = __builtin_amdgcn_readfirstlane(lrun[k]);
-            const int fl = __builtin_amdgcn_readfirstlane(nflag[r]);
-            const double *rr = rec + static_cast<size_t>(r) * 8;
-            PairRec q;
-            q.DX = rr[0];
-            q.DY = rr[1];
-            q.DX2 = rr[2];
-            q.DY2 = rr[3];
-            q.cT0m_DX = rr[4];
-            q.sT0m_DY = rr[5];
-            q.cT0p_DX = rr[6];
-            q.sT0p_DY = rr[7];
+        int hid[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) hid[c] = 0;
+        int k = pop_next<NCH>(mk);
+        double rv = 0.0;
+        double2 sv[CPL];
+        if (k >= 0) {
+            rv = load_rec(k);
+            load_sines(k, sv);
+        }
+        while (k >= 0) {
+            const int kc = k;
+            const double rc = rv;
+            double2 sc[CPL];
+#pragma unroll
+            for (int m = 0; m < CPL; ++m) sc[m] = sv[m];
+            k = pop_next<NCH>(mk);
+            if (k >= 0) {
+                rv = load_rec(k);
+                load_sines(k, sv);
+            }
+            const int fl = lane_get<NCH>(nfl, kc);
+            cvg::PairRec q;
+            q.DX = bcast(rc, 0);
+            q.DY = bcast(rc, 1);
+            q.DX2 = bcast(rc, 2);
+            q.DY2 = bcast(rc, 3);
+            q.cT0m_DX = bcast(rc, 4);
+            q.sT0m_DY = bcast(rc, 5);
+            q.cT0p_DX = bcast(rc, 6);
+            q.sT0p_DY = bcast(rc, 7);
             const bool qclose = (fl & 4) != 0;
             const bool qbr = (fl & 8) != 0;
-            const double2 *ps = sines + r * kt;
+            double lo[CPL], hi[CPL];
             bool any = false;
 #pragma unroll
             for (int m = 0; m < CPL; ++m) {
-                if (!live[m]) continue;
-                const int rho = lane + 64 * m;
-                double lo = -A.max_k, hi = A.max_k;
-                if (!qclose) {
-                    const double2 s = ps[cj[m]];
-                    cvg::bundle_cell(q, qbr, ox[m], sT2t[m], cT2t[m], s.x, s.y, A.dx, A.dx2, A.max_k, lo, hi);
+                lo[m] = -A.max_k;
+                hi[m] = A.max_k;
+                if (!qclose)
+                    cvg::bundle_cell(q, qbr, ox[m], sT2t[m], cT2t[m], sc[m].x, sc[m].y, A.dx, A.dx2, A.max_k, lo[m],
+                                     hi[m]);
+                any = any || (live[m] && hi[m] > lo[m]);
+            }
+            if (!qclose && __ballot(any) == 0ULL) continue;  // bundle_valid_check, :821-830
+            ++vp;
+            if (run == 2) continue;
+#pragma unroll
+            for (int m = 0; m < CPL; ++m) {
+                if (live[m]) {
+                    blo[static_cast<size_t>(nH) * K + coff[m]] = lo[m];
+                    bhi[static_cast<size_t>(nH) * K + coff[m]] = hi[m];
                 }
-                blo[static_cast<size_t>(k) * K + rho] = lo;
-                bhi[static_cast<size_t>(k) * K + rho] = hi;
-                any = any || (hi > lo);
             }
-            const bool hyp = qclose || (__ballot(any) != 0ULL);  // bundle_valid_check, :821-830
-            if (hyp) hyp_mask |= 1ULL << (k & 63);
-            // hypotheses in slot order, 64 slots at a time
-            if ((k & 63) == 63 || k == nL - 1) {
-                const int c0 = k & ~63;
-                if ((hyp_mask >> lane) & 1ULL) hlist[nH + __popcll(hyp_mask & lt_mask)] = c0 + lane;
-                nH += __popcll(hyp_mask);
-                hyp_mask = 0ULL;
-            }
+            lane_set<NCH>(hid, nH, lane_get<NCH>(nid, kc), lane);
+            ++nH;
         }
-        vp += nH;
-        if (run == 2 || nH == 0) continue;
-        wave_sync();
+        if (run == 2 || nH == 0 || G < 3) continue;  // no chain longer than 2 when G < 3
 
         // growth (:479-536), dedup (:546-560, 876-916) and record, seed by seed
         int nacc = 0;
-        for (int si = 0; si < nH; ++si) {
-            const int s = hlist[si];
-            double lo[CPL], hi[CPL];
+        int ahash[NCH], alen[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            ahash[c] = 0;
+            alen[c] = 0;
+        }
+        for (int s = 0; s < nH; ++s) {
+            double lo[CPL], hi[CPL], pl[CPL], ph[CPL];
 #pragma unroll
             for (int m = 0; m < CPL; ++m) {
-                const int rho = live[m] ? lane + 64 * m : 0;
-                lo[m] = blo[static_cast<size_t>(s) * K + rho];
-                hi[m] = bhi[static_cast<size_t>(s) * K + rho];
+                lo[m] = blo[static_cast<size_t>(s) * K + coff[m]];
+                hi[m] = bhi[static_cast<size_t>(s) * K + coff[m]];
+                pl[m] = blo[coff[m]];
+                ph[m] = bhi[coff[m]];
             }
             int len = 1;
-            if (lane == 0) cur[0] = a;
-            // next candidate bundle, loaded one step ahead
-            double pbl[CPL], pbh[CPL];
-            {
-                const int r0 = hlist[0];
-#pragma unroll
-                for (int m = 0; m < CPL; ++m) {
-                    const int rho = live[m] ? lane + 64 * m : 0;
-                    pbl[m] = blo[static_cast<size_t>(r0) * K + rho];
-                    pbh[m] = bhi[static_cast<size_t>(r0) * K + rho];
-                }
-            }
-            for (int ri = 0; ri < nH; ++ri) {
-                const int r = hlist[ri];
+            int cur = lane == 0 ? a : 0;  // chain position l in lane l
+            unsigned hsh = chain_hash(2166136261u, a);
+            for (int h = 0; h < nH; ++h) {
                 double bl[CPL], bh[CPL];
 #pragma unroll
                 for (int m = 0; m < CPL; ++m) {
-                    bl[m] = pbl[m];
-                    bh[m] = pbh[m];
+                    bl[m] = pl[m];
+                    bh[m] = ph[m];
                 }
-                if (ri + 1 < nH) {
-                    const int rn = hlist[ri + 1];
+                if (h + 1 < nH) {
 #pragma unroll
                     for (int m = 0; m < CPL; ++m) {
-                        const int rho = live[m] ? lane + 64 * m : 0;
-                        pbl[m] = blo[static_cast<size_t>(rn) * K + rho];
-                        pbh[m] = bhi[static_cast<size_t>(rn) * K + rho];
+                        pl[m] = blo[static_cast<size_t>(h + 1) * K + coff[m]];
+                        ph[m] = bhi[static_cast<size_t>(h + 1) * K + coff[m]];
                     }
                 }
-                if (ri == si) {  // the seed joins at its own distance rank (:496-510)
-                    if (lane == 0) cur[len] = nid[lrun[r]];
+                if (h == s) {  // the seed joins at its own distance rank (:496-510)
+                    const int id = lane_get<NCH>(hid, h);
+                    if (lane == len) cur = id;
+                    hsh = chain_hash(hsh, id);
                     ++len;
                     if (len >= G) break;
                     continue;
@@ -1045,7 +1066,9 @@ __global__ __launch_bounds__(64 * kFusedWaves) void k_curvelets(FusedArgs A)
                     ok = ok || (live[m] && nhi[m] > nlo[m]);
                 }
                 if (__ballot(ok) != 0ULL) {
-                    if (lane == 0) cur[len] = nid[lrun[r]];
+                    const int id = lane_get<NCH>(hid, h);
+                    if (lane == len) cur = id;
+                    hsh = chain_hash(hsh, id);
                     ++len;
 #pragma unroll
                     for (int m = 0; m < CPL; ++m) {
@@ -1056,28 +1079,32 @@ __global__ __launch_bounds__(64 * kFusedWaves) void k_curvelets(FusedArgs A)
                 if (len >= G) break;
             }
             if (len <= 2) continue;  // :549
-            wave_sync();
+
             // identical to an earlier accepted chain of this (anchor, run)?
             bool dup = false;
-            for (int k0 = 0; k0 < nacc && !dup; k0 += 64) {
-                const int k = k0 + lane;
-                bool same = k < nacc && acc_len[k] == len;
-                if (same) {
-                    const int *other = acc_chain + static_cast<size_t>(k) * CS;
-                    for (int l = 0; l < len; ++l) same = same && (other[l] == cur[l]);
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                unsigned long long mm = __ballot(64 * c + lane < nacc && static_cast<unsigned>(ahash[c]) == hsh &&
+                                                 alen[c] == len);
+                while (mm != 0ULL && !dup) {
+                    const int j = 64 * c + __builtin_ctzll(mm);
+                    mm &= mm - 1ULL;
+                    const bool diff = lane < len && accc[j * CS + lane] != cur;
+                    dup = __ballot(diff) == 0ULL;
                 }
-                dup = __ballot(same) != 0ULL;
             }
             if (dup) continue;
-            for (int l = lane; l < len; l += 64) acc_chain[static_cast<size_t>(nacc) * CS + l] = cur[l];
-            if (lane == 0) acc_len[nacc] = len;
+            if (lane < len) accc[nacc * CS + lane] = cur;
+            lane_set<NCH>(ahash, nacc, static_cast<int>(hsh), lane);
+            lane_set<NCH>(alen, nacc, len, lane);
             ++nacc;
+
             // best cell of the running bundle (fill_curvelet_info, :202-226)
             int best = -1;
 #pragma unroll
             for (int m = 0; m < CPL; ++m) {
-                const unsigned long long mask = __ballot(live[m] && elig[m] && hi[m] > lo[m]);
-                if (best < 0 && mask != 0ULL) best = 64 * m + __ffsll(static_cast<long long>(mask)) - 1;
+                const unsigned long long mask = __ballot(elig[m] && hi[m] > lo[m]);
+                if (best < 0 && mask != 0ULL) best = 64 * m + __builtin_ctzll(mask);
             }
             if (best < 0) best = A.rank_of_cell0;
             const int bm = best >> 6;
@@ -1093,7 +1120,7 @@ __global__ __launch_bounds__(64 * kFusedWaves) void k_curvelets(FusedArgs A)
                 }
             }
             const long long q = static_cast<long long>(base) + acc;
-            for (int l = lane; l < len; l += 64) A.st_ids[q * A.st_stride + l] = cur[l];
+            if (lane < len) A.st_ids[q * A.st_stride + lane] = cur;
             if (lane == 0) {
                 A.st_meta[q] = static_cast<unsigned>(len) | (run == 0 ? 0x100u : 0u) |
                                (static_cast<unsigned>(best_cell) << 16);
@@ -1101,7 +1128,6 @@ __global__ __launch_bounds__(64 * kFusedWaves) void k_curvelets(FusedArgs A)
                 A.st_k[2 * q + 1] = kmin;
             }
             ++acc;
-            wave_sync();
         }
     }
     if (lane == 0) {
@@ -1109,6 +1135,20 @@ __global__ __launch_bounds__(64 * kFusedWaves) void k_curvelets(FusedArgs A)
         if (vp) atomicAdd(A.valid_pairs, static_cast<unsigned>(vp));
         if (isect) atomicAdd(A.intersections, static_cast<unsigned long long>(isect));
     }
+}
+
+using FusedKernel = void (*)(FusedArgs);
+
+// k_curvelets<CPL, NCH, GB> by (cells per lane, 64-neighbour chunks, bundles in global memory)
+FusedKernel fused_kernel(int cpl, int nch, bool gb)
+{
+#define CVG_FK(C, N)                                                                        \
+    if (cpl == C && nch == N) return gb ? k_curvelets<C, N, true> : k_curvelets<C, N, false>;
+    CVG_FK(1, 1) CVG_FK(1, 2) CVG_FK(1, 4)
+    CVG_FK(2, 1) CVG_FK(2, 2) CVG_FK(2, 4)
+    CVG_FK(4, 1) CVG_FK(4, 2) CVG_FK(4, 4)
+#undef CVG_FK
+    return nullptr;
 }
 
 struct RecordArgs {
@@ -1409,7 +1449,7 @@ int derive(cvg_context *ctx, const cvg_params &p, Derived &d)
     d.kt = static_cast<unsigned>(ft);
     d.K = d.kx * d.kt;
     d.cpl = d.K <= 64 ? 1 : (d.K <= 128 ? 2 : 4);
-    if (p.group_max_sz > 255) return fail(ctx, CVG_ERR_UNSUPPORTED, "group_max_sz above 255");
+    if (p.group_max_sz > 64) return fail(ctx, CVG_ERR_UNSUPPORTED, "group_max_sz above 64");
     d.G = p.group_max_sz;
     d.off_x.resize(d.kx);
     d.off_t.resize(d.kt);
@@ -1491,6 +1531,7 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     const int ntrig = cvg::TRIG_GRID + 2 * static_cast<int>(d.kt);
     CVG_TRY(ensure(ctx, ctx->d_trig, static_cast<size_t>(ntrig) * n * sizeof(double)));
     CVG_TRY(ensure(ctx, ctx->d_cls, static_cast<size_t>(pairs) + 1));
+    CVG_TRY(ensure(ctx, ctx->d_prec, static_cast<size_t>(std::max(pairs, 1)) * 8 * sizeof(double)));
     CVG_TRY(ensure(ctx, ctx->d_psin, static_cast<size_t>(std::max(pairs, 1)) * d.kt * 2 * sizeof(double)));
     CVG_TRY(ensure(ctx, ctx->d_cnt3, static_cast<size_t>(3) * n * sizeof(unsigned)));
     CVG_TRY(ensure(ctx, ctx->d_lists, static_cast<size_t>(nb) * n * sizeof(int)));
@@ -1500,10 +1541,10 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     CVG_HIP(ctx, hipMemsetAsync(ctx->d_cnt3.ptr, 0, static_cast<size_t>(3) * n * sizeof(unsigned), st));
     CVG_HIP(ctx, hipMemsetAsync(scal + S_VALID, 0, (kScalarWords - S_VALID) * sizeof(unsigned), st));
     if (pairs > 0) {
-        hipLaunchKernelGGL(k_pair_class, dim3(blocks_for(pairs)), dim3(kBlock), 0, st, pairs, n,
+        hipLaunchKernelGGL(k_pair_prep, dim3(blocks_for(pairs)), dim3(kBlock), 0, st, pairs, n,
                            as<int>(ctx->d_pair_anchor), as<int>(ctx->d_ids), as<double>(ctx->d_x),
-                           as<double>(ctx->d_y), as<double>(ctx->d_trig), as<unsigned char>(ctx->d_cls),
-                           as<unsigned>(ctx->d_cnt3));
+                           as<double>(ctx->d_y), as<double>(ctx->d_trig), d.close_thr,
+                           as<unsigned char>(ctx->d_cls), as<double>(ctx->d_prec), as<unsigned>(ctx->d_cnt3));
         hipLaunchKernelGGL(k_pair_trig, dim3(blocks_for(static_cast<long long>(pairs) * d.kt)), dim3(kBlock), 0, st,
                            pairs, static_cast<int>(d.kt), as<int>(ctx->d_pair_anchor), as<int>(ctx->d_ids),
                            as<double>(ctx->d_x), as<double>(ctx->d_y), as<double>(ctx->d_th), T.off_t, d.dt,
@@ -1529,19 +1570,16 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
 
     FusedArgs A{};
     A.n = n;
-    A.x = as<double>(ctx->d_x);
-    A.y = as<double>(ctx->d_y);
-    A.th = as<double>(ctx->d_th);
+    A.prec = as<double>(ctx->d_prec);
     A.trig = as<double>(ctx->d_trig);
     A.psin = as<double>(ctx->d_psin);
     A.off = as<unsigned>(ctx->d_off);
     A.ids = as<int>(ctx->d_ids);
-    A.cls = as<unsigned char>(ctx->d_cls);
+    A.pflag = as<unsigned char>(ctx->d_cls);
     A.chain_stride = chain_stride;
     A.st_stride = st_stride;
     A.dx = d.dx;
     A.dx2 = d.dx2;
-    A.close_thr = d.close_thr;
     A.max_k = d.max_k;
     A.kt = static_cast<int>(d.kt);
     A.K = static_cast<int>(d.K);
@@ -1558,13 +1596,34 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     A.valid_pairs = scal + S_VALID;
     A.intersections = reinterpret_cast<unsigned long long *>(scal + S_COUNT);
 
-    // bundles of buckets too large for LDS go to global memory, one region each
+    // per bucket: 64-neighbour chunks of the lane-indexed lists (degree <= 3 x
+    // the class cap and <= the map's largest), waves per workgroup so that the
+    // workgroup's LDS fits, and bundles in global memory only when one wave's
+    // slice alone does not fit
+    struct Launch {
+        int nch, wpb;
+        bool gb;
+        size_t slice;
+    };
+    std::vector<Launch> plan(nb);
     std::vector<size_t> gb_off(nb, 0);
     size_t gb_total = 0;
-    auto deg_cap_of = [&](int b) { return std::min(3 * caps[b], static_cast<int>(std::max(ctx->max_deg, 1u))); };
     for (int b = 0; b < nb; ++b) {
-        if (bucket_count[b] == 0) continue;
-        if (wave_carve(caps[b], deg_cap_of(b), A.K, A.kt, chain_stride, true).bytes * kFusedWaves > ctx->lds_limit) {
+        const int dcap = std::min(3 * caps[b], static_cast<int>(std::max(ctx->max_deg, 1u)));
+        const int need = (dcap + 63) / 64;
+        if (need > 4) return fail(ctx, CVG_ERR_UNSUPPORTED, "more than 256 neighbours per anchor");
+        Launch &l = plan[b];
+        l.nch = need <= 1 ? 1 : (need <= 2 ? 2 : 4);
+        l.gb = false;
+        l.slice = fused_slice_bytes(caps[b], A.K, chain_stride, true);
+        l.wpb = 4;
+        while (l.wpb > 1 && l.slice * l.wpb > ctx->lds_limit) l.wpb >>= 1;
+        if (l.slice > ctx->lds_limit) {
+            l.gb = true;
+            l.wpb = 4;
+            l.slice = fused_slice_bytes(caps[b], A.K, chain_stride, false);
+            if (l.slice * l.wpb > ctx->lds_limit)
+                return fail(ctx, CVG_ERR_UNSUPPORTED, "neighbour lists too long for the workgroup's LDS");
             gb_off[b] = gb_total;
             gb_total += static_cast<size_t>(bucket_count[b]) * caps[b] * A.K * 2;
         }
@@ -1585,32 +1644,19 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
         const int slot = b == 0 ? 0 : 1 + (rr++ % 3);
         hipStream_t ls = slot == 0 ? st : ctx->aux[slot - 1];
         if (slot > 0) aux_used[slot - 1] = true;
+        const Launch &l = plan[b];
         A.list = as<int>(ctx->d_lists) + static_cast<size_t>(b) * n;
         A.list_len = len;
         A.nb_cap = caps[b];
-        A.deg_cap = deg_cap_of(b);
-        WaveCarve wc = wave_carve(A.nb_cap, A.deg_cap, A.K, A.kt, chain_stride, true);
-        if (wc.bytes * kFusedWaves > ctx->lds_limit) {
-            wc = wave_carve(A.nb_cap, A.deg_cap, A.K, A.kt, chain_stride, false);
-            if (wc.bytes * kFusedWaves > ctx->lds_limit)
-                return fail(ctx, CVG_ERR_UNSUPPORTED, "neighbour lists too long for the workgroup's LDS");
-            A.g_bundle = as<double>(ctx->d_gbundle) + gb_off[b];
-        } else {
-            A.g_bundle = nullptr;
-        }
-        const int blocks = (len + kFusedWaves - 1) / kFusedWaves;
+        A.g_bundle = l.gb ? as<double>(ctx->d_gbundle) + gb_off[b] : nullptr;
+        const int blocks = (len + l.wpb - 1) / l.wpb;
         const int grid = (blocks + 7) / 8 * 8;
-        const unsigned shmem = static_cast<unsigned>(wc.bytes * kFusedWaves);
-        const void *fn = d.cpl == 1 ? reinterpret_cast<const void *>(k_curvelets<1>)
-                                    : (d.cpl == 2 ? reinterpret_cast<const void *>(k_curvelets<2>)
-                                                  : reinterpret_cast<const void *>(k_curvelets<4>));
-        CVG_HIP(ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
-        if (d.cpl == 1)
-            hipLaunchKernelGGL(k_curvelets<1>, dim3(grid), dim3(64 * kFusedWaves), shmem, ls, A);
-        else if (d.cpl == 2)
-            hipLaunchKernelGGL(k_curvelets<2>, dim3(grid), dim3(64 * kFusedWaves), shmem, ls, A);
-        else
-            hipLaunchKernelGGL(k_curvelets<4>, dim3(grid), dim3(64 * kFusedWaves), shmem, ls, A);
+        const unsigned shmem = static_cast<unsigned>(l.slice * l.wpb);
+        const FusedKernel fn = fused_kernel(d.cpl, l.nch, l.gb);
+        if (!fn) return fail(ctx, CVG_ERR_STATE, "internal: no fused kernel for this grid");
+        CVG_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(64 * l.wpb), shmem, ls, A);
         CVG_HIP(ctx, hipGetLastError());
     }
     for (int i = 0; i < 3; ++i) {
@@ -1636,9 +1682,9 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     CVG_TRY(ensure(ctx, ctx->d_info, static_cast<size_t>(C) * kInfoWidth * sizeof(double)));
     RecordArgs R{};
     R.n = n;
-    R.x = A.x;
-    R.y = A.y;
-    R.th = A.th;
+    R.x = as<double>(ctx->d_x);
+    R.y = as<double>(ctx->d_y);
+    R.th = as<double>(ctx->d_th);
     R.off = A.off;
     R.acc_count = A.acc_count;
     R.row_off = as<unsigned>(ctx->d_rowoff);
@@ -1807,7 +1853,7 @@ void cvg_destroy(cvg_context *ctx)
                       &ctx->d_cy,       &ctx->d_scal,  &ctx->d_grid,    &ctx->d_tmp_ids, &ctx->d_sorted,
                       &ctx->d_sx,       &ctx->d_sy,    &ctx->d_deg,     &ctx->d_off,     &ctx->d_tiles,
                       &ctx->d_cand,     &ctx->d_ids,   &ctx->d_d2,      &ctx->d_pair_anchor, &ctx->d_trig,
-                      &ctx->d_cls,      &ctx->d_psin,  &ctx->d_cnt3,  &ctx->d_lists,   &ctx->d_tables,  &ctx->d_st_ids,
+                      &ctx->d_cls,      &ctx->d_prec,  &ctx->d_psin,  &ctx->d_cnt3,  &ctx->d_lists,   &ctx->d_tables,  &ctx->d_st_ids,
                       &ctx->d_st_meta,  &ctx->d_st_k,  &ctx->d_acc,     &ctx->d_rowoff,  &ctx->d_gbundle,
                       &ctx->d_chains,   &ctx->d_info,  &ctx->d_dbg};
     for (DevBuf *b : bufs) release(*b);
