@@ -13,11 +13,13 @@
 //           k_fill_csr        per anchor: gather in the reference's scan order,
 //                             distance sort (tie-exact std::sort replay), CSR
 //   curvelets k_edge_trig     per edge: sin/cos of theta, theta+-dt, theta+grid
-//           k_pair_class      per pair: direction-filter class
+//           k_pair_prep       per pair: direction-filter class, close/branch
+//                             flags, the bundle's cell-independent terms
+//           k_pair_trig       per (pair, grid angle): the transport's two sines
 //           k_bucket          anchors binned by their largest run (LDS capacity)
-//           k_curvelets       one 4-wave workgroup per anchor: bundles into LDS,
-//                             wave64 chain growth with ballot validity, dedup,
-//                             staging rows
+//           k_curvelets       one wave per anchor, bucket by bucket: bundles
+//                             (lane = cell) into the wave's LDS slice, chain
+//                             growth with ballot validity, dedup, staging rows
 //           scan              per-anchor accepted counts -> output row offsets
 //           k_record          chain rows (1-based) and the 10 info columns
 //

@@ -8,7 +8,8 @@ Run in a checkout where /root/reference exists (`make -C oracle ref` first):
 Writes, under tests/golden/:
   ref_cli/<case>.json          counts from the reference CLI's stdout
                                (CSR pairs, max degree, valid pairs, out_h/out_w)
-                               and SHA-256 of its chain/info text files
+                               and SHA-256 of its chain/info text files, plus
+                               per-column sums of the full-precision info
   ref_cli/<case>_{chain,info}.txt.gz
                                the reference CLI's own output files (small cases)
   full/<case>.npz              full-precision arrays from the reference's
@@ -52,6 +53,7 @@ CASES = [
     ("toedges_G4", "file:TO_edges", 4, False, False),
     ("toedges_G7", "file:TO_edges", 7, False, False),
     ("c1s1001_G4", "synth:C1:1001", 4, False, True),
+    ("c3s1003_G7", "synth:C3:1003", 7, False, False),   # the 1080p map: counts and hashes only
 ]
 
 
@@ -59,13 +61,14 @@ def sha(b: bytes) -> str:
     return hashlib.sha256(b).hexdigest()
 
 
-def edge_text(key: str) -> tuple[str, bytes]:
+def edge_text(key: str, keep_input: bool) -> tuple[str, bytes]:
     kind, _, rest = key.partition(":")
     if kind == "file":
         return rest + ".txt", shipped_text(rest)
     preset, seed = rest.split(":")
     e = synth_edges(preset, int(seed))
-    np.save(HERE / "inputs" / f"{preset}_{seed}.npy", e)
+    if keep_input:
+        np.save(HERE / "inputs" / f"{preset}_{seed}.npy", e)
     lines = "\n".join("\t".join(repr(float(v)) for v in row) for row in e) + "\n"
     return f"{preset}_{seed}.txt", lines.encode()
 
@@ -75,7 +78,8 @@ def edges_of(key: str) -> np.ndarray:
     if kind == "file":
         return shipped_edges(rest)
     preset, seed = rest.split(":")
-    return np.load(HERE / "inputs" / f"{preset}_{seed}.npy")
+    f = HERE / "inputs" / f"{preset}_{seed}.npy"
+    return np.load(f) if f.exists() else synth_edges(preset, int(seed))
 
 
 def run_cli(fname: str, text: bytes, G: int) -> tuple[dict, bytes, bytes]:
@@ -133,7 +137,7 @@ def main() -> int:
     ref = Ref()
     manifest = {}
     for name, key, G, keep_text, keep_full in CASES:
-        fname, text = edge_text(key)
+        fname, text = edge_text(key, keep_full)
         counts, chain, info = run_cli(fname, text, G)
         rec = dict(counts, input=key, group_max_sz=G, chain_sha256=sha(chain), info_sha256=sha(info),
                    input_sha256=sha(text))
@@ -148,6 +152,9 @@ def main() -> int:
         assert len(r["chains"]) == counts["out_h"]
         rec["full_chain_sha256"] = sha(np.ascontiguousarray(r["chains"]).tobytes())
         rec["full_info_sha256"] = sha(np.ascontiguousarray(r["info"]).tobytes())
+        # size-independent checks of the info columns (for maps too large to keep)
+        rec["info_colsum"] = [float(v) for v in r["info"].sum(axis=0)]
+        rec["info_colabssum"] = [float(v) for v in np.abs(r["info"]).sum(axis=0)]
         if keep_full:
             np.savez_compressed(HERE / "full" / f"{name}.npz", offsets=r["csr"]["offsets"], ids=r["csr"]["ids"],
                                 dist2=r["csr"]["dist2"], valid_pairs=np.array(r["valid_pairs"]),

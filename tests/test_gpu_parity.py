@@ -121,3 +121,25 @@ def test_curvelets_parity(ctx, oracle, oracle_cr, key, G):
     exact = oracle_cr.run(e, p, nthreads=0)
     assert np.array_equal(chains, exact["chains"])
     assert np.array_equal(bits(info), bits(exact["info"]))
+
+
+def test_full_size_map_matches_reference(ctx):
+    """The 1920x1080 benchmark map (207k edges, 2.04M pairs, G = 7) against the
+    reference's own run on it: CSR and valid-pair counts, the SHA-256 of every
+    chain row, and per-column sums of the info columns (1e-9 relative)."""
+    import hashlib
+    import json
+    from tests._inputs import GOLDEN
+    meta = json.loads((GOLDEN / "ref_cli" / "c3s1003_G7.json").read_text())
+    e = edges_for(meta["input"])
+    stats = ctx.run(e, CurveletParams(group_max_sz=meta["group_max_sz"]))
+    g = ctx.graph()
+    assert (g.num_edges, g.total_neighbor_pairs, g.max_num_of_neighbors) == (meta["edges"], meta["pairs"],
+                                                                             meta["max_deg"])
+    assert g.order_ambiguous_anchors == 0
+    assert stats["valid_pairs"] == meta["valid_pairs"]
+    chains, info = ctx.curvelets()
+    assert chains.shape == (meta["out_h"], meta["out_w"])
+    assert hashlib.sha256(np.ascontiguousarray(chains).tobytes()).hexdigest() == meta["full_chain_sha256"]
+    np.testing.assert_allclose(info.sum(axis=0), meta["info_colsum"], rtol=1e-9)
+    np.testing.assert_allclose(np.abs(info).sum(axis=0), meta["info_colabssum"], rtol=1e-9)

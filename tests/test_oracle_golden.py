@@ -32,7 +32,9 @@ def case_edges(meta):
     key = meta["input"]
     if key.startswith("synth:"):
         _, preset, seed = key.split(":")
-        return np.load(GOLDEN / "inputs" / f"{preset}_{seed}.npy")
+        f = GOLDEN / "inputs" / f"{preset}_{seed}.npy"
+        if f.exists():
+            return np.load(f)
     return edges_for(key)
 
 
@@ -53,7 +55,8 @@ def sha(b: bytes) -> str:
 
 
 def test_fixture_inventory():
-    assert {"digit1_G4", "digit1_G7", "abc_G4", "abc_G7", "toedges_G4", "toedges_G7", "c1s1001_G4"} <= set(CASES)
+    assert {"digit1_G4", "digit1_G7", "abc_G4", "abc_G7", "toedges_G4", "toedges_G7", "c1s1001_G4",
+            "c3s1003_G7"} <= set(CASES)
     assert set(FULL) <= set(CASES) and set(TEXT) <= set(CASES)
 
 
@@ -69,6 +72,7 @@ def test_counts_and_hashes(runs, name):
     assert r["chains"].shape == (meta["out_h"], meta["out_w"])
     assert sha(np.ascontiguousarray(r["chains"]).tobytes()) == meta["full_chain_sha256"]
     assert sha(np.ascontiguousarray(r["info"]).tobytes()) == meta["full_info_sha256"]
+    assert np.array_equal(r["info"].sum(axis=0), np.array(meta["info_colsum"]))
 
 
 @pytest.mark.parametrize("name", FULL)
