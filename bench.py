@@ -47,6 +47,17 @@ def host_cores() -> int:
     return max(1, n)
 
 
+def fused_traffic():
+    """HBM bytes per step of the fused kernel's launches, from the committed
+    rocprofv3 PMC summary of this workload (FETCH_SIZE + WRITE_SIZE, each scaled
+    by its copy-kernel calibration factor); None when it is absent."""
+    f = ROOT / "profiles" / "k_curvelets_1080p_traffic.json"
+    try:
+        return json.loads(f.read_text())["hbm_bytes_per_step"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(edges, params):
     """The reference binary's code (oracle/_ref) on host cores, span (ii)."""
     try:
@@ -81,9 +92,12 @@ def main(argv=None) -> int:
         src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(3):
-            dst.copy_(src)
+            dst.copy_(src)                   # the runtime's blit kernel
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)     # an elementwise kernel: reads nbytes, writes nbytes
         torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes": nbytes, "repeats": 3}))
+        print(json.dumps({"copy_calibration_bytes": nbytes, "repeats": 3,
+                          "kernels": ["__amd_rocclr_copyBuffer", "vectorized_elementwise_kernel (mul)"]}))
         return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); maps shard across GPUs "
@@ -112,6 +126,15 @@ def main(argv=None) -> int:
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
 
+    # the same map from host memory to host memory (PCIe both ways): reported
+    # beside `value`, never as it
+    pcie = []
+    for _ in range(5):
+        s0 = time.perf_counter()
+        ctx.run(edges, params)
+        ctx.curvelets()
+        pcie.append(time.perf_counter() - s0)
+
     pairs = st["total_neighbor_pairs"]
     ms_step = elapsed / args.steps * 1e3
     fused_ms = statistics.mean(fused)
@@ -135,12 +158,13 @@ def main(argv=None) -> int:
                    "edges": n, "neighbor_pairs": pairs, "valid_pairs": st["valid_pairs"],
                    "curvelets": st["num_curvelets"], "group_max_sz": WORKLOAD["group_max_sz"]},
         "bundle_pairs_per_s": pairs * args.steps / elapsed,
+        "pcie_inclusive_edges_per_s": n / statistics.median(pcie),
         "ms_per_step_median": statistics.median(per_step) * 1e3,
         "stage_ms": {"graph": st["ms_graph"], "curvelets": st["ms_curvelets"], "fused_kernel": fused_ms},
         "lds_bytes_per_workgroup": st["lds_bytes_per_workgroup"],
         "roofline": {"bound": "mfma", "pipe": "fp64 vector ALU (no MFMA in this path)",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": fused_traffic(),
                      "kernel": "k_curvelets", "kernel_ms": fused_ms,
                      "flop_per_launch": flops},
     }

@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -74,6 +75,12 @@ struct cvg_context {
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     std::string err;
     size_t lds_limit = 65536;
+    // fused-kernel launch shape: buckets whose class cap is at least
+    // coop_min_cap run one anchor per workgroup of coop_waves waves
+    // (CVG_COOP_MIN_CAP / CVG_COOP_WAVES override them; results never depend on it)
+    int coop_min_cap = 8;
+    int coop_waves = 4;
+    int num_cu = 1;
 
     unsigned *mailbox = nullptr;  // pinned host scalars read back after syncs
 
@@ -689,10 +696,12 @@ __global__ __launch_bounds__(kBlock) void k_pair_prep(int pairs, int n, const in
 }
 
 // Anchors in spatial (cell) order, binned by the largest of their three class
-// counts, which sizes the LDS slots the fused kernel needs.
+// counts, which sizes the LDS slots the fused kernel needs; each list entry
+// carries (anchor, CSR base, degree).
 __global__ __launch_bounds__(kBlock) void k_bucket(int n, const int *__restrict__ sorted,
-                                                   const unsigned *__restrict__ cnt3, int nb, const int *caps,
-                                                   int *__restrict__ lists, unsigned *__restrict__ scal)
+                                                   const unsigned *__restrict__ cnt3, const unsigned *__restrict__ off,
+                                                   int nb, const int *caps, int4 *__restrict__ lists,
+                                                   unsigned *__restrict__ scal)
 {
     __shared__ unsigned s_count[kMaxBuckets];
     __shared__ unsigned s_base[kMaxBuckets];
@@ -723,7 +732,11 @@ __global__ __launch_bounds__(kBlock) void k_bucket(int n, const int *__restrict_
     if (threadIdx.x < nb && s_count[threadIdx.x] > 0)
         s_base[threadIdx.x] = atomicAdd(&scal[S_BUCKET + threadIdx.x], s_count[threadIdx.x]);
     __syncthreads();
-    if (b >= 0) lists[static_cast<long long>(b) * n + s_base[b] + my_off] = a;
+    if (b >= 0) {
+        const unsigned o0 = off[a];
+        lists[static_cast<long long>(b) * n + s_base[b] + my_off] =
+            make_int4(a, static_cast<int>(o0), static_cast<int>(off[a + 1] - o0), 0);
+    }
 }
 
 // Per-pair sines of the transport (compute_curve_bundle, cpu_curvelet.hpp:728-729):
@@ -775,13 +788,13 @@ struct Tables {
 struct FusedArgs {
     int n;
     const double *trig, *psin, *prec;
-    const unsigned *off;
     const int *ids;
     const unsigned char *pflag;
-    const int *list;
+    const int4 *list;  // (anchor, CSR base, degree, -)
     int list_len;
-    int nb_cap;        // bundle slots per wave: the bucket's longest run
-    int chain_stride;  // LDS ints per accepted chain: max(G, 2)
+    int nb_cap;        // bundle slots per anchor: the bucket's longest run
+    int deg_cap;       // staged neighbour records per anchor
+    int chain_stride;  // ids per seed record: max(G, 1)
     int st_stride;     // staging ids per row: max(G, 1)
     double dx, dx2, max_k;
     int kt, K, G;
@@ -794,16 +807,39 @@ struct FusedArgs {
     unsigned *acc_count;
     unsigned *valid_pairs;
     unsigned long long *intersections;
-    double *g_bundle;  // global-bundle kernels: nb_cap*K*2 doubles per list entry
+    double *g_bundle;  // global-bundle kernels: nb_cap*K*2 doubles per anchor stream
 };
 
-// LDS bytes per wave: the run's hypothesis bundles (lo then hi; none when they
-// live in global memory) and the run's accepted chains.
-__host__ __device__ inline size_t fused_slice_bytes(int nb_cap, int K, int chain_stride, bool lds_bundles)
+// One anchor stream's LDS slice (byte offsets): two staging buffers (the
+// current and the next anchor's pair records and sines, copied from the
+// pre-passes' contiguous blocks), the run's bundles by slot (lo, hi; absent
+// when they live in global memory), the seeds' results (k_max, k_min; length,
+// hash, best cell, chain) and the slots' hypothesis flags.
+struct AnchorCarve {
+    size_t stg, stg_bytes, rec, sin, blo, bhi, skd, ski, hyp, bytes;
+};
+
+__host__ __device__ inline AnchorCarve anchor_carve(int nb, int dc, int K, int kt, int cs, bool lds_bundles)
 {
-    size_t b = lds_bundles ? static_cast<size_t>(nb_cap) * K * 2 * sizeof(double) : 0;
-    b += static_cast<size_t>(nb_cap) * chain_stride * sizeof(int);
-    return (b + 15) & ~static_cast<size_t>(15);
+    AnchorCarve c{};
+    c.rec = 0;  // within a staging buffer
+    c.sin = static_cast<size_t>(dc) * 8 * sizeof(double);
+    c.stg_bytes = c.sin + static_cast<size_t>(dc) * kt * 2 * sizeof(double);
+    size_t o = 0;
+    c.stg = o;
+    o += 2 * c.stg_bytes;
+    c.blo = o;
+    if (lds_bundles) o += static_cast<size_t>(nb) * K * sizeof(double);
+    c.bhi = o;
+    if (lds_bundles) o += static_cast<size_t>(nb) * K * sizeof(double);
+    c.skd = o;
+    o += static_cast<size_t>(nb) * 2 * sizeof(double);
+    c.ski = o;
+    o += static_cast<size_t>(nb) * (cs + 3) * sizeof(int);
+    c.hyp = o;
+    o += static_cast<size_t>(nb) * sizeof(int);
+    c.bytes = (o + 15) & ~static_cast<size_t>(15);
+    return c;
 }
 
 // Load from a wave-uniform address through the scalar data cache (read only).
@@ -811,6 +847,19 @@ template <typename T>
 __device__ __forceinline__ T uload(const T *p)
 {
     return *(const __attribute__((address_space(4))) T *)p;
+}
+
+// Asynchronous copy of cnt 16-byte entries from global memory into LDS
+// (global_load_lds: lane i of a chunk writes dst + 16 i); waves t of W split
+// the chunks.  Completion: s_waitcnt vmcnt(0), then a barrier.
+__device__ __forceinline__ void copy_to_lds(const double2 *src, char *dst, int cnt, int t, int W, int lane)
+{
+    for (int c0 = t * 64; c0 < cnt; c0 += W * 64) {
+        const int i = c0 + lane;
+        if (i < cnt)
+            __builtin_amdgcn_global_load_lds(static_cast<const void *>(src + i),
+                                             (__attribute__((address_space(3))) void *)(dst + 16 * c0), 16, 0, 0);
+    }
 }
 
 // Lane-indexed arrays: entry k of a wave-wide list lives in lane k % 64 of
@@ -855,64 +904,77 @@ __device__ __forceinline__ unsigned chain_hash(unsigned h, int id)
     return (h ^ static_cast<unsigned>(id)) * 16777619u;
 }
 
-// build_curvelets_greedy for one anchor per wavefront (cpu_curvelet.hpp:437-591)
-// plus the form_pairwise_bundles valid-pair count (:305-361), in one pass.
+// build_curvelets_greedy (cpu_curvelet.hpp:437-591) plus the
+// form_pairwise_bundles valid-pair count (:305-361), in one pass.
 //
-// Control state is wave-uniform (scalar registers); per-neighbour flags and ids,
-// hypothesis ids and the accepted chains' hashes sit in lane-indexed registers
-// and are read with readlane, so no loop waits on memory for control.
-// Runs: 0 = forward neighbours (class 1), 1 = backward (class 2), 2 = neither
-// (class 0: validity only, for the count).  For each neighbour of the run, in
-// CSR order, the lanes compute the bundle's cells (lane = cell in rank order,
-// CPL cells per lane) from the pair record and sines that k_pair_prep and
-// k_pair_trig wrote, prefetched one neighbour ahead; hypotheses (valid or close
-// pairs) are kept, compactly, in the wave's LDS slice.  Then every hypothesis
-// seeds a chain grown by intersecting the bundles in CSR order (validity = one
-// ballot); a chain longer than 2 is accepted unless an earlier accepted chain
-// of this (anchor, run) is identical (hash match, then element check), and is
-// written to the anchor's staging rows with its best cell (fill_curvelet_info,
-// :202-226: the first valid cell in rank order).
-template <int CPL, int NCH, bool GB>
+// Persistent: the grid holds as many anchor streams as the GPU keeps resident;
+// stream g processes list entries g, g + streams, ...  A stream is one wave
+// (COOP = false, blockDim/64 streams per workgroup) or a whole workgroup whose
+// W = blockDim/64 waves split each anchor's bundles (slot i on wave i % W) and
+// seeds (hypothesis j on wave j % W) (COOP = true).  While an anchor is
+// processed, the next one's pair records and sines are already being copied
+// into the other LDS staging buffer (global_load_lds), and its list entry,
+// flags, ids and per-lane trig values are loaded, so memory latency is paid
+// once per stream instead of once per anchor.
+//
+// Per anchor, runs 0 = forward neighbours (class 1), 1 = backward (class 2),
+// 2 = neither (class 0: validity only, for the count).  Phase A: the run's
+// bundles (lane = cell in rank order, CPL cells per lane) into LDS slots in CSR
+// order, with hypothesis flags.  Phase C: each hypothesis seeds a chain grown
+// by intersecting the hypotheses' bundles in CSR order (validity = one
+// ballot); the seed's chain, hash and best cell (fill_curvelet_info,
+// :202-226: the first valid cell in rank order) go to LDS.  Phase D (wave 0):
+// in seed order, a chain longer than 2 is accepted unless an accepted chain of
+// this (anchor, run) is identical (hash match, then element check), and
+// written to the anchor's staging rows.  Control state is wave-uniform;
+// per-neighbour flags and ids, slot ids and accepted hashes are lane-indexed
+// registers.
+template <int CPL, int NCH, bool GB, bool COOP>
 __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
 {
     extern __shared__ double lds[];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
     const int wpb = static_cast<int>(blockDim.x >> 6);
+    const int W = COOP ? wpb : 1;      // waves per anchor
+    const int t = COOP ? wave : 0;     // this wave's rank within its anchor
+    const int grp = COOP ? 0 : wave;   // anchor stream within the workgroup
+    const int apb = COOP ? 1 : wpb;    // anchor streams per workgroup
+    const int streams = static_cast<int>(gridDim.x) * apb;
+    const int g = static_cast<int>(blockIdx.x) * apb + grp;
+    if (g >= A.list_len) return;  // COOP: the whole workgroup (same g)
 
-    // XCD-aware order: the grid is a multiple of 8 and block b runs on XCD b % 8;
-    // each XCD gets a contiguous slice of the spatially ordered anchor list.
-    const int per = static_cast<int>(gridDim.x) >> 3;
-    const int blk = static_cast<int>(blockIdx.x & 7) * per + static_cast<int>(blockIdx.x >> 3);
-    const int L = blk * wpb + wave;
-    if (L >= A.list_len) return;  // whole wave; no workgroup barriers below
-    const int a = uload(A.list + L);
-
+    const int n = A.n;
     const int K = A.K;
     const int kt = A.kt;
     const int G = A.G;
     const int CS = A.chain_stride;
+    const int SR = CS + 3;  // ints per seed record
     const int NB = A.nb_cap;
-    char *sl = reinterpret_cast<char *>(lds) + static_cast<size_t>(wave) * fused_slice_bytes(NB, K, CS, !GB);
+    const int len_list = A.list_len;
+    const AnchorCarve cv = anchor_carve(NB, A.deg_cap, K, kt, CS, !GB);
+    char *sl = reinterpret_cast<char *>(lds) + static_cast<size_t>(grp) * cv.bytes;
+    double *skd = reinterpret_cast<double *>(sl + cv.skd);
+    int *ski = reinterpret_cast<int *>(sl + cv.ski);
+    int *hypf = reinterpret_cast<int *>(sl + cv.hyp);
     double *blo;
     double *bhi;
-    int *accc;
     if constexpr (GB) {
-        blo = A.g_bundle + static_cast<size_t>(L) * NB * K * 2;
+        blo = A.g_bundle + static_cast<size_t>(g) * NB * K * 2;
         bhi = blo + static_cast<size_t>(NB) * K;
-        accc = reinterpret_cast<int *>(sl);
     } else {
-        blo = reinterpret_cast<double *>(sl);
-        bhi = blo + static_cast<size_t>(NB) * K;
-        accc = reinterpret_cast<int *>(bhi + static_cast<size_t>(NB) * K);
+        blo = reinterpret_cast<double *>(sl + cv.blo);
+        bhi = reinterpret_cast<double *>(sl + cv.bhi);
     }
+    auto sync = [&]() {
+        if constexpr (COOP)
+            __syncthreads();
+        else
+            wave_sync();
+    };
 
-    const int n = A.n;
-    const unsigned base = uload(A.off + a);
-    const int deg = static_cast<int>(uload(A.off + a + 1) - base);
-
-    // per-lane cells (rank order)
-    double ox[CPL], sT2t[CPL], cT2t[CPL];
+    // per-lane cells (rank order): anchor-independent
+    double ox[CPL];
     int cj[CPL], ccell[CPL], coff[CPL];
     bool live[CPL], elig[CPL];
 #pragma unroll
@@ -925,227 +987,332 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
         cj[m] = A.rank_j[r];
         ccell[m] = A.rank_cell[r];
         elig[m] = live[m] && A.rank_elig[r] != 0;
-        sT2t[m] = A.trig[static_cast<long long>(cvg::TRIG_GRID + 2 * cj[m]) * n + a];
-        cT2t[m] = A.trig[static_cast<long long>(cvg::TRIG_GRID + 2 * cj[m] + 1) * n + a];
     }
 
-    // the anchor's neighbours: flags (class | close | branch) and ids
-    int nfl[NCH], nid[NCH];
-#pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-        const int r = 64 * c + lane;
-        nfl[c] = r < deg ? static_cast<int>(A.pflag[base + r]) : 3;
-        nid[c] = r < deg ? A.ids[base + r] : 0;
-    }
-
+    // the pipeline: entry, staging copy and per-lane values of the next anchor
+    const double2 *prec2 = reinterpret_cast<const double2 *>(A.prec);
     const double2 *psin2 = reinterpret_cast<const double2 *>(A.psin);
-    auto load_rec = [&](int k) { return A.prec[(static_cast<long long>(base) + k) * 8 + (lane & 7)]; };
-    auto load_sines = [&](int k, double2 (&s)[CPL]) {
-        const double2 *ps = psin2 + (static_cast<long long>(base) + k) * kt;
-#pragma unroll
-        for (int m = 0; m < CPL; ++m) s[m] = ps[cj[m]];
+    auto entry = [&](int L, int &ea, unsigned &eb, int &ed) {
+        const int *e = reinterpret_cast<const int *>(A.list + L);
+        ea = uload(e);
+        eb = static_cast<unsigned>(uload(e + 1));
+        ed = uload(e + 2);
     };
-
-    int acc = 0;        // accepted rows of this anchor (both runs)
-    int vp = 0;         // valid or close pairs (all three classes)
-    unsigned isect = 0; // intersections tested
-    for (int run = 0; run < 3; ++run) {
-        const int want = run == 0 ? 1 : (run == 1 ? 2 : 0);
-        unsigned long long mk[NCH];
+    auto stage = [&](int buf, unsigned b0, int d0) {
+        char *sb = sl + cv.stg + static_cast<size_t>(buf) * cv.stg_bytes;
+        copy_to_lds(prec2 + static_cast<size_t>(b0) * 4, sb + cv.rec, d0 * 4, t, W, lane);
+        copy_to_lds(psin2 + static_cast<size_t>(b0) * kt, sb + cv.sin, d0 * kt, t, W, lane);
+    };
+    double nsT2t[CPL], ncT2t[CPL];
+    int nnfl[NCH], nnid[NCH];
+    auto lane_vals = [&](int a0, unsigned b0, int d0) {
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) mk[c] = __ballot((nfl[c] & 3) == want);
-
-        // bundles of the run's neighbours (compute_curve_bundle, :636-819)
-        int nH = 0;
-        int hid[NCH];
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) hid[c] = 0;
-        int k = pop_next<NCH>(mk);
-        double rv = 0.0;
-        double2 sv[CPL];
-        if (k >= 0) {
-            rv = load_rec(k);
-            load_sines(k, sv);
+        for (int m = 0; m < CPL; ++m) {
+            nsT2t[m] = A.trig[static_cast<long long>(cvg::TRIG_GRID + 2 * cj[m]) * n + a0];
+            ncT2t[m] = A.trig[static_cast<long long>(cvg::TRIG_GRID + 2 * cj[m] + 1) * n + a0];
         }
-        while (k >= 0) {
-            const int kc = k;
-            const double rc = rv;
-            double2 sc[CPL];
-#pragma unroll
-            for (int m = 0; m < CPL; ++m) sc[m] = sv[m];
-            k = pop_next<NCH>(mk);
-            if (k >= 0) {
-                rv = load_rec(k);
-                load_sines(k, sv);
-            }
-            const int fl = lane_get<NCH>(nfl, kc);
-            cvg::PairRec q;
-            q.DX = bcast(rc, 0);
-            q.DY = bcast(rc, 1);
-            q.DX2 = bcast(rc, 2);
-            q.DY2 = bcast(rc, 3);
-            q.cT0m_DX = bcast(rc, 4);
-            q.sT0m_DY = bcast(rc, 5);
-            q.cT0p_DX = bcast(rc, 6);
-            q.sT0p_DY = bcast(rc, 7);
-            const bool qclose = (fl & 4) != 0;
-            const bool qbr = (fl & 8) != 0;
-            double lo[CPL], hi[CPL];
-            bool any = false;
-#pragma unroll
-            for (int m = 0; m < CPL; ++m) {
-                lo[m] = -A.max_k;
-                hi[m] = A.max_k;
-                if (!qclose)
-                    cvg::bundle_cell(q, qbr, ox[m], sT2t[m], cT2t[m], sc[m].x, sc[m].y, A.dx, A.dx2, A.max_k, lo[m],
-                                     hi[m]);
-                any = any || (live[m] && hi[m] > lo[m]);
-            }
-            if (!qclose && __ballot(any) == 0ULL) continue;  // bundle_valid_check, :821-830
-            ++vp;
-            if (run == 2) continue;
-#pragma unroll
-            for (int m = 0; m < CPL; ++m) {
-                if (live[m]) {
-                    blo[static_cast<size_t>(nH) * K + coff[m]] = lo[m];
-                    bhi[static_cast<size_t>(nH) * K + coff[m]] = hi[m];
-                }
-            }
-            lane_set<NCH>(hid, nH, lane_get<NCH>(nid, kc), lane);
-            ++nH;
-        }
-        if (run == 2 || nH == 0 || G < 3) continue;  // no chain longer than 2 when G < 3
-
-        // growth (:479-536), dedup (:546-560, 876-916) and record, seed by seed
-        int nacc = 0;
-        int ahash[NCH], alen[NCH];
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
-            ahash[c] = 0;
-            alen[c] = 0;
+            const int r = 64 * c + lane;
+            nnfl[c] = r < d0 ? static_cast<int>(A.pflag[b0 + r]) : 3;
+            nnid[c] = r < d0 ? A.ids[b0 + r] : 0;
         }
-        for (int s = 0; s < nH; ++s) {
-            double lo[CPL], hi[CPL], pl[CPL], ph[CPL];
-#pragma unroll
-            for (int m = 0; m < CPL; ++m) {
-                lo[m] = blo[static_cast<size_t>(s) * K + coff[m]];
-                hi[m] = bhi[static_cast<size_t>(s) * K + coff[m]];
-                pl[m] = blo[coff[m]];
-                ph[m] = bhi[coff[m]];
-            }
-            int len = 1;
-            int cur = lane == 0 ? a : 0;  // chain position l in lane l
-            unsigned hsh = chain_hash(2166136261u, a);
-            for (int h = 0; h < nH; ++h) {
-                double bl[CPL], bh[CPL];
-#pragma unroll
-                for (int m = 0; m < CPL; ++m) {
-                    bl[m] = pl[m];
-                    bh[m] = ph[m];
-                }
-                if (h + 1 < nH) {
-#pragma unroll
-                    for (int m = 0; m < CPL; ++m) {
-                        pl[m] = blo[static_cast<size_t>(h + 1) * K + coff[m]];
-                        ph[m] = bhi[static_cast<size_t>(h + 1) * K + coff[m]];
-                    }
-                }
-                if (h == s) {  // the seed joins at its own distance rank (:496-510)
-                    const int id = lane_get<NCH>(hid, h);
-                    if (lane == len) cur = id;
-                    hsh = chain_hash(hsh, id);
-                    ++len;
-                    if (len >= G) break;
-                    continue;
-                }
-                ++isect;
-                double nlo[CPL], nhi[CPL];
-                bool ok = false;
-#pragma unroll
-                for (int m = 0; m < CPL; ++m) {
-                    nlo[m] = (lo[m] < bl[m]) ? bl[m] : lo[m];  // bundle_intersection, :838-841
-                    nhi[m] = (hi[m] > bh[m]) ? bh[m] : hi[m];
-                    ok = ok || (live[m] && nhi[m] > nlo[m]);
-                }
-                if (__ballot(ok) != 0ULL) {
-                    const int id = lane_get<NCH>(hid, h);
-                    if (lane == len) cur = id;
-                    hsh = chain_hash(hsh, id);
-                    ++len;
-#pragma unroll
-                    for (int m = 0; m < CPL; ++m) {
-                        lo[m] = nlo[m];
-                        hi[m] = nhi[m];
-                    }
-                }
-                if (len >= G) break;
-            }
-            if (len <= 2) continue;  // :549
+    };
 
-            // identical to an earlier accepted chain of this (anchor, run)?
-            bool dup = false;
+    int L = g;
+    int a, deg;
+    unsigned base;
+    entry(L, a, base, deg);
+    stage(0, base, deg);
+    lane_vals(a, base, deg);
+    int Ln = L + streams;
+    int na = 0, ndeg = 0;
+    unsigned nbase = 0;
+    if (Ln < len_list) entry(Ln, na, nbase, ndeg);
+
+    unsigned vpw = 0;     // valid or close pairs computed by this wave
+    unsigned isect = 0;   // intersections tested by this wave
+    for (int it = 0; L < len_list; ++it) {
+        const int buf = it & 1;
+        double sT2t[CPL], cT2t[CPL];
+        int nfl[NCH], nid[NCH];
+#pragma unroll
+        for (int m = 0; m < CPL; ++m) {
+            sT2t[m] = nsT2t[m];
+            cT2t[m] = ncT2t[m];
+        }
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            nfl[c] = nnfl[c];
+            nid[c] = nnid[c];
+        }
+        __builtin_amdgcn_s_waitcnt(0);  // this anchor's staging copy and values have landed
+        sync();
+        // start the next anchor's loads
+        const int Lnn = Ln + streams;
+        int nna = 0, nndeg = 0;
+        unsigned nnbase = 0;
+        if (Ln < len_list) {
+            stage(buf ^ 1, nbase, ndeg);
+            lane_vals(na, nbase, ndeg);
+            if (Lnn < len_list) entry(Lnn, nna, nnbase, nndeg);
+        }
+
+        const char *sb = sl + cv.stg + static_cast<size_t>(buf) * cv.stg_bytes;
+        const double2 *srec = reinterpret_cast<const double2 *>(sb + cv.rec);
+        const double2 *ssin = reinterpret_cast<const double2 *>(sb + cv.sin);
+        int acc = 0;  // accepted rows of this anchor (wave 0)
+        for (int run = 0; run < 3; ++run) {
+            const int want = run == 0 ? 1 : (run == 1 ? 2 : 0);
+            unsigned long long mk[NCH];
+            int nL = 0;
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
-                unsigned long long mm = __ballot(64 * c + lane < nacc && static_cast<unsigned>(ahash[c]) == hsh &&
-                                                 alen[c] == len);
-                while (mm != 0ULL && !dup) {
-                    const int j = 64 * c + __builtin_ctzll(mm);
-                    mm &= mm - 1ULL;
-                    const bool diff = lane < len && accc[j * CS + lane] != cur;
-                    dup = __ballot(diff) == 0ULL;
-                }
+                mk[c] = __ballot((nfl[c] & 3) == want);
+                nL += __popcll(mk[c]);
             }
-            if (dup) continue;
-            if (lane < len) accc[nacc * CS + lane] = cur;
-            lane_set<NCH>(ahash, nacc, static_cast<int>(hsh), lane);
-            lane_set<NCH>(alen, nacc, len, lane);
-            ++nacc;
+            if (nL == 0) continue;
 
-            // best cell of the running bundle (fill_curvelet_info, :202-226)
-            int best = -1;
+            // phase A: the run's bundles (compute_curve_bundle, :636-819), slot
+            // i = i-th neighbour of the run in CSR order
+            int rid[NCH];
 #pragma unroll
-            for (int m = 0; m < CPL; ++m) {
-                const unsigned long long mask = __ballot(elig[m] && hi[m] > lo[m]);
-                if (best < 0 && mask != 0ULL) best = 64 * m + __builtin_ctzll(mask);
+            for (int c = 0; c < NCH; ++c) rid[c] = 0;
+            for (int i = 0; i < nL; ++i) {
+                const int r = pop_next<NCH>(mk);
+                lane_set<NCH>(rid, i, lane_get<NCH>(nid, r), lane);
+                if (COOP && (i % W) != t) continue;
+                const int fl = lane_get<NCH>(nfl, r);
+                const double2 *rr = srec + r * 4;
+                const double2 r0 = rr[0], r1 = rr[1], r2 = rr[2], r3 = rr[3];
+                cvg::PairRec q;
+                q.DX = r0.x;
+                q.DY = r0.y;
+                q.DX2 = r1.x;
+                q.DY2 = r1.y;
+                q.cT0m_DX = r2.x;
+                q.sT0m_DY = r2.y;
+                q.cT0p_DX = r3.x;
+                q.sT0p_DY = r3.y;
+                const bool qclose = (fl & 4) != 0;
+                const bool qbr = (fl & 8) != 0;
+                double lo[CPL], hi[CPL];
+                bool any = false;
+#pragma unroll
+                for (int m = 0; m < CPL; ++m) {
+                    lo[m] = -A.max_k;
+                    hi[m] = A.max_k;
+                    if (!qclose) {
+                        const double2 s = ssin[r * kt + cj[m]];
+                        cvg::bundle_cell(q, qbr, ox[m], sT2t[m], cT2t[m], s.x, s.y, A.dx, A.dx2, A.max_k, lo[m],
+                                         hi[m]);
+                    }
+                    any = any || (live[m] && hi[m] > lo[m]);
+                }
+                const bool hyp = qclose || __ballot(any) != 0ULL;  // bundle_valid_check, :821-830
+                vpw += hyp ? 1u : 0u;
+                if (run == 2) continue;
+                if (hyp) {
+#pragma unroll
+                    for (int m = 0; m < CPL; ++m) {
+                        if (live[m]) {
+                            blo[static_cast<size_t>(i) * K + coff[m]] = lo[m];
+                            bhi[static_cast<size_t>(i) * K + coff[m]] = hi[m];
+                        }
+                    }
+                }
+                if (lane == 0) hypf[i] = hyp ? 1 : 0;
             }
-            if (best < 0) best = A.rank_of_cell0;
-            const int bm = best >> 6;
-            const int bl = best & 63;
-            double kmax = 0.0, kmin = 0.0;
-            int best_cell = 0;
+            if (run == 2) continue;
+            sync();
+
+            // phase B: hypotheses = flagged slots, in CSR order
+            unsigned long long hm[NCH];
+            int nH = 0;
 #pragma unroll
-            for (int m = 0; m < CPL; ++m) {
-                if (m == bm) {
-                    kmax = bcast(hi[m], bl);
-                    kmin = bcast(lo[m], bl);
-                    best_cell = bcast(ccell[m], bl);
+            for (int c = 0; c < NCH; ++c) {
+                const int i = 64 * c + lane;
+                hm[c] = __ballot(i < nL && hypf[i] != 0);
+                nH += __popcll(hm[c]);
+            }
+
+            if (nH > 0 && G >= 3) {  // no chain longer than 2 when G < 3
+                // phase C: growth (:479-536), seed j on wave j % W
+                unsigned long long sm[NCH];
+#pragma unroll
+                for (int c = 0; c < NCH; ++c) sm[c] = hm[c];
+                for (int j = 0; j < nH; ++j) {
+                    const int s = pop_next<NCH>(sm);
+                    if (COOP && (j % W) != t) continue;
+                    double lo[CPL], hi[CPL], pl[CPL], ph[CPL];
+                    unsigned long long wm[NCH];
+#pragma unroll
+                    for (int c = 0; c < NCH; ++c) wm[c] = hm[c];
+                    int i = pop_next<NCH>(wm);
+#pragma unroll
+                    for (int m = 0; m < CPL; ++m) {
+                        lo[m] = blo[static_cast<size_t>(s) * K + coff[m]];
+                        hi[m] = bhi[static_cast<size_t>(s) * K + coff[m]];
+                        pl[m] = blo[static_cast<size_t>(i) * K + coff[m]];
+                        ph[m] = bhi[static_cast<size_t>(i) * K + coff[m]];
+                    }
+                    int len = 1;
+                    int cur = lane == 0 ? a : 0;  // chain position l in lane l
+                    unsigned hsh = chain_hash(2166136261u, a);
+                    while (i >= 0) {
+                        const int ci = i;
+                        double bl[CPL], bh[CPL];
+#pragma unroll
+                        for (int m = 0; m < CPL; ++m) {
+                            bl[m] = pl[m];
+                            bh[m] = ph[m];
+                        }
+                        i = pop_next<NCH>(wm);
+                        if (i >= 0) {
+#pragma unroll
+                            for (int m = 0; m < CPL; ++m) {
+                                pl[m] = blo[static_cast<size_t>(i) * K + coff[m]];
+                                ph[m] = bhi[static_cast<size_t>(i) * K + coff[m]];
+                            }
+                        }
+                        if (ci == s) {  // the seed joins at its own distance rank (:496-510)
+                            const int id = lane_get<NCH>(rid, ci);
+                            if (lane == len) cur = id;
+                            hsh = chain_hash(hsh, id);
+                            ++len;
+                            if (len >= G) break;
+                            continue;
+                        }
+                        ++isect;
+                        double nlo[CPL], nhi[CPL];
+                        bool ok = false;
+#pragma unroll
+                        for (int m = 0; m < CPL; ++m) {
+                            nlo[m] = (lo[m] < bl[m]) ? bl[m] : lo[m];  // bundle_intersection, :838-841
+                            nhi[m] = (hi[m] > bh[m]) ? bh[m] : hi[m];
+                            ok = ok || (live[m] && nhi[m] > nlo[m]);
+                        }
+                        if (__ballot(ok) != 0ULL) {
+                            const int id = lane_get<NCH>(rid, ci);
+                            if (lane == len) cur = id;
+                            hsh = chain_hash(hsh, id);
+                            ++len;
+#pragma unroll
+                            for (int m = 0; m < CPL; ++m) {
+                                lo[m] = nlo[m];
+                                hi[m] = nhi[m];
+                            }
+                        }
+                        if (len >= G) break;
+                    }
+                    int *rec = ski + j * SR;
+                    if (lane == 0) rec[0] = len;
+                    if (len <= 2) continue;  // :549
+                    // best cell of the running bundle (fill_curvelet_info, :202-226)
+                    int best = -1;
+#pragma unroll
+                    for (int m = 0; m < CPL; ++m) {
+                        const unsigned long long mask = __ballot(elig[m] && hi[m] > lo[m]);
+                        if (best < 0 && mask != 0ULL) best = 64 * m + __builtin_ctzll(mask);
+                    }
+                    if (best < 0) best = A.rank_of_cell0;
+                    const int bm = best >> 6;
+                    const int bl = best & 63;
+                    double kmax = 0.0, kmin = 0.0;
+                    int best_cell = 0;
+#pragma unroll
+                    for (int m = 0; m < CPL; ++m) {
+                        if (m == bm) {
+                            kmax = bcast(hi[m], bl);
+                            kmin = bcast(lo[m], bl);
+                            best_cell = bcast(ccell[m], bl);
+                        }
+                    }
+                    if (lane < len) rec[3 + lane] = cur;
+                    if (lane == 0) {
+                        rec[1] = static_cast<int>(hsh);
+                        rec[2] = best_cell;
+                        skd[2 * j] = kmax;
+                        skd[2 * j + 1] = kmin;
+                    }
+                }
+                sync();
+
+                // phase D: dedup (:546-560, 876-916) and staging, in seed order
+                if (t == 0) {
+                    int ah[NCH], al[NCH], aj[NCH];
+#pragma unroll
+                    for (int c = 0; c < NCH; ++c) {
+                        ah[c] = 0;
+                        al[c] = 0;
+                        aj[c] = 0;
+                    }
+                    int nacc = 0;
+                    for (int j = 0; j < nH; ++j) {
+                        const int *rec = ski + j * SR;
+                        const int len = __builtin_amdgcn_readfirstlane(rec[0]);
+                        if (len <= 2) continue;
+                        const int h = __builtin_amdgcn_readfirstlane(rec[1]);
+                        const int cur = lane < len ? rec[3 + lane] : 0;
+                        bool dup = false;
+#pragma unroll
+                        for (int c = 0; c < NCH; ++c) {
+                            unsigned long long mm = __ballot(64 * c + lane < nacc && ah[c] == h && al[c] == len);
+                            while (mm != 0ULL && !dup) {
+                                const int k = 64 * c + __builtin_ctzll(mm);
+                                mm &= mm - 1ULL;
+                                const int *other = ski + lane_get<NCH>(aj, k) * SR;
+                                const bool diff = lane < len && other[3 + lane] != cur;
+                                dup = __ballot(diff) == 0ULL;
+                            }
+                        }
+                        if (dup) continue;
+                        lane_set<NCH>(ah, nacc, h, lane);
+                        lane_set<NCH>(al, nacc, len, lane);
+                        lane_set<NCH>(aj, nacc, j, lane);
+                        ++nacc;
+                        const long long q = static_cast<long long>(base) + acc;
+                        if (lane < len) A.st_ids[q * A.st_stride + lane] = cur;
+                        if (lane == 0) {
+                            A.st_meta[q] = static_cast<unsigned>(len) | (run == 0 ? 0x100u : 0u) |
+                                           (static_cast<unsigned>(rec[2]) << 16);
+                            A.st_k[2 * q] = skd[2 * j];
+                            A.st_k[2 * q + 1] = skd[2 * j + 1];
+                        }
+                        ++acc;
+                    }
                 }
             }
-            const long long q = static_cast<long long>(base) + acc;
-            if (lane < len) A.st_ids[q * A.st_stride + lane] = cur;
-            if (lane == 0) {
-                A.st_meta[q] = static_cast<unsigned>(len) | (run == 0 ? 0x100u : 0u) |
-                               (static_cast<unsigned>(best_cell) << 16);
-                A.st_k[2 * q] = kmax;
-                A.st_k[2 * q + 1] = kmin;
-            }
-            ++acc;
+            sync();  // the next run reuses the slots and seed records
         }
+        if (t == 0 && lane == 0) A.acc_count[a] = static_cast<unsigned>(acc);
+
+        // advance the pipeline
+        L = Ln;
+        a = na;
+        base = nbase;
+        deg = ndeg;
+        Ln = Lnn;
+        na = nna;
+        nbase = nnbase;
+        ndeg = nndeg;
     }
     if (lane == 0) {
-        A.acc_count[a] = static_cast<unsigned>(acc);
-        if (vp) atomicAdd(A.valid_pairs, static_cast<unsigned>(vp));
+        if (vpw) atomicAdd(A.valid_pairs, vpw);
         if (isect) atomicAdd(A.intersections, static_cast<unsigned long long>(isect));
     }
 }
 
 using FusedKernel = void (*)(FusedArgs);
 
-// k_curvelets<CPL, NCH, GB> by (cells per lane, 64-neighbour chunks, bundles in global memory)
-FusedKernel fused_kernel(int cpl, int nch, bool gb)
+// k_curvelets<CPL, NCH, GB, COOP> by (cells per lane, 64-neighbour chunks,
+// bundles in global memory, waves sharing one anchor)
+FusedKernel fused_kernel(int cpl, int nch, bool gb, bool coop)
 {
-#define CVG_FK(C, N)                                                                        \
-    if (cpl == C && nch == N) return gb ? k_curvelets<C, N, true> : k_curvelets<C, N, false>;
+#define CVG_FK(C, N)                                                                                         \
+    if (cpl == C && nch == N)                                                                                \
+        return gb ? k_curvelets<C, N, true, false> : (coop ? k_curvelets<C, N, false, true>                  \
+                                                           : k_curvelets<C, N, false, false>);
     CVG_FK(1, 1) CVG_FK(1, 2) CVG_FK(1, 4)
     CVG_FK(2, 1) CVG_FK(2, 2) CVG_FK(2, 4)
     CVG_FK(4, 1) CVG_FK(4, 2) CVG_FK(4, 4)
@@ -1536,7 +1703,7 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     CVG_TRY(ensure(ctx, ctx->d_prec, static_cast<size_t>(std::max(pairs, 1)) * 8 * sizeof(double)));
     CVG_TRY(ensure(ctx, ctx->d_psin, static_cast<size_t>(std::max(pairs, 1)) * d.kt * 2 * sizeof(double)));
     CVG_TRY(ensure(ctx, ctx->d_cnt3, static_cast<size_t>(3) * n * sizeof(unsigned)));
-    CVG_TRY(ensure(ctx, ctx->d_lists, static_cast<size_t>(nb) * n * sizeof(int)));
+    CVG_TRY(ensure(ctx, ctx->d_lists, static_cast<size_t>(nb) * n * sizeof(int4)));
     unsigned *scal = as<unsigned>(ctx->d_scal);
     hipLaunchKernelGGL(k_edge_trig, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<double>(ctx->d_th), d.dt,
                        static_cast<int>(d.kt), T.off_t, as<double>(ctx->d_trig));
@@ -1553,7 +1720,8 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
                            d.close_thr, as<double>(ctx->d_psin));
     }
     hipLaunchKernelGGL(k_bucket, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_sorted),
-                       as<unsigned>(ctx->d_cnt3), nb, T.caps, as<int>(ctx->d_lists), scal);
+                       as<unsigned>(ctx->d_cnt3), as<unsigned>(ctx->d_off), nb, T.caps, as<int4>(ctx->d_lists),
+                       scal);
     CVG_HIP(ctx, hipGetLastError());
     CVG_TRY(read_scalars(ctx));
     std::vector<unsigned> bucket_count(nb);
@@ -1562,7 +1730,7 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
 
     // staging
     const int st_stride = static_cast<int>(std::max(d.G, 1u));
-    const int chain_stride = static_cast<int>(std::max(d.G, 2u));
+    const int chain_stride = static_cast<int>(std::max(d.G, 1u));
     const size_t np = static_cast<size_t>(std::max(pairs, 1));
     CVG_TRY(ensure(ctx, ctx->d_st_ids, np * st_stride * sizeof(int)));
     CVG_TRY(ensure(ctx, ctx->d_st_meta, np * sizeof(unsigned)));
@@ -1575,7 +1743,6 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     A.prec = as<double>(ctx->d_prec);
     A.trig = as<double>(ctx->d_trig);
     A.psin = as<double>(ctx->d_psin);
-    A.off = as<unsigned>(ctx->d_off);
     A.ids = as<int>(ctx->d_ids);
     A.pflag = as<unsigned char>(ctx->d_cls);
     A.chain_stride = chain_stride;
@@ -1599,43 +1766,70 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     A.intersections = reinterpret_cast<unsigned long long *>(scal + S_COUNT);
 
     // per bucket: 64-neighbour chunks of the lane-indexed lists (degree <= 3 x
-    // the class cap and <= the map's largest), waves per workgroup so that the
-    // workgroup's LDS fits, and bundles in global memory only when one wave's
-    // slice alone does not fit
+    // the class cap and <= the map's largest); heavy buckets share one anchor
+    // among the waves of a workgroup, light ones pack one anchor per wave;
+    // bundles go to global memory only when one anchor's slice alone exceeds
+    // LDS.  Each launch is persistent: as many anchor streams as stay resident
+    // (the occupancy query accounts for registers and LDS), each looping over
+    // its share of the bucket's list.
     struct Launch {
-        int nch, wpb;
-        bool gb;
+        int nch, wpb, apb, dcap, grid;
+        bool gb, coop;
         size_t slice;
+        unsigned shmem;
+        FusedKernel fn;
     };
     std::vector<Launch> plan(nb);
     std::vector<size_t> gb_off(nb, 0);
     size_t gb_total = 0;
     for (int b = 0; b < nb; ++b) {
-        const int dcap = std::min(3 * caps[b], static_cast<int>(std::max(ctx->max_deg, 1u)));
-        const int need = (dcap + 63) / 64;
-        if (need > 4) return fail(ctx, CVG_ERR_UNSUPPORTED, "more than 256 neighbours per anchor");
         Launch &l = plan[b];
+        l.dcap = std::min(3 * caps[b], static_cast<int>(std::max(ctx->max_deg, 1u)));
+        const int need = (l.dcap + 63) / 64;
+        if (need > 4) return fail(ctx, CVG_ERR_UNSUPPORTED, "more than 256 neighbours per anchor");
         l.nch = need <= 1 ? 1 : (need <= 2 ? 2 : 4);
         l.gb = false;
-        l.slice = fused_slice_bytes(caps[b], A.K, chain_stride, true);
-        l.wpb = 4;
-        while (l.wpb > 1 && l.slice * l.wpb > ctx->lds_limit) l.wpb >>= 1;
+        l.coop = caps[b] >= ctx->coop_min_cap;
+        l.slice = anchor_carve(caps[b], l.dcap, A.K, A.kt, chain_stride, true).bytes;
+        if (l.coop) {
+            l.wpb = ctx->coop_waves;
+        } else {
+            l.wpb = 4;
+            while (l.wpb > 1 && l.slice * l.wpb > ctx->lds_limit) l.wpb >>= 1;
+        }
         if (l.slice > ctx->lds_limit) {
             l.gb = true;
-            l.wpb = 4;
-            l.slice = fused_slice_bytes(caps[b], A.K, chain_stride, false);
-            if (l.slice * l.wpb > ctx->lds_limit)
+            l.coop = false;
+            l.wpb = 1;
+            l.slice = anchor_carve(caps[b], l.dcap, A.K, A.kt, chain_stride, false).bytes;
+            if (l.slice > ctx->lds_limit)
                 return fail(ctx, CVG_ERR_UNSUPPORTED, "neighbour lists too long for the workgroup's LDS");
+        }
+        l.apb = l.coop ? 1 : l.wpb;
+        l.shmem = static_cast<unsigned>(l.slice * l.apb);
+        l.fn = fused_kernel(d.cpl, l.nch, l.gb, l.coop);
+        if (!l.fn) return fail(ctx, CVG_ERR_STATE, "internal: no fused kernel for this grid");
+        l.grid = 0;
+        if (bucket_count[b] == 0) continue;
+        CVG_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(l.fn),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(l.shmem)));
+        int occ = 0;
+        CVG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(l.fn),
+                                                                  64 * l.wpb, l.shmem));
+        const long long resident = static_cast<long long>(std::max(occ, 1)) * ctx->num_cu;
+        const long long need_blocks = (static_cast<long long>(bucket_count[b]) + l.apb - 1) / l.apb;
+        l.grid = static_cast<int>(std::min(resident, need_blocks));
+        if (l.gb) {
             gb_off[b] = gb_total;
-            gb_total += static_cast<size_t>(bucket_count[b]) * caps[b] * A.K * 2;
+            gb_total += static_cast<size_t>(l.grid) * l.apb * caps[b] * A.K * 2;
         }
     }
     if (gb_total > 0) CVG_TRY(ensure(ctx, ctx->d_gbundle, gb_total * sizeof(double)));
 
     // bucket launches: the short-list bucket 0 (most anchors) alone on the main
     // stream, the long-list buckets round-robin over the auxiliary streams,
-    // longest lists first, so their long-latency waves start at once and
-    // overlap the bulk
+    // longest lists first, so their long-latency work starts at once and
+    // overlaps the bulk
     CVG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
     for (int i = 0; i < 3; ++i) CVG_HIP(ctx, hipStreamWaitEvent(ctx->aux[i], ctx->ev[3], 0));
     bool aux_used[3] = {false, false, false};
@@ -1647,18 +1841,12 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
         hipStream_t ls = slot == 0 ? st : ctx->aux[slot - 1];
         if (slot > 0) aux_used[slot - 1] = true;
         const Launch &l = plan[b];
-        A.list = as<int>(ctx->d_lists) + static_cast<size_t>(b) * n;
+        A.list = as<int4>(ctx->d_lists) + static_cast<size_t>(b) * n;
         A.list_len = len;
         A.nb_cap = caps[b];
+        A.deg_cap = l.dcap;
         A.g_bundle = l.gb ? as<double>(ctx->d_gbundle) + gb_off[b] : nullptr;
-        const int blocks = (len + l.wpb - 1) / l.wpb;
-        const int grid = (blocks + 7) / 8 * 8;
-        const unsigned shmem = static_cast<unsigned>(l.slice * l.wpb);
-        const FusedKernel fn = fused_kernel(d.cpl, l.nch, l.gb);
-        if (!fn) return fail(ctx, CVG_ERR_STATE, "internal: no fused kernel for this grid");
-        CVG_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
-        hipLaunchKernelGGL(fn, dim3(grid), dim3(64 * l.wpb), shmem, ls, A);
+        hipLaunchKernelGGL(l.fn, dim3(l.grid), dim3(64 * l.wpb), l.shmem, ls, A);
         CVG_HIP(ctx, hipGetLastError());
     }
     for (int i = 0; i < 3; ++i) {
@@ -1687,7 +1875,7 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     R.x = as<double>(ctx->d_x);
     R.y = as<double>(ctx->d_y);
     R.th = as<double>(ctx->d_th);
-    R.off = A.off;
+    R.off = as<unsigned>(ctx->d_off);
     R.acc_count = A.acc_count;
     R.row_off = as<unsigned>(ctx->d_rowoff);
     R.st_ids = A.st_ids;
@@ -1836,6 +2024,12 @@ int cvg_create(int device, cvg_context **out_ctx)
             if (prop.maxSharedMemoryPerMultiProcessor > 0)
                 lim = std::min(lim, static_cast<size_t>(prop.maxSharedMemoryPerMultiProcessor));
             if (lim > 0) ctx->lds_limit = lim;
+            if (prop.multiProcessorCount > 0) ctx->num_cu = prop.multiProcessorCount;
+        }
+        if (const char *v = std::getenv("CVG_COOP_MIN_CAP")) ctx->coop_min_cap = std::max(1, std::atoi(v));
+        if (const char *v = std::getenv("CVG_COOP_WAVES")) {
+            const int w = std::atoi(v);
+            ctx->coop_waves = (w == 1 || w == 2 || w == 4) ? w : 4;
         }
     } while (false);
     if (rc != CVG_OK) {

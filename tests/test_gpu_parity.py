@@ -143,3 +143,21 @@ def test_full_size_map_matches_reference(ctx):
     assert hashlib.sha256(np.ascontiguousarray(chains).tobytes()).hexdigest() == meta["full_chain_sha256"]
     np.testing.assert_allclose(info.sum(axis=0), meta["info_colsum"], rtol=1e-9)
     np.testing.assert_allclose(np.abs(info).sum(axis=0), meta["info_colabssum"], rtol=1e-9)
+
+
+@pytest.mark.parametrize("min_cap,waves", [(1, 4), (1, 2), (1, 1), (1000, 4)])
+def test_launch_shapes_same_results(oracle_cr, monkeypatch, min_cap, waves):
+    """Every launch shape of the fused kernel (one anchor per wave, or one
+    anchor per workgroup of 1, 2 or 4 waves) gives the same bits."""
+    from curvelet_construction_on_gpu_amd import Context
+    monkeypatch.setenv("CVG_COOP_MIN_CAP", str(min_cap))
+    monkeypatch.setenv("CVG_COOP_WAVES", str(waves))
+    with Context(0) as c:
+        for key, G in (("file:TO_edges", 7), ("synth:C2:1002", 5)):
+            e = edges_for(key)
+            p = CurveletParams(group_max_sz=G)
+            c.run(e, p)
+            chains, info = c.curvelets()
+            want = oracle_cr.run(e, p, nthreads=0)
+            assert np.array_equal(chains, want["chains"])
+            assert np.array_equal(bits(info), bits(want["info"]))
