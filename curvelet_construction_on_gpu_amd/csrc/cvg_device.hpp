@@ -17,10 +17,12 @@ struct PairRec {
     double cT0m_DX, sT0m_DY, cT0p_DX, sT0p_DY;
 };
 
-// Per-edge trig slots (structure of arrays, stride = number of edges):
-//   0 cos(theta)      1 sin(theta)
-//   2 sin(theta+dt)   3 cos(theta+dt)   4 sin(theta-dt)   5 cos(theta-dt)
-//   6+2j sin(theta + off_t[j])          7+2j cos(theta + off_t[j])
+// Per-edge trig slots: rows 0-5 are structure of arrays (stride = number of
+// edges n), then one block of 2 Kt values per edge (edge-major, so the fused
+// kernel reads an anchor's grid angles from one or two cache lines):
+//   row 0 cos(theta)      row 1 sin(theta)
+//   row 2 sin(theta+dt)   row 3 cos(theta+dt)   row 4 sin(theta-dt)   row 5 cos(theta-dt)
+//   [6n + 2(i Kt + j)] = sin(theta_i + off_t[j]),  [6n + 2(i Kt + j) + 1] = cos(theta_i + off_t[j])
 enum {
     TRIG_COS = 0,
     TRIG_SIN = 1,

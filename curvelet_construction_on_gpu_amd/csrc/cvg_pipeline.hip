@@ -78,7 +78,7 @@ struct cvg_context {
     // fused-kernel launch shape: buckets whose class cap is at least
     // coop_min_cap run one anchor per workgroup of coop_waves waves
     // (CVG_COOP_MIN_CAP / CVG_COOP_WAVES override them; results never depend on it)
-    int coop_min_cap = 8;
+    int coop_min_cap = 6;
     int coop_waves = 4;
     int num_cu = 1;
 
@@ -656,8 +656,8 @@ __global__ __launch_bounds__(kBlock) void k_edge_trig(int n, const double *__res
     trig[cvg::TRIG_COS_M * n + i] = c;
     for (int j = 0; j < kt; ++j) {
         cvg::cr_sincos(t + off_t[j], c_sincos_tab, &s, &c);  // T2t = T2 + delta_t2, :702
-        trig[static_cast<long long>(cvg::TRIG_GRID + 2 * j) * n + i] = s;
-        trig[static_cast<long long>(cvg::TRIG_GRID + 2 * j + 1) * n + i] = c;
+        trig[static_cast<long long>(cvg::TRIG_GRID) * n + (static_cast<long long>(i) * kt + j) * 2] = s;
+        trig[static_cast<long long>(cvg::TRIG_GRID) * n + (static_cast<long long>(i) * kt + j) * 2 + 1] = c;
     }
 }
 
@@ -1008,8 +1008,10 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
     auto lane_vals = [&](int a0, unsigned b0, int d0) {
 #pragma unroll
         for (int m = 0; m < CPL; ++m) {
-            nsT2t[m] = A.trig[static_cast<long long>(cvg::TRIG_GRID + 2 * cj[m]) * n + a0];
-            ncT2t[m] = A.trig[static_cast<long long>(cvg::TRIG_GRID + 2 * cj[m] + 1) * n + a0];
+            const double2 sc = reinterpret_cast<const double2 *>(A.trig + static_cast<long long>(cvg::TRIG_GRID) * n)
+                [static_cast<long long>(a0) * kt + cj[m]];
+            nsT2t[m] = sc.x;
+            ncT2t[m] = sc.y;
         }
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
@@ -1081,7 +1083,7 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
             for (int i = 0; i < nL; ++i) {
                 const int r = pop_next<NCH>(mk);
                 lane_set<NCH>(rid, i, lane_get<NCH>(nid, r), lane);
-                if (COOP && (i % W) != t) continue;
+                if (COOP && (i & (W - 1)) != t) continue;
                 const int fl = lane_get<NCH>(nfl, r);
                 const double2 *rr = srec + r * 4;
                 const double2 r0 = rr[0], r1 = rr[1], r2 = rr[2], r3 = rr[3];
@@ -1126,7 +1128,8 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
             if (run == 2) continue;
             sync();
 
-            // phase B: hypotheses = flagged slots, in CSR order
+            // phase B: hypotheses = flagged slots, in CSR order; lane h holds
+            // hypothesis h's bundle offset (hso, bytes) and neighbour id (hid)
             unsigned long long hm[NCH];
             int nH = 0;
 #pragma unroll
@@ -1135,52 +1138,62 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
                 hm[c] = __ballot(i < nL && hypf[i] != 0);
                 nH += __popcll(hm[c]);
             }
+            int hso[NCH], hid[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                hso[c] = 0;
+                hid[c] = 0;
+            }
+            {
+                unsigned long long m2[NCH];
+#pragma unroll
+                for (int c = 0; c < NCH; ++c) m2[c] = hm[c];
+                for (int h = 0; h < nH; ++h) {
+                    const int slot = pop_next<NCH>(m2);
+                    lane_set<NCH>(hso, h, slot * K * 8, lane);
+                    lane_set<NCH>(hid, h, lane_get<NCH>(rid, slot), lane);
+                }
+            }
 
             if (nH > 0 && G >= 3) {  // no chain longer than 2 when G < 3
-                // phase C: growth (:479-536), seed j on wave j % W
-                unsigned long long sm[NCH];
-#pragma unroll
-                for (int c = 0; c < NCH; ++c) sm[c] = hm[c];
-                for (int j = 0; j < nH; ++j) {
-                    const int s = pop_next<NCH>(sm);
-                    if (COOP && (j % W) != t) continue;
+                const char *blo_b = reinterpret_cast<const char *>(blo);
+                const char *bhi_b = reinterpret_cast<const char *>(bhi);
+                // phase C: growth (:479-536), seed j on wave j % W; the walk
+                // visits the hypotheses in CSR order, the seed at its own rank
+                for (int j = t; j < nH; j += W) {
                     double lo[CPL], hi[CPL], pl[CPL], ph[CPL];
-                    unsigned long long wm[NCH];
-#pragma unroll
-                    for (int c = 0; c < NCH; ++c) wm[c] = hm[c];
-                    int i = pop_next<NCH>(wm);
+                    const int soff = lane_get<NCH>(hso, j);
+                    int noff = lane_get<NCH>(hso, 0);
 #pragma unroll
                     for (int m = 0; m < CPL; ++m) {
-                        lo[m] = blo[static_cast<size_t>(s) * K + coff[m]];
-                        hi[m] = bhi[static_cast<size_t>(s) * K + coff[m]];
-                        pl[m] = blo[static_cast<size_t>(i) * K + coff[m]];
-                        ph[m] = bhi[static_cast<size_t>(i) * K + coff[m]];
+                        lo[m] = *reinterpret_cast<const double *>(blo_b + soff + coff[m] * 8);
+                        hi[m] = *reinterpret_cast<const double *>(bhi_b + soff + coff[m] * 8);
+                        pl[m] = *reinterpret_cast<const double *>(blo_b + noff + coff[m] * 8);
+                        ph[m] = *reinterpret_cast<const double *>(bhi_b + noff + coff[m] * 8);
                     }
                     int len = 1;
                     int cur = lane == 0 ? a : 0;  // chain position l in lane l
                     unsigned hsh = chain_hash(2166136261u, a);
-                    while (i >= 0) {
-                        const int ci = i;
+                    for (int h = 0; h < nH; ++h) {
                         double bl[CPL], bh[CPL];
 #pragma unroll
                         for (int m = 0; m < CPL; ++m) {
                             bl[m] = pl[m];
                             bh[m] = ph[m];
                         }
-                        i = pop_next<NCH>(wm);
-                        if (i >= 0) {
+                        if (h + 1 < nH) {
+                            noff = lane_get<NCH>(hso, h + 1);
 #pragma unroll
                             for (int m = 0; m < CPL; ++m) {
-                                pl[m] = blo[static_cast<size_t>(i) * K + coff[m]];
-                                ph[m] = bhi[static_cast<size_t>(i) * K + coff[m]];
+                                pl[m] = *reinterpret_cast<const double *>(blo_b + noff + coff[m] * 8);
+                                ph[m] = *reinterpret_cast<const double *>(bhi_b + noff + coff[m] * 8);
                             }
                         }
-                        if (ci == s) {  // the seed joins at its own distance rank (:496-510)
-                            const int id = lane_get<NCH>(rid, ci);
+                        if (h == j) {  // the seed joins at its own distance rank (:496-510)
+                            const int id = lane_get<NCH>(hid, h);
                             if (lane == len) cur = id;
                             hsh = chain_hash(hsh, id);
-                            ++len;
-                            if (len >= G) break;
+                            if (++len >= G) break;
                             continue;
                         }
                         ++isect;
@@ -1193,17 +1206,16 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
                             ok = ok || (live[m] && nhi[m] > nlo[m]);
                         }
                         if (__ballot(ok) != 0ULL) {
-                            const int id = lane_get<NCH>(rid, ci);
+                            const int id = lane_get<NCH>(hid, h);
                             if (lane == len) cur = id;
                             hsh = chain_hash(hsh, id);
-                            ++len;
 #pragma unroll
                             for (int m = 0; m < CPL; ++m) {
                                 lo[m] = nlo[m];
                                 hi[m] = nhi[m];
                             }
+                            if (++len >= G) break;
                         }
-                        if (len >= G) break;
                     }
                     int *rec = ski + j * SR;
                     if (lane == 0) rec[0] = len;
@@ -1238,48 +1250,48 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
                 }
                 sync();
 
-                // phase D: dedup (:546-560, 876-916) and staging, in seed order
+                // phase D (wave 0): dedup (:546-560, 876-916) and staging, seeds in
+                // lanes.  Seed j is dropped when an earlier seed j' (length > 2)
+                // has the identical chain: the first of equal chains is the one
+                // the reference accepted, so "identical to an earlier accepted
+                // chain" and "identical to an earlier seed's chain" coincide.
                 if (t == 0) {
-                    int ah[NCH], al[NCH], aj[NCH];
+                    int slen[NCH], shash[NCH];
 #pragma unroll
                     for (int c = 0; c < NCH; ++c) {
-                        ah[c] = 0;
-                        al[c] = 0;
-                        aj[c] = 0;
+                        const int j = 64 * c + lane;
+                        slen[c] = j < nH ? ski[j * SR] : 0;
+                        shash[c] = (j < nH && slen[c] > 2) ? ski[j * SR + 1] : 0;
                     }
-                    int nacc = 0;
-                    for (int j = 0; j < nH; ++j) {
-                        const int *rec = ski + j * SR;
-                        const int len = __builtin_amdgcn_readfirstlane(rec[0]);
-                        if (len <= 2) continue;
-                        const int h = __builtin_amdgcn_readfirstlane(rec[1]);
-                        const int cur = lane < len ? rec[3 + lane] : 0;
-                        bool dup = false;
 #pragma unroll
-                        for (int c = 0; c < NCH; ++c) {
-                            unsigned long long mm = __ballot(64 * c + lane < nacc && ah[c] == h && al[c] == len);
-                            while (mm != 0ULL && !dup) {
-                                const int k = 64 * c + __builtin_ctzll(mm);
-                                mm &= mm - 1ULL;
-                                const int *other = ski + lane_get<NCH>(aj, k) * SR;
-                                const bool diff = lane < len && other[3 + lane] != cur;
-                                dup = __ballot(diff) == 0ULL;
-                            }
+                    for (int c = 0; c < NCH; ++c) {
+                        if (64 * c >= nH) break;
+                        const int j = 64 * c + lane;
+                        const int lj = slen[c];
+                        bool dup = false;
+                        const int jmax = min(nH, 64 * c + 64) - 1;
+                        for (int jp = 0; jp < jmax; ++jp) {
+                            const int lp = lane_get<NCH>(slen, jp);
+                            if (lp <= 2) continue;
+                            const int hp = lane_get<NCH>(shash, jp);
+                            bool cand = j > jp && !dup && lj == lp && shash[c] == hp;
+                            if (__ballot(cand) == 0ULL) continue;
+                            for (int l = 0; l < lp; ++l)
+                                cand = cand && ski[j * SR + 3 + l] == ski[jp * SR + 3 + l];
+                            dup = dup || cand;
                         }
-                        if (dup) continue;
-                        lane_set<NCH>(ah, nacc, h, lane);
-                        lane_set<NCH>(al, nacc, len, lane);
-                        lane_set<NCH>(aj, nacc, j, lane);
-                        ++nacc;
-                        const long long q = static_cast<long long>(base) + acc;
-                        if (lane < len) A.st_ids[q * A.st_stride + lane] = cur;
-                        if (lane == 0) {
-                            A.st_meta[q] = static_cast<unsigned>(len) | (run == 0 ? 0x100u : 0u) |
-                                           (static_cast<unsigned>(rec[2]) << 16);
+                        const bool accept = j < nH && lj > 2 && !dup;
+                        const unsigned long long am = __ballot(accept);
+                        if (accept) {
+                            const long long q = static_cast<long long>(base) + acc +
+                                                __popcll(am & ((1ULL << lane) - 1ULL));
+                            A.st_meta[q] = static_cast<unsigned>(lj) | (run == 0 ? 0x100u : 0u) |
+                                           (static_cast<unsigned>(ski[j * SR + 2]) << 16);
                             A.st_k[2 * q] = skd[2 * j];
                             A.st_k[2 * q + 1] = skd[2 * j + 1];
+                            for (int l = 0; l < lj; ++l) A.st_ids[q * A.st_stride + l] = ski[j * SR + 3 + l];
                         }
-                        ++acc;
+                        acc += __popcll(am);
                     }
                 }
             }
