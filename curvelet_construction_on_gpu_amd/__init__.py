@@ -100,6 +100,10 @@ _SIGNATURES = {
     "cvg_device_curvelets": (C.c_int, [C.c_void_p, _P(C.c_void_p), _P(C.c_void_p)]),
     "cvg_get_stream": (C.c_int, [C.c_void_p, _P(C.c_void_p)]),
     "cvg_synchronize": (C.c_int, [C.c_void_p]),
+    "cvg_run_batch": (C.c_int, [C.c_void_p, C.c_int, _ip, C.c_int, _P(_dp), _P(_Params), _P(_RunStats)]),
+    "cvg_run_batch_device": (C.c_int, [C.c_void_p, C.c_int, _ip, C.c_int, C.c_void_p, _P(_Params), _P(_RunStats)]),
+    "cvg_batch_map_shape": (C.c_int, [C.c_void_p, C.c_int, _up, _up]),
+    "cvg_get_batch_map_curvelets": (C.c_int, [C.c_void_p, C.c_int, _up, _dp]),
     "cvg_synth_preset": (C.c_int, [C.c_char_p, _P(_SynthConfig)]),
     "cvg_synth_generate": (C.c_int, [_P(_SynthConfig), C.c_ulonglong, _dp, C.c_int, _ip]),
     "cvg_debug_anchor_bundles": (C.c_int, [C.c_void_p, _P(_Params), C.c_int, _ucp, _ucp, _dp, _dp, _ip, _ip]),
@@ -281,6 +285,40 @@ class Context:
 
     def synchronize(self) -> None:
         self.check(self._lib.cvg_synchronize(self._h), "synchronize")
+
+    # -- batched form --------------------------------------------------------
+    def run_batch(self, maps: list, params: CurveletParams) -> dict:
+        """Several edge maps (each N_m x sz) in one pass; results per map via batch_curvelets(m)."""
+        arrs = [np.ascontiguousarray(m, dtype=np.float64).reshape(-1, np.shape(m)[1] if np.ndim(m) == 2 else 4)
+                for m in maps]
+        sz = arrs[0].shape[1] if arrs else 4
+        if any(a.shape[1] != sz for a in arrs):
+            raise CurveletError("all maps of a batch need the same values per edge")
+        counts = np.array([a.shape[0] for a in arrs], dtype=np.int32)
+        ptrs = (_dp * len(arrs))(*[_ptr(a, C.c_double) for a in arrs])
+        st = _RunStats()
+        p = params._c()
+        self.check(self._lib.cvg_run_batch(self._h, len(arrs), _ptr(counts, C.c_int), sz, ptrs, C.byref(p),
+                                           C.byref(st)), "run_batch")
+        self._batch_keep = arrs
+        return {k: getattr(st, k) for k, _ in _RunStats._fields_}
+
+    def run_batch_device(self, dptr: int, counts, sz: int, params: CurveletParams) -> dict:
+        c = np.ascontiguousarray(counts, dtype=np.int32)
+        st = _RunStats()
+        p = params._c()
+        self.check(self._lib.cvg_run_batch_device(self._h, len(c), _ptr(c, C.c_int), sz, C.c_void_p(dptr),
+                                                  C.byref(p), C.byref(st)), "run_batch_device")
+        return {k: getattr(st, k) for k, _ in _RunStats._fields_}
+
+    def batch_curvelets(self, m: int) -> tuple[np.ndarray, np.ndarray]:
+        rows, w = C.c_uint(), C.c_uint()
+        self.check(self._lib.cvg_batch_map_shape(self._h, m, C.byref(rows), C.byref(w)), "batch_map_shape")
+        chains = np.zeros((rows.value, w.value), np.uint32)
+        info = np.zeros((rows.value, 10))
+        self.check(self._lib.cvg_get_batch_map_curvelets(self._h, m, _ptr(chains, C.c_uint), _ptr(info, C.c_double)),
+                   "get_batch_map_curvelets")
+        return chains, info
 
     def anchor_bundles(self, params: CurveletParams, anchor: int, max_deg: int):
         """Per-neighbour dir class, hypothesis flag and bundle cells of one anchor (device)."""

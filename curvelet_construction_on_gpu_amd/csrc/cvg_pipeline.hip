@@ -99,6 +99,13 @@ struct cvg_context {
     DevBuf d_edges_up, d_x, d_y, d_th, d_cx, d_cy, d_scal, d_grid, d_tmp_ids, d_sorted, d_sx, d_sy, d_deg,
         d_off, d_tiles, d_cand, d_ids, d_d2, d_pair_anchor;
 
+    // the maps of a batch (a single map outside batches): edge id ranges,
+    // per-edge map ids, per-map cell boxes and shifts, output row ranges
+    int num_maps = 1;
+    std::vector<long long> map_eoff;
+    std::vector<unsigned> map_rows;
+    DevBuf d_map, d_eoff, d_mapbox, d_maprow;
+
     // curvelets
     bool curvelets_ready = false;
     cvg_params cur_params{};
@@ -367,6 +374,84 @@ __global__ __launch_bounds__(kBlock) void k_prep_edges(int n, int sz, const doub
 }
 
 __device__ inline int cell_index(const GridDesc &g, int cx, int cy) { return (cy - g.y0) * g.w + (cx - g.x0); }
+
+// Batches: edge i belongs to map m with eoff[m] <= i < eoff[m+1].
+__global__ __launch_bounds__(kBlock) void k_map_ids(int n, int num_maps, const long long *__restrict__ eoff,
+                                                    int *__restrict__ map_of)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int lo = 0, hi = num_maps - 1;  // largest m with eoff[m] <= i
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (eoff[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    map_of[i] = lo;
+}
+
+// Per-map cell boxes (minx, maxx, miny, maxy); box must start at
+// (INT_MAX, INT_MIN, INT_MAX, INT_MIN).  A wave whose edges share one map
+// reduces first and updates once.
+__global__ __launch_bounds__(kBlock) void k_map_box(int n, const int *__restrict__ map_of,
+                                                    const int *__restrict__ cx, const int *__restrict__ cy,
+                                                    int *__restrict__ box)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = i < n;
+    const int m = in ? map_of[i] : -1;
+    int mnx = in ? cx[i] : INT_MAX, mxx = in ? cx[i] : INT_MIN;
+    int mny = in ? cy[i] : INT_MAX, mxy = in ? cy[i] : INT_MIN;
+    const int m0 = __shfl(m, 0, 64);
+    if (__ballot(in && m != m0) == 0ULL) {
+        for (int o = 32; o > 0; o >>= 1) {
+            mnx = min(mnx, __shfl_xor(mnx, o, 64));
+            mxx = max(mxx, __shfl_xor(mxx, o, 64));
+            mny = min(mny, __shfl_xor(mny, o, 64));
+            mxy = max(mxy, __shfl_xor(mxy, o, 64));
+        }
+        if ((threadIdx.x & 63) == 0 && m0 >= 0) {
+            atomicMin(&box[4 * m0], mnx);
+            atomicMax(&box[4 * m0 + 1], mxx);
+            atomicMin(&box[4 * m0 + 2], mny);
+            atomicMax(&box[4 * m0 + 3], mxy);
+        }
+    } else if (in) {
+        atomicMin(&box[4 * m], mnx);
+        atomicMax(&box[4 * m + 1], mxx);
+        atomicMin(&box[4 * m + 2], mny);
+        atomicMax(&box[4 * m + 3], mxy);
+    }
+}
+
+__global__ void k_map_box_init(int num_maps, int *box)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 4 * num_maps) box[i] = (i & 1) ? INT_MIN : INT_MAX;
+}
+
+// Each map's cells move into its own region of one dense grid: only the
+// integer cell coordinates shift; x, y and theta are untouched, so every
+// distance and bundle is computed from the map's own values, and the regions
+// are separated by more than the window radius, so no window reaches another map.
+__global__ __launch_bounds__(kBlock) void k_shift_cells(int n, const int *__restrict__ map_of,
+                                                        const int *__restrict__ shift, int *__restrict__ cx,
+                                                        int *__restrict__ cy)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int m = map_of[i];
+    cx[i] += shift[2 * m];
+    cy[i] += shift[2 * m + 1];
+}
+
+// First output row of every map (rows are anchor-major, so map-major).
+__global__ void k_map_rows(int num_maps, const long long *__restrict__ eoff, const unsigned *__restrict__ row_off,
+                           unsigned *__restrict__ rows)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m <= num_maps) rows[m] = row_off[eoff[m]];
+}
 
 __global__ __launch_bounds__(kBlock) void k_cell_count(int n, const int *__restrict__ cx, const int *__restrict__ cy,
                                                        GridDesc g, unsigned *__restrict__ cnt)
@@ -1333,8 +1418,11 @@ FusedKernel fused_kernel(int cpl, int nch, bool gb, bool coop)
 }
 
 struct RecordArgs {
-    int n;
+    int pairs;
     const double *x, *y, *th;
+    const int *pair_anchor;
+    const int *map_of;        // per edge: its map (batches), or null
+    const long long *map_eoff;  // per map: first edge id
     const unsigned *off, *acc_count, *row_off;
     const int *st_ids;
     const unsigned *st_meta;
@@ -1349,57 +1437,57 @@ struct RecordArgs {
 
 // record_curvelet_chain + fill_curvelet_info (cpu_curvelet.hpp:197-280) for
 // every accepted staging row, written at its compacted position
-// (compact_curvelet_output, :282-303).
+// (compact_curvelet_output, :282-303).  One thread per staging slot q: the
+// anchor's k-th accepted row sits in slot off[a] + k.  Chain ids are written
+// 1-based and relative to the anchor's map (batches concatenate maps).
 __global__ __launch_bounds__(kBlock) void k_record(RecordArgs R)
 {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= R.n) return;
-    const unsigned cnt = R.acc_count[a];
-    if (cnt == 0) return;
-    const unsigned base = R.off[a];
-    const unsigned o = R.row_off[a];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= R.pairs) return;
+    const int a = R.pair_anchor[q];
+    const unsigned k = static_cast<unsigned>(q) - R.off[a];
+    if (k >= R.acc_count[a]) return;
+    const unsigned o = R.row_off[a] + k;
+    const int id0 = R.map_of ? static_cast<int>(R.map_eoff[R.map_of[a]]) : 0;
     const double ax = R.x[a], ay = R.y[a], at = R.th[a];
     const int W = R.G + 1;
-    for (unsigned k = 0; k < cnt; ++k) {
-        const long long q = static_cast<long long>(base) + k;
-        const unsigned meta = R.st_meta[q];
-        const int len = static_cast<int>(meta & 0xffu);
-        const bool fwd = (meta >> 8) & 1u;
-        const int c = static_cast<int>(meta >> 16);
-        const int *chain = R.st_ids + q * R.st_stride;
-        unsigned *row = R.chains + static_cast<long long>(o + k) * W;
-        row[0] = static_cast<unsigned>(a) + 1;
-        for (int l = 0; l < R.G; ++l) row[l + 1] = l < len ? static_cast<unsigned>(chain[l]) + 1 : 0u;
+    const unsigned meta = R.st_meta[q];
+    const int len = static_cast<int>(meta & 0xffu);
+    const bool fwd = (meta >> 8) & 1u;
+    const int c = static_cast<int>(meta >> 16);
+    const int *chain = R.st_ids + static_cast<long long>(q) * R.st_stride;
+    unsigned *row = R.chains + static_cast<long long>(o) * W;
+    row[0] = static_cast<unsigned>(a - id0) + 1;
+    for (int l = 0; l < R.G; ++l) row[l + 1] = l < len ? static_cast<unsigned>(chain[l] - id0) + 1 : 0u;
 
-        const double ox = R.off_x[c / R.kt];
-        const double ot = R.off_t[c % R.kt];
-        const double k_max = R.st_k[2 * q];
-        const double k_min = R.st_k[2 * q + 1];
-        const double kk = (k_max + k_min) / 2.0;
-        const double theta = cvg::wrap_two_pi(at + ot);
-        double s, cth;
-        cvg::cr_sincos(theta, c_sincos_tab, &s, &cth);
-        const double px = ax - ox * s;
-        const double py = ay + ox * cth;
-        double length = 0.0;
-        for (int l = 0; l + 1 < len; ++l) {
-            const int e1 = chain[l], e2 = chain[l + 1];
-            const double x1 = R.x[e1], y1 = R.y[e1], x2 = R.x[e2], y2 = R.y[e2];
-            length += sqrt((x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2));
-        }
-        const double quality = (length > 0.0 && len > 0) ? 2.0 / (R.nrad / length + length / double(len)) : 0.0;
-        double *inf = R.info + static_cast<long long>(o + k) * kInfoWidth;
-        inf[0] = fwd ? 1.0 : 0.0;
-        inf[1] = k_max;
-        inf[2] = k_min;
-        inf[3] = at;
-        inf[4] = px;
-        inf[5] = py;
-        inf[6] = theta;
-        inf[7] = kk;
-        inf[8] = length;
-        inf[9] = quality;
+    const double ox = R.off_x[c / R.kt];
+    const double ot = R.off_t[c % R.kt];
+    const double k_max = R.st_k[2 * static_cast<long long>(q)];
+    const double k_min = R.st_k[2 * static_cast<long long>(q) + 1];
+    const double kk = (k_max + k_min) / 2.0;
+    const double theta = cvg::wrap_two_pi(at + ot);
+    double s, cth;
+    cvg::cr_sincos(theta, c_sincos_tab, &s, &cth);
+    const double px = ax - ox * s;
+    const double py = ay + ox * cth;
+    double length = 0.0;
+    for (int l = 0; l + 1 < len; ++l) {
+        const int e1 = chain[l], e2 = chain[l + 1];
+        const double x1 = R.x[e1], y1 = R.y[e1], x2 = R.x[e2], y2 = R.y[e2];
+        length += sqrt((x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2));
     }
+    const double quality = (length > 0.0 && len > 0) ? 2.0 / (R.nrad / length + length / double(len)) : 0.0;
+    double *inf = R.info + static_cast<long long>(o) * kInfoWidth;
+    inf[0] = fwd ? 1.0 : 0.0;
+    inf[1] = k_max;
+    inf[2] = k_min;
+    inf[3] = at;
+    inf[4] = px;
+    inf[5] = py;
+    inf[6] = theta;
+    inf[7] = kk;
+    inf[8] = length;
+    inf[9] = quality;
 }
 
 // ---------------------------------------------------------------------------
@@ -1481,7 +1569,8 @@ int read_scalars(cvg_context *ctx)
     return CVG_OK;
 }
 
-int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double rad, unsigned R, unsigned max_cand)
+int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double rad, unsigned R, unsigned max_cand,
+                int num_maps = 1, const long long *eoff = nullptr)
 {
     ctx->graph_ready = false;
     ctx->curvelets_ready = false;
@@ -1489,6 +1578,14 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
         return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "build_neighbor_csr_graph: need num_edges > 0, sz_edge_data >= 4");
     if (R > 4096) return fail(ctx, CVG_ERR_UNSUPPORTED, "neighbor_radius above 4096 cells");
     if (max_cand == 0) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "max_candidates must be positive");
+    if (num_maps < 1) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "a batch needs at least one map");
+    ctx->num_maps = num_maps;
+    ctx->map_eoff.assign(static_cast<size_t>(num_maps) + 1, 0);
+    if (eoff) {
+        for (int m = 0; m <= num_maps; ++m) ctx->map_eoff[m] = eoff[m];
+    } else {
+        ctx->map_eoff[1] = n;
+    }
     ctx->n = n;
     ctx->sz = sz;
     ctx->rad = rad;
@@ -1512,8 +1609,66 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     if (mb[S_BAD])
         return fail(ctx, CVG_ERR_BAD_INPUT,
                     "edge values out of range: need finite x, y with |x|,|y| < 2^30 and |theta| < 2^20");
-    const int minx = static_cast<int>(mb[S_MINX]), maxx = static_cast<int>(mb[S_MAXX]);
-    const int miny = static_cast<int>(mb[S_MINY]), maxy = static_cast<int>(mb[S_MAXY]);
+    int minx = static_cast<int>(mb[S_MINX]), maxx = static_cast<int>(mb[S_MAXX]);
+    int miny = static_cast<int>(mb[S_MINY]), maxy = static_cast<int>(mb[S_MAXY]);
+    CVG_TRY(ensure(ctx, ctx->d_eoff, (static_cast<size_t>(num_maps) + 1) * sizeof(long long)));
+    CVG_HIP(ctx, hipMemcpyAsync(ctx->d_eoff.ptr, ctx->map_eoff.data(), (static_cast<size_t>(num_maps) + 1) *
+                                sizeof(long long), hipMemcpyHostToDevice, st));
+    if (num_maps > 1) {
+        // per-map boxes, then a shelf packing of the boxes into one grid
+        CVG_TRY(ensure(ctx, ctx->d_map, ni));
+        CVG_TRY(ensure(ctx, ctx->d_mapbox, static_cast<size_t>(num_maps) * 4 * sizeof(int)));
+        hipLaunchKernelGGL(k_map_ids, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, num_maps,
+                           as<long long>(ctx->d_eoff), as<int>(ctx->d_map));
+        hipLaunchKernelGGL(k_map_box_init, dim3(blocks_for(4LL * num_maps)), dim3(kBlock), 0, st, num_maps,
+                           as<int>(ctx->d_mapbox));
+        hipLaunchKernelGGL(k_map_box, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_map),
+                           as<int>(ctx->d_cx), as<int>(ctx->d_cy), as<int>(ctx->d_mapbox));
+        std::vector<int> box(static_cast<size_t>(num_maps) * 4);
+        CVG_HIP(ctx, hipMemcpyAsync(box.data(), ctx->d_mapbox.ptr, box.size() * sizeof(int), hipMemcpyDeviceToHost,
+                                    st));
+        CVG_HIP(ctx, hipStreamSynchronize(st));
+        std::vector<int> shift(static_cast<size_t>(num_maps) * 2, 0);
+        const long long gap = 2LL * R + 1;
+        long long shelf_w = 8192;
+        for (int m = 0; m < num_maps; ++m) {
+            if (box[4 * m] > box[4 * m + 1]) continue;  // empty map
+            shelf_w = std::max(shelf_w, static_cast<long long>(box[4 * m + 1]) - box[4 * m] + 1);
+        }
+        std::vector<int> order(num_maps);
+        for (int m = 0; m < num_maps; ++m) order[m] = m;
+        std::stable_sort(order.begin(), order.end(), [&](int p, int q) {
+            return static_cast<long long>(box[4 * p + 3]) - box[4 * p + 2] >
+                   static_cast<long long>(box[4 * q + 3]) - box[4 * q + 2];
+        });
+        long long cx0 = 0, cy0 = 0, shelf_h = 0, used_w = 0;
+        for (int m : order) {
+            if (box[4 * m] > box[4 * m + 1]) continue;
+            const long long w_m = static_cast<long long>(box[4 * m + 1]) - box[4 * m] + 1;
+            const long long h_m = static_cast<long long>(box[4 * m + 3]) - box[4 * m + 2] + 1;
+            if (cx0 > 0 && cx0 + w_m > shelf_w) {
+                cy0 += shelf_h + gap;
+                cx0 = 0;
+                shelf_h = 0;
+            }
+            if (cy0 + h_m > (1LL << 30) || cx0 + w_m > (1LL << 30))
+                return fail(ctx, CVG_ERR_UNSUPPORTED, "batch too large for one cell grid");
+            shift[2 * m] = static_cast<int>(cx0 - box[4 * m]);
+            shift[2 * m + 1] = static_cast<int>(cy0 - box[4 * m + 2]);
+            used_w = std::max(used_w, cx0 + w_m);
+            shelf_h = std::max(shelf_h, h_m);
+            cx0 += w_m + gap;
+        }
+        minx = 0;
+        miny = 0;
+        maxx = static_cast<int>(used_w - 1);
+        maxy = static_cast<int>(cy0 + shelf_h - 1);
+        CVG_HIP(ctx, hipMemcpyAsync(ctx->d_mapbox.ptr, shift.data(), shift.size() * sizeof(int),
+                                    hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_shift_cells, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_map),
+                           as<int>(ctx->d_mapbox), as<int>(ctx->d_cx), as<int>(ctx->d_cy));
+        CVG_HIP(ctx, hipGetLastError());
+    }
     GridDesc g;
     g.x0 = minx - static_cast<int>(R);
     g.y0 = miny - static_cast<int>(R);
@@ -1879,11 +2034,26 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
                          (static_cast<unsigned long long>(ctx->mailbox[S_COUNT + 1]) << 32);
     ctx->num_curvelets = C;
     ctx->width = d.G + 1;
+    ctx->map_rows.assign(2, 0);
+    ctx->map_rows[1] = C;
+    if (ctx->num_maps > 1) {
+        const int M = ctx->num_maps;
+        CVG_TRY(ensure(ctx, ctx->d_maprow, (static_cast<size_t>(M) + 1) * sizeof(unsigned)));
+        hipLaunchKernelGGL(k_map_rows, dim3(blocks_for(M + 1LL)), dim3(kBlock), 0, st, M, as<long long>(ctx->d_eoff),
+                           as<unsigned>(ctx->d_rowoff), as<unsigned>(ctx->d_maprow));
+        ctx->map_rows.assign(static_cast<size_t>(M) + 1, 0);
+        CVG_HIP(ctx, hipMemcpyAsync(ctx->map_rows.data(), ctx->d_maprow.ptr, (static_cast<size_t>(M) + 1) *
+                                    sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        CVG_HIP(ctx, hipStreamSynchronize(st));
+    }
 
     CVG_TRY(ensure(ctx, ctx->d_chains, static_cast<size_t>(C) * ctx->width * sizeof(unsigned)));
     CVG_TRY(ensure(ctx, ctx->d_info, static_cast<size_t>(C) * kInfoWidth * sizeof(double)));
     RecordArgs R{};
-    R.n = n;
+    R.pairs = pairs;
+    R.pair_anchor = as<int>(ctx->d_pair_anchor);
+    R.map_of = ctx->num_maps > 1 ? as<int>(ctx->d_map) : nullptr;
+    R.map_eoff = as<long long>(ctx->d_eoff);
     R.x = as<double>(ctx->d_x);
     R.y = as<double>(ctx->d_y);
     R.th = as<double>(ctx->d_th);
@@ -1901,7 +2071,7 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     R.off_t = T.off_t;
     R.chains = as<unsigned>(ctx->d_chains);
     R.info = as<double>(ctx->d_info);
-    if (C > 0) hipLaunchKernelGGL(k_record, dim3(blocks_for(n)), dim3(kBlock), 0, st, R);
+    if (C > 0) hipLaunchKernelGGL(k_record, dim3(blocks_for(pairs)), dim3(kBlock), 0, st, R);
     CVG_HIP(ctx, hipGetLastError());
     ctx->cur_params = p;
     ctx->curvelets_ready = true;
@@ -1921,10 +2091,11 @@ int check_ctx(cvg_context *ctx)
     return CVG_OK;
 }
 
-int run_common(cvg_context *ctx, int n, int sz, const double *d_edges, const cvg_params *p, cvg_run_stats *stats)
+int run_common(cvg_context *ctx, int n, int sz, const double *d_edges, const cvg_params *p, cvg_run_stats *stats,
+               int num_maps = 1, const long long *eoff = nullptr)
 {
     CVG_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-    CVG_TRY(build_graph(ctx, n, sz, d_edges, p->nrad, 3u, p->max_candidates));  // main.cpp:48 passes 3
+    CVG_TRY(build_graph(ctx, n, sz, d_edges, p->nrad, 3u, p->max_candidates, num_maps, eoff));  // main.cpp:48: 3
     CVG_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     CVG_TRY(build_curvelets(ctx, *p));
     CVG_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
@@ -2063,7 +2234,8 @@ void cvg_destroy(cvg_context *ctx)
                       &ctx->d_cand,     &ctx->d_ids,   &ctx->d_d2,      &ctx->d_pair_anchor, &ctx->d_trig,
                       &ctx->d_cls,      &ctx->d_prec,  &ctx->d_psin,  &ctx->d_cnt3,  &ctx->d_lists,   &ctx->d_tables,  &ctx->d_st_ids,
                       &ctx->d_st_meta,  &ctx->d_st_k,  &ctx->d_acc,     &ctx->d_rowoff,  &ctx->d_gbundle,
-                      &ctx->d_chains,   &ctx->d_info,  &ctx->d_dbg};
+                      &ctx->d_chains,   &ctx->d_info,  &ctx->d_dbg,     &ctx->d_map,     &ctx->d_eoff,
+                      &ctx->d_mapbox,   &ctx->d_maprow};
     for (DevBuf *b : bufs) release(*b);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -2190,6 +2362,80 @@ int cvg_run_device(cvg_context *ctx, int num_edges, int sz_edge_data, const doub
     if (!params || !device_edges || num_edges <= 0 || sz_edge_data < 4)
         return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "cvg_run_device: bad arguments");
     return run_common(ctx, num_edges, sz_edge_data, device_edges, params, stats);
+}
+
+// Batches: maps concatenated in edge order; eoff[m] = first edge of map m.
+static int batch_offsets(cvg_context *ctx, int num_maps, const int *num_edges, std::vector<long long> &eoff)
+{
+    if (num_maps <= 0 || !num_edges) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "batch: need num_maps > 0");
+    eoff.assign(static_cast<size_t>(num_maps) + 1, 0);
+    for (int m = 0; m < num_maps; ++m) {
+        if (num_edges[m] < 0) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "batch: negative edge count");
+        eoff[m + 1] = eoff[m] + num_edges[m];
+    }
+    if (eoff[num_maps] <= 0) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "batch: no edges");
+    if (eoff[num_maps] > INT_MAX) return fail(ctx, CVG_ERR_UNSUPPORTED, "batch: more than 2^31-1 edges");
+    return CVG_OK;
+}
+
+int cvg_run_batch(cvg_context *ctx, int num_maps, const int *num_edges, int sz_edge_data, const double *const *edges,
+                  const cvg_params *params, cvg_run_stats *stats)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!params || !edges || sz_edge_data < 4) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "cvg_run_batch: bad arguments");
+    std::vector<long long> eoff;
+    CVG_TRY(batch_offsets(ctx, num_maps, num_edges, eoff));
+    const int n = static_cast<int>(eoff[num_maps]);
+    CVG_TRY(ensure(ctx, ctx->d_edges_up, static_cast<size_t>(n) * sz_edge_data * sizeof(double)));
+    double *d = as<double>(ctx->d_edges_up);
+    for (int m = 0; m < num_maps; ++m) {
+        if (num_edges[m] == 0) continue;
+        if (!edges[m]) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "cvg_run_batch: null edge array");
+        CVG_HIP(ctx, hipMemcpyAsync(d + eoff[m] * sz_edge_data, edges[m],
+                                    static_cast<size_t>(num_edges[m]) * sz_edge_data * sizeof(double),
+                                    hipMemcpyHostToDevice, ctx->stream));
+    }
+    return run_common(ctx, n, sz_edge_data, d, params, stats, num_maps, eoff.data());
+}
+
+int cvg_run_batch_device(cvg_context *ctx, int num_maps, const int *num_edges, int sz_edge_data,
+                         const double *device_edges, const cvg_params *params, cvg_run_stats *stats)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!params || !device_edges || sz_edge_data < 4)
+        return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "cvg_run_batch_device: bad arguments");
+    std::vector<long long> eoff;
+    CVG_TRY(batch_offsets(ctx, num_maps, num_edges, eoff));
+    return run_common(ctx, static_cast<int>(eoff[num_maps]), sz_edge_data, device_edges, params, stats, num_maps,
+                      eoff.data());
+}
+
+int cvg_batch_map_shape(cvg_context *ctx, int map, unsigned *num_curvelets, unsigned *chain_width)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!ctx->curvelets_ready) return fail(ctx, CVG_ERR_STATE, "no curvelets");
+    if (map < 0 || map >= static_cast<int>(ctx->map_rows.size()) - 1)
+        return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "map index out of range");
+    if (num_curvelets) *num_curvelets = ctx->map_rows[map + 1] - ctx->map_rows[map];
+    if (chain_width) *chain_width = ctx->width;
+    return CVG_OK;
+}
+
+int cvg_get_batch_map_curvelets(cvg_context *ctx, int map, unsigned *chains, double *info)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!ctx->curvelets_ready) return fail(ctx, CVG_ERR_STATE, "no curvelets");
+    if (map < 0 || map >= static_cast<int>(ctx->map_rows.size()) - 1)
+        return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "map index out of range");
+    const size_t r0 = ctx->map_rows[map], rows = ctx->map_rows[map + 1] - r0;
+    if (rows > 0 && chains)
+        CVG_HIP(ctx, hipMemcpyAsync(chains, as<unsigned>(ctx->d_chains) + r0 * ctx->width,
+                                    rows * ctx->width * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+    if (rows > 0 && info)
+        CVG_HIP(ctx, hipMemcpyAsync(info, as<double>(ctx->d_info) + r0 * kInfoWidth, rows * kInfoWidth * sizeof(double),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+    CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return CVG_OK;
 }
 
 int cvg_upload_edges(cvg_context *ctx, int num_edges, int sz_edge_data, const double *edges,

@@ -163,6 +163,23 @@ int cvg_device_curvelets(cvg_context *ctx, const unsigned **device_chains, const
 int cvg_get_stream(cvg_context *ctx, void **hip_stream);
 int cvg_synchronize(cvg_context *ctx);
 
+/* ---- batched form ---------------------------------------------------------
+ * Several maps in one pass: every kernel of the path runs once over all of
+ * them (each map's cells occupy their own region of one cell grid, so no
+ * neighbour window crosses maps; anchors of all maps share the fused kernel's
+ * launches).  Each map's chains and info are bit-identical to what cvg_run
+ * gives for that map alone; chain ids are 1-based within their map.  The
+ * reference has no batch interface: these stand in for a loop of
+ * run_curvelet<double> calls (cpu_curvelet/main.cpp:20-96), one per map. */
+int cvg_run_batch(cvg_context *ctx, int num_maps, const int *num_edges, int sz_edge_data,
+                  const double *const *edges, const cvg_params *params, cvg_run_stats *stats);
+/* Same, with the maps already concatenated (map-major) in device memory. */
+int cvg_run_batch_device(cvg_context *ctx, int num_maps, const int *num_edges, int sz_edge_data,
+                         const double *device_edges, const cvg_params *params, cvg_run_stats *stats);
+/* One map's output shape (num_curvelets(), chain_width()) and rows. */
+int cvg_batch_map_shape(cvg_context *ctx, int map, unsigned *num_curvelets, unsigned *chain_width);
+int cvg_get_batch_map_curvelets(cvg_context *ctx, int map, unsigned *chains, double *info);
+
 /* ---- synthetic third-order edge maps ------------------------------------ */
 
 /* Seeded generator of edge maps in the test_files format (x, y, theta,

@@ -1,0 +1,81 @@
+"""The batched form: several maps through one pass of the path.  Each map's
+chains and info must be the same bits alone and in the mix (the reference's
+semantics are per map; a batch only shares launches)."""
+import numpy as np
+import pytest
+
+from curvelet_construction_on_gpu_amd import CurveletParams, synth_edges
+from tests._inputs import edges_for
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint64)
+
+
+def lattice(nx, ny, seed):
+    rng = np.random.default_rng(seed)
+    xs, ys = np.meshgrid(np.arange(nx, dtype=float), np.arange(ny, dtype=float))
+    return np.stack([xs.ravel(), ys.ravel(), rng.uniform(0, np.pi, xs.size), np.ones(xs.size)], axis=1)
+
+
+def mixed_maps():
+    c1 = synth_edges("C1", 1001)
+    shifted = np.array(edges_for("synth:C2:1002")[:5000])
+    shifted[:, 0] -= 300.0   # negative coordinates overlap the other maps in x/y space
+    shifted[:, 1] -= 120.0
+    return [c1, synth_edges("C1", 1002), np.array(edges_for("file:TO_edges_digit1")), shifted,
+            lattice(30, 20, 5), np.zeros((0, 4)), c1[:1].copy(), synth_edges("C1", 1003)[:2000],
+            np.array(edges_for("file:TO_edges_ABC_0006_thresh1"))]
+
+
+@pytest.mark.parametrize("G", [4, 7])
+def test_batch_equals_alone(ctx, G):
+    maps = mixed_maps()
+    p = CurveletParams(group_max_sz=G)
+    alone = []
+    for m in maps:
+        if len(m) == 0:
+            alone.append((np.zeros((0, G + 1), np.uint32), np.zeros((0, 10))))
+            continue
+        ctx.run(m, p)
+        alone.append(ctx.curvelets())
+    st = ctx.run_batch(maps, p)
+    total = 0
+    for i, (ch, inf) in enumerate(alone):
+        bc, bi = ctx.batch_curvelets(i)
+        assert bc.shape == ch.shape, i
+        assert np.array_equal(bc, ch), i
+        assert np.array_equal(bits(bi), bits(inf)), i
+        total += len(ch)
+    assert st["num_curvelets"] == total
+
+
+def test_batch_matches_oracle_and_counts(ctx, oracle_cr):
+    maps = [synth_edges("C1", 1004), np.array(edges_for("file:TO_edges_digit1")), synth_edges("C2", 1002)[:3000]]
+    p = CurveletParams(group_max_sz=5)
+    st = ctx.run_batch(maps, p)
+    vp = 0
+    for i, m in enumerate(maps):
+        want = oracle_cr.run(m, p, nthreads=0)
+        ch, inf = ctx.batch_curvelets(i)
+        assert np.array_equal(ch, want["chains"])
+        assert np.array_equal(bits(inf), bits(want["info"]))
+        vp += want["valid_pairs"]
+    assert st["valid_pairs"] == vp
+
+
+def test_batch_of_one_and_repeat(ctx):
+    e = synth_edges("C2", 1001)
+    p = CurveletParams(group_max_sz=6)
+    ctx.run(e, p)
+    want = ctx.curvelets()
+    for _ in range(2):
+        ctx.run_batch([e], p)
+        got = ctx.batch_curvelets(0)
+        assert np.array_equal(got[0], want[0]) and np.array_equal(bits(got[1]), bits(want[1]))
+    # the single-map path after a batch is unaffected
+    ctx.run(e, p)
+    again = ctx.curvelets()
+    assert np.array_equal(again[0], want[0]) and np.array_equal(bits(again[1]), bits(want[1]))
