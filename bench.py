@@ -76,12 +76,191 @@ def cpu_baseline(edges, params):
         return {"error": repr(exc)}
 
 
+BATCH_SIZES = [(320, 240), (640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
+
+
+def batch_maps(count=64, seed=4001):
+    """64 synthetic maps: sizes uniform over BATCH_SIZES, clutter U[0, 0.4],
+    curves and junction patches in proportion to area (the 1080p map's
+    generator: 1500 curves and 40 patches per 1920x1080)."""
+    import numpy as np
+    from curvelet_construction_on_gpu_amd import synth_edges
+    rng = np.random.default_rng(seed)
+    maps, sizes = [], []
+    for m in range(count):
+        w, h = BATCH_SIZES[rng.integers(len(BATCH_SIZES))]
+        area = w * h / (1920 * 1080)
+        maps.append(synth_edges("C3", seed * 1000 + m, width=w, height=h, num_curves=max(1, round(1500 * area)),
+                                junction_patches=round(40 * area), clutter_fraction=float(rng.uniform(0, 0.4))))
+        sizes.append(f"{w}x{h}")
+    return maps, sizes
+
+
+def bench_batch(args) -> int:
+    """maps/s of the batched form (one pass over all 64 maps) against the plain
+    loop (one cvg_run_device per map), inputs resident in HBM for both."""
+    import numpy as np
+    from curvelet_construction_on_gpu_amd import Context, CurveletParams
+    maps, sizes = batch_maps()
+    params = CurveletParams()
+    counts = [len(m) for m in maps]
+    ctx = Context(0)
+    allm = np.concatenate(maps)
+    dall = ctx.upload(allm)
+    eoff = np.concatenate([[0], np.cumsum(counts)])
+    step = 8 * allm.shape[1]
+    for _ in range(max(1, args.warmup // 2)):
+        ctx.run_batch_device(dall, counts, 4, params)
+        for m in range(len(maps)):
+            ctx.run_device(dall + int(eoff[m]) * step, counts[m], 4, params)
+    ctx.synchronize()
+    reps = max(1, args.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        st = ctx.run_batch_device(dall, counts, 4, params)
+    ctx.synchronize()
+    t_batch = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for m in range(len(maps)):
+            ctx.run_device(dall + int(eoff[m]) * step, counts[m], 4, params)
+    ctx.synchronize()
+    t_plain = (time.perf_counter() - t0) / reps
+    # the batch's rows of a few maps against their plain-path rows
+    same = True
+    ctx.run_batch_device(dall, counts, 4, params)
+    got = {m: ctx.batch_curvelets(m) for m in (0, 1, len(maps) - 1)}
+    for m, (ch, inf) in got.items():
+        ctx.run_device(dall + int(eoff[m]) * step, counts[m], 4, params)
+        c2, i2 = ctx.curvelets()
+        same = same and np.array_equal(ch, c2) and np.array_equal(inf.view(np.uint64), i2.view(np.uint64))
+    ctx.close()
+    n = int(eoff[-1])
+    line = {"metric": "maps/s, 64-map batch (sizes 320x240..3840x2160)", "value": len(maps) / t_batch,
+            "unit": "maps/s", "n_gpus": 1, "steps": reps, "warmup": max(1, args.warmup // 2),
+            "ms_per_step": t_batch * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "64 synthetic maps, sizes uniform over 320x240/640x480/1280x720/1920x1080/"
+                                   "3840x2160, clutter U[0,0.4], curves and junctions per area, defaults (G=4)",
+                       "edges": n, "maps": len(maps), "sizes": {k: sizes.count(k) for k in sorted(set(sizes))},
+                       "neighbor_pairs": st["total_neighbor_pairs"], "curvelets": st["num_curvelets"]},
+            "plain_loop_maps_per_s": len(maps) / t_plain, "batch_speedup_vs_plain_loop": t_plain / t_batch,
+            "batch_edges_per_s": n / t_batch, "plain_loop_edges_per_s": n / t_plain,
+            "sampled_maps_identical": bool(same)}
+    print(json.dumps(line))
+    return 0
+
+
+def serve_pool(count=96, seed=5001):
+    """Request maps: area log-uniform between 320x240 and 3840x2160 (16:9),
+    curves and junction patches per area, clutter U[0, 0.4]."""
+    import math
+    import numpy as np
+    from curvelet_construction_on_gpu_amd import synth_edges
+    rng = np.random.default_rng(seed)
+    pool, areas = [], []
+    lo, hi = math.log(320 * 240), math.log(3840 * 2160)
+    for m in range(count):
+        area = math.exp(rng.uniform(lo, hi))
+        w = max(32, int(round(math.sqrt(area * 16 / 9))))
+        h = max(32, int(round(area / w)))
+        a = w * h / (1920 * 1080)
+        pool.append(synth_edges("C3", seed * 1000 + m, width=w, height=h, num_curves=max(1, round(1500 * a)),
+                                junction_patches=round(40 * a), clutter_fraction=float(rng.uniform(0, 0.4))))
+        areas.append(w * h)
+    return pool, areas
+
+
+def bench_serve(args) -> int:
+    """The served form: (1) saturation maps/s of continuous batching against
+    the plain one-map-at-a-time loop (the same server with one worker and
+    batches of one) on 400 requests submitted at once; (2) a Poisson stream at
+    200 maps/s (2,000 requests after 100 warm-up ones): p50/p99 latency from
+    submit to results in host memory, overall and for 1080p-class maps."""
+    import threading
+    import numpy as np
+    from curvelet_construction_on_gpu_amd import CurveletParams, Server
+    pool, areas = serve_pool()
+    params = CurveletParams()
+    rng = np.random.default_rng(6001)
+
+    def saturate(srv, nreq):
+        picks = rng.integers(len(pool), size=nreq)
+        t0 = time.perf_counter()
+        ids = [srv.submit(pool[k]) for k in picks]
+        for i in ids:
+            srv.result(i)
+        return nreq / (time.perf_counter() - t0), int(sum(len(pool[k]) for k in picks))
+
+    out = {}
+    with Server(params, workers=1, max_batch_maps=1) as plain:
+        saturate(plain, 20)
+        out["plain_loop_maps_per_s"], _ = saturate(plain, 200)
+    with Server(params) as srv:
+        saturate(srv, 40)
+        sat, sat_edges = saturate(srv, 400)
+        out["saturation_maps_per_s"] = sat
+        cfg = srv.config
+        # Poisson stream
+        rate, nwarm, nreq = 200.0, 100, 2000
+        gaps = rng.exponential(1.0 / rate, size=nwarm + nreq)
+        picks = rng.integers(len(pool), size=nwarm + nreq)
+        ids = [None] * (nwarm + nreq)
+        t_start = time.perf_counter()
+
+        def client():
+            t = t_start
+            for k in range(nwarm + nreq):
+                t += gaps[k]
+                d = t - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                ids[k] = srv.submit(pool[picks[k]])
+
+        th = threading.Thread(target=client)
+        th.start()
+        lat, lat_1080, batch_sizes = [], [], []
+        for k in range(nwarm + nreq):
+            while ids[k] is None:
+                time.sleep(0.0002)
+            ch, inf, r = srv.result(ids[k])
+            if k >= nwarm:
+                lat.append(r["latency_ms"])
+                batch_sizes.append(r["batch_maps"])
+                if 0.65 <= areas[picks[k]] / (1920 * 1080) <= 1.35:
+                    lat_1080.append(r["latency_ms"])
+            if k == nwarm:
+                t_first = time.perf_counter()
+        th.join()
+        t_end = time.perf_counter()
+    stream_rate = nreq / (t_end - t_first)
+    line = {"metric": "maps/s (served, continuous batching, saturation)", "value": out["saturation_maps_per_s"],
+            "unit": "maps/s", "n_gpus": 1, "steps": nreq, "warmup": nwarm, "ms_per_step": 1e3 / out["saturation_maps_per_s"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "request maps with area log-uniform 320x240..3840x2160, curves/junctions per "
+                                   "area, clutter U[0,0.4], defaults (G=4); pool of 96 seeded maps",
+                       "server": cfg, "saturation_requests": 400, "saturation_edges": sat_edges},
+            "plain_loop_maps_per_s": out["plain_loop_maps_per_s"],
+            "speedup_vs_plain_loop": out["saturation_maps_per_s"] / out["plain_loop_maps_per_s"],
+            "poisson": {"rate_maps_per_s": rate, "requests": nreq, "sustained_maps_per_s": stream_rate,
+                        "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99)),
+                        "latency_ms_p50_1080p": float(np.percentile(lat_1080, 50)) if lat_1080 else None,
+                        "latency_ms_p99_1080p": float(np.percentile(lat_1080, 99)) if lat_1080 else None,
+                        "requests_1080p": len(lat_1080), "mean_batch_maps": float(np.mean(batch_sizes))}}
+    print(json.dumps(line))
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["map", "batch", "serve"], default="map",
+                    help="map: the 1080p map (the headline line); batch: 64 mixed-size maps, batched "
+                         "form against the plain per-map loop; serve: a Poisson request stream through "
+                         "the served form (continuous batching), latency and maps/s")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -105,6 +284,10 @@ def main(argv=None) -> int:
         return 2
 
     from curvelet_construction_on_gpu_amd import Context, CurveletParams, synth_edges
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     edges = synth_edges(WORKLOAD["preset"], WORKLOAD["seed"])
     params = CurveletParams(group_max_sz=WORKLOAD["group_max_sz"])

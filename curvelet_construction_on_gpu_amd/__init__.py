@@ -63,6 +63,17 @@ class _RunStats(C.Structure):
                 ("lds_bytes_per_workgroup", C.c_uint)]
 
 
+class _ServerConfig(C.Structure):
+    _fields_ = [("workers", C.c_int), ("max_batch_maps", C.c_int), ("bytes_per_edge", C.c_double),
+                ("hbm_budget_bytes", C.c_double)]
+
+
+class _ServerResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("num_curvelets", C.c_uint), ("chain_width", C.c_uint),
+                ("num_edges", C.c_int), ("batch_maps", C.c_int), ("queue_ms", C.c_double),
+                ("latency_ms", C.c_double)]
+
+
 class _SynthConfig(C.Structure):
     _fields_ = [("width", C.c_int), ("height", C.c_int), ("num_curves", C.c_int), ("radius_min", C.c_double),
                 ("radius_max", C.c_double), ("radius_log_uniform", C.c_int), ("segment_min", C.c_double),
@@ -104,6 +115,14 @@ _SIGNATURES = {
     "cvg_run_batch_device": (C.c_int, [C.c_void_p, C.c_int, _ip, C.c_int, C.c_void_p, _P(_Params), _P(_RunStats)]),
     "cvg_batch_map_shape": (C.c_int, [C.c_void_p, C.c_int, _up, _up]),
     "cvg_get_batch_map_curvelets": (C.c_int, [C.c_void_p, C.c_int, _up, _dp]),
+    "cvg_server_config_default": (None, [_P(_ServerConfig)]),
+    "cvg_server_create": (C.c_int, [C.c_int, _P(_Params), _P(_ServerConfig), _P(C.c_void_p)]),
+    "cvg_server_submit": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _P(C.c_ulonglong)]),
+    "cvg_server_wait": (C.c_int, [C.c_void_p, C.c_ulonglong, _P(_ServerResult)]),
+    "cvg_server_fetch": (C.c_int, [C.c_void_p, C.c_ulonglong, _up, _dp]),
+    "cvg_server_counters": (C.c_int, [C.c_void_p, _P(C.c_ulonglong), _P(C.c_ulonglong), _P(C.c_ulonglong)]),
+    "cvg_server_last_error": (C.c_char_p, [C.c_void_p]),
+    "cvg_server_destroy": (None, [C.c_void_p]),
     "cvg_synth_preset": (C.c_int, [C.c_char_p, _P(_SynthConfig)]),
     "cvg_synth_generate": (C.c_int, [_P(_SynthConfig), C.c_ulonglong, _dp, C.c_int, _ip]),
     "cvg_debug_anchor_bundles": (C.c_int, [C.c_void_p, _P(_Params), C.c_int, _ucp, _ucp, _dp, _dp, _ip, _ip]),
@@ -335,6 +354,78 @@ class Context:
                    "debug_anchor_bundles")
         d, k = deg.value, cells.value
         return dir_[:d], hyp[:d], bmin[:d * k].reshape(d, k), bmax[:d * k].reshape(d, k)
+
+
+class Server:
+    """The served form: continuous batching over worker contexts (cvg_server_*)."""
+
+    def __init__(self, params: CurveletParams | None = None, device: int = 0, workers: int | None = None,
+                 max_batch_maps: int | None = None, hbm_budget_bytes: float | None = None,
+                 bytes_per_edge: float | None = None):
+        self._lib = load_library()
+        cfg = _ServerConfig()
+        self._lib.cvg_server_config_default(C.byref(cfg))
+        if workers is not None:
+            cfg.workers = workers
+        if max_batch_maps is not None:
+            cfg.max_batch_maps = max_batch_maps
+        if hbm_budget_bytes is not None:
+            cfg.hbm_budget_bytes = hbm_budget_bytes
+        if bytes_per_edge is not None:
+            cfg.bytes_per_edge = bytes_per_edge
+        self.config = {k: getattr(cfg, k) for k, _ in _ServerConfig._fields_}
+        h = C.c_void_p()
+        p = (params or CurveletParams())._c()
+        rc = self._lib.cvg_server_create(device, C.byref(p), C.byref(cfg), C.byref(h))
+        if rc != 0:
+            raise CurveletError(f"cvg_server_create failed ({rc})", rc)
+        self._h = h
+
+    def submit(self, edges: np.ndarray) -> int:
+        e = np.ascontiguousarray(edges, dtype=np.float64)
+        rid = C.c_ulonglong()
+        rc = self._lib.cvg_server_submit(self._h, e.shape[0], e.shape[1], _ptr(e, C.c_double), C.byref(rid))
+        if rc != 0:
+            raise CurveletError(f"cvg_server_submit failed ({rc})", rc)
+        return rid.value
+
+    def wait(self, rid: int) -> dict:
+        res = _ServerResult()
+        rc = self._lib.cvg_server_wait(self._h, rid, C.byref(res))
+        if rc != 0:
+            raise CurveletError(f"request {rid} failed: {self._lib.cvg_server_last_error(self._h).decode()}", rc)
+        return {k: getattr(res, k) for k, _ in _ServerResult._fields_}
+
+    def result(self, rid: int) -> tuple[np.ndarray, np.ndarray, dict]:
+        r = self.wait(rid)
+        chains = np.zeros((r["num_curvelets"], r["chain_width"]), np.uint32)
+        info = np.zeros((r["num_curvelets"], 10))
+        rc = self._lib.cvg_server_fetch(self._h, rid, _ptr(chains, C.c_uint), _ptr(info, C.c_double))
+        if rc != 0:
+            raise CurveletError(f"cvg_server_fetch failed ({rc})", rc)
+        return chains, info, r
+
+    def counters(self) -> dict:
+        b, m, e = C.c_ulonglong(), C.c_ulonglong(), C.c_ulonglong()
+        self._lib.cvg_server_counters(self._h, C.byref(b), C.byref(m), C.byref(e))
+        return {"batches": b.value, "maps": m.value, "edges": e.value}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.cvg_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 _default_ctx: Context | None = None

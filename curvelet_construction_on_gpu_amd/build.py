@@ -51,9 +51,9 @@ def build_lib() -> Path:
         obj = objdir / (Path(src).stem + ".o")
         run([HIPCC, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(obj)
-    for src in ["cvg_io.cpp", "cvg_synth.cpp"]:
+    for src in ["cvg_io.cpp", "cvg_synth.cpp", "cvg_server.cpp"]:
         obj = objdir / (Path(src).stem + ".o")
-        run([CXX, "-O2", "-std=c++17", "-fPIC", "-Wall", "-c", str(CSRC / src), "-o", str(obj)])
+        run([CXX, "-O2", "-std=c++17", "-fPIC", "-Wall", "-pthread", "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(obj)
     tmp = LIB.with_suffix(".so.tmp")
     run([HIPCC, "-shared", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp)])

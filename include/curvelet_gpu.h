@@ -180,6 +180,48 @@ int cvg_run_batch_device(cvg_context *ctx, int num_maps, const int *num_edges, i
 int cvg_batch_map_shape(cvg_context *ctx, int map, unsigned *num_curvelets, unsigned *chain_width);
 int cvg_get_batch_map_curvelets(cvg_context *ctx, int map, unsigned *chains, double *info);
 
+/* ---- served form ----------------------------------------------------------
+ * A request loop with continuous batching.  Requests (one edge map each) queue
+ * as they arrive; each of `workers` threads owns a context (streams and device
+ * pools resident across batches) and, whenever free, admits the oldest
+ * pending requests while their estimated HBM footprint (num_edges x
+ * bytes_per_edge) fits hbm_budget_bytes, at most max_batch_maps, and runs them
+ * as one batch.  Each result equals cvg_run on that map alone.  The reference
+ * has no serving layer: this is the loop a service would run around
+ * run_curvelet<double> (cpu_curvelet/main.cpp:20-96). */
+typedef struct cvg_server cvg_server;
+typedef struct cvg_server_config {
+    int workers;             /* contexts (and host threads) running batches concurrently, 1..8 */
+    int max_batch_maps;      /* maps per batch; 1 gives the plain one-map-at-a-time loop */
+    double bytes_per_edge;   /* HBM footprint estimate per edge (edge arrays + neighbours x per-pair buffers) */
+    double hbm_budget_bytes; /* per batch */
+} cvg_server_config;
+typedef struct cvg_server_result {
+    int status;
+    unsigned num_curvelets;
+    unsigned chain_width;
+    int num_edges;
+    int batch_maps;          /* maps in the batch that carried this request */
+    double queue_ms;         /* submit -> batch start */
+    double latency_ms;       /* submit -> results in host memory */
+} cvg_server_result;
+
+void cvg_server_config_default(cvg_server_config *config);
+int cvg_server_create(int device, const cvg_params *params, const cvg_server_config *config, cvg_server **out);
+/* Copies the edges; returns at once with the request's id. */
+int cvg_server_submit(cvg_server *server, int num_edges, int sz_edge_data, const double *edges,
+                      unsigned long long *request_id);
+/* Blocks until the request is done; fills its shape and timings. */
+int cvg_server_wait(cvg_server *server, unsigned long long request_id, cvg_server_result *result);
+/* Copies a finished request's chains (num_curvelets x chain_width, 1-based ids)
+ * and info (num_curvelets x 10) and releases it. */
+int cvg_server_fetch(cvg_server *server, unsigned long long request_id, unsigned *chains, double *info);
+int cvg_server_counters(cvg_server *server, unsigned long long *batches, unsigned long long *maps,
+                        unsigned long long *edges);
+const char *cvg_server_last_error(cvg_server *server);
+/* Finishes the queued requests, then stops the workers. */
+void cvg_server_destroy(cvg_server *server);
+
 /* ---- synthetic third-order edge maps ------------------------------------ */
 
 /* Seeded generator of edge maps in the test_files format (x, y, theta,

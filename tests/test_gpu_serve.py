@@ -1,0 +1,51 @@
+"""The served form: requests arriving while others run are batched together
+(continuous batching); every result must be the same bits alone and in the mix."""
+import numpy as np
+import pytest
+
+from curvelet_construction_on_gpu_amd import CurveletParams, Server, synth_edges
+from tests._inputs import edges_for
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint64)
+
+
+@pytest.mark.parametrize("workers", [1, 2])
+def test_served_results_equal_alone(ctx, workers):
+    p = CurveletParams(group_max_sz=5)
+    big = synth_edges("C3", 1003)
+    maps = [big] + [synth_edges("C1", 1000 + s) for s in range(8)] + [
+        np.array(edges_for("file:TO_edges_digit1")), np.array(edges_for("synth:C2:1002")[:4000]),
+        np.array(edges_for("file:TO_edges_ABC_0006_thresh1")), synth_edges("C1", 1001)[:1].copy()]
+    alone = []
+    for m in maps:
+        ctx.run(m, p)
+        alone.append(ctx.curvelets())
+    with Server(p, workers=workers, max_batch_maps=16) as srv:
+        ids = [srv.submit(m) for m in maps]   # the big map first keeps a worker busy while the rest queue
+        res = [srv.result(i) for i in ids]
+        counters = srv.counters()
+    for (ch, inf, r), (ach, ainf) in zip(res, alone):
+        assert r["status"] == 0
+        assert np.array_equal(ch, ach)
+        assert np.array_equal(bits(inf), bits(ainf))
+    assert counters["maps"] == len(maps)
+    assert max(r["batch_maps"] for _, _, r in res) > 1  # some requests really were served in a mix
+    assert counters["batches"] < len(maps)
+
+
+def test_plain_loop_configuration(ctx):
+    p = CurveletParams()
+    maps = [synth_edges("C1", 1100 + s) for s in range(4)]
+    with Server(p, workers=1, max_batch_maps=1) as srv:
+        ids = [srv.submit(m) for m in maps]
+        res = [srv.result(i) for i in ids]
+        assert srv.counters()["batches"] == len(maps)
+    for m, (ch, inf, r) in zip(maps, res):
+        assert r["batch_maps"] == 1
+        ctx.run(m, p)
+        a, b = ctx.curvelets()
+        assert np.array_equal(ch, a) and np.array_equal(bits(inf), bits(b))

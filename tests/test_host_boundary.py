@@ -283,3 +283,18 @@ def test_cli_errors(workdir, args, code, needle):
 def test_cli_float_rejected(workdir):
     r = run_cli(CLI, ["--float", "--edge-file", "bad.txt"], workdir)
     assert r.returncode == 1 and "--float" in r.stderr
+
+
+def test_server_rejects_bad_config_without_a_device():
+    lib = cvg.load_library()
+    cfg = cvg._ServerConfig()
+    lib.cvg_server_config_default(C.byref(cfg))
+    assert cfg.workers >= 1 and cfg.max_batch_maps >= 1 and cfg.hbm_budget_bytes > 0
+    p = cvg._Params()
+    lib.cvg_params_default(C.byref(p))
+    out = C.c_void_p()
+    assert lib.cvg_server_create(0, None, C.byref(cfg), C.byref(out)) == 1
+    bad = cvg._ServerConfig(0, 1, 1.0, 1.0)
+    assert lib.cvg_server_create(0, C.byref(p), C.byref(bad), C.byref(out)) == 1
+    assert lib.cvg_server_submit(None, 1, 4, None, None) == 1
+    lib.cvg_server_destroy(None)
