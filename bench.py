@@ -189,7 +189,8 @@ def bench_serve(args) -> int:
         t0 = time.perf_counter()
         ids = [srv.submit(pool[k]) for k in picks]
         for i in ids:
-            srv.result(i)
+            srv.view(i)
+            srv.release(i)
         return nreq / (time.perf_counter() - t0), int(sum(len(pool[k]) for k in picks))
 
     out = {}
@@ -223,7 +224,8 @@ def bench_serve(args) -> int:
         for k in range(nwarm + nreq):
             while ids[k] is None:
                 time.sleep(0.0002)
-            ch, inf, r = srv.result(ids[k])
+            ch, inf, r = srv.view(ids[k])
+            srv.release(ids[k])
             if k >= nwarm:
                 lat.append(r["latency_ms"])
                 batch_sizes.append(r["batch_maps"])

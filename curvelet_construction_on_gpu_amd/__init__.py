@@ -120,6 +120,10 @@ _SIGNATURES = {
     "cvg_server_submit": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _P(C.c_ulonglong)]),
     "cvg_server_wait": (C.c_int, [C.c_void_p, C.c_ulonglong, _P(_ServerResult)]),
     "cvg_server_fetch": (C.c_int, [C.c_void_p, C.c_ulonglong, _up, _dp]),
+    "cvg_server_view": (C.c_int, [C.c_void_p, C.c_ulonglong, _P(_up), _P(_dp)]),
+    "cvg_server_release": (C.c_int, [C.c_void_p, C.c_ulonglong]),
+    "cvg_host_alloc": (C.c_int, [C.c_size_t, _P(C.c_void_p)]),
+    "cvg_host_free": (None, [C.c_void_p]),
     "cvg_server_counters": (C.c_int, [C.c_void_p, _P(C.c_ulonglong), _P(C.c_ulonglong), _P(C.c_ulonglong)]),
     "cvg_server_last_error": (C.c_char_p, [C.c_void_p]),
     "cvg_server_destroy": (None, [C.c_void_p]),
@@ -404,6 +408,21 @@ class Server:
         if rc != 0:
             raise CurveletError(f"cvg_server_fetch failed ({rc})", rc)
         return chains, info, r
+
+    def view(self, rid: int) -> tuple[np.ndarray, np.ndarray, dict]:
+        """Zero-copy arrays over the server's page-locked result rows; valid until release(rid)."""
+        r = self.wait(rid)
+        cp, ip = _up(), _dp()
+        rc = self._lib.cvg_server_view(self._h, rid, C.byref(cp), C.byref(ip))
+        if rc != 0:
+            raise CurveletError(f"cvg_server_view failed ({rc})", rc)
+        rows, w = r["num_curvelets"], r["chain_width"]
+        if rows == 0:
+            return np.zeros((0, w), np.uint32), np.zeros((0, 10)), r
+        return (np.ctypeslib.as_array(cp, shape=(rows, w)), np.ctypeslib.as_array(ip, shape=(rows, 10)), r)
+
+    def release(self, rid: int) -> None:
+        self._lib.cvg_server_release(self._h, rid)
 
     def counters(self) -> dict:
         b, m, e = C.c_ulonglong(), C.c_ulonglong(), C.c_ulonglong()

@@ -2457,6 +2457,22 @@ int cvg_device_curvelets(cvg_context *ctx, const unsigned **device_chains, const
     return CVG_OK;
 }
 
+int cvg_host_alloc(size_t bytes, void **ptr)
+{
+    if (!ptr || bytes == 0) return CVG_ERR_INVALID_ARGUMENT;
+    *ptr = nullptr;
+    if (hipHostMalloc(ptr, bytes, hipHostMallocDefault) != hipSuccess) {
+        *ptr = nullptr;
+        return CVG_ERR_OUT_OF_MEMORY;
+    }
+    return CVG_OK;
+}
+
+void cvg_host_free(void *ptr)
+{
+    if (ptr) (void)hipHostFree(ptr);
+}
+
 int cvg_get_stream(cvg_context *ctx, void **hip_stream)
 {
     if (!ctx || !hip_stream) return CVG_ERR_INVALID_ARGUMENT;

@@ -10,10 +10,17 @@
 // overlaps on the device.  A map's results are those of cvg_run on it alone:
 // the batch shares launches, not arithmetic.
 //
+// Host memory: request edges are copied at submit into page-locked blocks,
+// and each batch's chains and info come back in one transfer into a
+// page-locked result block shared by the batch's requests (released when the
+// last of them is), both from a pool, so transfers run by DMA at link speed
+// and a request's rows are read in place (cvg_server_view) or copied once.
+//
 // The reference has no serving layer; this is the loop around
 // run_curvelet<double> (cpu_curvelet/main.cpp:20-96) a service would run.
 #include <chrono>
 #include <condition_variable>
+#include <map>
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -29,23 +36,72 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// Page-locked host blocks, reused by size (smallest free block that fits).
+class PinnedPool {
+  public:
+    ~PinnedPool()
+    {
+        for (auto &f : free_) cvg_host_free(f.second);
+    }
+    void *get(size_t bytes, size_t *got)
+    {
+        bytes = (bytes + (1u << 20) - 1) & ~static_cast<size_t>((1u << 20) - 1);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            auto it = free_.lower_bound(bytes);
+            if (it != free_.end() && it->first <= 4 * bytes) {
+                void *p = it->second;
+                *got = it->first;
+                free_.erase(it);
+                return p;
+            }
+        }
+        void *p = nullptr;
+        if (cvg_host_alloc(bytes, &p) != CVG_OK) return nullptr;
+        *got = bytes;
+        return p;
+    }
+    void put(void *p, size_t bytes)
+    {
+        if (!p) return;
+        std::lock_guard<std::mutex> lk(mu_);
+        free_.emplace(bytes, p);
+    }
+
+  private:
+    std::mutex mu_;
+    std::multimap<size_t, void *> free_;
+};
+
+struct Block {
+    PinnedPool *pool = nullptr;
+    void *mem = nullptr;
+    size_t bytes = 0;
+    ~Block()
+    {
+        if (pool) pool->put(mem, bytes);
+    }
+};
+
 struct Request {
     unsigned long long id = 0;
-    std::vector<double> edges;
+    std::unique_ptr<Block> edges;  // page-locked copy of the map
     int n = 0;
     int sz = 4;
     Clock::time_point t_submit, t_start, t_done;
     bool done = false;
     int status = CVG_OK;
     unsigned rows = 0, width = 0;
-    std::vector<unsigned> chains;
-    std::vector<double> info;
+    std::shared_ptr<Block> result;  // the batch's chains and info
+    const unsigned *chains = nullptr;
+    const double *info = nullptr;
     int batch_maps = 0;
 };
 
 }  // namespace
 
 struct cvg_server {
+    PinnedPool pool;
     cvg_params params{};
     cvg_server_config cfg{};
     std::vector<cvg_context *> ctxs;
@@ -89,28 +145,48 @@ void worker(cvg_server *s, int w)
         std::vector<const double *> ptrs(batch.size());
         for (size_t i = 0; i < batch.size(); ++i) {
             counts[i] = batch[i]->n;
-            ptrs[i] = batch[i]->edges.data();
+            ptrs[i] = static_cast<const double *>(batch[i]->edges->mem);
             batch[i]->t_start = t0;
         }
         int rc = cvg_run_batch(ctx, static_cast<int>(batch.size()), counts.data(), batch[0]->sz, ptrs.data(),
                                &s->params, nullptr);
-        for (size_t i = 0; i < batch.size(); ++i) {
-            Request &r = *batch[i];
-            r.status = rc;
-            if (rc == CVG_OK) {
-                unsigned rows = 0, width = 0;
-                r.status = cvg_batch_map_shape(ctx, static_cast<int>(i), &rows, &width);
-                if (r.status == CVG_OK) {
+        // all of the batch's rows in one transfer each for chains and info
+        unsigned total = 0, width = 0;
+        std::shared_ptr<Block> blk;
+        if (rc == CVG_OK) rc = cvg_curvelet_shape(ctx, &total, &width);
+        if (rc == CVG_OK) {
+            const size_t cbytes = (static_cast<size_t>(total) * width * sizeof(unsigned) + 63) & ~static_cast<size_t>(63);
+            const size_t bytes = cbytes + static_cast<size_t>(total) * 10 * sizeof(double) + 64;
+            blk = std::make_shared<Block>();
+            blk->mem = s->pool.get(bytes, &blk->bytes);
+            blk->pool = &s->pool;
+            if (!blk->mem) {
+                rc = CVG_ERR_OUT_OF_MEMORY;
+            } else {
+                unsigned *chains = static_cast<unsigned *>(blk->mem);
+                double *info = reinterpret_cast<double *>(static_cast<char *>(blk->mem) + cbytes);
+                rc = cvg_get_curvelets(ctx, chains, info);
+                unsigned row = 0;
+                for (size_t i = 0; i < batch.size() && rc == CVG_OK; ++i) {
+                    Request &r = *batch[i];
+                    unsigned rows = 0, w = 0;
+                    rc = cvg_batch_map_shape(ctx, static_cast<int>(i), &rows, &w);
                     r.rows = rows;
-                    r.width = width;
-                    r.chains.resize(static_cast<size_t>(rows) * width);
-                    r.info.resize(static_cast<size_t>(rows) * 10);
-                    r.status = cvg_get_batch_map_curvelets(ctx, static_cast<int>(i), r.chains.data(), r.info.data());
+                    r.width = w;
+                    r.chains = chains + static_cast<size_t>(row) * width;
+                    r.info = info + static_cast<size_t>(row) * 10;
+                    r.result = blk;
+                    row += rows;
                 }
             }
-            r.batch_maps = static_cast<int>(batch.size());
-            std::vector<double>().swap(r.edges);
         }
+        for (auto &r : batch) {
+            r->status = rc;
+            r->batch_maps = static_cast<int>(batch.size());
+            r->edges.reset();
+            if (rc != CVG_OK) r->result.reset();
+        }
+        blk.reset();
         const auto t1 = Clock::now();
         {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -184,7 +260,12 @@ int cvg_server_submit(cvg_server *s, int num_edges, int sz_edge_data, const doub
     r->t_submit = Clock::now();
     r->n = num_edges;
     r->sz = sz_edge_data;
-    r->edges.assign(edges, edges + static_cast<size_t>(num_edges) * sz_edge_data);
+    const size_t bytes = static_cast<size_t>(num_edges) * sz_edge_data * sizeof(double);
+    r->edges.reset(new Block);
+    r->edges->mem = s->pool.get(bytes, &r->edges->bytes);
+    r->edges->pool = &s->pool;
+    if (!r->edges->mem) return CVG_ERR_OUT_OF_MEMORY;
+    std::memcpy(r->edges->mem, edges, bytes);
     {
         std::lock_guard<std::mutex> lk(s->mu);
         r->id = s->next_id++;
@@ -227,8 +308,34 @@ int cvg_server_fetch(cvg_server *s, unsigned long long request_id, unsigned *cha
         s->table.erase(it);
     }
     if (r->status != CVG_OK) return r->status;
-    if (chains && !r->chains.empty()) std::memcpy(chains, r->chains.data(), r->chains.size() * sizeof(unsigned));
-    if (info && !r->info.empty()) std::memcpy(info, r->info.data(), r->info.size() * sizeof(double));
+    if (chains && r->rows) std::memcpy(chains, r->chains, static_cast<size_t>(r->rows) * r->width * sizeof(unsigned));
+    if (info && r->rows) std::memcpy(info, r->info, static_cast<size_t>(r->rows) * 10 * sizeof(double));
+    return CVG_OK;
+}
+
+int cvg_server_view(cvg_server *s, unsigned long long request_id, const unsigned **chains, const double **info)
+{
+    if (!s) return CVG_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->table.find(request_id);
+    if (it == s->table.end() || !it->second->done) return CVG_ERR_STATE;
+    if (it->second->status != CVG_OK) return it->second->status;
+    if (chains) *chains = it->second->chains;
+    if (info) *info = it->second->info;
+    return CVG_OK;
+}
+
+int cvg_server_release(cvg_server *s, unsigned long long request_id)
+{
+    if (!s) return CVG_ERR_INVALID_ARGUMENT;
+    std::shared_ptr<Request> r;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->table.find(request_id);
+        if (it == s->table.end() || !it->second->done) return CVG_ERR_STATE;
+        r = it->second;
+        s->table.erase(it);
+    }
     return CVG_OK;
 }
 

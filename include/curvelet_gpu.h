@@ -159,6 +159,10 @@ int cvg_upload_edges(cvg_context *ctx, int num_edges, int sz_edge_data, const do
                      const double **device_edges);
 /* Device pointers to the last run's outputs (valid until the next run). */
 int cvg_device_curvelets(cvg_context *ctx, const unsigned **device_chains, const double **device_info);
+/* Page-locked host memory: edge arrays and result buffers in it transfer by
+ * DMA at full link speed (the served form keeps pools of it). */
+int cvg_host_alloc(size_t bytes, void **ptr);
+void cvg_host_free(void *ptr);
 /* The context's HIP stream (hipStream_t), for callers that order their own work after it. */
 int cvg_get_stream(cvg_context *ctx, void **hip_stream);
 int cvg_synchronize(cvg_context *ctx);
@@ -216,6 +220,10 @@ int cvg_server_wait(cvg_server *server, unsigned long long request_id, cvg_serve
 /* Copies a finished request's chains (num_curvelets x chain_width, 1-based ids)
  * and info (num_curvelets x 10) and releases it. */
 int cvg_server_fetch(cvg_server *server, unsigned long long request_id, unsigned *chains, double *info);
+/* Zero-copy access to a finished request's rows in the server's page-locked
+ * result block; valid until cvg_server_release. */
+int cvg_server_view(cvg_server *server, unsigned long long request_id, const unsigned **chains, const double **info);
+int cvg_server_release(cvg_server *server, unsigned long long request_id);
 int cvg_server_counters(cvg_server *server, unsigned long long *batches, unsigned long long *maps,
                         unsigned long long *edges);
 const char *cvg_server_last_error(cvg_server *server);

@@ -49,3 +49,17 @@ def test_plain_loop_configuration(ctx):
         ctx.run(m, p)
         a, b = ctx.curvelets()
         assert np.array_equal(ch, a) and np.array_equal(bits(inf), bits(b))
+
+
+def test_view_and_release(ctx):
+    p = CurveletParams()
+    maps = [synth_edges("C1", 1200 + s) for s in range(6)]
+    with Server(p, workers=2, max_batch_maps=4) as srv:
+        ids = [srv.submit(m) for m in maps]
+        views = [srv.view(i) for i in ids]
+        for m, (ch, inf, r) in zip(maps, views):
+            ctx.run(m, p)
+            a, b = ctx.curvelets()
+            assert np.array_equal(ch, a) and np.array_equal(bits(inf), bits(b))
+        for i in ids:
+            srv.release(i)
