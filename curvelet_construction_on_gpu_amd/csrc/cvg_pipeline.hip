@@ -104,6 +104,9 @@ struct cvg_context {
     int num_maps = 1;
     std::vector<long long> map_eoff;
     std::vector<unsigned> map_rows;
+    unsigned *h_maprow = nullptr;  // page-locked
+    size_t h_maprow_bytes = 0;
+    bool counts_pending = false;
     DevBuf d_map, d_eoff, d_mapbox, d_maprow;
 
     // curvelets
@@ -828,7 +831,8 @@ __global__ __launch_bounds__(kBlock) void k_bucket(int n, const int *__restrict_
 // psin[q*2Kt + 2j] = sin(T2t_j - (theta_n - dt)), psin[q*2Kt + 2j + 1] =
 // sin(T2t_j - (theta_n + dt)) with T2t_j = theta_a + delta_t2[j], for every
 // pair that is not "close" (close pairs keep the default bundle and never read
-// them).  One thread per (pair, j); the sin/cos table lives in LDS.
+// them).  One thread per pair (the pair's loads once, 2 Kt independent sines);
+// the sin/cos table lives in LDS.
 __global__ __launch_bounds__(kBlock) void k_pair_trig(int pairs, int kt, const int *__restrict__ pair_anchor,
                                                       const int *__restrict__ ids, const double *__restrict__ x,
                                                       const double *__restrict__ y, const double *__restrict__ th,
@@ -838,21 +842,24 @@ __global__ __launch_bounds__(kBlock) void k_pair_trig(int pairs, int kt, const i
     __shared__ double tab[4 * CVG_SIN_ROWS];
     for (int i = threadIdx.x; i < 4 * CVG_SIN_ROWS; i += blockDim.x) tab[i] = c_sincos_tab[i];
     __syncthreads();
-    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= static_cast<long long>(pairs) * kt) return;
-    const int q = static_cast<int>(t / kt);
-    const int j = static_cast<int>(t - static_cast<long long>(q) * kt);
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= pairs) return;
     const int a = pair_anchor[q];
     const int b = ids[q];
     const double DX = x[a] - x[b];
     const double DY = y[a] - y[b];
     if (DX * DX + DY * DY < close_thr) return;  // :682
-    const double T2t = th[a] + off_t[j];
+    const double ta = th[a];
     const double nt = th[b];
-    double2 v;
-    v.x = cvg::cr_sin(T2t - (nt - dt), tab);
-    v.y = cvg::cr_sin(T2t - (nt + dt), tab);
-    reinterpret_cast<double2 *>(psin)[static_cast<long long>(q) * kt + j] = v;
+    const double tm = nt - dt, tp = nt + dt;
+    double2 *out = reinterpret_cast<double2 *>(psin) + static_cast<long long>(q) * kt;
+    for (int j = 0; j < kt; ++j) {
+        const double T2t = ta + off_t[j];
+        double2 v;
+        v.x = cvg::cr_sin(T2t - tm, tab);
+        v.y = cvg::cr_sin(T2t - tp, tab);
+        out[j] = v;
+    }
 }
 
 // Host-computed per-parameter tables, laid out in one device block.  Lanes of
@@ -876,7 +883,7 @@ struct FusedArgs {
     const int *ids;
     const unsigned char *pflag;
     const int4 *list;  // (anchor, CSR base, degree, -)
-    int list_len;
+    const unsigned *list_len;  // the bucket's entry count (device; written by k_bucket)
     int nb_cap;        // bundle slots per anchor: the bucket's longest run
     int deg_cap;       // staged neighbour records per anchor
     int chain_stride;  // ids per seed record: max(G, 1)
@@ -1027,7 +1034,8 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
     const int apb = COOP ? 1 : wpb;    // anchor streams per workgroup
     const int streams = static_cast<int>(gridDim.x) * apb;
     const int g = static_cast<int>(blockIdx.x) * apb + grp;
-    if (g >= A.list_len) return;  // COOP: the whole workgroup (same g)
+    const int len_list = static_cast<int>(uload(A.list_len));
+    if (g >= len_list) return;  // COOP: the whole workgroup (same g)
 
     const int n = A.n;
     const int K = A.K;
@@ -1036,7 +1044,6 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
     const int CS = A.chain_stride;
     const int SR = CS + 3;  // ints per seed record
     const int NB = A.nb_cap;
-    const int len_list = A.list_len;
     const AnchorCarve cv = anchor_carve(NB, A.deg_cap, K, kt, CS, !GB);
     char *sl = reinterpret_cast<char *>(lds) + static_cast<size_t>(grp) * cv.bytes;
     double *skd = reinterpret_cast<double *>(sl + cv.skd);
@@ -1574,6 +1581,7 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
 {
     ctx->graph_ready = false;
     ctx->curvelets_ready = false;
+    ctx->counts_pending = false;
     if (n <= 0 || sz < 4 || d_edges == nullptr)  // preprocess.hpp:197
         return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "build_neighbor_csr_graph: need num_edges > 0, sz_edge_data >= 4");
     if (R > 4096) return fail(ctx, CVG_ERR_UNSUPPORTED, "neighbor_radius above 4096 cells");
@@ -1824,6 +1832,42 @@ size_t put(std::vector<char> &blob, const std::vector<T> &v)
     return off;
 }
 
+// Page-locked host buffer for per-map row offsets.
+int ensure_host(cvg_context *ctx, size_t bytes)
+{
+    if (ctx->h_maprow_bytes >= bytes) return CVG_OK;
+    if (ctx->h_maprow) (void)hipHostFree(ctx->h_maprow);
+    ctx->h_maprow = nullptr;
+    ctx->h_maprow_bytes = 0;
+    if (hipHostMalloc(reinterpret_cast<void **>(&ctx->h_maprow), bytes, hipHostMallocDefault) != hipSuccess) {
+        ctx->h_maprow = nullptr;
+        return fail(ctx, CVG_ERR_OUT_OF_MEMORY, "hipHostMalloc failed");
+    }
+    ctx->h_maprow_bytes = bytes;
+    return CVG_OK;
+}
+
+// The counts build_curvelets left in flight: waits for the stream once.
+int finish_curvelets(cvg_context *ctx)
+{
+    if (!ctx->counts_pending) return CVG_OK;
+    CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->counts_pending = false;
+    const unsigned C = ctx->mailbox[32];
+    ctx->valid_pairs = ctx->mailbox[S_VALID];
+    ctx->intersections = static_cast<unsigned long long>(ctx->mailbox[S_COUNT]) |
+                         (static_cast<unsigned long long>(ctx->mailbox[S_COUNT + 1]) << 32);
+    ctx->ambiguous = ctx->mailbox[S_AMBIG];
+    ctx->num_curvelets = C;
+    if (ctx->num_maps > 1) {
+        ctx->map_rows.assign(ctx->h_maprow, ctx->h_maprow + ctx->num_maps + 1);
+    } else {
+        ctx->map_rows.assign(2, 0);
+        ctx->map_rows[1] = C;
+    }
+    return CVG_OK;
+}
+
 int build_curvelets(cvg_context *ctx, const cvg_params &p)
 {
     if (!ctx->graph_ready) return fail(ctx, CVG_ERR_STATE, "build the neighbour graph first");
@@ -1881,7 +1925,7 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
                            as<int>(ctx->d_pair_anchor), as<int>(ctx->d_ids), as<double>(ctx->d_x),
                            as<double>(ctx->d_y), as<double>(ctx->d_trig), d.close_thr,
                            as<unsigned char>(ctx->d_cls), as<double>(ctx->d_prec), as<unsigned>(ctx->d_cnt3));
-        hipLaunchKernelGGL(k_pair_trig, dim3(blocks_for(static_cast<long long>(pairs) * d.kt)), dim3(kBlock), 0, st,
+        hipLaunchKernelGGL(k_pair_trig, dim3(blocks_for(pairs)), dim3(kBlock), 0, st,
                            pairs, static_cast<int>(d.kt), as<int>(ctx->d_pair_anchor), as<int>(ctx->d_ids),
                            as<double>(ctx->d_x), as<double>(ctx->d_y), as<double>(ctx->d_th), T.off_t, d.dt,
                            d.close_thr, as<double>(ctx->d_psin));
@@ -1890,10 +1934,6 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
                        as<unsigned>(ctx->d_cnt3), as<unsigned>(ctx->d_off), nb, T.caps, as<int4>(ctx->d_lists),
                        scal);
     CVG_HIP(ctx, hipGetLastError());
-    CVG_TRY(read_scalars(ctx));
-    std::vector<unsigned> bucket_count(nb);
-    for (int b = 0; b < nb; ++b) bucket_count[b] = ctx->mailbox[S_BUCKET + b];
-    ctx->ambiguous = ctx->mailbox[S_AMBIG];
 
     // staging
     const int st_stride = static_cast<int>(std::max(d.G, 1u));
@@ -1976,16 +2016,17 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
         l.shmem = static_cast<unsigned>(l.slice * l.apb);
         l.fn = fused_kernel(d.cpl, l.nch, l.gb, l.coop);
         if (!l.fn) return fail(ctx, CVG_ERR_STATE, "internal: no fused kernel for this grid");
-        l.grid = 0;
-        if (bucket_count[b] == 0) continue;
         CVG_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(l.fn),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(l.shmem)));
         int occ = 0;
         CVG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(l.fn),
                                                                   64 * l.wpb, l.shmem));
+        // the bucket sizes stay on the device: every launch gets the resident
+        // grid (streams beyond the bucket's length exit at once), at most one
+        // stream per anchor of the map
         const long long resident = static_cast<long long>(std::max(occ, 1)) * ctx->num_cu;
-        const long long need_blocks = (static_cast<long long>(bucket_count[b]) + l.apb - 1) / l.apb;
-        l.grid = static_cast<int>(std::min(resident, need_blocks));
+        const long long max_blocks = (static_cast<long long>(n) + l.apb - 1) / l.apb;
+        l.grid = static_cast<int>(std::min(resident, max_blocks));
         if (l.gb) {
             gb_off[b] = gb_total;
             gb_total += static_cast<size_t>(l.grid) * l.apb * caps[b] * A.K * 2;
@@ -2002,14 +2043,12 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     bool aux_used[3] = {false, false, false};
     int rr = 0;
     for (int b = nb - 1; b >= 0; --b) {
-        const int len = static_cast<int>(bucket_count[b]);
-        if (len == 0) continue;
         const int slot = b == 0 ? 0 : 1 + (rr++ % 3);
         hipStream_t ls = slot == 0 ? st : ctx->aux[slot - 1];
         if (slot > 0) aux_used[slot - 1] = true;
         const Launch &l = plan[b];
         A.list = as<int4>(ctx->d_lists) + static_cast<size_t>(b) * n;
-        A.list_len = len;
+        A.list_len = scal + S_BUCKET + b;
         A.nb_cap = caps[b];
         A.deg_cap = l.dcap;
         A.g_bundle = l.gb ? as<double>(ctx->d_gbundle) + gb_off[b] : nullptr;
@@ -2025,30 +2064,12 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     // every anchor is in exactly one bucket, so every acc_count entry is written
 
     CVG_TRY(exclusive_scan(ctx, as<unsigned>(ctx->d_acc), n, as<unsigned>(ctx->d_rowoff), ctx->d_tiles));
-    CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox + 32, as<unsigned>(ctx->d_rowoff) + n, sizeof(unsigned),
-                                hipMemcpyDeviceToHost, st));
-    CVG_TRY(read_scalars(ctx));
-    const unsigned C = ctx->mailbox[32];
-    ctx->valid_pairs = ctx->mailbox[S_VALID];
-    ctx->intersections = static_cast<unsigned long long>(ctx->mailbox[S_COUNT]) |
-                         (static_cast<unsigned long long>(ctx->mailbox[S_COUNT + 1]) << 32);
-    ctx->num_curvelets = C;
+    // outputs sized by their bound (one row per CSR pair at most), so the
+    // record runs without a host round trip; the counts come back with the
+    // stream (finish_curvelets)
     ctx->width = d.G + 1;
-    ctx->map_rows.assign(2, 0);
-    ctx->map_rows[1] = C;
-    if (ctx->num_maps > 1) {
-        const int M = ctx->num_maps;
-        CVG_TRY(ensure(ctx, ctx->d_maprow, (static_cast<size_t>(M) + 1) * sizeof(unsigned)));
-        hipLaunchKernelGGL(k_map_rows, dim3(blocks_for(M + 1LL)), dim3(kBlock), 0, st, M, as<long long>(ctx->d_eoff),
-                           as<unsigned>(ctx->d_rowoff), as<unsigned>(ctx->d_maprow));
-        ctx->map_rows.assign(static_cast<size_t>(M) + 1, 0);
-        CVG_HIP(ctx, hipMemcpyAsync(ctx->map_rows.data(), ctx->d_maprow.ptr, (static_cast<size_t>(M) + 1) *
-                                    sizeof(unsigned), hipMemcpyDeviceToHost, st));
-        CVG_HIP(ctx, hipStreamSynchronize(st));
-    }
-
-    CVG_TRY(ensure(ctx, ctx->d_chains, static_cast<size_t>(C) * ctx->width * sizeof(unsigned)));
-    CVG_TRY(ensure(ctx, ctx->d_info, static_cast<size_t>(C) * kInfoWidth * sizeof(double)));
+    CVG_TRY(ensure(ctx, ctx->d_chains, np * ctx->width * sizeof(unsigned)));
+    CVG_TRY(ensure(ctx, ctx->d_info, np * kInfoWidth * sizeof(double)));
     RecordArgs R{};
     R.pairs = pairs;
     R.pair_anchor = as<int>(ctx->d_pair_anchor);
@@ -2071,8 +2092,22 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     R.off_t = T.off_t;
     R.chains = as<unsigned>(ctx->d_chains);
     R.info = as<double>(ctx->d_info);
-    if (C > 0) hipLaunchKernelGGL(k_record, dim3(blocks_for(pairs)), dim3(kBlock), 0, st, R);
+    if (pairs > 0) hipLaunchKernelGGL(k_record, dim3(blocks_for(pairs)), dim3(kBlock), 0, st, R);
     CVG_HIP(ctx, hipGetLastError());
+    CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox + 32, as<unsigned>(ctx->d_rowoff) + n, sizeof(unsigned),
+                                hipMemcpyDeviceToHost, st));
+    CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox, ctx->d_scal.ptr, kScalarWords * sizeof(unsigned),
+                                hipMemcpyDeviceToHost, st));
+    if (ctx->num_maps > 1) {
+        const int M = ctx->num_maps;
+        CVG_TRY(ensure(ctx, ctx->d_maprow, (static_cast<size_t>(M) + 1) * sizeof(unsigned)));
+        CVG_TRY(ensure_host(ctx, (static_cast<size_t>(M) + 1) * sizeof(unsigned)));
+        hipLaunchKernelGGL(k_map_rows, dim3(blocks_for(M + 1LL)), dim3(kBlock), 0, st, M, as<long long>(ctx->d_eoff),
+                           as<unsigned>(ctx->d_rowoff), as<unsigned>(ctx->d_maprow));
+        CVG_HIP(ctx, hipMemcpyAsync(ctx->h_maprow, ctx->d_maprow.ptr, (static_cast<size_t>(M) + 1) * sizeof(unsigned),
+                                    hipMemcpyDeviceToHost, st));
+    }
+    ctx->counts_pending = true;
     ctx->cur_params = p;
     ctx->curvelets_ready = true;
     return CVG_OK;
@@ -2100,6 +2135,7 @@ int run_common(cvg_context *ctx, int n, int sz, const double *d_edges, const cvg
     CVG_TRY(build_curvelets(ctx, *p));
     CVG_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     CVG_HIP(ctx, hipEventSynchronize(ctx->ev[2]));
+    CVG_TRY(finish_curvelets(ctx));
     if (stats) {
         float t01 = 0, t12 = 0, t02 = 0;
         (void)hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]);
@@ -2240,6 +2276,7 @@ void cvg_destroy(cvg_context *ctx)
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->mailbox) (void)hipHostFree(ctx->mailbox);
+    if (ctx->h_maprow) (void)hipHostFree(ctx->h_maprow);
     for (int i = 0; i < 3; ++i) {
         if (ctx->aux[i]) (void)hipStreamSynchronize(ctx->aux[i]);
         if (ctx->ev_join[i]) (void)hipEventDestroy(ctx->ev_join[i]);
@@ -2303,7 +2340,7 @@ int cvg_form_pairwise_bundles(cvg_context *ctx, const cvg_params *params, unsign
     CVG_TRY(check_ctx(ctx));
     if (!params) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "null params");
     CVG_TRY(build_curvelets(ctx, *params));
-    CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CVG_TRY(finish_curvelets(ctx));
     if (valid_pairs) *valid_pairs = ctx->valid_pairs;
     return CVG_OK;
 }
@@ -2314,8 +2351,8 @@ int cvg_build_curvelets_greedy(cvg_context *ctx, const cvg_params *params, unsig
     if (!params) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "null params");
     if (!(ctx->curvelets_ready && same_params(ctx->cur_params, *params))) {
         CVG_TRY(build_curvelets(ctx, *params));
-        CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
+    CVG_TRY(finish_curvelets(ctx));
     if (num_curvelets) *num_curvelets = ctx->num_curvelets;
     return CVG_OK;
 }
@@ -2324,6 +2361,7 @@ int cvg_curvelet_shape(cvg_context *ctx, unsigned *num_curvelets, unsigned *chai
 {
     CVG_TRY(check_ctx(ctx));
     if (!ctx->curvelets_ready) return fail(ctx, CVG_ERR_STATE, "no curvelets");
+    CVG_TRY(finish_curvelets(ctx));
     if (num_curvelets) *num_curvelets = ctx->num_curvelets;
     if (chain_width) *chain_width = ctx->width;
     return CVG_OK;
@@ -2333,6 +2371,7 @@ int cvg_get_curvelets(cvg_context *ctx, unsigned *chains, double *info)
 {
     CVG_TRY(check_ctx(ctx));
     if (!ctx->curvelets_ready) return fail(ctx, CVG_ERR_STATE, "no curvelets");
+    CVG_TRY(finish_curvelets(ctx));
     const size_t C = ctx->num_curvelets;
     if (chains && C > 0)
         CVG_HIP(ctx, hipMemcpyAsync(chains, ctx->d_chains.ptr, C * ctx->width * sizeof(unsigned),
@@ -2414,6 +2453,7 @@ int cvg_batch_map_shape(cvg_context *ctx, int map, unsigned *num_curvelets, unsi
 {
     CVG_TRY(check_ctx(ctx));
     if (!ctx->curvelets_ready) return fail(ctx, CVG_ERR_STATE, "no curvelets");
+    CVG_TRY(finish_curvelets(ctx));
     if (map < 0 || map >= static_cast<int>(ctx->map_rows.size()) - 1)
         return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "map index out of range");
     if (num_curvelets) *num_curvelets = ctx->map_rows[map + 1] - ctx->map_rows[map];
@@ -2425,6 +2465,7 @@ int cvg_get_batch_map_curvelets(cvg_context *ctx, int map, unsigned *chains, dou
 {
     CVG_TRY(check_ctx(ctx));
     if (!ctx->curvelets_ready) return fail(ctx, CVG_ERR_STATE, "no curvelets");
+    CVG_TRY(finish_curvelets(ctx));
     if (map < 0 || map >= static_cast<int>(ctx->map_rows.size()) - 1)
         return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "map index out of range");
     const size_t r0 = ctx->map_rows[map], rows = ctx->map_rows[map + 1] - r0;
@@ -2452,6 +2493,7 @@ int cvg_device_curvelets(cvg_context *ctx, const unsigned **device_chains, const
 {
     CVG_TRY(check_ctx(ctx));
     if (!ctx->curvelets_ready) return fail(ctx, CVG_ERR_STATE, "no curvelets");
+    CVG_TRY(finish_curvelets(ctx));
     if (device_chains) *device_chains = as<unsigned>(ctx->d_chains);
     if (device_info) *device_info = as<double>(ctx->d_info);
     return CVG_OK;
