@@ -584,7 +584,7 @@ def synth_edges(preset: str | _SynthConfig, seed: int, **overrides) -> np.ndarra
     return out
 
 
-MATH_OPS = {"sin": 0, "cos": 1, "atan2": 2, "sqrt": 3, "to2pi": 4, "fmod": 5}
+MATH_OPS = {"sin": 0, "cos": 1, "atan2": 2, "sqrt": 3, "to2pi": 4, "fmod": 5, "sincos_sin": 6, "sincos_cos": 7}
 
 
 def math_eval(op: str, x: np.ndarray, y: np.ndarray | None = None, ctx: Context | None = None,

@@ -1553,6 +1553,12 @@ __host__ __device__ inline double math_op(int op, double x, double y, const doub
     case 3: return sqrt(x);
     case 4: return cvg::wrap_two_pi(x);
     case 5: return fmod(x, y);
+    case 6:
+    case 7: {
+        double sv, cv;
+        cvg::cr_sincos(x, sc, &sv, &cv);
+        return op == 6 ? sv : cv;
+    }
     default: return 0.0;
     }
 }
@@ -2583,7 +2589,7 @@ int cvg_debug_anchor_bundles(cvg_context *ctx, const cvg_params *params, int anc
 
 int cvg_math_eval(cvg_context *ctx, int op, int n, const double *x, const double *y, double *out)
 {
-    if (!x || !out || n < 0 || op < 0 || op > 5) return CVG_ERR_INVALID_ARGUMENT;
+    if (!x || !out || n < 0 || op < 0 || op > 7) return CVG_ERR_INVALID_ARGUMENT;
     if ((op == 2 || op == 5) && !y) return CVG_ERR_INVALID_ARGUMENT;
     if (!ctx) {
         for (int i = 0; i < n; ++i)

@@ -273,7 +273,8 @@ int cvg_synth_generate(const cvg_synth_config *config, unsigned long long seed, 
 int cvg_debug_anchor_bundles(cvg_context *ctx, const cvg_params *params, int anchor, unsigned char *dir,
                              unsigned char *hyp, double *bmin, double *bmax, int *deg, int *cells);
 /* Evaluate the library's correctly rounded math on n arguments.
- * op: 0 sin(x), 1 cos(x), 2 atan2(y, x), 3 sqrt(x), 4 To2Pi(x), 5 fmod(x, y).
+ * op: 0 sin(x), 1 cos(x), 2 atan2(y, x), 3 sqrt(x), 4 To2Pi(x), 5 fmod(x, y),
+ * 6 and 7 the sine and cosine of the joint sincos(x).
  * ctx == NULL evaluates the host build of the same code; otherwise the device. */
 int cvg_math_eval(cvg_context *ctx, int op, int n, const double *x, const double *y, double *out);
 

@@ -306,3 +306,16 @@ def test_host_sqrt_fmod_exact():
     assert np.array_equal(math_eval("fmod", a, b, device=False), np.fmod(a, b))
     t = math_eval("to2pi", a, device=False)
     assert np.array_equal(t, np.array([to2pi(v) for v in a]))
+
+
+def test_host_sin_cos_agree_with_joint_sincos():
+    """sin and cos are evaluated by one branch-free expression (the cosine as the
+    sine of the table pair rotated by pi/2); they must carry the bits of the
+    joint sincos evaluation, which keeps the two expressions apart."""
+    rng = np.random.default_rng(8)
+    x = np.concatenate([rng.uniform(-8, 8, 300000), rng.uniform(-1e5, 1e5, 50000),
+                        rng.uniform(-1e-3, 1e-3, 20000), np.array([0.0, -0.0, 1e-9, math.pi / 4, -math.pi / 4])])
+    assert np.array_equal(math_eval("sin", x, device=False).view(np.uint64),
+                          math_eval("sincos_sin", x, device=False).view(np.uint64))
+    assert np.array_equal(math_eval("cos", x, device=False).view(np.uint64),
+                          math_eval("sincos_cos", x, device=False).view(np.uint64))
