@@ -7,11 +7,14 @@
 //           k_cell_count      counting sort of edges into a dense pixel-cell grid
 //           scan              exclusive prefix sum over the grid (cell starts)
 //           k_cell_scatter    edges into cell order
-//           k_cell_order      within-cell order by edge id, coalesced SoA copies
-//           k_discover_count  per anchor: in-radius candidates in the 7x7 window
+//           k_cell_order      within-cell order by edge id; cell-sorted id,
+//                             cell and (x, y) arrays
+//           k_discover_count  per anchor (a 16-lane segment): in-radius
+//                             candidates in the 7x7 window
 //           scan              CSR offsets
-//           k_fill_csr        per anchor: gather in the reference's scan order,
-//                             distance sort (tie-exact std::sort replay), CSR
+//           k_fill_csr        per anchor (a 16-lane segment): gather in the
+//                             reference's scan order, distance sort
+//                             (tie-exact std::sort replay), CSR
 //   curvelets k_edge_trig     per edge: sin/cos of theta, theta+-dt, theta+grid
 //           k_pair_prep       per pair: direction-filter class, close/branch
 //                             flags, the bundle's cell-independent terms
@@ -81,6 +84,9 @@ struct cvg_context {
     int coop_min_cap = 6;
     int coop_waves = 4;
     int num_cu = 1;
+    // lanes per anchor in the neighbour-window scans (8, 16, 32 or 64;
+    // CVG_FILL_LANES overrides it; results never depend on it)
+    int fill_lanes = 16;
 
     unsigned *mailbox = nullptr;  // pinned host scalars read back after syncs
 
@@ -96,7 +102,7 @@ struct cvg_context {
     unsigned truncated = 0;
     unsigned ambiguous = 0;
     int gx0 = 0, gy0 = 0, gw = 0, gh = 0;
-    DevBuf d_edges_up, d_x, d_y, d_th, d_cx, d_cy, d_scal, d_grid, d_tmp_ids, d_sorted, d_sx, d_sy, d_deg,
+    DevBuf d_edges_up, d_x, d_y, d_th, d_cx, d_cy, d_scal, d_grid, d_tmp_ids, d_sorted, d_sxy, d_scell, d_part, d_deg,
         d_off, d_tiles, d_cand, d_ids, d_d2, d_pair_anchor;
 
     // the maps of a batch (a single map outside batches): edge id ranges,
@@ -116,7 +122,7 @@ struct cvg_context {
     unsigned long long intersections = 0;
     unsigned num_curvelets = 0;
     unsigned width = 0;
-    DevBuf d_trig, d_cls, d_prec, d_psin, d_cnt3, d_lists, d_tables, d_st_ids, d_st_meta, d_st_k, d_acc, d_rowoff, d_gbundle,
+    DevBuf d_trig, d_cls, d_prec, d_psin, d_lists, d_tables, d_st_ids, d_st_meta, d_st_k, d_acc, d_rowoff, d_gbundle,
         d_chains, d_info;
 
     // diagnostics
@@ -211,33 +217,95 @@ __device__ unsigned block_exclusive_scan(unsigned v, unsigned *lds_waves, unsign
     return pre + x - v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_tiles(const unsigned *__restrict__ in, long long n,
-                                                       unsigned *__restrict__ tile_sums)
+// Workgroup max and sum of (max, count) pairs; the result is valid in thread 0.
+__device__ void block_max_sum(unsigned &mx, unsigned &cnt, unsigned *lds_max, unsigned *lds_cnt)
 {
-    __shared__ unsigned lw[kBlock / 64];
-    const long long base = static_cast<long long>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
-    unsigned s = 0;
-    for (int k = 0; k < kScanItems; ++k)
-        if (base + k < n) s += in[base + k];
-    unsigned tot;
-    block_exclusive_scan(s, lw, &tot);
-    if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = max(mx, static_cast<unsigned>(__shfl_xor(static_cast<int>(mx), o, 64)));
+        cnt += static_cast<unsigned>(__shfl_xor(static_cast<int>(cnt), o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        lds_max[threadIdx.x >> 6] = mx;
+        lds_cnt[threadIdx.x >> 6] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < static_cast<int>(blockDim.x >> 6); ++w) {
+            mx = max(mx, lds_max[w]);
+            cnt += lds_cnt[w];
+        }
+    }
+    __syncthreads();
 }
 
-// Single workgroup: exclusive scan of the tile sums in place, total at [ntiles].
-__global__ __launch_bounds__(1024) void k_scan_tile_sums(unsigned *tile_sums, int ntiles)
+// Tile sums; with STATS also each tile's largest value and its count of values
+// >= thr (tile_max / tile_cnt), so a scan over counts also yields their
+// maximum and how many reached a cap, without contended atomics.
+template <bool STATS>
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(const unsigned *__restrict__ in, long long n,
+                                                       unsigned *__restrict__ tile_sums, unsigned thr,
+                                                       unsigned *__restrict__ tile_max, unsigned *__restrict__ tile_cnt)
+{
+    __shared__ unsigned lw[kBlock / 64];
+    __shared__ unsigned lm[kBlock / 64], lc[kBlock / 64];
+    const long long base = static_cast<long long>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
+    unsigned s = 0, mx = 0, c = 0;
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < n) {
+            const unsigned v = in[base + k];
+            s += v;
+            if constexpr (STATS) {
+                mx = max(mx, v);
+                c += v >= thr ? 1u : 0u;
+            }
+        }
+    unsigned tot;
+    block_exclusive_scan(s, lw, &tot);
+    if constexpr (STATS) block_max_sum(mx, c, lm, lc);
+    if (threadIdx.x == 0) {
+        tile_sums[blockIdx.x] = tot;
+        if constexpr (STATS) {
+            tile_max[blockIdx.x] = mx;
+            tile_cnt[blockIdx.x] = c;
+        }
+    }
+}
+
+// Single workgroup: exclusive scan of the tile sums in place, total at
+// [ntiles]; with STATS also the maximum and count over the tiles into
+// *stat_max and *stat_cnt.
+template <bool STATS>
+__global__ __launch_bounds__(1024) void k_scan_tile_sums(unsigned *tile_sums, int ntiles,
+                                                         const unsigned *__restrict__ tile_max,
+                                                         const unsigned *__restrict__ tile_cnt,
+                                                         unsigned *__restrict__ stat_max,
+                                                         unsigned *__restrict__ stat_cnt)
 {
     __shared__ unsigned lw[1024 / 64];
-    unsigned carry = 0;
+    __shared__ unsigned lm[1024 / 64], lc[1024 / 64];
+    unsigned carry = 0, mx = 0, c = 0;
     for (int base = 0; base < ntiles; base += 1024) {
         const int i = base + threadIdx.x;
         const unsigned v = i < ntiles ? tile_sums[i] : 0u;
+        if constexpr (STATS) {
+            if (i < ntiles) {
+                mx = max(mx, tile_max[i]);
+                c += tile_cnt[i];
+            }
+        }
         unsigned tot;
         const unsigned ex = block_exclusive_scan(v, lw, &tot);
         if (i < ntiles) tile_sums[i] = carry + ex;
         carry += tot;
     }
-    if (threadIdx.x == 0) tile_sums[ntiles] = carry;
+    if constexpr (STATS) block_max_sum(mx, c, lm, lc);
+    if (threadIdx.x == 0) {
+        tile_sums[ntiles] = carry;
+        if constexpr (STATS) {
+            *stat_max = mx;
+            *stat_cnt = c;
+        }
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_scan_apply(const unsigned *in, long long n, const unsigned *tile_prefix,
@@ -260,14 +328,29 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const unsigned *in, long 
     if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = tile_prefix[ntiles];
 }
 
-// out has n + 1 entries; tiles must hold ceil(n / kScanTile) + 1 entries.
-int exclusive_scan(cvg_context *ctx, const unsigned *in, long long n, unsigned *out, DevBuf &tiles)
+// out has n + 1 entries (the tile buffer is sized here).  With stat_max set,
+// the maximum of the inputs goes to *stat_max and the number of inputs >= thr
+// to *stat_cnt.
+int exclusive_scan(cvg_context *ctx, const unsigned *in, long long n, unsigned *out, DevBuf &tiles,
+                   unsigned *stat_max = nullptr, unsigned *stat_cnt = nullptr, unsigned thr = 0)
 {
     const int ntiles = static_cast<int>((n + kScanTile - 1) / kScanTile);
-    CVG_TRY(ensure(ctx, tiles, static_cast<size_t>(ntiles + 1) * sizeof(unsigned)));
+    CVG_TRY(ensure(ctx, tiles, static_cast<size_t>(3) * (ntiles + 1) * sizeof(unsigned)));
     unsigned *t = as<unsigned>(tiles);
-    if (ntiles > 0) hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, ctx->stream, in, n, t);
-    hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(1024), 0, ctx->stream, t, ntiles);
+    unsigned *tmax = t + (ntiles + 1), *tcnt = t + 2 * (ntiles + 1);
+    if (stat_max) {
+        if (ntiles > 0)
+            hipLaunchKernelGGL(k_scan_tiles<true>, dim3(ntiles), dim3(kBlock), 0, ctx->stream, in, n, t, thr, tmax,
+                               tcnt);
+        hipLaunchKernelGGL(k_scan_tile_sums<true>, dim3(1), dim3(1024), 0, ctx->stream, t, ntiles, tmax, tcnt,
+                           stat_max, stat_cnt);
+    } else {
+        if (ntiles > 0)
+            hipLaunchKernelGGL(k_scan_tiles<false>, dim3(ntiles), dim3(kBlock), 0, ctx->stream, in, n, t, thr, tmax,
+                               tcnt);
+        hipLaunchKernelGGL(k_scan_tile_sums<false>, dim3(1), dim3(1024), 0, ctx->stream, t, ntiles, tmax, tcnt,
+                           stat_max, stat_cnt);
+    }
     if (ntiles > 0)
         hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(kBlock), 0, ctx->stream, in, n, t, ntiles, out);
     else
@@ -287,7 +370,9 @@ struct GridDesc {
 // Scalars block layout (d_scal, uint32): [0..3] bbox (minx, maxx, miny, maxy as
 // int), [4] bad-input flag, [5] max degree, [6] truncated anchors,
 // [7] order-ambiguous anchors, [8] valid pairs, [9] anchors needing the
-// std::sort replay, [10..23] bucket counts.
+// std::sort replay, [10..11] the 64-bit count of bundle intersections.  The
+// fused kernel's bucket list lengths follow, one 128-byte line each (atomics
+// from many workgroups on one line serialize in L2).
 enum {
     S_MINX = 0,
     S_MAXX = 1,
@@ -299,13 +384,13 @@ enum {
     S_AMBIG = 7,
     S_VALID = 8,
     S_TIES = 9,
-    S_BUCKET = 10,
-    S_COUNT = 24
+    S_COUNT = 10
 };
-constexpr int kMaxBuckets = S_COUNT - S_BUCKET;
-
-// A 64-bit counter of bundle intersections follows the S_COUNT scalars.
-constexpr int kScalarWords = S_COUNT + 2;
+constexpr int kScalarWords = S_COUNT + 2;  // the words read back to the host
+constexpr int kMaxBuckets = 14;
+constexpr int kBucketStride = 32;  // words
+constexpr int kBucketBase = 32;    // bucket b's list length at [kBucketBase + b * kBucketStride]
+constexpr int kScalarBufWords = kBucketBase + kMaxBuckets * kBucketStride;
 
 __global__ void k_init_scalars(unsigned *s)
 {
@@ -322,10 +407,12 @@ __global__ void k_init_scalars(unsigned *s)
 // Cells are (lrint(x), lrint(y)) (preprocess.hpp:85-86).  Coordinates must be
 // finite with |x|, |y| < 2^30 and |theta| < 2^20 (beyond that the reference's
 // int casts and the trig reduction leave their domain); others raise S_BAD.
+// Each workgroup leaves its cell box and bad flag in part[5 * block]
+// (k_bbox_finish reduces them).
 __global__ __launch_bounds__(kBlock) void k_prep_edges(int n, int sz, const double *__restrict__ e,
                                                        double *__restrict__ x, double *__restrict__ y,
                                                        double *__restrict__ th, int *__restrict__ cx,
-                                                       int *__restrict__ cy, unsigned *__restrict__ scal)
+                                                       int *__restrict__ cy, int *__restrict__ part)
 {
     __shared__ int smin_x, smax_x, smin_y, smax_y, sbad;
     if (threadIdx.x == 0) {
@@ -368,11 +455,54 @@ __global__ __launch_bounds__(kBlock) void k_prep_edges(int n, int sz, const doub
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        atomicMin(reinterpret_cast<int *>(&scal[S_MINX]), smin_x);
-        atomicMax(reinterpret_cast<int *>(&scal[S_MAXX]), smax_x);
-        atomicMin(reinterpret_cast<int *>(&scal[S_MINY]), smin_y);
-        atomicMax(reinterpret_cast<int *>(&scal[S_MAXY]), smax_y);
-        if (sbad) atomicOr(&scal[S_BAD], 1u);
+        int *o = part + 5 * static_cast<long long>(blockIdx.x);
+        o[0] = smin_x;
+        o[1] = smax_x;
+        o[2] = smin_y;
+        o[3] = smax_y;
+        o[4] = sbad;
+    }
+}
+
+// One workgroup: the cell box and bad flag over k_prep_edges' nparts partials
+// into the scalars (a tree, not same-address atomics, which serialize).
+__global__ __launch_bounds__(1024) void k_bbox_finish(int nparts, const int *__restrict__ part,
+                                                      unsigned *__restrict__ scal)
+{
+    __shared__ int l[5][1024 / 64];
+    int v[5] = {INT_MAX, INT_MIN, INT_MAX, INT_MIN, 0};
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+        const int *p = part + 5 * static_cast<long long>(i);
+        v[0] = min(v[0], p[0]);
+        v[1] = max(v[1], p[1]);
+        v[2] = min(v[2], p[2]);
+        v[3] = max(v[3], p[3]);
+        v[4] |= p[4];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        v[0] = min(v[0], __shfl_xor(v[0], o, 64));
+        v[1] = max(v[1], __shfl_xor(v[1], o, 64));
+        v[2] = min(v[2], __shfl_xor(v[2], o, 64));
+        v[3] = max(v[3], __shfl_xor(v[3], o, 64));
+        v[4] |= __shfl_xor(v[4], o, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 5; ++k) l[k][w] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int u = 1; u < static_cast<int>(blockDim.x >> 6); ++u) {
+            v[0] = min(v[0], l[0][u]);
+            v[1] = max(v[1], l[1][u]);
+            v[2] = min(v[2], l[2][u]);
+            v[3] = max(v[3], l[3][u]);
+            v[4] |= l[4][u];
+        }
+        scal[S_MINX] = static_cast<unsigned>(v[0]);
+        scal[S_MAXX] = static_cast<unsigned>(v[1]);
+        scal[S_MINY] = static_cast<unsigned>(v[2]);
+        scal[S_MAXY] = static_cast<unsigned>(v[3]);
+        scal[S_BAD] = v[4] ? 1u : 0u;
     }
 }
 
@@ -479,12 +609,14 @@ __global__ __launch_bounds__(kBlock) void k_cell_scatter(int n, const int *__res
 // Within-cell order: ascending edge id (the reference's order is whatever its
 // unstable std::sort of the index permutation yields, preprocess.hpp:90-92;
 // the two agree for up to 16 edges and only matter for the cases that
-// k_fill_csr counts as order-ambiguous).
+// k_fill_csr counts as order-ambiguous).  The cell-sorted arrays are the edge
+// id, its cell index and its (x, y) pair, so one window candidate costs one
+// int, one int and one 16-byte load, all independent.
 __global__ __launch_bounds__(kBlock) void k_cell_order(int n, const int *__restrict__ cx, const int *__restrict__ cy,
                                                        GridDesc g, const unsigned *__restrict__ start,
                                                        const int *__restrict__ tmp, const double *__restrict__ x,
                                                        const double *__restrict__ y, int *__restrict__ sorted,
-                                                       double *__restrict__ sx, double *__restrict__ sy)
+                                                       int *__restrict__ scell, double2 *__restrict__ sxy)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -494,41 +626,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_order(int n, const int *__restr
     for (unsigned k = s; k < e; ++k) r += (tmp[k] < i) ? 1u : 0u;
     const unsigned p = s + r;
     sorted[p] = i;
-    sx[p] = x[i];
-    sy[p] = y[i];
-}
-
-// discover_neighbors_csr counting pass (preprocess.hpp:114-184): the window is
-// rows py = cy-R..cy+R, and in each row the cells cx-R..cx+R are one
-// contiguous range of the cell-sorted arrays.  Accept d2 <= rad^2 in the
-// reference's form `!(d2 > rad2)`; keep at most max_candidates.
-__global__ __launch_bounds__(kBlock) void k_discover_count(int n, const double *__restrict__ x,
-                                                           const double *__restrict__ y, const int *__restrict__ cx,
-                                                           const int *__restrict__ cy, GridDesc g, int R, double rad2,
-                                                           unsigned max_cand, const unsigned *__restrict__ start,
-                                                           const int *__restrict__ sorted,
-                                                           const double *__restrict__ sx,
-                                                           const double *__restrict__ sy, unsigned *__restrict__ deg,
-                                                           unsigned *__restrict__ scal)
-{
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= n) return;
-    const double ax = x[a], ay = y[a];
-    const int ix = cx[a] - g.x0, iy = cy[a] - g.y0;
-    unsigned cnt = 0;
-    for (int dy = -R; dy <= R; ++dy) {
-        const int row = (iy + dy) * g.w;
-        const unsigned s = start[row + ix - R], e = start[row + ix + R + 1];
-        for (unsigned k = s; k < e; ++k) {
-            if (sorted[k] == a) continue;
-            const double d = (ax - sx[k]) * (ax - sx[k]) + (ay - sy[k]) * (ay - sy[k]);
-            if (!(d > rad2)) ++cnt;
-        }
-    }
-    const unsigned d = cnt < max_cand ? cnt : max_cand;
-    deg[a] = d;
-    atomicMax(&scal[S_MAXDEG], d);
-    if (d >= max_cand) atomicAdd(&scal[S_TRUNC], 1u);  // preprocess.hpp:224, a >= test
+    scell[p] = c;
+    sxy[p] = make_double2(x[i], y[i]);
 }
 
 // LDS or global writes by some lanes made visible to the other lanes of the
@@ -551,154 +650,248 @@ __device__ inline double bcast(double v, int lane)
 
 __device__ inline int bcast(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 
-constexpr int kFillWaves = 4;  // anchors (waves) per workgroup in k_fill_csr
+// The neighbour-window scans give each anchor a segment of G lanes (64 / G
+// anchors per wave).  The window is rows py = cy-R..cy+R; in each row the cells
+// cx-R..cx+R are one contiguous range of the cell-sorted arrays
+// (preprocess.hpp:141-150).  Rows rb .. rb+G-1 are loaded at once, lane s of
+// the segment holding row rb+s: its first sorted position `rs` and the
+// exclusive prefix `pre` of the row lengths; `tot` is the rows' total.  Row
+// loads are independent, so the whole window costs one load latency.
+template <int G>
+__device__ inline void window_rows(const unsigned *__restrict__ start, GridDesc g, int ix, int iy, int R, int rb,
+                                   int sl, bool valid, int &rs, int &pre, int &tot)
+{
+    const int r = rb + sl;
+    int len = 0;
+    rs = 0;
+    if (valid && r <= 2 * R) {
+        const int rr = (iy - R + r) * g.w;
+        rs = static_cast<int>(start[rr + ix - R]);
+        len = static_cast<int>(start[rr + ix + R + 1]) - rs;
+    }
+    int inc = len;
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) {
+        const int v = __shfl_up(inc, o, G);
+        if (sl >= o) inc += v;
+    }
+    pre = inc - len;
+    tot = __shfl(inc, G - 1, G);
+}
+
+// Sorted position of the t-th candidate (0 <= t < tot) of rows rb..rb+nrows-1,
+// or -1.  All lanes must call it (nrows is wave-uniform).
+template <int G>
+__device__ inline int window_pos(int rs, int pre, int tot, int nrows, int t)
+{
+    int k = -1;
+    for (int j = 0; j < nrows; ++j) {
+        const int pj = __shfl(pre, j, G);
+        const int sj = __shfl(rs, j, G);
+        if (t >= pj) k = sj + (t - pj);
+    }
+    return t < tot ? k : -1;
+}
+
+template <int G>
+__device__ inline unsigned long long segment_mask(int lane)
+{
+    if constexpr (G == 64) return ~0ULL;
+    else return ((1ULL << G) - 1ULL) << (lane & ~(G - 1));
+}
+
+// Largest value of a segment-uniform quantity over the wave's segments.
+template <int G>
+__device__ inline int wave_max_over_segments(int v)
+{
+    int m = bcast(v, 0);
+#pragma unroll
+    for (int s = 1; s < 64 / G; ++s) m = max(m, bcast(v, s * G));
+    return m;
+}
+
+// discover_neighbors_csr counting pass (preprocess.hpp:114-184): in-radius
+// candidates in the reference's form `!(d2 > rad2)`, the anchor itself
+// excluded, at most max_candidates kept.  One G-lane segment per anchor.
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_discover_count(int n, const double *__restrict__ x,
+                                                           const double *__restrict__ y, const int *__restrict__ cx,
+                                                           const int *__restrict__ cy, GridDesc g, int R, double rad2,
+                                                           unsigned max_cand, const unsigned *__restrict__ start,
+                                                           const int *__restrict__ sorted,
+                                                           const double2 *__restrict__ sxy,
+                                                           unsigned *__restrict__ deg)
+{
+    const int lane = threadIdx.x & 63;
+    const int sl = lane & (G - 1);
+    const int a = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / G);
+    const bool valid = a < n;
+    if (__ballot(valid) == 0ULL) return;  // wave-uniform
+    const unsigned long long segmask = segment_mask<G>(lane);
+    double ax = 0.0, ay = 0.0;
+    int ix = 0, iy = 0;
+    if (valid) {
+        ax = x[a];
+        ay = y[a];
+        ix = cx[a] - g.x0;
+        iy = cy[a] - g.y0;
+    }
+    unsigned cnt = 0;
+    for (int rb = 0; rb <= 2 * R; rb += G) {
+        int rs, pre, tot;
+        window_rows<G>(start, g, ix, iy, R, rb, sl, valid, rs, pre, tot);
+        const int nrows = min(G, 2 * R + 1 - rb);
+        for (int t0 = 0; __ballot(t0 < tot) != 0ULL; t0 += G) {
+            const int k = window_pos<G>(rs, pre, tot, nrows, t0 + sl);
+            bool pass = false;
+            if (k >= 0) {
+                const double2 p = sxy[k];
+                const double d = (ax - p.x) * (ax - p.x) + (ay - p.y) * (ay - p.y);
+                pass = sorted[k] != a && !(d > rad2);
+            }
+            cnt += static_cast<unsigned>(__popcll(__ballot(pass) & segmask));
+        }
+    }
+    // the largest degree and the anchors that reached max_candidates
+    // (preprocess.hpp:224, a >= test) come out of the offsets scan
+    if (valid && sl == 0) deg[a] = cnt < max_cand ? cnt : max_cand;
+}
+
+constexpr int kFillWaves = 4;  // waves per workgroup in k_fill_csr
 
 // discover_neighbors_csr gathering pass + the distance sort (preprocess.hpp:141-183)
-// + the CSR fill (:240-262), one wave per anchor.  The lanes walk the window's
-// candidates in the reference's scan order (py outer, px inner, then
-// within-cell order): the 2R+1 rows are contiguous ranges of the cell-sorted
-// arrays, flattened and taken 64 at a time; a ballot prefix keeps the first
-// max_candidates in-radius ones.  Then every lane ranks its candidates by d2
-// against the whole list (ties by scan position, which is the insertion-sort
-// order std::sort yields up to 16 entries); lists with ties and more than 16
-// entries are queued for the std::sort replay (k_tie_sort).
+// + the CSR fill (:240-262), one G-lane segment per anchor.  The segment walks
+// the window's candidates in the reference's scan order (py outer, px inner,
+// then within-cell order), G at a time; a ballot prefix keeps the first
+// max_candidates in-radius ones, in scan order, in the segment's LDS.  Then
+// every lane ranks its candidates by d2 against the whole list (ties by scan
+// position, which is the insertion-sort order std::sort yields up to 16
+// entries); lists with ties and more than 16 entries get their scan-order
+// copy in `cand` and are queued for the std::sort replay (k_tie_sort).
+template <int G>
 __global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const double *__restrict__ x,
                                                               const double *__restrict__ y, const int *__restrict__ cx,
                                                               const int *__restrict__ cy, GridDesc g, int R, double rad2,
                                                               const unsigned *__restrict__ start,
                                                               const int *__restrict__ sorted,
-                                                              const double *__restrict__ sx,
-                                                              const double *__restrict__ sy,
+                                                              const int *__restrict__ scell,
+                                                              const double2 *__restrict__ sxy,
                                                               const unsigned *__restrict__ off, Cand *__restrict__ cand,
                                                               int *__restrict__ ids, double *__restrict__ d2,
                                                               int *__restrict__ pair_anchor, int *__restrict__ tie_list,
                                                               unsigned max_cand, int cap,
                                                               unsigned *__restrict__ scal)
 {
-    // per wave: cap candidates (d2, id, cell) in LDS for the ranking
+    // per segment: cap candidates (d2, id, cell) in LDS for the ranking
     extern __shared__ double fill_lds[];
     const int lane = threadIdx.x & 63;
-    const int w = static_cast<int>(threadIdx.x >> 6);
-    double *ld = fill_lds + static_cast<size_t>(w) * cap * 2;
+    const int sl = lane & (G - 1);
+    const int seg = static_cast<int>(threadIdx.x) / G;
+    double *ld = fill_lds + static_cast<size_t>(seg) * cap * 2;
     int *lid = reinterpret_cast<int *>(ld + cap);
     int *lcell = lid + cap;
-    const int a = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x >> 6) + w;
-    if (a >= n) return;
+    const int a = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x / G) + seg;
+    const bool valid = a < n;
+    if (__ballot(valid) == 0ULL) return;  // wave-uniform
+    const unsigned long long segmask = segment_mask<G>(lane);
     const unsigned long long lt_mask = (1ULL << lane) - 1ULL;
-    const double ax = x[a], ay = y[a];
-    const int ix = cx[a] - g.x0, iy = cy[a] - g.y0;
-    const unsigned o = off[a];
-    const int dg = static_cast<int>(off[a + 1] - o);
-    // scan: rows dy = -R..R, each [rs, re) in the cell-sorted arrays.  Lane t
-    // of a 64-window finds its row by walking the (uniform) row lengths.
+    double ax = 0.0, ay = 0.0;
+    int ix = 0, iy = 0, dg = 0;
+    unsigned o = 0;
+    if (valid) {
+        ax = x[a];
+        ay = y[a];
+        ix = cx[a] - g.x0;
+        iy = cy[a] - g.y0;
+        o = off[a];
+        dg = static_cast<int>(off[a + 1] - o);
+    }
     int kept = 0;
-    int last_cell = -1;        // cell of the last kept candidate
+    int last_cell = -1;  // cell of the last kept candidate
     bool ambiguous = false;
-    bool done = false;
-    int row = 0;               // first row not yet exhausted, and its consumed count
-    int row_used = 0;
-    while (!done && row <= 2 * R) {
-        // the 64 next candidates in scan order
-        int r = row, used = row_used, t = lane, k = -1;
-        while (r <= 2 * R) {
-            const int rr = (iy - R + r) * g.w;
-            const int rs = static_cast<int>(start[rr + ix - R]);
-            const int rlen = static_cast<int>(start[rr + ix + R + 1]) - rs;
-            if (t < rlen - used) {
-                k = rs + used + t;
-                break;
+    bool done = !valid;
+    for (int rb = 0; rb <= 2 * R && __ballot(!done) != 0ULL; rb += G) {
+        int rs, pre, tot;
+        window_rows<G>(start, g, ix, iy, R, rb, sl, valid, rs, pre, tot);
+        const int nrows = min(G, 2 * R + 1 - rb);
+        for (int t0 = 0; __ballot(!done && t0 < tot) != 0ULL; t0 += G) {
+            int k = window_pos<G>(rs, pre, tot, nrows, t0 + sl);
+            if (done) k = -1;
+            bool pass = false;
+            double d = 0.0;
+            int b = -1, cell = -1;
+            if (k >= 0) {
+                b = sorted[k];
+                cell = scell[k];
+                const double2 p = sxy[k];
+                d = (ax - p.x) * (ax - p.x) + (ay - p.y) * (ay - p.y);
+                pass = b != a && !(d > rad2);
             }
-            t -= rlen - used;
-            used = 0;
-            ++r;
-        }
-        bool pass = false;
-        double d = 0.0;
-        int b = -1, cell = -1;
-        if (k >= 0) {
-            b = sorted[k];
-            d = (ax - sx[k]) * (ax - sx[k]) + (ay - sy[k]) * (ay - sy[k]);
-            pass = b != a && !(d > rad2);
-            if (pass) cell = (cy[b] - g.y0) * g.w + (cx[b] - g.x0);
-        }
-        const unsigned long long m = __ballot(pass);
-        const int pos = kept + __popcll(m & lt_mask);
-        if (pass && pos < dg) {
-            cand[o + pos] = Cand{d, b};  // scan order, for the std::sort replay
-            ld[pos] = d;
-            lid[pos] = b;
-            lcell[pos] = cell;
-        }
-        // a max_candidates cut inside one cell makes the kept set depend on
-        // the within-cell order
-        const bool beyond = pass && pos == dg;
-        const unsigned long long bm = __ballot(beyond);
-        if (bm != 0ULL) {
-            const int bl = __ffsll(static_cast<long long>(bm)) - 1;
-            const int bcell = bcast(cell, bl);
-            int prev = last_cell;
-            const unsigned long long pm = __ballot(pass && pos == dg - 1);
-            if (pm != 0ULL) prev = bcast(cell, __ffsll(static_cast<long long>(pm)) - 1);
-            if (dg > 0 && prev == bcell) ambiguous = true;
-            done = true;
-        } else if (m != 0ULL) {
-            const int hl = 63 - __clzll(static_cast<long long>(m));  // last passing lane
-            if (kept + __popcll(m) <= dg) last_cell = bcast(cell, hl);
-        }
-        kept += __popcll(m);
-        // without a cut (dg < max_candidates) nothing passes after the dg-th
-        if (kept >= dg && static_cast<unsigned>(dg) < max_cand) done = true;
-        // advance the row cursor past the 64 candidates just taken
-        const unsigned long long valid = __ballot(k >= 0);
-        if (valid != ~0ULL) break;  // fewer than 64 left: the window is exhausted
-        int adv = 64;
-        while (row <= 2 * R && adv > 0) {
-            const int rr = (iy - R + row) * g.w;
-            const int rlen = static_cast<int>(start[rr + ix + R + 1]) - static_cast<int>(start[rr + ix - R]);
-            const int left = rlen - row_used;
-            if (adv < left) {
-                row_used += adv;
-                adv = 0;
-            } else {
-                adv -= left;
-                row_used = 0;
-                ++row;
+            const unsigned long long m = __ballot(pass) & segmask;
+            const int pos = kept + __popcll(m & lt_mask);
+            if (pass && pos < dg) {
+                ld[pos] = d;
+                lid[pos] = b;
+                lcell[pos] = cell;
             }
+            // a max_candidates cut inside one cell makes the kept set depend on
+            // the within-cell order: compare the first dropped candidate's cell
+            // with the last kept one's
+            const unsigned long long bm = __ballot(pass && pos == dg) & segmask;
+            const unsigned long long pm = __ballot(pass && pos == dg - 1) & segmask;
+            const int bcell = __shfl(cell, bm != 0ULL ? __ffsll(static_cast<long long>(bm)) - 1 : lane);
+            const int pcell = __shfl(cell, pm != 0ULL ? __ffsll(static_cast<long long>(pm)) - 1 : lane);
+            const int hcell = __shfl(cell, m != 0ULL ? 63 - __clzll(static_cast<long long>(m)) : lane);
+            if (bm != 0ULL) {
+                const int prev = pm != 0ULL ? pcell : last_cell;
+                if (dg > 0 && prev == bcell) ambiguous = true;
+                done = true;
+            } else if (m != 0ULL) {
+                if (kept + __popcll(m) <= dg) last_cell = hcell;
+            }
+            kept += __popcll(m);
+            // without a cut (dg < max_candidates) nothing passes after the dg-th
+            if (kept >= dg && static_cast<unsigned>(dg) < max_cand) done = true;
         }
     }
     wave_sync();
     // rank sort by d2 with ties by scan position; detect ties and same-cell ties
     bool tie = false;
-    for (int i0 = 0; i0 < dg; i0 += 64) {
-        const int i = i0 + lane;
+    const int dmax = wave_max_over_segments<G>(dg);
+    for (int i0 = 0; i0 < dmax; i0 += G) {
+        const int i = i0 + sl;
         const bool active = i < dg;
-        Cand ci{0.0, 0};
-        int celli = -1;
+        double di = 0.0;
+        int idi = 0, celli = -1;
         if (active) {
-            ci = Cand{ld[i], lid[i]};
+            di = ld[i];
+            idi = lid[i];
             celli = lcell[i];
         }
         int rank = 0;
-        for (int j = 0; j < dg; ++j) {
-            const double dj = ld[j];
-            const int cellj = lcell[j];
-            rank += (dj < ci.d || (dj == ci.d && j < i)) ? 1 : 0;
-            const bool eq = active && dj == ci.d && j != i;
-            tie = tie || eq;
-            ambiguous = ambiguous || (eq && cellj == celli);
+        for (int j = 0; j < dmax; ++j) {
+            if (j < dg) {
+                const double dj = ld[j];
+                const int cellj = lcell[j];
+                rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
+                const bool eq = active && dj == di && j != i;
+                tie = tie || eq;
+                ambiguous = ambiguous || (eq && cellj == celli);
+            }
         }
         if (active) {
-            ids[o + rank] = ci.id;
-            d2[o + rank] = ci.d;
+            ids[o + rank] = idi;
+            d2[o + rank] = di;
         }
     }
-    const bool any_tie = __ballot(tie) != 0ULL;
-    const bool any_amb = __ballot(ambiguous) != 0ULL;
-    for (int i = lane; i < dg; i += 64) pair_anchor[o + i] = a;
-    if (lane == 0) {
-        if (any_tie && dg > 16) tie_list[atomicAdd(&scal[S_TIES], 1u)] = a;
-        if (any_amb && n > 16) atomicAdd(&scal[S_AMBIG], 1u);
+    const bool any_tie = (__ballot(tie) & segmask) != 0ULL;
+    const bool any_amb = (__ballot(ambiguous) & segmask) != 0ULL;
+    for (int i = sl; i < dg; i += G) pair_anchor[o + i] = a;
+    if (any_tie && dg > 16) {
+        for (int i = sl; i < dg; i += G) cand[o + i] = Cand{ld[i], lid[i]};  // scan order
+        if (sl == 0) tie_list[atomicAdd(&scal[S_TIES], 1u)] = a;
     }
+    if (sl == 0 && any_amb && n > 16) atomicAdd(&scal[S_AMBIG], 1u);
 }
 
 // Anchors whose (more than 16) candidates contain exact distance ties: replay
@@ -750,8 +943,7 @@ __global__ __launch_bounds__(kBlock) void k_edge_trig(int n, const double *__res
 }
 
 // Per CSR pair q = (a, b): the direction filter (cpu_curvelet.hpp:459-461),
-// the cell-independent part of compute_curve_bundle (:646-656, 682, 690) and
-// per-anchor class counts cnt3[3a + {0: class 1, 1: class 2, 2: class 0}].
+// and the cell-independent part of compute_curve_bundle (:646-656, 682, 690).
 //   pflag[q] = class | close << 2 | branch_a << 3
 //   prec[8q .. 8q+7] = DX, DY, DX2, DY2, DX cos(T0-dt), DY sin(T0-dt),
 //                      DX cos(T0+dt), DY sin(T0+dt)
@@ -759,7 +951,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_prep(int pairs, int n, const in
                                                       const int *__restrict__ ids, const double *__restrict__ x,
                                                       const double *__restrict__ y, const double *__restrict__ trig,
                                                       double close_thr, unsigned char *__restrict__ pflag,
-                                                      double *__restrict__ prec, unsigned *__restrict__ cnt3)
+                                                      double *__restrict__ prec)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= pairs) return;
@@ -780,14 +972,16 @@ __global__ __launch_bounds__(kBlock) void k_pair_prep(int pairs, int n, const in
     o[1] = make_double2(p.DX2, p.DY2);
     o[2] = make_double2(p.cT0m_DX, p.sT0m_DY);
     o[3] = make_double2(p.cT0p_DX, p.sT0p_DY);
-    atomicAdd(&cnt3[3 * a + (k == 1 ? 0 : (k == 2 ? 1 : 2))], 1u);
 }
 
 // Anchors in spatial (cell) order, binned by the largest of their three class
-// counts, which sizes the LDS slots the fused kernel needs; each list entry
-// carries (anchor, CSR base, degree).
-__global__ __launch_bounds__(kBlock) void k_bucket(int n, const int *__restrict__ sorted,
-                                                   const unsigned *__restrict__ cnt3, const unsigned *__restrict__ off,
+// counts (read from the anchor's run of pair flags), which sizes the LDS slots
+// the fused kernel needs; each list entry carries (anchor, CSR base, degree).
+constexpr int kBucketBlock = 1024;  // few workgroups: one list-length atomic each per bucket
+
+__global__ __launch_bounds__(kBucketBlock) void k_bucket(int n, const int *__restrict__ sorted,
+                                                   const unsigned char *__restrict__ pflag,
+                                                   const unsigned *__restrict__ off,
                                                    int nb, const int *caps, int4 *__restrict__ lists,
                                                    unsigned *__restrict__ scal)
 {
@@ -798,9 +992,19 @@ __global__ __launch_bounds__(kBlock) void k_bucket(int n, const int *__restrict_
     if (threadIdx.x < kMaxBuckets) s_count[threadIdx.x] = 0;
     __syncthreads();
     int a = -1, b = -1;
+    unsigned o0 = 0, o1 = 0;
     if (p < n) {
         a = sorted[p];
-        const unsigned m = max(cnt3[3 * a], max(cnt3[3 * a + 1], cnt3[3 * a + 2]));
+        o0 = off[a];
+        o1 = off[a + 1];
+        unsigned c[3] = {0u, 0u, 0u};
+        for (unsigned q = o0; q < o1; ++q) {
+            const unsigned k = pflag[q] & 3u;
+            c[0] += k == 0u ? 1u : 0u;
+            c[1] += k == 1u ? 1u : 0u;
+            c[2] += k == 2u ? 1u : 0u;
+        }
+        const unsigned m = max(c[0], max(c[1], c[2]));
         b = 0;
         while (b < nb - 1 && m > static_cast<unsigned>(caps[b])) ++b;
     }
@@ -818,13 +1022,11 @@ __global__ __launch_bounds__(kBlock) void k_bucket(int n, const int *__restrict_
     }
     __syncthreads();
     if (threadIdx.x < nb && s_count[threadIdx.x] > 0)
-        s_base[threadIdx.x] = atomicAdd(&scal[S_BUCKET + threadIdx.x], s_count[threadIdx.x]);
+        s_base[threadIdx.x] = atomicAdd(&scal[kBucketBase + threadIdx.x * kBucketStride], s_count[threadIdx.x]);
     __syncthreads();
-    if (b >= 0) {
-        const unsigned o0 = off[a];
+    if (b >= 0)
         lists[static_cast<long long>(b) * n + s_base[b] + my_off] =
-            make_int4(a, static_cast<int>(o0), static_cast<int>(off[a + 1] - o0), 0);
-    }
+            make_int4(a, static_cast<int>(o0), static_cast<int>(o1 - o0), 0);
 }
 
 // Per-pair sines of the transport (compute_curve_bundle, cpu_curvelet.hpp:728-729):
@@ -1612,11 +1814,15 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     CVG_TRY(ensure(ctx, ctx->d_th, nd));
     CVG_TRY(ensure(ctx, ctx->d_cx, ni));
     CVG_TRY(ensure(ctx, ctx->d_cy, ni));
-    CVG_TRY(ensure(ctx, ctx->d_scal, kScalarWords * sizeof(unsigned)));
+    CVG_TRY(ensure(ctx, ctx->d_scal, kScalarBufWords * sizeof(unsigned)));
     unsigned *scal = as<unsigned>(ctx->d_scal);
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, st, scal);
-    hipLaunchKernelGGL(k_prep_edges, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, sz, d_edges, as<double>(ctx->d_x),
-                       as<double>(ctx->d_y), as<double>(ctx->d_th), as<int>(ctx->d_cx), as<int>(ctx->d_cy), scal);
+    const int prep_blocks = blocks_for(n);
+    CVG_TRY(ensure(ctx, ctx->d_part, static_cast<size_t>(prep_blocks) * 5 * sizeof(int)));
+    hipLaunchKernelGGL(k_prep_edges, dim3(prep_blocks), dim3(kBlock), 0, st, n, sz, d_edges, as<double>(ctx->d_x),
+                       as<double>(ctx->d_y), as<double>(ctx->d_th), as<int>(ctx->d_cx), as<int>(ctx->d_cy),
+                       as<int>(ctx->d_part));
+    hipLaunchKernelGGL(k_bbox_finish, dim3(1), dim3(1024), 0, st, prep_blocks, as<int>(ctx->d_part), scal);
     CVG_HIP(ctx, hipGetLastError());
     CVG_TRY(read_scalars(ctx));
     const unsigned *mb = ctx->mailbox;
@@ -1710,25 +1916,36 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     CVG_TRY(exclusive_scan(ctx, cnt, cells, start, ctx->d_tiles));
     CVG_TRY(ensure(ctx, ctx->d_tmp_ids, ni));
     CVG_TRY(ensure(ctx, ctx->d_sorted, ni));
-    CVG_TRY(ensure(ctx, ctx->d_sx, nd));
-    CVG_TRY(ensure(ctx, ctx->d_sy, nd));
+    CVG_TRY(ensure(ctx, ctx->d_sxy, 2 * nd));
+    CVG_TRY(ensure(ctx, ctx->d_scell, ni));
     hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
                        as<int>(ctx->d_cy), g, start, cnt, as<int>(ctx->d_tmp_ids));
     hipLaunchKernelGGL(k_cell_order, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
                        as<int>(ctx->d_cy), g, start, as<int>(ctx->d_tmp_ids), as<double>(ctx->d_x),
-                       as<double>(ctx->d_y), as<int>(ctx->d_sorted), as<double>(ctx->d_sx), as<double>(ctx->d_sy));
+                       as<double>(ctx->d_y), as<int>(ctx->d_sorted), as<int>(ctx->d_scell),
+                       as<double2>(ctx->d_sxy));
     CVG_HIP(ctx, hipGetLastError());
 
     // neighbour counts and CSR offsets
     CVG_TRY(ensure(ctx, ctx->d_deg, ni));
     CVG_TRY(ensure(ctx, ctx->d_off, static_cast<size_t>(n + 1) * sizeof(unsigned)));
     const double rad2 = rad * rad;  // preprocess.hpp:221
-    hipLaunchKernelGGL(k_discover_count, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<double>(ctx->d_x),
-                       as<double>(ctx->d_y), as<int>(ctx->d_cx), as<int>(ctx->d_cy), g, static_cast<int>(R), rad2,
-                       max_cand, start, as<int>(ctx->d_sorted), as<double>(ctx->d_sx), as<double>(ctx->d_sy),
-                       as<unsigned>(ctx->d_deg), scal);
+    {
+        using CountFn = void (*)(int, const double *, const double *, const int *, const int *, GridDesc, int, double,
+                                 unsigned, const unsigned *, const int *, const double2 *, unsigned *);
+        const int G = ctx->fill_lanes;
+        const CountFn fn = G == 8    ? k_discover_count<8>
+                           : G == 32 ? k_discover_count<32>
+                           : G == 64 ? k_discover_count<64>
+                                     : k_discover_count<16>;
+        hipLaunchKernelGGL(fn, dim3(blocks_for(static_cast<long long>(n) * G)), dim3(kBlock), 0, st, n,
+                           as<double>(ctx->d_x), as<double>(ctx->d_y), as<int>(ctx->d_cx), as<int>(ctx->d_cy), g,
+                           static_cast<int>(R), rad2, max_cand, start, as<int>(ctx->d_sorted),
+                           as<double2>(ctx->d_sxy), as<unsigned>(ctx->d_deg));
+    }
     CVG_HIP(ctx, hipGetLastError());
-    CVG_TRY(exclusive_scan(ctx, as<unsigned>(ctx->d_deg), n, as<unsigned>(ctx->d_off), ctx->d_tiles));
+    CVG_TRY(exclusive_scan(ctx, as<unsigned>(ctx->d_deg), n, as<unsigned>(ctx->d_off), ctx->d_tiles,
+                           scal + S_MAXDEG, scal + S_TRUNC, max_cand));
     unsigned pairs_u = 0;
     CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox + 32, as<unsigned>(ctx->d_off) + n, sizeof(unsigned),
                                 hipMemcpyDeviceToHost, st));
@@ -1749,21 +1966,30 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     CVG_TRY(ensure(ctx, ctx->d_ids, np * sizeof(int)));
     CVG_TRY(ensure(ctx, ctx->d_d2, np * sizeof(double)));
     CVG_TRY(ensure(ctx, ctx->d_pair_anchor, np * sizeof(int)));
-    // one wave per anchor; each wave keeps its candidates in LDS
-    int fill_waves = kFillWaves;
+    // one G-lane segment per anchor; each segment keeps its candidates in LDS
+    using FillFn = void (*)(int, const double *, const double *, const int *, const int *, GridDesc, int, double,
+                            const unsigned *, const int *, const int *, const double2 *, const unsigned *, Cand *,
+                            int *, double *, int *, int *, unsigned, int, unsigned *);
     const int fill_cap = static_cast<int>(std::max(ctx->max_deg, 1u));
-    const size_t per_wave = static_cast<size_t>(fill_cap) * (2 * sizeof(double));
-    while (fill_waves > 1 && per_wave * fill_waves > ctx->lds_limit) fill_waves /= 2;
-    if (per_wave * fill_waves > ctx->lds_limit)
+    const size_t per_seg = static_cast<size_t>(fill_cap) * (2 * sizeof(double));
+    int G = ctx->fill_lanes;
+    // fewer anchors per wave when the lists are long
+    while (G < 64 && per_seg * (64 / G) * kFillWaves > ctx->lds_limit) G *= 2;
+    int fill_waves = kFillWaves;
+    while (fill_waves > 1 && per_seg * (64 / G) * fill_waves > ctx->lds_limit) fill_waves /= 2;
+    if (per_seg * (64 / G) * fill_waves > ctx->lds_limit)
         return fail(ctx, CVG_ERR_UNSUPPORTED, "an anchor has too many neighbours for one workgroup's LDS");
-    const unsigned fill_shmem = static_cast<unsigned>(per_wave * fill_waves);
-    CVG_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(k_fill_csr),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(fill_shmem)));
-    hipLaunchKernelGGL(k_fill_csr, dim3((n + fill_waves - 1) / fill_waves), dim3(64 * fill_waves), fill_shmem, st, n, as<double>(ctx->d_x),
-                       as<double>(ctx->d_y), as<int>(ctx->d_cx), as<int>(ctx->d_cy), g, static_cast<int>(R), rad2,
-                       start, as<int>(ctx->d_sorted), as<double>(ctx->d_sx), as<double>(ctx->d_sy),
-                       as<unsigned>(ctx->d_off), as<Cand>(ctx->d_cand), as<int>(ctx->d_ids), as<double>(ctx->d_d2),
-                       as<int>(ctx->d_pair_anchor), as<int>(ctx->d_tmp_ids), max_cand, fill_cap, scal);
+    const FillFn fill = G == 8 ? k_fill_csr<8> : G == 16 ? k_fill_csr<16> : G == 32 ? k_fill_csr<32> : k_fill_csr<64>;
+    const int anchors_per_block = (64 / G) * fill_waves;
+    const unsigned fill_shmem = static_cast<unsigned>(per_seg * anchors_per_block);
+    CVG_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(fill), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     static_cast<int>(fill_shmem)));
+    hipLaunchKernelGGL(fill, dim3((n + anchors_per_block - 1) / anchors_per_block), dim3(64 * fill_waves), fill_shmem,
+                       st, n, as<double>(ctx->d_x), as<double>(ctx->d_y), as<int>(ctx->d_cx), as<int>(ctx->d_cy), g,
+                       static_cast<int>(R), rad2, start, as<int>(ctx->d_sorted), as<int>(ctx->d_scell),
+                       as<double2>(ctx->d_sxy), as<unsigned>(ctx->d_off), as<Cand>(ctx->d_cand),
+                       as<int>(ctx->d_ids), as<double>(ctx->d_d2), as<int>(ctx->d_pair_anchor),
+                       as<int>(ctx->d_tmp_ids), max_cand, fill_cap, scal);
     hipLaunchKernelGGL(k_tie_sort, dim3(std::min(blocks_for(n), 64)), dim3(kBlock), 0, st, as<int>(ctx->d_tmp_ids),
                        scal, as<unsigned>(ctx->d_off), as<Cand>(ctx->d_cand), as<int>(ctx->d_ids),
                        as<double>(ctx->d_d2));
@@ -1919,25 +2145,23 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     CVG_TRY(ensure(ctx, ctx->d_cls, static_cast<size_t>(pairs) + 1));
     CVG_TRY(ensure(ctx, ctx->d_prec, static_cast<size_t>(std::max(pairs, 1)) * 8 * sizeof(double)));
     CVG_TRY(ensure(ctx, ctx->d_psin, static_cast<size_t>(std::max(pairs, 1)) * d.kt * 2 * sizeof(double)));
-    CVG_TRY(ensure(ctx, ctx->d_cnt3, static_cast<size_t>(3) * n * sizeof(unsigned)));
     CVG_TRY(ensure(ctx, ctx->d_lists, static_cast<size_t>(nb) * n * sizeof(int4)));
     unsigned *scal = as<unsigned>(ctx->d_scal);
     hipLaunchKernelGGL(k_edge_trig, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<double>(ctx->d_th), d.dt,
                        static_cast<int>(d.kt), T.off_t, as<double>(ctx->d_trig));
-    CVG_HIP(ctx, hipMemsetAsync(ctx->d_cnt3.ptr, 0, static_cast<size_t>(3) * n * sizeof(unsigned), st));
-    CVG_HIP(ctx, hipMemsetAsync(scal + S_VALID, 0, (kScalarWords - S_VALID) * sizeof(unsigned), st));
+    CVG_HIP(ctx, hipMemsetAsync(scal + S_VALID, 0, (kScalarBufWords - S_VALID) * sizeof(unsigned), st));
     if (pairs > 0) {
         hipLaunchKernelGGL(k_pair_prep, dim3(blocks_for(pairs)), dim3(kBlock), 0, st, pairs, n,
                            as<int>(ctx->d_pair_anchor), as<int>(ctx->d_ids), as<double>(ctx->d_x),
                            as<double>(ctx->d_y), as<double>(ctx->d_trig), d.close_thr,
-                           as<unsigned char>(ctx->d_cls), as<double>(ctx->d_prec), as<unsigned>(ctx->d_cnt3));
+                           as<unsigned char>(ctx->d_cls), as<double>(ctx->d_prec));
         hipLaunchKernelGGL(k_pair_trig, dim3(blocks_for(pairs)), dim3(kBlock), 0, st,
                            pairs, static_cast<int>(d.kt), as<int>(ctx->d_pair_anchor), as<int>(ctx->d_ids),
                            as<double>(ctx->d_x), as<double>(ctx->d_y), as<double>(ctx->d_th), T.off_t, d.dt,
                            d.close_thr, as<double>(ctx->d_psin));
     }
-    hipLaunchKernelGGL(k_bucket, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_sorted),
-                       as<unsigned>(ctx->d_cnt3), as<unsigned>(ctx->d_off), nb, T.caps, as<int4>(ctx->d_lists),
+    hipLaunchKernelGGL(k_bucket, dim3(blocks_for(n, kBucketBlock)), dim3(kBucketBlock), 0, st, n, as<int>(ctx->d_sorted),
+                       as<unsigned char>(ctx->d_cls), as<unsigned>(ctx->d_off), nb, T.caps, as<int4>(ctx->d_lists),
                        scal);
     CVG_HIP(ctx, hipGetLastError());
 
@@ -2054,7 +2278,7 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
         if (slot > 0) aux_used[slot - 1] = true;
         const Launch &l = plan[b];
         A.list = as<int4>(ctx->d_lists) + static_cast<size_t>(b) * n;
-        A.list_len = scal + S_BUCKET + b;
+        A.list_len = scal + kBucketBase + b * kBucketStride;
         A.nb_cap = caps[b];
         A.deg_cap = l.dcap;
         A.g_bundle = l.gb ? as<double>(ctx->d_gbundle) + gb_off[b] : nullptr;
@@ -2256,6 +2480,10 @@ int cvg_create(int device, cvg_context **out_ctx)
             const int w = std::atoi(v);
             ctx->coop_waves = (w == 1 || w == 2 || w == 4) ? w : 4;
         }
+        if (const char *v = std::getenv("CVG_FILL_LANES")) {
+            const int f = std::atoi(v);
+            ctx->fill_lanes = (f == 8 || f == 16 || f == 32 || f == 64) ? f : 16;
+        }
     } while (false);
     if (rc != CVG_OK) {
         cvg_destroy(ctx);
@@ -2272,9 +2500,9 @@ void cvg_destroy(cvg_context *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf *bufs[] = {&ctx->d_edges_up, &ctx->d_x,     &ctx->d_y,       &ctx->d_th,      &ctx->d_cx,
                       &ctx->d_cy,       &ctx->d_scal,  &ctx->d_grid,    &ctx->d_tmp_ids, &ctx->d_sorted,
-                      &ctx->d_sx,       &ctx->d_sy,    &ctx->d_deg,     &ctx->d_off,     &ctx->d_tiles,
+                      &ctx->d_sxy,      &ctx->d_scell,   &ctx->d_part,    &ctx->d_deg,     &ctx->d_off,     &ctx->d_tiles,
                       &ctx->d_cand,     &ctx->d_ids,   &ctx->d_d2,      &ctx->d_pair_anchor, &ctx->d_trig,
-                      &ctx->d_cls,      &ctx->d_prec,  &ctx->d_psin,  &ctx->d_cnt3,  &ctx->d_lists,   &ctx->d_tables,  &ctx->d_st_ids,
+                      &ctx->d_cls,      &ctx->d_prec,  &ctx->d_psin,  &ctx->d_lists,   &ctx->d_tables,  &ctx->d_st_ids,
                       &ctx->d_st_meta,  &ctx->d_st_k,  &ctx->d_acc,     &ctx->d_rowoff,  &ctx->d_gbundle,
                       &ctx->d_chains,   &ctx->d_info,  &ctx->d_dbg,     &ctx->d_map,     &ctx->d_eoff,
                       &ctx->d_mapbox,   &ctx->d_maprow};
