@@ -1108,12 +1108,13 @@ struct FusedArgs {
 // current and the next anchor's pair records and sines, copied from the
 // pre-passes' contiguous blocks), the run's bundles by slot (lo, hi; absent
 // when they live in global memory), the seeds' results (k_max, k_min; length,
-// hash, best cell, chain) and the slots' hypothesis flags.
+// hash, best cell, chain), the slots' hypothesis flags and a list area for
+// compacting the run's slots and hypotheses (2 x 64 x nch ints).
 struct AnchorCarve {
-    size_t stg, stg_bytes, rec, sin, blo, bhi, skd, ski, hyp, bytes;
+    size_t stg, stg_bytes, rec, sin, blo, bhi, skd, ski, hyp, lst, bytes;
 };
 
-__host__ __device__ inline AnchorCarve anchor_carve(int nb, int dc, int K, int kt, int cs, bool lds_bundles)
+__host__ __device__ inline AnchorCarve anchor_carve(int nb, int dc, int K, int kt, int cs, bool lds_bundles, int nch)
 {
     AnchorCarve c{};
     c.rec = 0;  // within a staging buffer
@@ -1132,6 +1133,8 @@ __host__ __device__ inline AnchorCarve anchor_carve(int nb, int dc, int K, int k
     o += static_cast<size_t>(nb) * (cs + 3) * sizeof(int);
     c.hyp = o;
     o += static_cast<size_t>(nb) * sizeof(int);
+    c.lst = o;
+    o += static_cast<size_t>(2) * 64 * nch * sizeof(int);
     c.bytes = (o + 15) & ~static_cast<size_t>(15);
     return c;
 }
@@ -1246,11 +1249,13 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
     const int CS = A.chain_stride;
     const int SR = CS + 3;  // ints per seed record
     const int NB = A.nb_cap;
-    const AnchorCarve cv = anchor_carve(NB, A.deg_cap, K, kt, CS, !GB);
+    const AnchorCarve cv = anchor_carve(NB, A.deg_cap, K, kt, CS, !GB, NCH);
     char *sl = reinterpret_cast<char *>(lds) + static_cast<size_t>(grp) * cv.bytes;
     double *skd = reinterpret_cast<double *>(sl + cv.skd);
     int *ski = reinterpret_cast<int *>(sl + cv.ski);
     int *hypf = reinterpret_cast<int *>(sl + cv.hyp);
+    int *lst = reinterpret_cast<int *>(sl + cv.lst);
+    const unsigned long long lt_mask = (1ULL << lane) - 1ULL;
     double *blo;
     double *bhi;
     if constexpr (GB) {
@@ -1370,14 +1375,33 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
             if (nL == 0) continue;
 
             // phase A: the run's bundles (compute_curve_bundle, :636-819), slot
-            // i = i-th neighbour of the run in CSR order
-            int rid[NCH];
+            // i = i-th neighbour of the run in CSR order.  The slots' neighbour
+            // indices are compacted through the list area (every wave writes
+            // the same values) into lane-indexed rr, their ids into rid.
+            {
+                int pb = 0;
 #pragma unroll
-            for (int c = 0; c < NCH; ++c) rid[c] = 0;
-            for (int i = 0; i < nL; ++i) {
-                const int r = pop_next<NCH>(mk);
-                lane_set<NCH>(rid, i, lane_get<NCH>(nid, r), lane);
-                if (COOP && (i & (W - 1)) != t) continue;
+                for (int c = 0; c < NCH; ++c) {
+                    if ((mk[c] >> lane) & 1ULL) lst[pb + __popcll(mk[c] & lt_mask)] = 64 * c + lane;
+                    pb += __popcll(mk[c]);
+                }
+            }
+            wave_sync();
+            int rr[NCH], rid[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) rr[c] = 64 * c + lane < nL ? lst[64 * c + lane] : 0;
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                int v = 0;
+#pragma unroll
+                for (int cc = 0; cc < NCH; ++cc) {
+                    const int g = __shfl(nid[cc], rr[c] & 63, 64);
+                    if ((rr[c] >> 6) == cc) v = g;
+                }
+                rid[c] = v;
+            }
+            for (int i = COOP ? t : 0; i < nL; i += W) {
+                const int r = lane_get<NCH>(rr, i);
                 const int fl = lane_get<NCH>(nfl, r);
                 const double2 *rr = srec + r * 4;
                 const double2 r0 = rr[0], r1 = rr[1], r2 = rr[2], r3 = rr[3];
@@ -1432,21 +1456,27 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
                 hm[c] = __ballot(i < nL && hypf[i] != 0);
                 nH += __popcll(hm[c]);
             }
+            // compacted through the list area like the slots (the barrier above
+            // ends every wave's reads of the slot list)
+            {
+                int pb = 0;
+#pragma unroll
+                for (int c = 0; c < NCH; ++c) {
+                    if ((hm[c] >> lane) & 1ULL) {
+                        const int h = pb + __popcll(hm[c] & lt_mask);
+                        lst[h] = (64 * c + lane) * K * 8;
+                        lst[64 * NCH + h] = rid[c];
+                    }
+                    pb += __popcll(hm[c]);
+                }
+            }
+            wave_sync();
             int hso[NCH], hid[NCH];
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
-                hso[c] = 0;
-                hid[c] = 0;
-            }
-            {
-                unsigned long long m2[NCH];
-#pragma unroll
-                for (int c = 0; c < NCH; ++c) m2[c] = hm[c];
-                for (int h = 0; h < nH; ++h) {
-                    const int slot = pop_next<NCH>(m2);
-                    lane_set<NCH>(hso, h, slot * K * 8, lane);
-                    lane_set<NCH>(hid, h, lane_get<NCH>(rid, slot), lane);
-                }
+                const int h = 64 * c + lane;
+                hso[c] = h < nH ? lst[h] : 0;
+                hid[c] = h < nH ? lst[64 * NCH + h] : 0;
             }
 
             if (nH > 0 && G >= 3) {  // no chain longer than 2 when G < 3
@@ -2227,7 +2257,7 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
         l.nch = need <= 1 ? 1 : (need <= 2 ? 2 : 4);
         l.gb = false;
         l.coop = caps[b] >= ctx->coop_min_cap;
-        l.slice = anchor_carve(caps[b], l.dcap, A.K, A.kt, chain_stride, true).bytes;
+        l.slice = anchor_carve(caps[b], l.dcap, A.K, A.kt, chain_stride, true, l.nch).bytes;
         if (l.coop) {
             l.wpb = ctx->coop_waves;
         } else {
@@ -2238,7 +2268,7 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
             l.gb = true;
             l.coop = false;
             l.wpb = 1;
-            l.slice = anchor_carve(caps[b], l.dcap, A.K, A.kt, chain_stride, false).bytes;
+            l.slice = anchor_carve(caps[b], l.dcap, A.K, A.kt, chain_stride, false, l.nch).bytes;
             if (l.slice > ctx->lds_limit)
                 return fail(ctx, CVG_ERR_UNSUPPORTED, "neighbour lists too long for the workgroup's LDS");
         }
