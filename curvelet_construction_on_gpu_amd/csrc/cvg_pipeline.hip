@@ -122,7 +122,7 @@ struct cvg_context {
     unsigned long long intersections = 0;
     unsigned num_curvelets = 0;
     unsigned width = 0;
-    DevBuf d_trig, d_cls, d_prec, d_psin, d_lists, d_tables, d_st_ids, d_st_meta, d_st_k, d_acc, d_rowoff, d_gbundle,
+    DevBuf d_trig, d_cls, d_prec, d_psin, d_lists, d_tables, d_st_ids, d_st_meta, d_st_k, d_st_rank, d_acc, d_rowoff, d_gbundle,
         d_chains, d_info;
 
     // diagnostics
@@ -1098,6 +1098,7 @@ struct FusedArgs {
     int *st_ids;
     unsigned *st_meta;
     double *st_k;
+    int *st_rank;         // per CSR slot: row among the anchor's accepted chains, or -1
     unsigned *acc_count;
     unsigned *valid_pairs;
     unsigned long long *intersections;
@@ -1107,11 +1108,12 @@ struct FusedArgs {
 // One anchor stream's LDS slice (byte offsets): two staging buffers (the
 // current and the next anchor's pair records and sines, copied from the
 // pre-passes' contiguous blocks), the run's bundles by slot (lo, hi; absent
-// when they live in global memory), the seeds' results (k_max, k_min; length,
-// hash, best cell, chain), the slots' hypothesis flags and a list area for
-// compacting the run's slots and hypotheses (2 x 64 x nch ints).
+// when they live in global memory), the seeds' results (length, hash, best
+// cell, chain), the slots' hypothesis flags, a list area for compacting the
+// run's slots and hypotheses (2 x 64 x nch ints) and the anchor's per-seed
+// accept flags (dc ints).
 struct AnchorCarve {
-    size_t stg, stg_bytes, rec, sin, blo, bhi, skd, ski, hyp, lst, bytes;
+    size_t stg, stg_bytes, rec, sin, blo, bhi, ski, hyp, lst, acf, bytes;
 };
 
 __host__ __device__ inline AnchorCarve anchor_carve(int nb, int dc, int K, int kt, int cs, bool lds_bundles, int nch)
@@ -1127,14 +1129,14 @@ __host__ __device__ inline AnchorCarve anchor_carve(int nb, int dc, int K, int k
     if (lds_bundles) o += static_cast<size_t>(nb) * K * sizeof(double);
     c.bhi = o;
     if (lds_bundles) o += static_cast<size_t>(nb) * K * sizeof(double);
-    c.skd = o;
-    o += static_cast<size_t>(nb) * 2 * sizeof(double);
     c.ski = o;
     o += static_cast<size_t>(nb) * (cs + 3) * sizeof(int);
     c.hyp = o;
     o += static_cast<size_t>(nb) * sizeof(int);
     c.lst = o;
     o += static_cast<size_t>(2) * 64 * nch * sizeof(int);
+    c.acf = o;
+    o += static_cast<size_t>(dc) * sizeof(int);
     c.bytes = (o + 15) & ~static_cast<size_t>(15);
     return c;
 }
@@ -1251,10 +1253,10 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
     const int NB = A.nb_cap;
     const AnchorCarve cv = anchor_carve(NB, A.deg_cap, K, kt, CS, !GB, NCH);
     char *sl = reinterpret_cast<char *>(lds) + static_cast<size_t>(grp) * cv.bytes;
-    double *skd = reinterpret_cast<double *>(sl + cv.skd);
     int *ski = reinterpret_cast<int *>(sl + cv.ski);
     int *hypf = reinterpret_cast<int *>(sl + cv.hyp);
     int *lst = reinterpret_cast<int *>(sl + cv.lst);
+    int *accf = reinterpret_cast<int *>(sl + cv.acf);
     const unsigned long long lt_mask = (1ULL << lane) - 1ULL;
     double *blo;
     double *bhi;
@@ -1362,7 +1364,7 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
         const char *sb = sl + cv.stg + static_cast<size_t>(buf) * cv.stg_bytes;
         const double2 *srec = reinterpret_cast<const double2 *>(sb + cv.rec);
         const double2 *ssin = reinterpret_cast<const double2 *>(sb + cv.sin);
-        int acc = 0;  // accepted rows of this anchor (wave 0)
+        int nseed = 0;  // hypotheses of the earlier runs (their seeds' slots come first)
         for (int run = 0; run < 3; ++run) {
             const int want = run == 0 ? 1 : (run == 1 ? 2 : 0);
             unsigned long long mk[NCH];
@@ -1568,60 +1570,73 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
                     if (lane == 0) {
                         rec[1] = static_cast<int>(hsh);
                         rec[2] = best_cell;
-                        skd[2 * j] = kmax;
-                        skd[2 * j + 1] = kmin;
+                    }
+                    // the seed's row, staged at its own slot of the anchor; it is
+                    // kept unless an earlier seed has the same chain
+                    const long long q = static_cast<long long>(base) + nseed + j;
+                    if (lane < len) A.st_ids[q * A.st_stride + lane] = cur;
+                    if (lane == 0) {
+                        A.st_meta[q] = static_cast<unsigned>(len) | (run == 0 ? 0x100u : 0u) |
+                                       (static_cast<unsigned>(best_cell) << 16);
+                        A.st_k[2 * q] = kmax;
+                        A.st_k[2 * q + 1] = kmin;
                     }
                 }
                 sync();
 
-                // phase D (wave 0): dedup (:546-560, 876-916) and staging, seeds in
-                // lanes.  Seed j is dropped when an earlier seed j' (length > 2)
-                // has the identical chain: the first of equal chains is the one
-                // the reference accepted, so "identical to an earlier accepted
-                // chain" and "identical to an earlier seed's chain" coincide.
-                if (t == 0) {
+                // dedup (:546-560, 876-916), seed j on wave j % W: seed j is
+                // dropped when an earlier seed j' (length > 2) has the identical
+                // chain; the first of equal chains is the one the reference
+                // accepted, so "identical to an earlier accepted chain" and
+                // "identical to an earlier seed's chain" coincide.  Earlier seeds
+                // with the same length and hash are found in one ballot (lane =
+                // seed), then their chains compared element by element.
+                {
                     int slen[NCH], shash[NCH];
 #pragma unroll
                     for (int c = 0; c < NCH; ++c) {
-                        const int j = 64 * c + lane;
-                        slen[c] = j < nH ? ski[j * SR] : 0;
-                        shash[c] = (j < nH && slen[c] > 2) ? ski[j * SR + 1] : 0;
+                        const int jj = 64 * c + lane;
+                        slen[c] = jj < nH ? ski[jj * SR] : 0;
+                        shash[c] = (jj < nH && slen[c] > 2) ? ski[jj * SR + 1] : 0;
                     }
-#pragma unroll
-                    for (int c = 0; c < NCH; ++c) {
-                        if (64 * c >= nH) break;
-                        const int j = 64 * c + lane;
-                        const int lj = slen[c];
-                        bool dup = false;
-                        const int jmax = min(nH, 64 * c + 64) - 1;
-                        for (int jp = 0; jp < jmax; ++jp) {
-                            const int lp = lane_get<NCH>(slen, jp);
-                            if (lp <= 2) continue;
-                            const int hp = lane_get<NCH>(shash, jp);
-                            bool cand = j > jp && !dup && lj == lp && shash[c] == hp;
-                            if (__ballot(cand) == 0ULL) continue;
-                            for (int l = 0; l < lp; ++l)
-                                cand = cand && ski[j * SR + 3 + l] == ski[jp * SR + 3 + l];
-                            dup = dup || cand;
-                        }
-                        const bool accept = j < nH && lj > 2 && !dup;
-                        const unsigned long long am = __ballot(accept);
+                    for (int j = t; j < nH; j += W) {
+                        const int lj = lane_get<NCH>(slen, j);
+                        bool accept = lj > 2;
                         if (accept) {
-                            const long long q = static_cast<long long>(base) + acc +
-                                                __popcll(am & ((1ULL << lane) - 1ULL));
-                            A.st_meta[q] = static_cast<unsigned>(lj) | (run == 0 ? 0x100u : 0u) |
-                                           (static_cast<unsigned>(ski[j * SR + 2]) << 16);
-                            A.st_k[2 * q] = skd[2 * j];
-                            A.st_k[2 * q + 1] = skd[2 * j + 1];
-                            for (int l = 0; l < lj; ++l) A.st_ids[q * A.st_stride + l] = ski[j * SR + 3 + l];
+                            const int hj = lane_get<NCH>(shash, j);
+#pragma unroll
+                            for (int c = 0; c < NCH; ++c) {
+                                unsigned long long cm =
+                                    __ballot(64 * c + lane < j && slen[c] == lj && shash[c] == hj);
+                                while (accept && cm != 0ULL) {
+                                    const int jp = 64 * c + __builtin_ctzll(cm);
+                                    cm &= cm - 1ULL;
+                                    const bool ne = lane < lj && ski[j * SR + 3 + lane] != ski[jp * SR + 3 + lane];
+                                    if (__ballot(ne) == 0ULL) accept = false;
+                                }
+                            }
                         }
-                        acc += __popcll(am);
+                        if (lane == 0) accf[nseed + j] = accept ? 1 : 0;
                     }
                 }
             }
             sync();  // the next run reuses the slots and seed records
+            nseed += nH;
         }
-        if (t == 0 && lane == 0) A.acc_count[a] = static_cast<unsigned>(acc);
+        // every slot of the anchor: its row among the anchor's accepted chains
+        // (run-0 seeds, then run-1 seeds, each in seed order), or -1
+        if (t == 0) {
+            int accn = 0;
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                const int k = 64 * c + lane;
+                const bool ac = G >= 3 && k < nseed && accf[k] != 0;
+                const unsigned long long am = __ballot(ac);
+                if (k < deg) A.st_rank[base + k] = ac ? accn + __popcll(am & lt_mask) : -1;
+                accn += __popcll(am);
+            }
+            if (lane == 0) A.acc_count[a] = static_cast<unsigned>(accn);
+        }
 
         // advance the pipeline
         L = Ln;
@@ -1662,7 +1677,8 @@ struct RecordArgs {
     const int *pair_anchor;
     const int *map_of;        // per edge: its map (batches), or null
     const long long *map_eoff;  // per map: first edge id
-    const unsigned *off, *acc_count, *row_off;
+    const unsigned *row_off;
+    const int *st_rank;
     const int *st_ids;
     const unsigned *st_meta;
     const double *st_k;
@@ -1676,17 +1692,18 @@ struct RecordArgs {
 
 // record_curvelet_chain + fill_curvelet_info (cpu_curvelet.hpp:197-280) for
 // every accepted staging row, written at its compacted position
-// (compact_curvelet_output, :282-303).  One thread per staging slot q: the
-// anchor's k-th accepted row sits in slot off[a] + k.  Chain ids are written
-// 1-based and relative to the anchor's map (batches concatenate maps).
+// (compact_curvelet_output, :282-303).  One thread per staging slot q (one per
+// seed of the anchor, the rest unused): an accepted row carries its rank among
+// the anchor's accepted rows.  Chain ids are written 1-based and relative to
+// the anchor's map (batches concatenate maps).
 __global__ __launch_bounds__(kBlock) void k_record(RecordArgs R)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= R.pairs) return;
+    const int rk = R.st_rank[q];
+    if (rk < 0) return;
     const int a = R.pair_anchor[q];
-    const unsigned k = static_cast<unsigned>(q) - R.off[a];
-    if (k >= R.acc_count[a]) return;
-    const unsigned o = R.row_off[a] + k;
+    const unsigned o = R.row_off[a] + static_cast<unsigned>(rk);
     const int id0 = R.map_of ? static_cast<int>(R.map_eoff[R.map_of[a]]) : 0;
     const double ax = R.x[a], ay = R.y[a], at = R.th[a];
     const int W = R.G + 1;
@@ -2202,6 +2219,7 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     CVG_TRY(ensure(ctx, ctx->d_st_ids, np * st_stride * sizeof(int)));
     CVG_TRY(ensure(ctx, ctx->d_st_meta, np * sizeof(unsigned)));
     CVG_TRY(ensure(ctx, ctx->d_st_k, np * 2 * sizeof(double)));
+    CVG_TRY(ensure(ctx, ctx->d_st_rank, np * sizeof(int)));
     CVG_TRY(ensure(ctx, ctx->d_acc, static_cast<size_t>(n) * sizeof(unsigned)));
     CVG_TRY(ensure(ctx, ctx->d_rowoff, static_cast<size_t>(n + 1) * sizeof(unsigned)));
 
@@ -2228,6 +2246,7 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     A.st_ids = as<int>(ctx->d_st_ids);
     A.st_meta = as<unsigned>(ctx->d_st_meta);
     A.st_k = as<double>(ctx->d_st_k);
+    A.st_rank = as<int>(ctx->d_st_rank);
     A.acc_count = as<unsigned>(ctx->d_acc);
     A.valid_pairs = scal + S_VALID;
     A.intersections = reinterpret_cast<unsigned long long *>(scal + S_COUNT);
@@ -2338,8 +2357,7 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     R.x = as<double>(ctx->d_x);
     R.y = as<double>(ctx->d_y);
     R.th = as<double>(ctx->d_th);
-    R.off = as<unsigned>(ctx->d_off);
-    R.acc_count = A.acc_count;
+    R.st_rank = A.st_rank;
     R.row_off = as<unsigned>(ctx->d_rowoff);
     R.st_ids = A.st_ids;
     R.st_meta = A.st_meta;
@@ -2533,7 +2551,7 @@ void cvg_destroy(cvg_context *ctx)
                       &ctx->d_sxy,      &ctx->d_scell,   &ctx->d_part,    &ctx->d_deg,     &ctx->d_off,     &ctx->d_tiles,
                       &ctx->d_cand,     &ctx->d_ids,   &ctx->d_d2,      &ctx->d_pair_anchor, &ctx->d_trig,
                       &ctx->d_cls,      &ctx->d_prec,  &ctx->d_psin,  &ctx->d_lists,   &ctx->d_tables,  &ctx->d_st_ids,
-                      &ctx->d_st_meta,  &ctx->d_st_k,  &ctx->d_acc,     &ctx->d_rowoff,  &ctx->d_gbundle,
+                      &ctx->d_st_meta,  &ctx->d_st_k,  &ctx->d_st_rank, &ctx->d_acc,     &ctx->d_rowoff,  &ctx->d_gbundle,
                       &ctx->d_chains,   &ctx->d_info,  &ctx->d_dbg,     &ctx->d_map,     &ctx->d_eoff,
                       &ctx->d_mapbox,   &ctx->d_maprow};
     for (DevBuf *b : bufs) release(*b);
