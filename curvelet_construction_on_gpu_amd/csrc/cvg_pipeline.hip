@@ -1487,42 +1487,28 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
                 // phase C: growth (:479-536), seed j on wave j % W; the walk
                 // visits the hypotheses in CSR order, the seed at its own rank
                 for (int j = t; j < nH; j += W) {
-                    double lo[CPL], hi[CPL], pl[CPL], ph[CPL];
+                    double lo[CPL], hi[CPL];
                     const int soff = lane_get<NCH>(hso, j);
-                    int noff = lane_get<NCH>(hso, 0);
 #pragma unroll
                     for (int m = 0; m < CPL; ++m) {
                         lo[m] = *reinterpret_cast<const double *>(blo_b + soff + coff[m] * 8);
                         hi[m] = *reinterpret_cast<const double *>(bhi_b + soff + coff[m] * 8);
-                        pl[m] = *reinterpret_cast<const double *>(blo_b + noff + coff[m] * 8);
-                        ph[m] = *reinterpret_cast<const double *>(bhi_b + noff + coff[m] * 8);
                     }
                     int len = 1;
                     int cur = lane == 0 ? a : 0;  // chain position l in lane l
                     unsigned hsh = chain_hash(2166136261u, a);
-                    for (int h = 0; h < nH; ++h) {
+                    // The seed joins at its own distance rank (:496-510); its step
+                    // is an intersection with its own bundle, which leaves the
+                    // running bundle (inside the seed's) bit for bit unchanged.
+                    int h = 0;
+                    for (; h < nH && len < G; ++h) {
                         double bl[CPL], bh[CPL];
+                        const int noff = lane_get<NCH>(hso, h);
 #pragma unroll
                         for (int m = 0; m < CPL; ++m) {
-                            bl[m] = pl[m];
-                            bh[m] = ph[m];
+                            bl[m] = *reinterpret_cast<const double *>(blo_b + noff + coff[m] * 8);
+                            bh[m] = *reinterpret_cast<const double *>(bhi_b + noff + coff[m] * 8);
                         }
-                        if (h + 1 < nH) {
-                            noff = lane_get<NCH>(hso, h + 1);
-#pragma unroll
-                            for (int m = 0; m < CPL; ++m) {
-                                pl[m] = *reinterpret_cast<const double *>(blo_b + noff + coff[m] * 8);
-                                ph[m] = *reinterpret_cast<const double *>(bhi_b + noff + coff[m] * 8);
-                            }
-                        }
-                        if (h == j) {  // the seed joins at its own distance rank (:496-510)
-                            const int id = lane_get<NCH>(hid, h);
-                            if (lane == len) cur = id;
-                            hsh = chain_hash(hsh, id);
-                            if (++len >= G) break;
-                            continue;
-                        }
-                        ++isect;
                         double nlo[CPL], nhi[CPL];
                         bool ok = false;
 #pragma unroll
@@ -1531,18 +1517,19 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
                             nhi[m] = (hi[m] > bh[m]) ? bh[m] : hi[m];
                             ok = ok || (live[m] && nhi[m] > nlo[m]);
                         }
-                        if (__ballot(ok) != 0ULL) {
+                        if (h == j || __ballot(ok) != 0ULL) {
                             const int id = lane_get<NCH>(hid, h);
-                            if (lane == len) cur = id;
+                            cur = lane == len ? id : cur;
                             hsh = chain_hash(hsh, id);
 #pragma unroll
                             for (int m = 0; m < CPL; ++m) {
                                 lo[m] = nlo[m];
                                 hi[m] = nhi[m];
                             }
-                            if (++len >= G) break;
+                            ++len;
                         }
                     }
+                    isect += static_cast<unsigned>(h - (j < h ? 1 : 0));  // steps other than the seed's
                     int *rec = ski + j * SR;
                     if (lane == 0) rec[0] = len;
                     if (len <= 2) continue;  // :549
