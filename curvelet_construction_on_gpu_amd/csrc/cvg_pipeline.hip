@@ -1149,15 +1149,23 @@ __device__ __forceinline__ T uload(const T *p)
 }
 
 // Asynchronous copy of cnt 16-byte entries from global memory into LDS
-// (global_load_lds: lane i of a chunk writes dst + 16 i); waves t of W split
-// the chunks.  Completion: s_waitcnt vmcnt(0), then a barrier.
+// (global_load_lds_dwordx4: lane i of a chunk writes M0 + 16 i); waves t of W
+// split the chunks.  Completion: s_waitcnt vmcnt(0), then a barrier, both at
+// the top of the next anchor.  Issued as inline assembly: through the builtin
+// the compiler cannot tell the staging buffer from the LDS the current anchor
+// reads, and waits for the copy before every such read, which serialises the
+// prefetch with the work it is meant to overlap.
 __device__ __forceinline__ void copy_to_lds(const double2 *src, char *dst, int cnt, int t, int W, int lane)
 {
     for (int c0 = t * 64; c0 < cnt; c0 += W * 64) {
         const int i = c0 + lane;
+        const unsigned lds = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(
+            reinterpret_cast<size_t>((__attribute__((address_space(3))) char *)(dst + 16 * c0))));
         if (i < cnt)
-            __builtin_amdgcn_global_load_lds(static_cast<const void *>(src + i),
-                                             (__attribute__((address_space(3))) void *)(dst + 16 * c0), 16, 0, 0);
+            asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+                         :
+                         : "v"(src + i), "s"(lds)
+                         : "memory", "m0");
     }
 }
 
