@@ -15,6 +15,16 @@
 // (data_io.hpp:80-133): every value followed by a tab, a newline per row,
 // doubles in default ostream notation, which is printf's %.6g; std::to_chars
 // with chars_format::general and precision 6 produces the same text.
+//
+// Both directions run on several host threads for large files: the reader
+// splits the text at line boundaries, parses the pieces independently and
+// reports the first error in file order (its line number counted over the
+// pieces before it); the writers format row ranges independently and write
+// them in order.  Numbers are converted by strtod_l in the "C" locale from a
+// stack copy of the token (no allocation per number).
+#include <locale.h>
+
+#include <algorithm>
 #include <cerrno>
 #include <charconv>
 #include <cmath>
@@ -22,6 +32,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/curvelet_gpu.h"
@@ -41,6 +52,38 @@ bool is_stream_space(char c)
 }
 
 bool is_digit(char c) { return c >= '0' && c <= '9'; }
+
+// The "C" locale, created once (istream >> double uses the classic locale;
+// the process's LC_NUMERIC may differ).
+locale_t c_locale()
+{
+    static const locale_t loc = newlocale(LC_ALL_MASK, "C", static_cast<locale_t>(0));
+    return loc;
+}
+
+// Host threads for a job of `bytes` (at least 1 MiB per thread, at most 16).
+int io_threads(size_t bytes)
+{
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t by_size = bytes / (1u << 20) + 1;
+    return static_cast<int>(std::min<size_t>({static_cast<size_t>(hw), by_size, static_cast<size_t>(16)}));
+}
+
+// Runs f(0) .. f(n-1) on up to n threads (f(0) on the caller's); the parts
+// whose thread cannot be started run on the caller's thread.
+template <typename F>
+void run_parallel(int n, F f)
+{
+    std::vector<std::thread> th;
+    int started = 1;
+    try {
+        for (; started < n; ++started) th.emplace_back(f, started);
+    } catch (...) {
+    }
+    f(0);
+    for (int i = started; i < n; ++i) f(i);
+    for (auto &t : th) t.join();
+}
 
 // One istream >> double on [*p, e).  Advances *p past what the extraction
 // consumed.  Returns false on failure (as the stream's failbit).
@@ -72,14 +115,102 @@ bool extract_double(const char **p, const char *e, double *out)
     }
     *p = q;
     if (q == start) return false;
-    std::string tok(start, q);
+    // strtod must see exactly the collected characters (it would read
+    // further, e.g. a hexadecimal form, on the raw buffer)
+    const size_t len = static_cast<size_t>(q - start);
+    char small[128];
+    std::string big;
+    const char *tok;
+    if (len < sizeof(small)) {
+        std::memcpy(small, start, len);
+        small[len] = '\0';
+        tok = small;
+    } else {
+        big.assign(start, q);
+        tok = big.c_str();
+    }
     char *end = nullptr;
-    errno = 0;
-    const double v = std::strtod(tok.c_str(), &end);
-    if (end == tok.c_str() || *end != '\0') return false;
+    const double v = strtod_l(tok, &end, c_locale());
+    if (end == tok || *end != '\0') return false;
     if (std::isinf(v)) return false;
     *out = v;
     return true;
+}
+
+enum ParseError { kNone = 0, kBlank, kShort, kExtra };
+
+// Parses the lines of [b, e) (whole lines).  Stops at the first error.
+struct Piece {
+    std::vector<double> values;
+    long lines = 0;   // lines parsed (the failing one included)
+    long edges = 0;
+    ParseError error = kNone;
+    int found = 0;    // values read on the failing line
+};
+
+void parse_piece(const char *b, const char *e, int values_per_edge, Piece &out)
+{
+    out.values.reserve(static_cast<size_t>(e - b) / 16 + 16);
+    const char *p = b;
+    while (p < e) {
+        const char *nl = static_cast<const char *>(std::memchr(p, '\n', static_cast<size_t>(e - p)));
+        const char *le = nl ? nl : e;
+        ++out.lines;
+        if (is_blank_line(p, le)) {
+            out.error = kBlank;
+            return;
+        }
+        const char *q = p;
+        for (int j = 0; j < values_per_edge; ++j) {
+            double v;
+            if (!extract_double(&q, le, &v)) {
+                out.error = kShort;
+                out.found = j;
+                return;
+            }
+            out.values.push_back(v);
+        }
+        if (!is_blank_line(q, le)) {
+            out.error = kExtra;
+            return;
+        }
+        ++out.edges;
+        p = nl ? nl + 1 : e;
+    }
+}
+
+// Formats rows [r0, r1) of each thread's share with fmt(row, p), which writes
+// at most max_row bytes at p and returns the end.  Rows go through a small
+// local buffer appended to the thread's output in bulk (per-value vector
+// appends do not scale across threads); the outputs are written in order.
+template <typename RowFmt>
+int write_rows(const char *path, unsigned rows, size_t max_row, size_t bytes_per_row_guess, RowFmt fmt)
+{
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return CVG_ERR_IO;
+    const int nt = io_threads(static_cast<size_t>(rows) * bytes_per_row_guess);
+    std::vector<std::vector<char>> out(nt);
+    run_parallel(nt, [&](int t) {
+        const unsigned r0 = static_cast<unsigned>(static_cast<unsigned long long>(rows) * t / nt);
+        const unsigned r1 = static_cast<unsigned>(static_cast<unsigned long long>(rows) * (t + 1) / nt);
+        std::vector<char> &o = out[t];
+        o.reserve(static_cast<size_t>(r1 - r0) * bytes_per_row_guess + 16);
+        const size_t cap = std::max<size_t>(size_t(1) << 18, 2 * max_row);
+        std::vector<char> local(cap);
+        size_t used = 0;
+        for (unsigned i = r0; i < r1; ++i) {
+            if (cap - used < max_row) {
+                o.insert(o.end(), local.data(), local.data() + used);
+                used = 0;
+            }
+            used = static_cast<size_t>(fmt(i, local.data() + used) - local.data());
+        }
+        o.insert(o.end(), local.data(), local.data() + used);
+    });
+    bool ok = true;
+    for (const auto &o : out)
+        if (!o.empty()) ok = ok && std::fwrite(o.data(), 1, o.size(), f) == o.size();
+    return (std::fclose(f) == 0 && ok) ? CVG_OK : CVG_ERR_IO;
 }
 
 void set_err(char *err, size_t err_len, const std::string &msg)
@@ -110,49 +241,57 @@ int cvg_read_edge_file(const char *path, int values_per_edge, double **edges, in
     }
     std::vector<char> buf;
     {
+        // one read of the whole file when its size is known, else chunks
+        long size = -1;
+        if (std::fseek(f, 0, SEEK_END) == 0) size = std::ftell(f);
+        if (size >= 0 && std::fseek(f, 0, SEEK_SET) == 0) {
+            buf.resize(static_cast<size_t>(size));
+            buf.resize(std::fread(buf.data(), 1, buf.size(), f));
+        }
         char chunk[1 << 16];
         size_t got;
         while ((got = std::fread(chunk, 1, sizeof(chunk), f)) > 0) buf.insert(buf.end(), chunk, chunk + got);
     }
     std::fclose(f);
 
-    std::vector<double> data;
-    data.reserve(buf.size() / 16);
-    const char *p = buf.data();
-    const char *end = p + buf.size();
+    // pieces end at line boundaries
+    const char *begin = buf.data();
+    const char *end = begin + buf.size();
+    const int nt = io_threads(buf.size());
+    std::vector<const char *> cut(nt + 1, end);
+    cut[0] = begin;
+    for (int t = 1; t < nt; ++t) {
+        const char *c = std::max(cut[t - 1], begin + buf.size() * t / nt);
+        if (c > begin && c < end && c[-1] != '\n') {
+            const char *nl = static_cast<const char *>(std::memchr(c, '\n', static_cast<size_t>(end - c)));
+            c = nl ? nl + 1 : end;
+        }
+        cut[t] = c;
+    }
+    std::vector<Piece> pieces(nt);
+    run_parallel(nt, [&](int t) { parse_piece(cut[t], cut[t + 1], values_per_edge, pieces[t]); });
+
     long line_num = 0;
     long edge_count = 0;
-    while (p < end) {
-        const char *nl = static_cast<const char *>(std::memchr(p, '\n', static_cast<size_t>(end - p)));
-        const char *le = nl ? nl : end;
-        ++line_num;
-        if (is_blank_line(p, le)) {
+    for (const Piece &pc : pieces) {
+        line_num += pc.lines;
+        edge_count += pc.edges;
+        if (pc.error == kNone) continue;
+        if (pc.error == kBlank)
             set_err(err, err_len,
                     "Error: blank line at line " + std::to_string(line_num) + " in " + path +
                         " (each line must contain exactly " + std::to_string(values_per_edge) + " numeric values)");
-            return CVG_ERR_BAD_INPUT;
-        }
-        const char *q = p;
-        for (int j = 0; j < values_per_edge; ++j) {
-            double v;
-            if (!extract_double(&q, le, &v)) {
-                set_err(err, err_len,
-                        std::string("Error: invalid edge file format in ") + path + " at line " +
-                            std::to_string(line_num) + ": expected " + std::to_string(values_per_edge) +
-                            " numeric values per line, found " + std::to_string(j) + ".");
-                return CVG_ERR_BAD_INPUT;
-            }
-            data.push_back(v);
-        }
-        if (!is_blank_line(q, le)) {
+        else if (pc.error == kShort)
+            set_err(err, err_len,
+                    std::string("Error: invalid edge file format in ") + path + " at line " +
+                        std::to_string(line_num) + ": expected " + std::to_string(values_per_edge) +
+                        " numeric values per line, found " + std::to_string(pc.found) + ".");
+        else
             set_err(err, err_len,
                     std::string("Error: invalid edge file format in ") + path + " at line " +
                         std::to_string(line_num) + ": expected exactly " + std::to_string(values_per_edge) +
                         " numeric values per line, found extra data.");
-            return CVG_ERR_BAD_INPUT;
-        }
-        ++edge_count;
-        p = nl ? nl + 1 : end;
+        return CVG_ERR_BAD_INPUT;
     }
     if (edge_count == 0) {
         set_err(err, err_len, std::string("Error: input file contains no valid edges: ") + path);
@@ -162,12 +301,17 @@ int cvg_read_edge_file(const char *path, int values_per_edge, double **edges, in
         set_err(err, err_len, "Error: too many edges");
         return CVG_ERR_BAD_INPUT;
     }
-    double *out = static_cast<double *>(std::malloc(data.size() * sizeof(double)));
+    const size_t total = static_cast<size_t>(edge_count) * values_per_edge;
+    double *out = static_cast<double *>(std::malloc(total * sizeof(double)));
     if (!out) {
         set_err(err, err_len, "Error: out of host memory");
         return CVG_ERR_OUT_OF_MEMORY;
     }
-    std::memcpy(out, data.data(), data.size() * sizeof(double));
+    size_t o = 0;
+    for (const Piece &pc : pieces) {
+        if (!pc.values.empty()) std::memcpy(out + o, pc.values.data(), pc.values.size() * sizeof(double));
+        o += pc.values.size();
+    }
     *edges = out;
     *num_edges = static_cast<int>(edge_count);
     return CVG_OK;
@@ -178,44 +322,33 @@ void cvg_free(void *ptr) { std::free(ptr); }
 int cvg_write_chain_file(const char *path, const unsigned *chains, unsigned rows, unsigned width)
 {
     if (!path || (!chains && rows > 0 && width > 0)) return CVG_ERR_INVALID_ARGUMENT;
-    FILE *f = std::fopen(path, "wb");
-    if (!f) return CVG_ERR_IO;
-    std::vector<char> out;
-    out.reserve(static_cast<size_t>(rows) * width * 8 + 16);
-    char tmp[32];
-    for (unsigned i = 0; i < rows; ++i) {
-        for (unsigned j = 0; j < width; ++j) {
-            // main.cpp:76 casts each unsigned entry to int before printing.
-            const int v = static_cast<int>(chains[static_cast<size_t>(i) * width + j]);
-            const auto r = std::to_chars(tmp, tmp + sizeof(tmp), v);
-            out.insert(out.end(), tmp, r.ptr);
-            out.push_back('\t');
-        }
-        out.push_back('\n');
-    }
-    const bool ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
-    return (std::fclose(f) == 0 && ok) ? CVG_OK : CVG_ERR_IO;
+    return write_rows(path, rows, static_cast<size_t>(width) * 12 + 1, static_cast<size_t>(width) * 8 + 1,
+                      [&](unsigned i, char *p) {
+                          for (unsigned j = 0; j < width; ++j) {
+                              // main.cpp:76 casts each unsigned entry to int before printing.
+                              const int v = static_cast<int>(chains[static_cast<size_t>(i) * width + j]);
+                              p = std::to_chars(p, p + 11, v).ptr;
+                              *p++ = '\t';
+                          }
+                          *p++ = '\n';
+                          return p;
+                      });
 }
 
 int cvg_write_info_file(const char *path, const double *info, unsigned rows, unsigned width)
 {
     if (!path || (!info && rows > 0 && width > 0)) return CVG_ERR_INVALID_ARGUMENT;
-    FILE *f = std::fopen(path, "wb");
-    if (!f) return CVG_ERR_IO;
-    std::vector<char> out;
-    out.reserve(static_cast<size_t>(rows) * width * 12 + 16);
-    char tmp[64];
-    for (unsigned i = 0; i < rows; ++i) {
-        for (unsigned j = 0; j < width; ++j) {
-            const double v = info[static_cast<size_t>(i) * width + j];
-            const auto r = std::to_chars(tmp, tmp + sizeof(tmp), v, std::chars_format::general, 6);
-            out.insert(out.end(), tmp, r.ptr);
-            out.push_back('\t');
-        }
-        out.push_back('\n');
-    }
-    const bool ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
-    return (std::fclose(f) == 0 && ok) ? CVG_OK : CVG_ERR_IO;
+    // %.6g of a double is at most 13 characters ("-1.23457e-308"); 24 leaves room
+    return write_rows(path, rows, static_cast<size_t>(width) * 25 + 1, static_cast<size_t>(width) * 12 + 1,
+                      [&](unsigned i, char *p) {
+                          for (unsigned j = 0; j < width; ++j) {
+                              const double v = info[static_cast<size_t>(i) * width + j];
+                              p = std::to_chars(p, p + 24, v, std::chars_format::general, 6).ptr;
+                              *p++ = '\t';
+                          }
+                          *p++ = '\n';
+                          return p;
+                      });
 }
 
 }  // extern "C"

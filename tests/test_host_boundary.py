@@ -173,6 +173,51 @@ def test_reader_correctly_rounded(tmp_path):
     assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
 
 
+def _big_text(rows=60000, seed=5):
+    # several MiB: the reader splits it into pieces parsed on separate threads
+    rng = np.random.default_rng(seed)
+    vals = rng.uniform(-2e3, 2e3, (rows, 4))
+    lines = [" ".join(f"{v:.17g}" for v in r) for r in vals]
+    return vals, lines
+
+
+def test_reader_many_pieces_values(tmp_path):
+    vals, lines = _big_text()
+    text = "\n".join(lines) + "\n"
+    assert len(text) > 3 << 20
+    got = read_TO_edges_from_file(write(tmp_path, text))
+    assert np.array_equal(got.view(np.uint64), vals.view(np.uint64))
+
+
+@pytest.mark.parametrize("at", [1, 2, 17, 29999, 30000, 44444, 59999, 60000])
+@pytest.mark.parametrize("kind", ["blank", "short", "extra"])
+def test_reader_many_pieces_first_error(tmp_path, at, kind):
+    # an error on line `at` (1-based) of a multi-MiB file, and a second error
+    # further on: the message names the first one, with its global line number
+    _, lines = _big_text()
+    lines = list(lines)
+    bad = {"blank": "", "short": "1 2 3", "extra": "1 2 3 4 5"}[kind]
+    lines[at - 1] = bad
+    if at + 5000 <= len(lines):
+        lines[at + 5000 - 1] = "1 2 3 4 5"
+    with pytest.raises(CurveletError) as ei:
+        read_TO_edges_from_file(write(tmp_path, "\n".join(lines) + "\n"))
+    msg = str(ei.value)
+    if kind == "blank":
+        assert f"blank line at line {at} " in msg
+    elif kind == "short":
+        assert f"at line {at}: expected 4 numeric values per line, found 3." in msg
+    else:
+        assert f"at line {at}: expected exactly 4 numeric values per line, found extra data." in msg
+
+
+def test_reader_many_pieces_no_final_newline_and_crlf(tmp_path):
+    vals, lines = _big_text(rows=40000, seed=6)
+    text = "\r\n".join(lines)  # CRLF, no newline at the end
+    got = read_TO_edges_from_file(write(tmp_path, text))
+    assert np.array_equal(got.view(np.uint64), vals.view(np.uint64))
+
+
 # ------------------------------------------------------------------ writers
 
 def test_chain_writer_format(tmp_path):
@@ -190,6 +235,27 @@ def test_info_writer_format(tmp_path):
     # default ostream formatting: %g with 6 significant digits
     want = "\t".join("%g" % v for v in vals[0]) + "\t\n"
     assert p.read_text() == want
+
+
+def test_writers_many_threads_match_sequential(tmp_path):
+    rng = np.random.default_rng(11)
+    rows = 200000
+    chains = rng.integers(0, 2**32, size=(rows, 8), dtype=np.uint64).astype(np.uint32)
+    info = rng.standard_normal((rows, 10)) * 10.0 ** rng.integers(-12, 12, (rows, 10))
+    pc, pi = tmp_path / "c.txt", tmp_path / "i.txt"
+    write_chain_file(pc, chains)
+    write_info_file(pi, info)
+    want_c = "".join("".join(f"{int(np.int32(np.uint32(v)))}\t" for v in r) + "\n" for r in chains[:2000])
+    got_c = pc.read_text()
+    assert got_c.startswith(want_c)
+    assert got_c.count("\n") == rows
+    lines_i = pi.read_text().split("\n")
+    assert len(lines_i) == rows + 1 and lines_i[-1] == ""
+    for k in list(range(0, rows, 9973)) + [rows - 1]:
+        assert lines_i[k] == "".join("%g\t" % v for v in info[k])
+    lines_c = got_c.split("\n")
+    for k in list(range(0, rows, 9973)) + [rows - 1]:
+        assert lines_c[k] == "".join(f"{int(np.int32(np.uint32(v)))}\t" for v in chains[k])
 
 
 def test_info_writer_matches_printf_g(tmp_path):
