@@ -1691,12 +1691,8 @@ struct RecordArgs {
 // seed of the anchor, the rest unused): an accepted row carries its rank among
 // the anchor's accepted rows.  Chain ids are written 1-based and relative to
 // the anchor's map (batches concatenate maps).
-__global__ __launch_bounds__(kBlock) void k_record(RecordArgs R)
+__device__ inline void record_slot(const RecordArgs &R, int q, int rk)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= R.pairs) return;
-    const int rk = R.st_rank[q];
-    if (rk < 0) return;
     const int a = R.pair_anchor[q];
     const unsigned o = R.row_off[a] + static_cast<unsigned>(rk);
     const int id0 = R.map_of ? static_cast<int>(R.map_eoff[R.map_of[a]]) : 0;
@@ -1739,6 +1735,37 @@ __global__ __launch_bounds__(kBlock) void k_record(RecordArgs R)
     inf[7] = kk;
     inf[8] = length;
     inf[9] = quality;
+}
+
+// One thread per staging slot finds the accepted rows (about one slot in
+// five); a workgroup compacts them in LDS (one ballot and one LDS atomic per
+// wave) and its first threads record them, so the record work runs on full
+// waves.
+constexpr int kRecordBlock = 1024;
+
+__global__ __launch_bounds__(kRecordBlock) void k_record(RecordArgs R)
+{
+    __shared__ int sq[kRecordBlock];
+    __shared__ int srk[kRecordBlock];
+    __shared__ int scount;
+    if (threadIdx.x == 0) scount = 0;
+    __syncthreads();
+    const int q = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int rk = q < R.pairs ? R.st_rank[q] : -1;
+    const bool on = rk >= 0;
+    const unsigned long long m = __ballot(on);
+    const int lane = threadIdx.x & 63;
+    int base = 0;
+    if (lane == 0 && m != 0ULL) base = atomicAdd(&scount, __popcll(m));
+    base = __shfl(base, 0, 64);
+    if (on) {
+        const int pos = base + __popcll(m & ((1ULL << lane) - 1ULL));
+        sq[pos] = q;
+        srk[pos] = rk;
+    }
+    __syncthreads();
+    const int cnt = scount;
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) record_slot(R, sq[i], srk[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -2365,7 +2392,8 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     R.off_t = T.off_t;
     R.chains = as<unsigned>(ctx->d_chains);
     R.info = as<double>(ctx->d_info);
-    if (pairs > 0) hipLaunchKernelGGL(k_record, dim3(blocks_for(pairs)), dim3(kBlock), 0, st, R);
+    if (pairs > 0)
+        hipLaunchKernelGGL(k_record, dim3(blocks_for(pairs, kRecordBlock)), dim3(kRecordBlock), 0, st, R);
     CVG_HIP(ctx, hipGetLastError());
     CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox + 32, as<unsigned>(ctx->d_rowoff) + n, sizeof(unsigned),
                                 hipMemcpyDeviceToHost, st));
