@@ -685,6 +685,7 @@ template <int G>
 __device__ inline int window_pos(int rs, int pre, int tot, int nrows, int t)
 {
     int k = -1;
+#pragma unroll 8
     for (int j = 0; j < nrows; ++j) {
         const int pj = __shfl(pre, j, G);
         const int sj = __shfl(rs, j, G);
@@ -868,16 +869,18 @@ __global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const doubl
             idi = lid[i];
             celli = lcell[i];
         }
+        // entries j < dmax lie inside the segment's cap-sized area, so they
+        // are read unconditionally (branch-free, unrollable) and masked
         int rank = 0;
+#pragma unroll 4
         for (int j = 0; j < dmax; ++j) {
-            if (j < dg) {
-                const double dj = ld[j];
-                const int cellj = lcell[j];
-                rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
-                const bool eq = active && dj == di && j != i;
-                tie = tie || eq;
-                ambiguous = ambiguous || (eq && cellj == celli);
-            }
+            const double dj = ld[j];
+            const int cellj = lcell[j];
+            const bool in = j < dg;
+            rank += (in && (dj < di || (dj == di && j < i))) ? 1 : 0;
+            const bool eq = in && active && dj == di && j != i;
+            tie = tie || eq;
+            ambiguous = ambiguous || (eq && cellj == celli);
         }
         if (active) {
             ids[o + rank] = idi;
