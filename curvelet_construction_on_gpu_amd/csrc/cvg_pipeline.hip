@@ -977,9 +977,11 @@ __global__ __launch_bounds__(kBlock) void k_pair_prep(int pairs, int n, const in
     o[3] = make_double2(p.cT0p_DX, p.sT0p_DY);
 }
 
-// Anchors in spatial (cell) order, binned by the largest of their three class
-// counts (read from the anchor's run of pair flags), which sizes the LDS slots
-// the fused kernel needs; each list entry carries (anchor, CSR base, degree).
+// Anchors in spatial (cell) order, binned by max(forward, backward) + neither
+// of their class counts (read from the anchor's run of pair flags): that key
+// bounds every run's slot count and half the degree (the third class is
+// almost always empty), which size the fused kernel's LDS slots and staging.
+// Each list entry carries (anchor, CSR base, degree).
 constexpr int kBucketBlock = 1024;  // few workgroups: one list-length atomic each per bucket
 
 __global__ __launch_bounds__(kBucketBlock) void k_bucket(int n, const int *__restrict__ sorted,
@@ -1007,7 +1009,7 @@ __global__ __launch_bounds__(kBucketBlock) void k_bucket(int n, const int *__res
             c[1] += k == 1u ? 1u : 0u;
             c[2] += k == 2u ? 1u : 0u;
         }
-        const unsigned m = max(c[0], max(c[1], c[2]));
+        const unsigned m = max(c[1], c[2]) + c[0];
         b = 0;
         while (b < nb - 1 && m > static_cast<unsigned>(caps[b])) ++b;
     }
@@ -2276,8 +2278,8 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     A.valid_pairs = scal + S_VALID;
     A.intersections = reinterpret_cast<unsigned long long *>(scal + S_COUNT);
 
-    // per bucket: 64-neighbour chunks of the lane-indexed lists (degree <= 3 x
-    // the class cap and <= the map's largest); heavy buckets share one anchor
+    // per bucket: 64-neighbour chunks of the lane-indexed lists (degree <= 2 x
+    // the bucket's key and <= the map's largest); heavy buckets share one anchor
     // among the waves of a workgroup, light ones pack one anchor per wave;
     // bundles go to global memory only when one anchor's slice alone exceeds
     // LDS.  Each launch is persistent: as many anchor streams as stay resident
@@ -2295,7 +2297,7 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     size_t gb_total = 0;
     for (int b = 0; b < nb; ++b) {
         Launch &l = plan[b];
-        l.dcap = std::min(3 * caps[b], static_cast<int>(std::max(ctx->max_deg, 1u)));
+        l.dcap = std::min(2 * caps[b], static_cast<int>(std::max(ctx->max_deg, 1u)));
         const int need = (l.dcap + 63) / 64;
         if (need > 4) return fail(ctx, CVG_ERR_UNSUPPORTED, "more than 256 neighbours per anchor");
         l.nch = need <= 1 ? 1 : (need <= 2 ? 2 : 4);
