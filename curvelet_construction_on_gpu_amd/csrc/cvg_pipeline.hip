@@ -1110,9 +1110,9 @@ struct FusedArgs {
     double *g_bundle;  // global-bundle kernels: nb_cap*K*2 doubles per anchor stream
 };
 
-// One anchor stream's LDS slice (byte offsets): two staging buffers (the
-// current and the next anchor's pair records and sines, copied from the
-// pre-passes' contiguous blocks), the run's bundles by slot (lo, hi; absent
+// One anchor stream's LDS slice (byte offsets): the staging buffer (the
+// anchor's pair records and sines, copied from the pre-passes' contiguous
+// blocks), the run's bundles by slot (lo, hi; absent
 // when they live in global memory), the seeds' results (length, hash, best
 // cell, chain), the slots' hypothesis flags, a list area for compacting the
 // run's slots and hypotheses (2 x 64 x nch ints) and the anchor's per-seed
@@ -1129,7 +1129,7 @@ __host__ __device__ inline AnchorCarve anchor_carve(int nb, int dc, int K, int k
     c.stg_bytes = c.sin + static_cast<size_t>(dc) * kt * 2 * sizeof(double);
     size_t o = 0;
     c.stg = o;
-    o += 2 * c.stg_bytes;
+    o += c.stg_bytes;
     c.blo = o;
     if (lds_bundles) o += static_cast<size_t>(nb) * K * sizeof(double);
     c.bhi = o;
@@ -1155,11 +1155,10 @@ __device__ __forceinline__ T uload(const T *p)
 
 // Asynchronous copy of cnt 16-byte entries from global memory into LDS
 // (global_load_lds_dwordx4: lane i of a chunk writes M0 + 16 i); waves t of W
-// split the chunks.  Completion: s_waitcnt vmcnt(0), then a barrier, both at
-// the top of the next anchor.  Issued as inline assembly: through the builtin
-// the compiler cannot tell the staging buffer from the LDS the current anchor
-// reads, and waits for the copy before every such read, which serialises the
-// prefetch with the work it is meant to overlap.
+// split the chunks.  Completion: s_waitcnt vmcnt(0), then a barrier.  Issued
+// as inline assembly: through the builtin the compiler cannot tell the
+// staging buffer from the rest of the LDS and adds vmcnt(0) waits before
+// unrelated LDS reads.
 __device__ __forceinline__ void copy_to_lds(const double2 *src, char *dst, int cnt, int t, int W, int lane)
 {
     for (int c0 = t * 64; c0 < cnt; c0 += W * 64) {
@@ -1223,11 +1222,12 @@ __device__ __forceinline__ unsigned chain_hash(unsigned h, int id)
 // stream g processes list entries g, g + streams, ...  A stream is one wave
 // (COOP = false, blockDim/64 streams per workgroup) or a whole workgroup whose
 // W = blockDim/64 waves split each anchor's bundles (slot i on wave i % W) and
-// seeds (hypothesis j on wave j % W) (COOP = true).  While an anchor is
-// processed, the next one's pair records and sines are already being copied
-// into the other LDS staging buffer (global_load_lds), and its list entry,
-// flags, ids and per-lane trig values are loaded, so memory latency is paid
-// once per stream instead of once per anchor.
+// seeds (hypothesis j on wave j % W) (COOP = true).  An anchor's pair records
+// and sines are copied into the stream's LDS staging buffer (global_load_lds)
+// when it starts; its list entry, flags, ids and per-lane trig values were
+// loaded while the previous anchor was processed.  One staging buffer, not
+// two: the copy's latency is covered by the other resident streams, and the
+// smaller slice keeps more streams resident.
 //
 // Per anchor, runs 0 = forward neighbours (class 1), 1 = backward (class 2),
 // 2 = neither (class 0: validity only, for the count).  Phase A: the run's
@@ -1339,7 +1339,6 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
     int a, deg;
     unsigned base;
     entry(L, a, base, deg);
-    stage(0, base, deg);
     lane_vals(a, base, deg);
     int Ln = L + streams;
     int na = 0, ndeg = 0;
@@ -1349,7 +1348,7 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
     unsigned vpw = 0;     // valid or close pairs computed by this wave
     unsigned isect = 0;   // intersections tested by this wave
     for (int it = 0; L < len_list; ++it) {
-        const int buf = it & 1;
+        const int buf = 0;
         double sT2t[CPL], cT2t[CPL];
         int nfl[NCH], nid[NCH];
 #pragma unroll
@@ -1362,6 +1361,8 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
             nfl[c] = nnfl[c];
             nid[c] = nnid[c];
         }
+        sync();  // every reader of the previous anchor's staging is done
+        stage(0, base, deg);
         __builtin_amdgcn_s_waitcnt(0);  // this anchor's staging copy and values have landed
         sync();
         // start the next anchor's loads
@@ -1369,7 +1370,6 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
         int nna = 0, nndeg = 0;
         unsigned nnbase = 0;
         if (Ln < len_list) {
-            stage(buf ^ 1, nbase, ndeg);
             lane_vals(na, nbase, ndeg);
             if (Lnn < len_list) entry(Lnn, nna, nnbase, nndeg);
         }
