@@ -81,7 +81,7 @@ struct cvg_context {
     // fused-kernel launch shape: buckets whose class cap is at least
     // coop_min_cap run one anchor per workgroup of coop_waves waves
     // (CVG_COOP_MIN_CAP / CVG_COOP_WAVES override them; results never depend on it)
-    int coop_min_cap = 6;
+    int coop_min_cap = 8;
     int coop_waves = 4;
     int num_cu = 1;
     // lanes per anchor in the neighbour-window scans (8, 16, 32 or 64;
