@@ -114,6 +114,9 @@ def bench_batch(args) -> int:
         for m in range(len(maps)):
             ctx.run_device(dall + int(eoff[m]) * step, counts[m], 4, params)
     ctx.synchronize()
+    # launch shape for this mix (untimed; results do not depend on it)
+    default_tuning = ctx.tuning
+    tuning, tuned_ms, trials = ctx.autotune_batch(dall, counts, 4, params, reps=1)
     reps = max(1, args.steps // 4)
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -146,7 +149,9 @@ def bench_batch(args) -> int:
                        "neighbor_pairs": st["total_neighbor_pairs"], "curvelets": st["num_curvelets"]},
             "plain_loop_maps_per_s": len(maps) / t_plain, "batch_speedup_vs_plain_loop": t_plain / t_batch,
             "batch_edges_per_s": n / t_batch, "plain_loop_edges_per_s": n / t_plain,
-            "sampled_maps_identical": bool(same)}
+            "sampled_maps_identical": bool(same),
+            "autotune": {"default": default_tuning, "default_ms": trials[0][1], "chosen": tuning,
+                         "chosen_ms": tuned_ms, "trials": len(trials)}}
     print(json.dumps(line))
     return 0
 

@@ -74,6 +74,14 @@ class _ServerResult(C.Structure):
                 ("latency_ms", C.c_double)]
 
 
+class _Tuning(C.Structure):
+    _fields_ = [("coop_min_key", C.c_int), ("coop_waves", C.c_int), ("window_lanes", C.c_int)]
+
+
+class _TuningTrial(C.Structure):
+    _fields_ = [("tuning", _Tuning), ("ms", C.c_double)]
+
+
 class _SynthConfig(C.Structure):
     _fields_ = [("width", C.c_int), ("height", C.c_int), ("num_curves", C.c_int), ("radius_min", C.c_double),
                 ("radius_max", C.c_double), ("radius_log_uniform", C.c_int), ("segment_min", C.c_double),
@@ -115,6 +123,10 @@ _SIGNATURES = {
     "cvg_run_batch_device": (C.c_int, [C.c_void_p, C.c_int, _ip, C.c_int, C.c_void_p, _P(_Params), _P(_RunStats)]),
     "cvg_batch_map_shape": (C.c_int, [C.c_void_p, C.c_int, _up, _up]),
     "cvg_get_batch_map_curvelets": (C.c_int, [C.c_void_p, C.c_int, _up, _dp]),
+    "cvg_get_tuning": (C.c_int, [C.c_void_p, _P(_Tuning)]),
+    "cvg_set_tuning": (C.c_int, [C.c_void_p, _P(_Tuning)]),
+    "cvg_autotune_batch": (C.c_int, [C.c_void_p, C.c_int, _ip, C.c_int, C.c_void_p, _P(_Params), C.c_int,
+                                     _P(_Tuning), _dp, _P(_TuningTrial), C.c_int, _ip]),
     "cvg_server_config_default": (None, [_P(_ServerConfig)]),
     "cvg_server_create": (C.c_int, [C.c_int, _P(_Params), _P(_ServerConfig), _P(C.c_void_p)]),
     "cvg_server_submit": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _P(C.c_ulonglong)]),
@@ -333,6 +345,31 @@ class Context:
         self.check(self._lib.cvg_run_batch_device(self._h, len(c), _ptr(c, C.c_int), sz, C.c_void_p(dptr),
                                                   C.byref(p), C.byref(st)), "run_batch_device")
         return {k: getattr(st, k) for k, _ in _RunStats._fields_}
+
+    @property
+    def tuning(self) -> dict:
+        """Launch shape (results never depend on it): coop_min_key, coop_waves, window_lanes."""
+        t = _Tuning()
+        self.check(self._lib.cvg_get_tuning(self._h, C.byref(t)), "get_tuning")
+        return {k: getattr(t, k) for k, _ in _Tuning._fields_}
+
+    @tuning.setter
+    def tuning(self, value: dict) -> None:
+        t = _Tuning(**{k: int(value[k]) for k, _ in _Tuning._fields_})
+        self.check(self._lib.cvg_set_tuning(self._h, C.byref(t)), "set_tuning")
+
+    def autotune_batch(self, dptr: int, counts, sz: int, params: CurveletParams, reps: int = 2):
+        """Times the batched path on these maps under candidate tunings, keeps the fastest.
+        Returns (best tuning, its ms per run, [(tuning, ms), ...] in the order tried)."""
+        c = np.ascontiguousarray(counts, dtype=np.int32)
+        p = params._c()
+        best, best_ms, n = _Tuning(), C.c_double(), C.c_int()
+        trials = (_TuningTrial * 32)()
+        self.check(self._lib.cvg_autotune_batch(self._h, len(c), _ptr(c, C.c_int), sz, C.c_void_p(dptr), C.byref(p),
+                                                reps, C.byref(best), C.byref(best_ms), trials, 32, C.byref(n)),
+                   "autotune_batch")
+        as_dict = lambda t: {k: getattr(t, k) for k, _ in _Tuning._fields_}
+        return as_dict(best), best_ms.value, [(as_dict(trials[i].tuning), trials[i].ms) for i in range(min(n.value, 32))]
 
     def batch_curvelets(self, m: int) -> tuple[np.ndarray, np.ndarray]:
         rows, w = C.c_uint(), C.c_uint()

@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdint>
@@ -78,14 +79,14 @@ struct cvg_context {
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     std::string err;
     size_t lds_limit = 65536;
-    // fused-kernel launch shape: buckets whose class cap is at least
-    // coop_min_cap run one anchor per workgroup of coop_waves waves
-    // (CVG_COOP_MIN_CAP / CVG_COOP_WAVES override them; results never depend on it)
+    // fused-kernel launch shape: buckets whose key is at least coop_min_cap
+    // run one anchor per workgroup of coop_waves waves (cvg_set_tuning, or
+    // CVG_COOP_MIN_CAP / CVG_COOP_WAVES at creation; results never depend on it)
     int coop_min_cap = 8;
     int coop_waves = 4;
     int num_cu = 1;
     // lanes per anchor in the neighbour-window scans (8, 16, 32 or 64;
-    // CVG_FILL_LANES overrides it; results never depend on it)
+    // cvg_set_tuning, or CVG_FILL_LANES at creation; results never depend on it)
     int fill_lanes = 16;
 
     unsigned *mailbox = nullptr;  // pinned host scalars read back after syncs
@@ -2757,6 +2758,87 @@ int cvg_run_batch_device(cvg_context *ctx, int num_maps, const int *num_edges, i
     CVG_TRY(batch_offsets(ctx, num_maps, num_edges, eoff));
     return run_common(ctx, static_cast<int>(eoff[num_maps]), sz_edge_data, device_edges, params, stats, num_maps,
                       eoff.data());
+}
+
+int cvg_get_tuning(cvg_context *ctx, cvg_tuning *tuning)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!tuning) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "cvg_get_tuning: null tuning");
+    tuning->coop_min_key = ctx->coop_min_cap;
+    tuning->coop_waves = ctx->coop_waves;
+    tuning->window_lanes = ctx->fill_lanes;
+    return CVG_OK;
+}
+
+int cvg_set_tuning(cvg_context *ctx, const cvg_tuning *tuning)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!tuning) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "cvg_set_tuning: null tuning");
+    const int w = tuning->coop_waves, f = tuning->window_lanes;
+    if (tuning->coop_min_key < 1 || !(w == 1 || w == 2 || w == 4) || !(f == 8 || f == 16 || f == 32 || f == 64))
+        return fail(ctx, CVG_ERR_INVALID_ARGUMENT,
+                    "cvg_set_tuning: need coop_min_key >= 1, coop_waves in {1, 2, 4}, window_lanes in {8, 16, 32, 64}");
+    ctx->coop_min_cap = tuning->coop_min_key;
+    ctx->coop_waves = w;
+    ctx->fill_lanes = f;
+    return CVG_OK;
+}
+
+int cvg_autotune_batch(cvg_context *ctx, int num_maps, const int *num_edges, int sz_edge_data,
+                       const double *device_edges, const cvg_params *params, int reps, cvg_tuning *best,
+                       double *best_ms, cvg_tuning_trial *trials, int max_trials, int *num_trials)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (reps < 1 || (max_trials > 0 && !trials))
+        return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "cvg_autotune_batch: need reps >= 1 and room for the trials");
+    int count = 0;
+    // mean wall time of `reps` runs (after one warm-up) under tuning t
+    auto time_of = [&](const cvg_tuning &t, double *ms) -> int {
+        CVG_TRY(cvg_set_tuning(ctx, &t));
+        CVG_TRY(cvg_run_batch_device(ctx, num_maps, num_edges, sz_edge_data, device_edges, params, nullptr));
+        CVG_TRY(cvg_synchronize(ctx));
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int r = 0; r < reps; ++r) {
+            CVG_TRY(cvg_run_batch_device(ctx, num_maps, num_edges, sz_edge_data, device_edges, params, nullptr));
+            CVG_TRY(finish_curvelets(ctx));
+        }
+        const auto t1 = std::chrono::steady_clock::now();
+        *ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / reps;
+        if (count < max_trials) trials[count] = cvg_tuning_trial{t, *ms};
+        ++count;
+        return CVG_OK;
+    };
+    cvg_tuning cur{};
+    CVG_TRY(cvg_get_tuning(ctx, &cur));
+    double cur_ms = 0.0;
+    CVG_TRY(time_of(cur, &cur_ms));
+    // coordinate descent: the cooperative threshold, its waves, the scan lanes
+    const int keys[] = {4, 6, 8, 11, 16, 1 << 30};
+    const int waves[] = {2, 4};
+    const int lanes[] = {8, 16, 32};
+    for (int field = 0; field < 3; ++field) {
+        const int *vals = field == 0 ? keys : (field == 1 ? waves : lanes);
+        const int nv = field == 0 ? 6 : (field == 1 ? 2 : 3);
+        cvg_tuning base = cur;
+        for (int k = 0; k < nv; ++k) {
+            cvg_tuning t = base;
+            int &slot = field == 0 ? t.coop_min_key : (field == 1 ? t.coop_waves : t.window_lanes);
+            const int old = field == 0 ? base.coop_min_key : (field == 1 ? base.coop_waves : base.window_lanes);
+            if (vals[k] == old) continue;
+            slot = vals[k];
+            double ms = 0.0;
+            CVG_TRY(time_of(t, &ms));
+            if (ms < cur_ms) {
+                cur = t;
+                cur_ms = ms;
+            }
+        }
+    }
+    CVG_TRY(cvg_set_tuning(ctx, &cur));
+    if (best) *best = cur;
+    if (best_ms) *best_ms = cur_ms;
+    if (num_trials) *num_trials = count;
+    return CVG_OK;
 }
 
 int cvg_batch_map_shape(cvg_context *ctx, int map, unsigned *num_curvelets, unsigned *chain_width)

@@ -184,6 +184,35 @@ int cvg_run_batch_device(cvg_context *ctx, int num_maps, const int *num_edges, i
 int cvg_batch_map_shape(cvg_context *ctx, int map, unsigned *num_curvelets, unsigned *chain_width);
 int cvg_get_batch_map_curvelets(cvg_context *ctx, int map, unsigned *chains, double *info);
 
+/* ---- launch tuning ---------------------------------------------------------
+ * The launch shape of a context's kernels.  Results never depend on it; only
+ * the time does.  The reference has no counterpart (it is a CPU loop); these
+ * serve the batched and served forms, whose map mixes differ from the
+ * defaults' 1920x1080 tuning.  The environment variables CVG_COOP_MIN_CAP,
+ * CVG_COOP_WAVES and CVG_FILL_LANES set the initial values of a new context. */
+typedef struct cvg_tuning {
+    int coop_min_key;  /* anchor buckets whose key (max(forward, backward) + neither
+                          neighbours) is at least this run one anchor per workgroup of
+                          coop_waves waves; lighter ones one anchor per wave (>= 1) */
+    int coop_waves;    /* 1, 2 or 4 */
+    int window_lanes;  /* lanes per anchor in the neighbour-window scans: 8, 16, 32 or 64 */
+} cvg_tuning;
+typedef struct cvg_tuning_trial {
+    cvg_tuning tuning;
+    double ms;         /* mean wall time per run of the whole path, results in HBM */
+} cvg_tuning_trial;
+int cvg_get_tuning(cvg_context *ctx, cvg_tuning *tuning);
+int cvg_set_tuning(cvg_context *ctx, const cvg_tuning *tuning);
+/* Autotuner for the batched form: times cvg_run_batch_device on the caller's
+ * maps under candidate tunings (coordinate descent over the three fields,
+ * one warm-up and `reps` timed runs each) and keeps the fastest on the
+ * context.  best/best_ms receive the result; trials (capacity max_trials, may
+ * be null) every candidate tried, in order; num_trials their count.  The last
+ * run's results stay on the context as after cvg_run_batch_device. */
+int cvg_autotune_batch(cvg_context *ctx, int num_maps, const int *num_edges, int sz_edge_data,
+                       const double *device_edges, const cvg_params *params, int reps, cvg_tuning *best,
+                       double *best_ms, cvg_tuning_trial *trials, int max_trials, int *num_trials);
+
 /* ---- served form ----------------------------------------------------------
  * A request loop with continuous batching.  Requests (one edge map each) queue
  * as they arrive; each of `workers` threads owns a context (streams and device

@@ -79,3 +79,45 @@ def test_batch_of_one_and_repeat(ctx):
     ctx.run(e, p)
     again = ctx.curvelets()
     assert np.array_equal(again[0], want[0]) and np.array_equal(bits(again[1]), bits(want[1]))
+
+
+def test_tuning_set_get_and_rejects(ctx):
+    from curvelet_construction_on_gpu_amd import CurveletError
+    old = ctx.tuning
+    try:
+        ctx.tuning = {"coop_min_key": 11, "coop_waves": 2, "window_lanes": 32}
+        assert ctx.tuning == {"coop_min_key": 11, "coop_waves": 2, "window_lanes": 32}
+        for bad in ({"coop_min_key": 0, "coop_waves": 4, "window_lanes": 16},
+                    {"coop_min_key": 8, "coop_waves": 3, "window_lanes": 16},
+                    {"coop_min_key": 8, "coop_waves": 4, "window_lanes": 12}):
+            with pytest.raises(CurveletError):
+                ctx.tuning = bad
+        assert ctx.tuning == {"coop_min_key": 11, "coop_waves": 2, "window_lanes": 32}
+    finally:
+        ctx.tuning = old
+
+
+def test_autotune_batch_keeps_results(ctx):
+    # the autotuner only chooses a launch shape: every map's bits are the same
+    # under the default tuning and under the one it keeps
+    maps = [synth_edges("C1", 1001), synth_edges("C2", 1002)[:6000], lattice(30, 20, 5),
+            np.array(edges_for("file:TO_edges_digit1"))]
+    p = CurveletParams(group_max_sz=5)
+    old = ctx.tuning
+    try:
+        ctx.run_batch(maps, p)
+        want = [ctx.batch_curvelets(i) for i in range(len(maps))]
+        flat = np.ascontiguousarray(np.concatenate(maps, axis=0))
+        d = ctx.upload(flat)
+        best, best_ms, trials = ctx.autotune_batch(d, [len(m) for m in maps], 4, p, reps=1)
+        assert best_ms > 0 and len(trials) >= 8
+        assert min(ms for _, ms in trials) == pytest.approx(best_ms)
+        assert ctx.tuning == best
+        assert best["coop_waves"] in (1, 2, 4) and best["window_lanes"] in (8, 16, 32, 64)
+        ctx.run_batch(maps, p)
+        for i in range(len(maps)):
+            got = ctx.batch_curvelets(i)
+            assert np.array_equal(got[0], want[i][0]), i
+            assert np.array_equal(bits(got[1]), bits(want[i][1])), i
+    finally:
+        ctx.tuning = old
