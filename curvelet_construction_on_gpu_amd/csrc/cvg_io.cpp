@@ -13,8 +13,9 @@
 //
 // Writers: write_int_array_to_file / write_double_array_to_file
 // (data_io.hpp:80-133): every value followed by a tab, a newline per row,
-// doubles in default ostream notation, which is printf's %.6g; std::to_chars
-// with chars_format::general and precision 6 produces the same text.
+// doubles in default ostream notation, which is printf's %.6g (format_g6
+// below: a scaled-integer fast path, std::to_chars for the cases it cannot
+// decide).
 //
 // Both directions run on several host threads for large files: the reader
 // splits the text at line boundaries, parses the pieces independently and
@@ -28,6 +29,7 @@
 #include <cerrno>
 #include <charconv>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -36,6 +38,7 @@
 #include <vector>
 
 #include "../../include/curvelet_gpu.h"
+#include "cvg_pow10_table.hpp"
 
 namespace {
 
@@ -135,6 +138,92 @@ bool extract_double(const char **p, const char *e, double *out)
     if (std::isinf(v)) return false;
     *out = v;
     return true;
+}
+
+// printf("%.6g") of a double, as std::ostream writes it.  Fast path: with
+// X = floor(log10 |x|), s = |x| * RN(10^(5 - X)) lies in [99999.5, 999999.5)
+// and carries a relative error below 2^-52 (one rounding of the table value,
+// one of the product), so it is within 2.3e-10 of the exact scaled value; when
+// its fraction is more than 1e-9 away from one half, rounding s to an integer
+// rounds the exact value the same way and gives the 6 significant digits.
+// Everything else (near a tie or a power of ten, zero's sign aside tiny or
+// huge magnitudes, inf, nan) goes through std::to_chars, which is exact.
+char *format_g6(char *p, double x)
+{
+    auto slow = [&]() { return std::to_chars(p, p + 32, x, std::chars_format::general, 6).ptr; };
+    uint64_t u;
+    std::memcpy(&u, &x, sizeof u);
+    const int be = static_cast<int>((u >> 52) & 0x7ff);
+    if (be < 60 || be > 1990) {
+        if ((u << 1) == 0) {  // +-0
+            if (u >> 63) *p++ = '-';
+            *p++ = '0';
+            return p;
+        }
+        return slow();
+    }
+    const double v = std::fabs(x);
+    int X = ((be - 1023) * 78913) >> 18;  // floor(e2 log10 2) up to one
+    if (v >= cvg::kPow10[X + 1 - cvg::kPow10Min]) ++X;
+    double s = v * cvg::kPow10[5 - X - cvg::kPow10Min];
+    if (s < 99999.5) {
+        --X;
+        s = v * cvg::kPow10[5 - X - cvg::kPow10Min];
+    }
+    const long n0 = static_cast<long>(s);
+    const double f = s - static_cast<double>(n0);
+    if (!(s >= 99999.5 && s < 999999.4999) || std::fabs(f - 0.5) < 1e-9) return slow();
+    unsigned n = static_cast<unsigned>(n0) + (f > 0.5 ? 1u : 0u);
+    if (n >= 1000000u) {
+        n /= 10u;
+        ++X;
+    }
+    char d[6];
+    const unsigned hi = n / 1000u, lo = n - hi * 1000u;
+    d[0] = static_cast<char>('0' + hi / 100u);
+    d[1] = static_cast<char>('0' + (hi / 10u) % 10u);
+    d[2] = static_cast<char>('0' + hi % 10u);
+    d[3] = static_cast<char>('0' + lo / 100u);
+    d[4] = static_cast<char>('0' + (lo / 10u) % 10u);
+    d[5] = static_cast<char>('0' + lo % 10u);
+    int nd = 6;  // significant digits after trailing zeros are dropped
+    while (nd > 1 && d[nd - 1] == '0') --nd;
+    if (u >> 63) *p++ = '-';
+    if (X >= -4 && X < 6) {  // fixed notation
+        if (X >= 0) {
+            for (int i = 0; i <= X; ++i) *p++ = i < nd ? d[i] : '0';
+            if (nd > X + 1) {
+                *p++ = '.';
+                for (int i = X + 1; i < nd; ++i) *p++ = d[i];
+            }
+        } else {
+            *p++ = '0';
+            *p++ = '.';
+            for (int i = 0; i < -X - 1; ++i) *p++ = '0';
+            for (int i = 0; i < nd; ++i) *p++ = d[i];
+        }
+    } else {  // d.ddddde+XX
+        *p++ = d[0];
+        if (nd > 1) {
+            *p++ = '.';
+            for (int i = 1; i < nd; ++i) *p++ = d[i];
+        }
+        *p++ = 'e';
+        int ex = X;
+        if (ex < 0) {
+            *p++ = '-';
+            ex = -ex;
+        } else {
+            *p++ = '+';
+        }
+        if (ex >= 100) {
+            *p++ = static_cast<char>('0' + ex / 100);
+            ex %= 100;
+        }
+        *p++ = static_cast<char>('0' + ex / 10);
+        *p++ = static_cast<char>('0' + ex % 10);
+    }
+    return p;
 }
 
 enum ParseError { kNone = 0, kBlank, kShort, kExtra };
@@ -343,7 +432,7 @@ int cvg_write_info_file(const char *path, const double *info, unsigned rows, uns
                       [&](unsigned i, char *p) {
                           for (unsigned j = 0; j < width; ++j) {
                               const double v = info[static_cast<size_t>(i) * width + j];
-                              p = std::to_chars(p, p + 24, v, std::chars_format::general, 6).ptr;
+                              p = format_g6(p, v);
                               *p++ = '\t';
                           }
                           *p++ = '\n';

@@ -237,6 +237,28 @@ def test_info_writer_format(tmp_path):
     assert p.read_text() == want
 
 
+def test_info_writer_hard_cases_match_printf_g(tmp_path):
+    # the fast %.6g path must defer to the exact one at every rounding tie,
+    # at powers of ten and at the exponent extremes
+    rng = np.random.default_rng(21)
+    ties = (rng.integers(100000, 1000000, 20000) + 0.5) * 10.0 ** rng.integers(-25, 25, 20000).astype(float) / 1e5
+    near = np.concatenate([np.nextafter(ties, np.inf), np.nextafter(ties, -np.inf), ties])
+    p10 = 10.0 ** np.arange(-300, 301, dtype=float)
+    around = np.concatenate([p10, np.nextafter(p10, np.inf), np.nextafter(p10, 0), p10 * 0.9999995, p10 * 0.99999949])
+    bits_rand = rng.integers(0, 2**63, 60000, dtype=np.uint64).view(np.float64)
+    bits_rand = bits_rand[np.isfinite(bits_rand)]
+    misc = np.array([0.0, -0.0, 5e-324, -5e-324, 2.2250738585072014e-308, 1.7976931348623157e308,
+                     999999.5, 999999.4999999999, 99999.95, 0.0001, 0.00009999995, 123456.5, 1e-5, 1e16])
+    vals = np.concatenate([near, -near, around, -around, bits_rand, misc])
+    vals = vals[: len(vals) // 10 * 10].reshape(-1, 10)
+    p = tmp_path / "i.txt"
+    write_info_file(p, vals)
+    lines = p.read_text().split("\n")
+    assert len(lines) == len(vals) + 1
+    for k, row in enumerate(vals):
+        assert lines[k] == "".join("%g\t" % v for v in row), k
+
+
 def test_writers_many_threads_match_sequential(tmp_path):
     rng = np.random.default_rng(11)
     rows = 200000
