@@ -21,8 +21,9 @@
 // splits the text at line boundaries, parses the pieces independently and
 // reports the first error in file order (its line number counted over the
 // pieces before it); the writers format row ranges independently and write
-// them in order.  Numbers are converted by strtod_l in the "C" locale from a
-// stack copy of the token (no allocation per number).
+// them in order.  Numbers are converted by the Eisel-Lemire algorithm (up to
+// 19 significant digits, correctly rounded like strtod), else by strtod_l in
+// the "C" locale from a stack copy of the token; no allocation per number.
 #include <locale.h>
 
 #include <algorithm>
@@ -39,6 +40,7 @@
 
 #include "../../include/curvelet_gpu.h"
 #include "cvg_pow10_table.hpp"
+#include "cvg_pow5_table.hpp"
 
 namespace {
 
@@ -88,6 +90,107 @@ void run_parallel(int n, F f)
     for (auto &t : th) t.join();
 }
 
+// Decimal to double for w * 10^q (w < 10^19): the Eisel-Lemire algorithm
+// (D. Lemire, "Number Parsing at a Gigabyte per Second", Software: Practice
+// and Experience 51(8), 2021; N. Mushtak and D. Lemire, "Fast number parsing
+// without fallback", 2023): w, normalized, times a 128-bit approximation of
+// 5^q (cvg_pow5_table.hpp) gives the correctly rounded significand, ties to
+// even.  Returns false for a subnormal result, which is left to strtod.
+bool eisel_lemire(uint64_t w, int q, bool neg, double *out)
+{
+    uint64_t bits;
+    if (w == 0 || q < cvg::kPow5Min) {
+        bits = 0;
+    } else if (q > cvg::kPow5Max) {
+        bits = uint64_t(0x7ff) << 52;  // infinity
+    } else {
+        const int lz = __builtin_clzll(w);
+        w <<= lz;
+        const int idx = 2 * (q - cvg::kPow5Min);
+        const unsigned __int128 p = static_cast<unsigned __int128>(w) * cvg::kPow5[idx];
+        uint64_t hi = static_cast<uint64_t>(p >> 64), lo = static_cast<uint64_t>(p);
+        if ((hi & 0x1FF) == 0x1FF) {  // the low 9 bits could carry: add the next 64 bits of 5^q
+            const uint64_t h2 = static_cast<uint64_t>((static_cast<unsigned __int128>(w) * cvg::kPow5[idx + 1]) >> 64);
+            lo += h2;
+            if (h2 > lo) ++hi;
+        }
+        const int upper = static_cast<int>(hi >> 63);
+        const int shift = upper + 9;  // keep 52 + 3 bits
+        uint64_t m = hi >> shift;
+        int e2 = ((((152170 + 65536) * q) >> 16) + 63) + upper - lz + 1023;
+        if (e2 <= 0) return false;
+        // exactly halfway (only possible for small |q|): round to even
+        if (lo <= 1 && q >= -4 && q <= 23 && (m & 3) == 1 && (m << shift) == hi) m &= ~uint64_t(1);
+        m += m & 1;
+        m >>= 1;
+        if (m >= (uint64_t(2) << 52)) {
+            m = uint64_t(1) << 52;
+            ++e2;
+        }
+        m &= ~(uint64_t(1) << 52);
+        bits = e2 >= 0x7ff ? uint64_t(0x7ff) << 52 : (static_cast<uint64_t>(e2) << 52) | m;
+    }
+    if (neg) bits |= uint64_t(1) << 63;
+    std::memcpy(out, &bits, sizeof bits);
+    return true;
+}
+
+// The token [s, e) has the scanner's form [sign] digits [. digits] [(e|E)
+// [sign] digits].  Returns false when it is outside the fast path (no
+// mantissa digit, an exponent without digits, more than 19 significant
+// digits, a huge exponent, a subnormal result); the caller then uses strtod.
+bool parse_fast(const char *s, const char *e, double *out)
+{
+    bool neg = false;
+    if (s < e && (*s == '+' || *s == '-')) {
+        neg = *s == '-';
+        ++s;
+    }
+    uint64_t w = 0;
+    int nd = 0, q = 0;
+    bool any = false, nonzero = false;
+    for (; s < e && is_digit(*s); ++s) {
+        any = true;
+        const unsigned dg = static_cast<unsigned>(*s - '0');
+        if (nonzero || dg != 0) {
+            if (nd == 19) return false;
+            w = w * 10 + dg;
+            ++nd;
+            nonzero = true;
+        }
+    }
+    if (s < e && *s == '.') {
+        for (++s; s < e && is_digit(*s); ++s) {
+            any = true;
+            const unsigned dg = static_cast<unsigned>(*s - '0');
+            if (nonzero || dg != 0) {
+                if (nd == 19) return false;
+                w = w * 10 + dg;
+                ++nd;
+                nonzero = true;
+            }
+            --q;
+        }
+    }
+    if (!any) return false;
+    if (s < e && (*s == 'e' || *s == 'E')) {
+        ++s;
+        bool eneg = false;
+        if (s < e && (*s == '+' || *s == '-')) {
+            eneg = *s == '-';
+            ++s;
+        }
+        if (s == e) return false;
+        int x = 0;
+        for (; s < e && is_digit(*s); ++s)
+            if (x < 100000) x = x * 10 + (*s - '0');
+        if (x >= 100000) return false;
+        q += eneg ? -x : x;
+    }
+    if (s != e) return false;
+    return eisel_lemire(w, q, neg, out);
+}
+
 // One istream >> double on [*p, e).  Advances *p past what the extraction
 // consumed.  Returns false on failure (as the stream's failbit).
 bool extract_double(const char **p, const char *e, double *out)
@@ -118,6 +221,12 @@ bool extract_double(const char **p, const char *e, double *out)
     }
     *p = q;
     if (q == start) return false;
+    double fast;
+    if (parse_fast(start, q, &fast)) {
+        if (std::isinf(fast)) return false;  // overflow: the stream's failbit
+        *out = fast;
+        return true;
+    }
     // strtod must see exactly the collected characters (it would read
     // further, e.g. a hexadecimal form, on the raw buffer)
     const size_t len = static_cast<size_t>(q - start);

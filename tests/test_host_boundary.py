@@ -150,6 +150,43 @@ def test_reader_number_grammar(tmp_path, text, want):
     assert np.array_equal(got.view(np.uint64), np.array(want, dtype=np.float64).view(np.uint64))
 
 
+def test_reader_fast_conversion_matches_correct_rounding(tmp_path):
+    # numbers of every shape the grammar admits, against Python's correctly
+    # rounded float() (what strtod returns): 17-digit and short forms,
+    # halfway points between adjacent doubles, 19 and 20 significant digits,
+    # subnormals, huge and tiny exponents
+    rng = np.random.default_rng(31)
+    toks = []
+    u = rng.integers(0, 2**63, 3000, dtype=np.uint64).view(np.float64)
+    u = u[np.isfinite(u)]
+    toks += [f"{v:.17g}" for v in u] + [f"{v:.{int(k)}e}" for v, k in zip(u, rng.integers(0, 20, len(u)))]
+    a = rng.uniform(-2e3, 2e3, 3000)
+    toks += [f"{v:.17g}" for v in a] + [f"{v:.13f}" for v in a]
+    from decimal import Decimal
+    lo = rng.uniform(1, 2, 2000) * 10.0 ** rng.integers(-30, 30, 2000)
+    hi = np.nextafter(lo, np.inf)
+    for x, y in zip(lo, hi):   # exact midpoint, and a hair to either side
+        m = (Decimal(float(x)) + Decimal(float(y))) / 2
+        toks += [f"{m:.40e}", f"{m:.18e}", f"{m:.19e}"]
+    toks += ["1234567890123456789", "12345678901234567890", "9007199254740993", "9007199254740992.5",
+             "2.4703282292062327e-324", "2.4703282292062328e-324", "4.9406564584124654e-324", "1e-400",
+             "2.2250738585072011e-308", "2.2250738585072014e-308", "1.7976931348623157e308", "1e23",
+             "0.000000000000000000000000000000001", "+.5e-3", "-0.0", "00000000000000000000000001.5", "7e0"]
+    rows = [toks[i:i + 4] for i in range(0, len(toks) // 4 * 4, 4)]
+    text = "\n".join(" ".join(r) for r in rows) + "\n"
+    got = read_TO_edges_from_file(write(tmp_path, text))
+    want = np.array([[float(t) for t in r] for r in rows])
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+
+
+@pytest.mark.parametrize("tok,found", [(".", 2), ("-", 2), ("+", 2), ("1e", 2), ("1e+", 2), ("1.5E-", 2),
+                                       ("1e400", 2), ("-1e309", 2)])
+def test_reader_malformed_and_overflowing_tokens(tmp_path, tok, found):
+    with pytest.raises(CurveletError) as ei:
+        read_TO_edges_from_file(write(tmp_path, f"1 2 {tok} 4\n"))
+    assert f"found {found}." in str(ei.value)
+
+
 def test_reader_values_per_edge(tmp_path):
     p = write(tmp_path, "1 2 3 4 5 6\n7 8 9 10 11 12\n")
     assert read_TO_edges_from_file(p, 6).shape == (2, 6)
