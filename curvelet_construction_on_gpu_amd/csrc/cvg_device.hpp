@@ -36,10 +36,21 @@ enum {
 // Direction class of a neighbour: 1 when dot(To2Pi(atan2(ny-ay, nx-ax)), theta_a)
 // > 0 (forward run), 2 when < 0 (backward run), 0 otherwise
 // (cpu_curvelet.hpp:459-461, curvelet_utils.hpp:58-69).
+//
+// The reference's dot is cos(phi - theta_a), phi = atan2(vy, vx), v = n - a,
+// up to its evaluation errors (atan2, To2Pi, cos, sin within an ulp each, the
+// products and the sum), below 2e-15 in all.  The projection p = vx cos_at +
+// vy sin_at, from the correctly rounded cos/sin of theta_a, is within 6e-16 |v|
+// of |v| cos(phi - theta_a).  So when |p| > 1e-12 |v| the two signs agree and
+// the class is sign(p); only near-perpendicular neighbours evaluate the
+// reference's expression.
 __device__ inline int direction_class(double ax, double ay, double cos_at, double sin_at, double nx, double ny,
                                       const double *sc_tab, const double *at_tab)
 {
-    const double dir = wrap_two_pi(cr_atan2(ny - ay, nx - ax, at_tab));
+    const double vx = nx - ax, vy = ny - ay;
+    const double p = vx * cos_at + vy * sin_at;
+    if (fabs(p) > 1e-12 * sqrt(vx * vx + vy * vy)) return p > 0 ? 1 : 2;
+    const double dir = wrap_two_pi(cr_atan2(vy, vx, at_tab));
     double sd, cd;
     cr_sincos(dir, sc_tab, &sd, &cd);
     const double dot = cd * cos_at + sd * sin_at;
