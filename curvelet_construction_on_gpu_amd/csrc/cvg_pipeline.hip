@@ -1303,6 +1303,13 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
         ccell[m] = A.rank_cell[r];
         elig[m] = live[m] && A.rank_elig[r] != 0;
     }
+    // the same as wave masks: a validity test is then one compare ballot
+    unsigned long long lmask[CPL], emask[CPL];
+#pragma unroll
+    for (int m = 0; m < CPL; ++m) {
+        lmask[m] = __ballot(live[m]);
+        emask[m] = __ballot(elig[m]);
+    }
 
     // the pipeline: entry, staging copy and per-lane values of the next anchor
     const double2 *prec2 = reinterpret_cast<const double2 *>(A.prec);
@@ -1433,7 +1440,7 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
                 const bool qclose = (fl & 4) != 0;
                 const bool qbr = (fl & 8) != 0;
                 double lo[CPL], hi[CPL];
-                bool any = false;
+                unsigned long long anym = 0ULL;
 #pragma unroll
                 for (int m = 0; m < CPL; ++m) {
                     lo[m] = -A.max_k;
@@ -1443,9 +1450,9 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
                         cvg::bundle_cell(q, qbr, ox[m], sT2t[m], cT2t[m], s.x, s.y, A.dx, A.dx2, A.max_k, lo[m],
                                          hi[m]);
                     }
-                    any = any || (live[m] && hi[m] > lo[m]);
+                    anym |= __builtin_amdgcn_ballot_w64(hi[m] > lo[m]) & lmask[m];
                 }
-                const bool hyp = qclose || __ballot(any) != 0ULL;  // bundle_valid_check, :821-830
+                const bool hyp = qclose || anym != 0ULL;  // bundle_valid_check, :821-830
                 vpw += hyp ? 1u : 0u;
                 if (run == 2) continue;
                 if (hyp) {
@@ -1524,14 +1531,14 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
                             bh[m] = *reinterpret_cast<const double *>(bhi_b + noff + coff[m] * 8);
                         }
                         double nlo[CPL], nhi[CPL];
-                        bool ok = false;
+                        unsigned long long okm = 0ULL;
 #pragma unroll
                         for (int m = 0; m < CPL; ++m) {
                             nlo[m] = (lo[m] < bl[m]) ? bl[m] : lo[m];  // bundle_intersection, :838-841
                             nhi[m] = (hi[m] > bh[m]) ? bh[m] : hi[m];
-                            ok = ok || (live[m] && nhi[m] > nlo[m]);
+                            okm |= __builtin_amdgcn_ballot_w64(nhi[m] > nlo[m]) & lmask[m];
                         }
-                        if (h == j || __ballot(ok) != 0ULL) {
+                        if (h == j || okm != 0ULL) {
                             const int id = lane_get<NCH>(hid, h);
                             cur = lane == len ? id : cur;
                             hsh = chain_hash(hsh, id);
@@ -1551,7 +1558,7 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
                     int best = -1;
 #pragma unroll
                     for (int m = 0; m < CPL; ++m) {
-                        const unsigned long long mask = __ballot(elig[m] && hi[m] > lo[m]);
+                        const unsigned long long mask = __builtin_amdgcn_ballot_w64(hi[m] > lo[m]) & emask[m];
                         if (best < 0 && mask != 0ULL) best = 64 * m + __builtin_ctzll(mask);
                     }
                     if (best < 0) best = A.rank_of_cell0;
