@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -40,6 +41,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/curvelet_gpu.h"
@@ -90,6 +92,9 @@ struct cvg_context {
     int fill_lanes = 16;
 
     unsigned *mailbox = nullptr;  // pinned host scalars read back after syncs
+    // page-locked staging blocks for downloads into pageable caller memory
+    void *h_stage[2] = {nullptr, nullptr};
+    hipEvent_t ev_stage[2] = {nullptr, nullptr};
 
     // graph
     int n = 0;
@@ -2483,6 +2488,107 @@ int upload(cvg_context *ctx, int n, int sz, const double *host, const double **d
     return CVG_OK;
 }
 
+// Device-to-host copy into caller memory, complete on return.  Page-locked
+// or small destinations take one DMA.  A pageable destination would make the
+// runtime stage the copy itself at about a third of the link's rate, so it
+// goes through two page-locked blocks of kStageBytes: while block k is
+// copied out on host threads, block k + 1's DMA runs.
+constexpr size_t kStageBytes = static_cast<size_t>(8) << 20;
+constexpr size_t kStageMinBytes = static_cast<size_t>(2) << 20;
+
+bool page_locked(const void *p)
+{
+    hipPointerAttribute_t a;
+    const hipError_t e = hipPointerGetAttributes(&a, p);
+    (void)hipGetLastError();  // an unregistered pointer reports an error: it must not stay pending
+    return e == hipSuccess && a.type == hipMemoryTypeHost;
+}
+
+int download(cvg_context *ctx, void *dst, const void *src, size_t bytes)
+{
+    hipStream_t st = ctx->stream;
+    if (bytes == 0) return CVG_OK;
+    if (bytes < kStageMinBytes || page_locked(dst)) {
+        CVG_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+        CVG_HIP(ctx, hipStreamSynchronize(st));
+        return CVG_OK;
+    }
+    for (int b = 0; b < 2; ++b) {
+        if (!ctx->h_stage[b]) {
+            if (hipHostMalloc(&ctx->h_stage[b], kStageBytes, hipHostMallocDefault) != hipSuccess) {
+                ctx->h_stage[b] = nullptr;
+                return fail(ctx, CVG_ERR_OUT_OF_MEMORY, "hipHostMalloc (download staging) failed");
+            }
+        }
+        if (!ctx->ev_stage[b]) CVG_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_stage[b], hipEventDisableTiming));
+    }
+    const size_t nchunk = (bytes + kStageBytes - 1) / kStageBytes;
+    const char *s = static_cast<const char *>(src);
+    char *d = static_cast<char *>(dst);
+    auto chunk_len = [&](size_t k) { return std::min(kStageBytes, bytes - k * kStageBytes); };
+    auto issue = [&](size_t k) -> int {
+        const int b = static_cast<int>(k & 1);
+        CVG_HIP(ctx, hipMemcpyAsync(ctx->h_stage[b], s + k * kStageBytes, chunk_len(k), hipMemcpyDeviceToHost, st));
+        CVG_HIP(ctx, hipEventRecord(ctx->ev_stage[b], st));
+        return CVG_OK;
+    };
+    // host copy-out: nw helper threads and this one split each block; the
+    // block index a helper may copy is published in `ready`, and `done`
+    // counts finished slices (every helper does one slice per block)
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int nw = static_cast<int>(std::min<unsigned>(hw > 1 ? hw - 1 : 0, 7));
+    const int parts = nw + 1;
+    std::atomic<long long> ready{-1};
+    std::atomic<long long> done{0};
+    std::atomic<bool> quit{false};
+    auto slice = [&](size_t k, int part) {
+        const size_t len = chunk_len(k);
+        const size_t per = ((len + parts - 1) / parts + 63) & ~static_cast<size_t>(63);
+        const size_t o = std::min(len, per * part), e = std::min(len, o + per);
+        if (e > o) std::memcpy(d + k * kStageBytes + o, static_cast<const char *>(ctx->h_stage[k & 1]) + o, e - o);
+    };
+    std::vector<std::thread> pool;
+    for (int w = 0; w < nw; ++w) {
+        try {
+            pool.emplace_back([&, w] {
+                for (size_t k = 0; k < nchunk; ++k) {
+                    long long r;
+                    while ((r = ready.load(std::memory_order_acquire)) < static_cast<long long>(k)) {
+                        if (quit.load(std::memory_order_relaxed)) return;
+                        std::this_thread::yield();
+                    }
+                    slice(k, w + 1);
+                    done.fetch_add(1, std::memory_order_release);
+                }
+            });
+        } catch (...) {
+            break;  // fewer helpers: the slices of missing ones are copied below
+        }
+    }
+    const int started = static_cast<int>(pool.size());
+    int rc = issue(0);
+    for (size_t k = 0; rc == CVG_OK && k < nchunk; ++k) {
+        // block (k + 1) & 1 last held chunk k - 1: every slice of it is copied
+        while (done.load(std::memory_order_acquire) < static_cast<long long>(k) * started) std::this_thread::yield();
+        if (k + 1 < nchunk) rc = issue(k + 1);
+        if (rc != CVG_OK) break;
+        if (hipEventSynchronize(ctx->ev_stage[k & 1]) != hipSuccess) {
+            rc = fail(ctx, CVG_ERR_DEVICE, "download: staging copy failed");
+            break;
+        }
+        ready.store(static_cast<long long>(k), std::memory_order_release);
+        slice(k, 0);
+        for (int w = started + 1; w < parts; ++w) slice(k, w);
+    }
+    if (rc == CVG_OK)
+        while (done.load(std::memory_order_acquire) < static_cast<long long>(nchunk) * started)
+            std::this_thread::yield();
+    quit.store(true, std::memory_order_relaxed);
+    for (auto &t : pool) t.join();
+    if (rc != CVG_OK) (void)hipStreamSynchronize(st);  // no DMA may still target a staging block
+    return rc;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -2595,6 +2701,10 @@ void cvg_destroy(cvg_context *ctx)
         if (e) (void)hipEventDestroy(e);
     if (ctx->mailbox) (void)hipHostFree(ctx->mailbox);
     if (ctx->h_maprow) (void)hipHostFree(ctx->h_maprow);
+    for (int b = 0; b < 2; ++b) {
+        if (ctx->h_stage[b]) (void)hipHostFree(ctx->h_stage[b]);
+        if (ctx->ev_stage[b]) (void)hipEventDestroy(ctx->ev_stage[b]);
+    }
     for (int i = 0; i < 3; ++i) {
         if (ctx->aux[i]) (void)hipStreamSynchronize(ctx->aux[i]);
         if (ctx->ev_join[i]) (void)hipEventDestroy(ctx->ev_join[i]);
@@ -2644,11 +2754,9 @@ int cvg_get_csr(cvg_context *ctx, int *offsets, int *ids, double *dist2)
         CVG_HIP(ctx, hipMemcpyAsync(offsets, ctx->d_off.ptr, static_cast<size_t>(ctx->n + 1) * sizeof(int),
                                     hipMemcpyDeviceToHost, st));
     if (ids && ctx->pairs > 0)
-        CVG_HIP(ctx, hipMemcpyAsync(ids, ctx->d_ids.ptr, static_cast<size_t>(ctx->pairs) * sizeof(int),
-                                    hipMemcpyDeviceToHost, st));
+        CVG_TRY(download(ctx, ids, ctx->d_ids.ptr, static_cast<size_t>(ctx->pairs) * sizeof(int)));
     if (dist2 && ctx->pairs > 0)
-        CVG_HIP(ctx, hipMemcpyAsync(dist2, ctx->d_d2.ptr, static_cast<size_t>(ctx->pairs) * sizeof(double),
-                                    hipMemcpyDeviceToHost, st));
+        CVG_TRY(download(ctx, dist2, ctx->d_d2.ptr, static_cast<size_t>(ctx->pairs) * sizeof(double)));
     CVG_HIP(ctx, hipStreamSynchronize(st));
     return CVG_OK;
 }
@@ -2691,12 +2799,8 @@ int cvg_get_curvelets(cvg_context *ctx, unsigned *chains, double *info)
     if (!ctx->curvelets_ready) return fail(ctx, CVG_ERR_STATE, "no curvelets");
     CVG_TRY(finish_curvelets(ctx));
     const size_t C = ctx->num_curvelets;
-    if (chains && C > 0)
-        CVG_HIP(ctx, hipMemcpyAsync(chains, ctx->d_chains.ptr, C * ctx->width * sizeof(unsigned),
-                                    hipMemcpyDeviceToHost, ctx->stream));
-    if (info && C > 0)
-        CVG_HIP(ctx, hipMemcpyAsync(info, ctx->d_info.ptr, C * kInfoWidth * sizeof(double), hipMemcpyDeviceToHost,
-                                    ctx->stream));
+    if (chains && C > 0) CVG_TRY(download(ctx, chains, ctx->d_chains.ptr, C * ctx->width * sizeof(unsigned)));
+    if (info && C > 0) CVG_TRY(download(ctx, info, ctx->d_info.ptr, C * kInfoWidth * sizeof(double)));
     CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return CVG_OK;
 }
@@ -2869,11 +2973,9 @@ int cvg_get_batch_map_curvelets(cvg_context *ctx, int map, unsigned *chains, dou
         return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "map index out of range");
     const size_t r0 = ctx->map_rows[map], rows = ctx->map_rows[map + 1] - r0;
     if (rows > 0 && chains)
-        CVG_HIP(ctx, hipMemcpyAsync(chains, as<unsigned>(ctx->d_chains) + r0 * ctx->width,
-                                    rows * ctx->width * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+        CVG_TRY(download(ctx, chains, as<unsigned>(ctx->d_chains) + r0 * ctx->width, rows * ctx->width * sizeof(unsigned)));
     if (rows > 0 && info)
-        CVG_HIP(ctx, hipMemcpyAsync(info, as<double>(ctx->d_info) + r0 * kInfoWidth, rows * kInfoWidth * sizeof(double),
-                                    hipMemcpyDeviceToHost, ctx->stream));
+        CVG_TRY(download(ctx, info, as<double>(ctx->d_info) + r0 * kInfoWidth, rows * kInfoWidth * sizeof(double)));
     CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return CVG_OK;
 }

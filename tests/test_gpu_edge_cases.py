@@ -210,3 +210,41 @@ def test_results_independent_of_history(ctx):
     ctx.run(a, p)
     ca2, ia2 = ctx.curvelets()
     assert np.array_equal(ca, ca2) and np.array_equal(ia.view(np.uint64), ia2.view(np.uint64))
+
+
+def test_pageable_downloads_match_page_locked(ctx):
+    """Results copied into pageable memory go through page-locked staging blocks
+    (several for the 1080p map's 47 MB); they must equal a direct DMA into
+    page-locked memory, for the curvelets and the neighbour graph."""
+    import ctypes as C
+    e = synthetic("C3", 1003)
+    ctx.run(e, CurveletParams(group_max_sz=7))
+    chains, info = ctx.curvelets()
+    lib = ctx._lib
+    blocks = []
+
+    def page_locked(shape, dtype):
+        p = C.c_void_p()
+        nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        assert lib.cvg_host_alloc(nbytes, C.byref(p)) == 0
+        blocks.append(p)
+        return np.frombuffer((C.c_char * nbytes).from_address(p.value), dtype=dtype).reshape(shape)
+
+    try:
+        pc = page_locked(chains.shape, np.uint32)
+        pi = page_locked(info.shape, np.float64)
+        ctx.check(lib.cvg_get_curvelets(ctx.handle, pc.ctypes.data_as(C.POINTER(C.c_uint)),
+                                        pi.ctypes.data_as(C.POINTER(C.c_double))), "get_curvelets")
+        assert info.nbytes > 3 * (8 << 20)  # four staging blocks
+        assert np.array_equal(chains, pc) and np.array_equal(info.view(np.uint64), pi.view(np.uint64))
+        g = ctx.graph()
+        assert g.neighbor_dist2.nbytes > (8 << 20)
+        pids = page_locked(g.neighbor_ids.shape, np.int32)
+        pd2 = page_locked(g.neighbor_dist2.shape, np.float64)
+        ctx.check(lib.cvg_get_csr(ctx.handle, None, pids.ctypes.data_as(C.POINTER(C.c_int)),
+                                  pd2.ctypes.data_as(C.POINTER(C.c_double))), "get_csr")
+        assert np.array_equal(g.neighbor_ids, pids)
+        assert np.array_equal(g.neighbor_dist2.view(np.uint64), pd2.view(np.uint64))
+    finally:
+        for p in blocks:
+            lib.cvg_host_free(p)
