@@ -39,6 +39,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <new>
 #include <string>
 #include <thread>
@@ -2488,13 +2489,20 @@ int upload(cvg_context *ctx, int n, int sz, const double *host, const double **d
     return CVG_OK;
 }
 
-// Device-to-host copy into caller memory, complete on return.  Page-locked
-// or small destinations take one DMA.  A pageable destination would make the
-// runtime stage the copy itself at about a third of the link's rate, so it
-// goes through two page-locked blocks of kStageBytes: while block k is
-// copied out on host threads, block k + 1's DMA runs.
-constexpr size_t kStageBytes = static_cast<size_t>(8) << 20;
-constexpr size_t kStageMinBytes = static_cast<size_t>(2) << 20;
+// Device-to-host copies into caller memory, complete on return.  Page-locked
+// or small destinations take one DMA each.  A pageable destination would make
+// the runtime stage the copy itself at about a third of the link's rate, so
+// the pageable segments go, as one stream of kStageBytes chunks, through two
+// page-locked blocks: while chunk k is copied out on host threads, chunk
+// k + 1's DMA runs.
+constexpr size_t kStageBytes = static_cast<size_t>(4) << 20;
+constexpr size_t kStageMinBytes = static_cast<size_t>(1) << 20;
+
+struct Segment {
+    void *dst;
+    const void *src;
+    size_t bytes;
+};
 
 bool page_locked(const void *p)
 {
@@ -2504,12 +2512,26 @@ bool page_locked(const void *p)
     return e == hipSuccess && a.type == hipMemoryTypeHost;
 }
 
-int download(cvg_context *ctx, void *dst, const void *src, size_t bytes)
+int download(cvg_context *ctx, std::initializer_list<Segment> segs)
 {
     hipStream_t st = ctx->stream;
-    if (bytes == 0) return CVG_OK;
-    if (bytes < kStageMinBytes || page_locked(dst)) {
-        CVG_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+    struct Chunk {
+        char *dst;
+        const char *src;
+        size_t len;
+    };
+    std::vector<Chunk> chunks;
+    for (const Segment &g : segs) {
+        if (g.bytes == 0 || !g.dst) continue;
+        if (g.bytes < kStageMinBytes || page_locked(g.dst)) {
+            CVG_HIP(ctx, hipMemcpyAsync(g.dst, g.src, g.bytes, hipMemcpyDeviceToHost, st));
+            continue;
+        }
+        for (size_t o = 0; o < g.bytes; o += kStageBytes)
+            chunks.push_back({static_cast<char *>(g.dst) + o, static_cast<const char *>(g.src) + o,
+                              std::min(kStageBytes, g.bytes - o)});
+    }
+    if (chunks.empty()) {
         CVG_HIP(ctx, hipStreamSynchronize(st));
         return CVG_OK;
     }
@@ -2522,38 +2544,34 @@ int download(cvg_context *ctx, void *dst, const void *src, size_t bytes)
         }
         if (!ctx->ev_stage[b]) CVG_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_stage[b], hipEventDisableTiming));
     }
-    const size_t nchunk = (bytes + kStageBytes - 1) / kStageBytes;
-    const char *s = static_cast<const char *>(src);
-    char *d = static_cast<char *>(dst);
-    auto chunk_len = [&](size_t k) { return std::min(kStageBytes, bytes - k * kStageBytes); };
+    const size_t nchunk = chunks.size();
     auto issue = [&](size_t k) -> int {
         const int b = static_cast<int>(k & 1);
-        CVG_HIP(ctx, hipMemcpyAsync(ctx->h_stage[b], s + k * kStageBytes, chunk_len(k), hipMemcpyDeviceToHost, st));
+        CVG_HIP(ctx, hipMemcpyAsync(ctx->h_stage[b], chunks[k].src, chunks[k].len, hipMemcpyDeviceToHost, st));
         CVG_HIP(ctx, hipEventRecord(ctx->ev_stage[b], st));
         return CVG_OK;
     };
-    // host copy-out: nw helper threads and this one split each block; the
-    // block index a helper may copy is published in `ready`, and `done`
-    // counts finished slices (every helper does one slice per block)
+    // host copy-out: nw helper threads and this one split each chunk; the
+    // chunk a helper may copy is published in `ready`, and `done` counts
+    // finished slices (every helper does one slice per chunk)
     const unsigned hw = std::thread::hardware_concurrency();
-    const int nw = static_cast<int>(std::min<unsigned>(hw > 1 ? hw - 1 : 0, 7));
+    const int nw = static_cast<int>(std::min<unsigned>(hw > 1 ? hw - 1 : 0, 11));
     const int parts = nw + 1;
     std::atomic<long long> ready{-1};
     std::atomic<long long> done{0};
     std::atomic<bool> quit{false};
     auto slice = [&](size_t k, int part) {
-        const size_t len = chunk_len(k);
+        const size_t len = chunks[k].len;
         const size_t per = ((len + parts - 1) / parts + 63) & ~static_cast<size_t>(63);
         const size_t o = std::min(len, per * part), e = std::min(len, o + per);
-        if (e > o) std::memcpy(d + k * kStageBytes + o, static_cast<const char *>(ctx->h_stage[k & 1]) + o, e - o);
+        if (e > o) std::memcpy(chunks[k].dst + o, static_cast<const char *>(ctx->h_stage[k & 1]) + o, e - o);
     };
     std::vector<std::thread> pool;
     for (int w = 0; w < nw; ++w) {
         try {
             pool.emplace_back([&, w] {
                 for (size_t k = 0; k < nchunk; ++k) {
-                    long long r;
-                    while ((r = ready.load(std::memory_order_acquire)) < static_cast<long long>(k)) {
+                    while (ready.load(std::memory_order_acquire) < static_cast<long long>(k)) {
                         if (quit.load(std::memory_order_relaxed)) return;
                         std::this_thread::yield();
                     }
@@ -2585,7 +2603,9 @@ int download(cvg_context *ctx, void *dst, const void *src, size_t bytes)
             std::this_thread::yield();
     quit.store(true, std::memory_order_relaxed);
     for (auto &t : pool) t.join();
-    if (rc != CVG_OK) (void)hipStreamSynchronize(st);  // no DMA may still target a staging block
+    // the direct copies, and after a failure any DMA still aimed at a block
+    if (hipStreamSynchronize(st) != hipSuccess && rc == CVG_OK)
+        rc = fail(ctx, CVG_ERR_DEVICE, "download: device-to-host copy failed");
     return rc;
 }
 
@@ -2749,16 +2769,10 @@ int cvg_get_csr(cvg_context *ctx, int *offsets, int *ids, double *dist2)
 {
     CVG_TRY(check_ctx(ctx));
     if (!ctx->graph_ready) return fail(ctx, CVG_ERR_STATE, "no neighbour graph");
-    hipStream_t st = ctx->stream;
-    if (offsets)
-        CVG_HIP(ctx, hipMemcpyAsync(offsets, ctx->d_off.ptr, static_cast<size_t>(ctx->n + 1) * sizeof(int),
-                                    hipMemcpyDeviceToHost, st));
-    if (ids && ctx->pairs > 0)
-        CVG_TRY(download(ctx, ids, ctx->d_ids.ptr, static_cast<size_t>(ctx->pairs) * sizeof(int)));
-    if (dist2 && ctx->pairs > 0)
-        CVG_TRY(download(ctx, dist2, ctx->d_d2.ptr, static_cast<size_t>(ctx->pairs) * sizeof(double)));
-    CVG_HIP(ctx, hipStreamSynchronize(st));
-    return CVG_OK;
+    const size_t P = ctx->pairs > 0 ? static_cast<size_t>(ctx->pairs) : 0;
+    return download(ctx, {{offsets, ctx->d_off.ptr, static_cast<size_t>(ctx->n + 1) * sizeof(int)},
+                          {ids, ctx->d_ids.ptr, P * sizeof(int)},
+                          {dist2, ctx->d_d2.ptr, P * sizeof(double)}});
 }
 
 int cvg_form_pairwise_bundles(cvg_context *ctx, const cvg_params *params, unsigned *valid_pairs)
@@ -2799,10 +2813,8 @@ int cvg_get_curvelets(cvg_context *ctx, unsigned *chains, double *info)
     if (!ctx->curvelets_ready) return fail(ctx, CVG_ERR_STATE, "no curvelets");
     CVG_TRY(finish_curvelets(ctx));
     const size_t C = ctx->num_curvelets;
-    if (chains && C > 0) CVG_TRY(download(ctx, chains, ctx->d_chains.ptr, C * ctx->width * sizeof(unsigned)));
-    if (info && C > 0) CVG_TRY(download(ctx, info, ctx->d_info.ptr, C * kInfoWidth * sizeof(double)));
-    CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return CVG_OK;
+    return download(ctx, {{chains, ctx->d_chains.ptr, C * ctx->width * sizeof(unsigned)},
+                          {info, ctx->d_info.ptr, C * kInfoWidth * sizeof(double)}});
 }
 
 int cvg_run(cvg_context *ctx, int num_edges, int sz_edge_data, const double *edges, const cvg_params *params,
@@ -2972,11 +2984,8 @@ int cvg_get_batch_map_curvelets(cvg_context *ctx, int map, unsigned *chains, dou
     if (map < 0 || map >= static_cast<int>(ctx->map_rows.size()) - 1)
         return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "map index out of range");
     const size_t r0 = ctx->map_rows[map], rows = ctx->map_rows[map + 1] - r0;
-    if (rows > 0 && chains)
-        CVG_TRY(download(ctx, chains, as<unsigned>(ctx->d_chains) + r0 * ctx->width, rows * ctx->width * sizeof(unsigned)));
-    if (rows > 0 && info)
-        CVG_TRY(download(ctx, info, as<double>(ctx->d_info) + r0 * kInfoWidth, rows * kInfoWidth * sizeof(double)));
-    CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CVG_TRY(download(ctx, {{chains, as<unsigned>(ctx->d_chains) + r0 * ctx->width, rows * ctx->width * sizeof(unsigned)},
+                           {info, as<double>(ctx->d_info) + r0 * kInfoWidth, rows * kInfoWidth * sizeof(double)}}));
     return CVG_OK;
 }
 
