@@ -119,6 +119,7 @@ _SIGNATURES = {
     "cvg_device_curvelets": (C.c_int, [C.c_void_p, _P(C.c_void_p), _P(C.c_void_p)]),
     "cvg_get_stream": (C.c_int, [C.c_void_p, _P(C.c_void_p)]),
     "cvg_synchronize": (C.c_int, [C.c_void_p]),
+    "cvg_recent_fused_ms": (C.c_int, [C.c_void_p, C.c_int, _P(C.c_float), _P(C.c_int)]),
     "cvg_run_batch": (C.c_int, [C.c_void_p, C.c_int, _ip, C.c_int, _P(_dp), _P(_Params), _P(_RunStats)]),
     "cvg_run_batch_device": (C.c_int, [C.c_void_p, C.c_int, _ip, C.c_int, C.c_void_p, _P(_Params), _P(_RunStats)]),
     "cvg_batch_map_shape": (C.c_int, [C.c_void_p, C.c_int, _up, _up]),
@@ -311,12 +312,21 @@ class Context:
         self.check(self._lib.cvg_upload_edges(self._h, n, sz, _ptr(e, C.c_double), C.byref(d)), "upload_edges")
         return d.value
 
-    def run_device(self, dptr: int, n: int, sz: int, params: CurveletParams) -> dict:
+    def run_device(self, dptr: int, n: int, sz: int, params: CurveletParams, stats: bool = True) -> dict | None:
+        """With stats=False the call returns once the work is queued (no wait
+        for the curvelet stage); results are complete at their first query."""
         st = _RunStats()
         p = params._c()
-        self.check(self._lib.cvg_run_device(self._h, n, sz, C.c_void_p(dptr), C.byref(p), C.byref(st)),
-                   "run_device")
-        return {k: getattr(st, k) for k, _ in _RunStats._fields_}
+        self.check(self._lib.cvg_run_device(self._h, n, sz, C.c_void_p(dptr), C.byref(p),
+                                            C.byref(st) if stats else None), "run_device")
+        return {k: getattr(st, k) for k, _ in _RunStats._fields_} if stats else None
+
+    def recent_fused_ms(self, count: int) -> list[float]:
+        """Durations (ms, HIP events) of the fused kernel in the last runs, oldest first (at most 64)."""
+        buf = (C.c_float * max(count, 0))()
+        got = C.c_int()
+        self.check(self._lib.cvg_recent_fused_ms(self._h, count, buf, C.byref(got)), "recent_fused_ms")
+        return list(buf)[:got.value]
 
     def synchronize(self) -> None:
         self.check(self._lib.cvg_synchronize(self._h), "synchronize")

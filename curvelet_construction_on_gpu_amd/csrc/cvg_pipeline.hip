@@ -80,6 +80,10 @@ struct cvg_context {
     hipStream_t aux[3] = {nullptr, nullptr, nullptr};
     hipEvent_t ev_join[3] = {nullptr, nullptr, nullptr};
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    // the fused kernel's start and end events of the last kFusedRing runs
+    static constexpr int kFusedRing = 64;
+    hipEvent_t fused_ev[kFusedRing][2] = {};
+    long long fused_runs = 0;
     std::string err;
     size_t lds_limit = 65536;
     // fused-kernel launch shape: buckets whose key is at least coop_min_cap
@@ -2358,7 +2362,9 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     // stream, the long-list buckets round-robin over the auxiliary streams,
     // longest lists first, so their long-latency work starts at once and
     // overlaps the bulk
+    const int ring = static_cast<int>(ctx->fused_runs % cvg_context::kFusedRing);
     CVG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+    CVG_HIP(ctx, hipEventRecord(ctx->fused_ev[ring][0], st));
     for (int i = 0; i < 3; ++i) CVG_HIP(ctx, hipStreamWaitEvent(ctx->aux[i], ctx->ev[3], 0));
     bool aux_used[3] = {false, false, false};
     int rr = 0;
@@ -2381,6 +2387,8 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
         CVG_HIP(ctx, hipStreamWaitEvent(st, ctx->ev_join[i], 0));
     }
     CVG_HIP(ctx, hipEventRecord(ctx->ev[4], st));
+    CVG_HIP(ctx, hipEventRecord(ctx->fused_ev[ring][1], st));
+    ++ctx->fused_runs;
     // every anchor is in exactly one bucket, so every acc_count entry is written
 
     CVG_TRY(exclusive_scan(ctx, as<unsigned>(ctx->d_acc), n, as<unsigned>(ctx->d_rowoff), ctx->d_tiles));
@@ -2454,9 +2462,12 @@ int run_common(cvg_context *ctx, int n, int sz, const double *d_edges, const cvg
     CVG_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     CVG_TRY(build_curvelets(ctx, *p));
     CVG_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+    // without stats the call returns with the work queued: the counts come
+    // back with the stream and are read at the first query of a result
+    if (!stats) return CVG_OK;
     CVG_HIP(ctx, hipEventSynchronize(ctx->ev[2]));
     CVG_TRY(finish_curvelets(ctx));
-    if (stats) {
+    {
         float t01 = 0, t12 = 0, t02 = 0;
         (void)hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]);
         (void)hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]);
@@ -2671,6 +2682,9 @@ int cvg_create(int device, cvg_context **out_ctx)
         if (rc != CVG_OK) break;
         for (auto &e : ctx->ev)
             if (hipEventCreate(&e) != hipSuccess) rc = CVG_ERR_DEVICE;
+        for (auto &pr : ctx->fused_ev)
+            for (auto &e : pr)
+                if (rc == CVG_OK && hipEventCreate(&e) != hipSuccess) rc = CVG_ERR_DEVICE;
         if (rc != CVG_OK) break;
         if (hipHostMalloc(reinterpret_cast<void **>(&ctx->mailbox), 64 * sizeof(unsigned), hipHostMallocDefault) !=
             hipSuccess) {
@@ -2719,6 +2733,9 @@ void cvg_destroy(cvg_context *ctx)
     for (DevBuf *b : bufs) release(*b);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &pr : ctx->fused_ev)
+        for (auto &e : pr)
+            if (e) (void)hipEventDestroy(e);
     if (ctx->mailbox) (void)hipHostFree(ctx->mailbox);
     if (ctx->h_maprow) (void)hipHostFree(ctx->h_maprow);
     for (int b = 0; b < 2; ++b) {
@@ -3029,6 +3046,23 @@ int cvg_get_stream(cvg_context *ctx, void **hip_stream)
 {
     if (!ctx || !hip_stream) return CVG_ERR_INVALID_ARGUMENT;
     *hip_stream = ctx->stream;
+    return CVG_OK;
+}
+
+int cvg_recent_fused_ms(cvg_context *ctx, int max, float *ms, int *count)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!ms || !count || max < 0) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "recent_fused_ms: bad arguments");
+    CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const long long have = std::min<long long>(ctx->fused_runs, cvg_context::kFusedRing);
+    const int k = static_cast<int>(std::min<long long>(have, max));
+    for (int i = 0; i < k; ++i) {
+        const int r = static_cast<int>((ctx->fused_runs - k + i) % cvg_context::kFusedRing);
+        float t = 0.0f;
+        CVG_HIP(ctx, hipEventElapsedTime(&t, ctx->fused_ev[r][0], ctx->fused_ev[r][1]));
+        ms[i] = t;
+    }
+    *count = k;
     return CVG_OK;
 }
 

@@ -146,7 +146,12 @@ int cvg_get_curvelets(cvg_context *ctx, unsigned *chains, double *info);
 
 /* The timed span of run_curvelet<double> (main.cpp:43-66): edges in host
  * memory -> graph -> bundles -> chains and info on the device.  Fetch the
- * results with cvg_get_curvelets. */
+ * results with cvg_get_curvelets.  With stats == NULL this and the other
+ * run calls return once the work is queued (after the graph stage's two host
+ * reads, so the host edge arrays are no longer needed); the results and
+ * counts are complete at their first query (cvg_curvelet_shape,
+ * cvg_get_curvelets, cvg_device_curvelets, ...).  With stats the call waits
+ * and fills them. */
 int cvg_run(cvg_context *ctx, int num_edges, int sz_edge_data, const double *edges, const cvg_params *params,
             cvg_run_stats *stats);
 /* Same, with the edge array already in device memory (for instance from
@@ -166,6 +171,12 @@ void cvg_host_free(void *ptr);
 /* The context's HIP stream (hipStream_t), for callers that order their own work after it. */
 int cvg_get_stream(cvg_context *ctx, void **hip_stream);
 int cvg_synchronize(cvg_context *ctx);
+/* Durations in ms (HIP events on the context stream around the fused kernel's
+ * launches, k_curvelets) of the most recent runs, oldest first: up to
+ * min(max, 64) values, *count = how many.  Waits for the stream.  No
+ * counterpart in the reference: it times the device work of runs made
+ * without stats. */
+int cvg_recent_fused_ms(cvg_context *ctx, int max, float *ms, int *count);
 
 /* ---- batched form ---------------------------------------------------------
  * Several maps in one pass: every kernel of the path runs once over all of

@@ -248,3 +248,24 @@ def test_pageable_downloads_match_page_locked(ctx):
     finally:
         for p in blocks:
             lib.cvg_host_free(p)
+
+
+def test_runs_without_stats_return_queued_and_match(ctx):
+    """A run without stats returns with its work queued; back-to-back runs
+    (a different map in between, never queried) and the first query after
+    them must give the results and counts of a run that waits."""
+    a = synthetic("C1", 1001)
+    b = synthetic("C2", 1002)
+    p = CurveletParams(group_max_sz=7)
+    da = ctx.upload(np.concatenate([a, b]))  # one device array: the upload buffer is reused per call
+    db = da + a.size * 8
+    st = ctx.run_device(da, len(a), a.shape[1], p)
+    ca, ia = ctx.curvelets()
+    for _ in range(3):
+        assert ctx.run_device(db, len(b), b.shape[1], p, stats=False) is None
+        assert ctx.run_device(da, len(a), a.shape[1], p, stats=False) is None
+    ca2, ia2 = ctx.curvelets()
+    assert ca2.shape[0] == st["num_curvelets"]
+    assert np.array_equal(ca, ca2) and np.array_equal(ia.view(np.uint64), ia2.view(np.uint64))
+    ms = ctx.recent_fused_ms(6)
+    assert len(ms) == 6 and all(m > 0 for m in ms)
