@@ -631,7 +631,9 @@ def synth_edges(preset: str | _SynthConfig, seed: int, **overrides) -> np.ndarra
     return out
 
 
-MATH_OPS = {"sin": 0, "cos": 1, "atan2": 2, "sqrt": 3, "to2pi": 4, "fmod": 5, "sincos_sin": 6, "sincos_cos": 7}
+MATH_OPS = {"sin": 0, "cos": 1, "atan2": 2, "sqrt": 3, "to2pi": 4, "fmod": 5, "sincos_sin": 6, "sincos_cos": 7,
+            "sin_difference": 8, "sin_difference_settled": 9, "sincos_dd_sin_hi": 10, "sincos_dd_sin_lo": 11,
+            "sincos_dd_cos_hi": 12, "sincos_dd_cos_lo": 13}
 
 
 def math_eval(op: str, x: np.ndarray, y: np.ndarray | None = None, ctx: Context | None = None,

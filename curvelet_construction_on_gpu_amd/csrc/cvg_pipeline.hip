@@ -133,7 +133,7 @@ struct cvg_context {
     unsigned long long intersections = 0;
     unsigned num_curvelets = 0;
     unsigned width = 0;
-    DevBuf d_trig, d_cls, d_prec, d_psin, d_lists, d_tables, d_st_ids, d_st_meta, d_st_k, d_st_rank, d_acc, d_rowoff, d_gbundle,
+    DevBuf d_trig, d_tdd, d_cls, d_prec, d_psin, d_lists, d_tables, d_st_ids, d_st_meta, d_st_k, d_st_rank, d_acc, d_rowoff, d_gbundle,
         d_chains, d_info;
 
     // diagnostics
@@ -933,26 +933,41 @@ __global__ __launch_bounds__(kBlock) void k_tie_sort(const int *__restrict__ tie
 // Curvelet stage
 // ---------------------------------------------------------------------------
 
+// The rows of trig (cvg_device.hpp), and in tdd the same values as
+// double-doubles for the per-pair sines: tdd[8i .. 8i+7] = sin, cos of
+// theta_i + dt, then of theta_i - dt (hi, lo each); from tdd + 8n, edge i's
+// grid angles, 4 per j: sin hi, sin lo, cos hi, cos lo of theta_i + off_t[j].
+// One thread per (edge i, angle k), k fastest: k < Kt the grid angle j = k,
+// k = Kt theta + dt, Kt + 1 theta - dt, Kt + 2 theta itself.
 __global__ __launch_bounds__(kBlock) void k_edge_trig(int n, const double *__restrict__ th, double dt, int kt,
-                                                      const double *__restrict__ off_t, double *__restrict__ trig)
+                                                      const double *__restrict__ off_t, double *__restrict__ trig,
+                                                      double *__restrict__ tdd)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const unsigned na = static_cast<unsigned>(kt) + 3u;
+    const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= static_cast<unsigned>(n) * na) return;
+    const int i = static_cast<int>(g / na);
+    const int k = static_cast<int>(g - static_cast<unsigned>(i) * na);
     const double t = th[i];
-    double s, c;
-    cvg::cr_sincos(t, c_sincos_tab, &s, &c);
-    trig[cvg::TRIG_COS * n + i] = c;
-    trig[cvg::TRIG_SIN * n + i] = s;
-    cvg::cr_sincos(t + dt, c_sincos_tab, &s, &c);  // T0_p_dt, cpu_curvelet.hpp:655
-    trig[cvg::TRIG_SIN_P * n + i] = s;
-    trig[cvg::TRIG_COS_P * n + i] = c;
-    cvg::cr_sincos(t - dt, c_sincos_tab, &s, &c);  // T0_m_dt, :656
-    trig[cvg::TRIG_SIN_M * n + i] = s;
-    trig[cvg::TRIG_COS_M * n + i] = c;
-    for (int j = 0; j < kt; ++j) {
-        cvg::cr_sincos(t + off_t[j], c_sincos_tab, &s, &c);  // T2t = T2 + delta_t2, :702
-        trig[static_cast<long long>(cvg::TRIG_GRID) * n + (static_cast<long long>(i) * kt + j) * 2] = s;
-        trig[static_cast<long long>(cvg::TRIG_GRID) * n + (static_cast<long long>(i) * kt + j) * 2 + 1] = c;
+    const double ang = k < kt ? t + off_t[k]      // T2t = T2 + delta_t2, cpu_curvelet.hpp:702
+                       : k == kt ? t + dt         // T0_p_dt, :655
+                       : k == kt + 1 ? t - dt     // T0_m_dt, :656
+                                     : t;
+    cvg::dd sd, cd;
+    cvg::cr_sincos_dd(ang, c_sincos_tab, &sd, &cd);
+    if (k < kt) {
+        const long long e = static_cast<long long>(i) * kt + k;
+        reinterpret_cast<double2 *>(trig + static_cast<long long>(cvg::TRIG_GRID) * n)[e] = make_double2(sd.hi, cd.hi);
+        reinterpret_cast<double4 *>(tdd + static_cast<long long>(n) * 8)[e] = make_double4(sd.hi, sd.lo, cd.hi, cd.lo);
+    } else if (k < kt + 2) {
+        const bool plus = k == kt;
+        trig[(plus ? cvg::TRIG_SIN_P : cvg::TRIG_SIN_M) * n + i] = sd.hi;
+        trig[(plus ? cvg::TRIG_COS_P : cvg::TRIG_COS_M) * n + i] = cd.hi;
+        reinterpret_cast<double4 *>(tdd)[2 * static_cast<long long>(i) + (plus ? 0 : 1)] =
+            make_double4(sd.hi, sd.lo, cd.hi, cd.lo);
+    } else {
+        trig[cvg::TRIG_COS * n + i] = cd.hi;
+        trig[cvg::TRIG_SIN * n + i] = sd.hi;
     }
 }
 
@@ -1048,36 +1063,75 @@ __global__ __launch_bounds__(kBucketBlock) void k_bucket(int n, const int *__res
 // Per-pair sines of the transport (compute_curve_bundle, cpu_curvelet.hpp:728-729):
 // psin[q*2Kt + 2j] = sin(T2t_j - (theta_n - dt)), psin[q*2Kt + 2j + 1] =
 // sin(T2t_j - (theta_n + dt)) with T2t_j = theta_a + delta_t2[j], for every
-// pair that is not "close" (close pairs keep the default bundle and never read
-// them).  One thread per pair (the pair's loads once, 2 Kt independent sines);
-// the sin/cos table lives in LDS.
-__global__ __launch_bounds__(kBlock) void k_pair_trig(int pairs, int kt, const int *__restrict__ pair_anchor,
-                                                      const int *__restrict__ ids, const double *__restrict__ x,
-                                                      const double *__restrict__ y, const double *__restrict__ th,
-                                                      const double *__restrict__ off_t, double dt, double close_thr,
-                                                      double *__restrict__ psin)
+// pair that is not "close" (pair flag bit 2; close pairs keep the default
+// bundle and never read them, and get zeros).  One thread per pair; KT > 0
+// fixes Kt at compile time so that every load is issued before the arithmetic
+// (KT = 0: any Kt up to kPairTrigMaxKt).  Each sine is the correctly rounded
+// sine of the rounded difference, formed from the double-double sin/cos of
+// both angles in tdd by the subtraction formula; the rare lane whose rounding
+// that does not settle evaluates the sine directly.  For Kt up to
+// kPairTrigMaxKt the block's results go through LDS (dynamic, kBlock x Kt
+// double2) so that its output span is written contiguously.
+constexpr int kPairTrigMaxKt = 16;
+template <int KT>
+__global__ __launch_bounds__(kBlock) void k_pair_trig(int pairs, int n, int kt_rt, const int *__restrict__ pair_anchor,
+                                                      const int *__restrict__ ids,
+                                                      const unsigned char *__restrict__ pflag,
+                                                      const double *__restrict__ th,
+                                                      const double *__restrict__ off_t, double dt,
+                                                      const double *__restrict__ tdd, double *__restrict__ psin)
 {
-    __shared__ double tab[4 * CVG_SIN_ROWS];
-    for (int i = threadIdx.x; i < 4 * CVG_SIN_ROWS; i += blockDim.x) tab[i] = c_sincos_tab[i];
-    __syncthreads();
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= pairs) return;
-    const int a = pair_anchor[q];
-    const int b = ids[q];
-    const double DX = x[a] - x[b];
-    const double DY = y[a] - y[b];
-    if (DX * DX + DY * DY < close_thr) return;  // :682
-    const double ta = th[a];
-    const double nt = th[b];
-    const double tm = nt - dt, tp = nt + dt;
-    double2 *out = reinterpret_cast<double2 *>(psin) + static_cast<long long>(q) * kt;
-    for (int j = 0; j < kt; ++j) {
-        const double T2t = ta + off_t[j];
-        double2 v;
-        v.x = cvg::cr_sin(T2t - tm, tab);
-        v.y = cvg::cr_sin(T2t - tp, tab);
-        out[j] = v;
+    extern __shared__ double2 stage[];
+    const int kt = KT > 0 ? KT : kt_rt;
+    const bool staged = kt <= kPairTrigMaxKt;
+    const int q0 = blockIdx.x * blockDim.x;
+    const int q = q0 + threadIdx.x;
+    double2 *out = reinterpret_cast<double2 *>(psin) + static_cast<long long>(q0) * kt;
+    double2 *row = (staged ? stage : out) + threadIdx.x * kt;
+    if (q < pairs) {
+        if (pflag[q] & 4) {  // close, :682
+            for (int j = 0; j < kt; ++j) row[j] = make_double2(0.0, 0.0);
+        } else {
+            const int a = pair_anchor[q];
+            const int b = ids[q];
+            const double ta = th[a];
+            const double nt = th[b];
+            const double tm = nt - dt, tp = nt + dt;
+            const double4 *nr = reinterpret_cast<const double4 *>(tdd) + 2 * static_cast<long long>(b);
+            const double4 p = nr[0], m = nr[1];
+            const cvg::dd sp = {p.x, p.y}, cp = {p.z, p.w}, sm = {m.x, m.y}, cm = {m.z, m.w};
+            const double4 *gr =
+                reinterpret_cast<const double4 *>(tdd + static_cast<long long>(n) * 8) + static_cast<long long>(a) * kt;
+            if (KT > 0) {
+                double4 g[KT > 0 ? KT : 1];
+#pragma unroll
+                for (int j = 0; j < (KT > 0 ? KT : 1); ++j) g[j] = gr[j];
+#pragma unroll
+                for (int j = 0; j < (KT > 0 ? KT : 1); ++j) {
+                    const double T2t = ta + off_t[j];
+                    const cvg::dd st = {g[j].x, g[j].y}, ct = {g[j].z, g[j].w};
+                    double2 v;
+                    v.x = cvg::sin_difference(T2t, tm, st, ct, sm, cm, c_sincos_tab);
+                    v.y = cvg::sin_difference(T2t, tp, st, ct, sp, cp, c_sincos_tab);
+                    row[j] = v;
+                }
+            } else {
+                for (int j = 0; j < kt; ++j) {
+                    const double T2t = ta + off_t[j];
+                    const double4 g = gr[j];
+                    const cvg::dd st = {g.x, g.y}, ct = {g.z, g.w};
+                    double2 v;
+                    v.x = cvg::sin_difference(T2t, tm, st, ct, sm, cm, c_sincos_tab);
+                    v.y = cvg::sin_difference(T2t, tp, st, ct, sp, cp, c_sincos_tab);
+                    row[j] = v;
+                }
+            }
+        }
     }
+    if (!staged) return;
+    __syncthreads();
+    const int cnt = min(static_cast<int>(blockDim.x), pairs - q0) * kt;
+    for (int e = threadIdx.x; e < cnt; e += blockDim.x) out[e] = stage[e];
 }
 
 // Host-computed per-parameter tables, laid out in one device block.  Lanes of
@@ -1853,6 +1907,23 @@ __host__ __device__ inline double math_op(int op, double x, double y, const doub
         cvg::cr_sincos(x, sc, &sv, &cv);
         return op == 6 ? sv : cv;
     }
+    case 8:
+    case 9: {
+        cvg::dd sa, ca, sb, cb;
+        cvg::cr_sincos_dd(x, sc, &sa, &ca);
+        cvg::cr_sincos_dd(y, sc, &sb, &cb);
+        if (op == 8) return cvg::sin_difference(x, y, sa, ca, sb, cb, sc);
+        double v;
+        return cvg::sin_of_difference(x, y, sa, ca, sb, cb, &v) ? 1.0 : 0.0;
+    }
+    case 10:
+    case 11:
+    case 12:
+    case 13: {
+        cvg::dd sv, cv;
+        cvg::cr_sincos_dd(x, sc, &sv, &cv);
+        return op == 10 ? sv.hi : op == 11 ? sv.lo : op == 12 ? cv.hi : cv.lo;
+    }
     default: return 0.0;
     }
 }
@@ -2234,23 +2305,30 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     // per-edge trig, per-pair classes and sines, buckets
     const int ntrig = cvg::TRIG_GRID + 2 * static_cast<int>(d.kt);
     CVG_TRY(ensure(ctx, ctx->d_trig, static_cast<size_t>(ntrig) * n * sizeof(double)));
+    CVG_TRY(ensure(ctx, ctx->d_tdd, static_cast<size_t>(8 + 4 * d.kt) * n * sizeof(double)));
     CVG_TRY(ensure(ctx, ctx->d_cls, static_cast<size_t>(pairs) + 1));
     CVG_TRY(ensure(ctx, ctx->d_prec, static_cast<size_t>(std::max(pairs, 1)) * 8 * sizeof(double)));
     CVG_TRY(ensure(ctx, ctx->d_psin, static_cast<size_t>(std::max(pairs, 1)) * d.kt * 2 * sizeof(double)));
     CVG_TRY(ensure(ctx, ctx->d_lists, static_cast<size_t>(nb) * n * sizeof(int4)));
     unsigned *scal = as<unsigned>(ctx->d_scal);
-    hipLaunchKernelGGL(k_edge_trig, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<double>(ctx->d_th), d.dt,
-                       static_cast<int>(d.kt), T.off_t, as<double>(ctx->d_trig));
+    if (static_cast<unsigned long long>(n) * (d.kt + 3) > 0xffffffffull ||
+        static_cast<unsigned long long>(pairs) * d.kt > 0xffffffffull)
+        return fail(ctx, CVG_ERR_UNSUPPORTED, "more than 2^32 (edge, angle) items in one run");
+    hipLaunchKernelGGL(k_edge_trig, dim3(blocks_for(static_cast<long long>(n) * (d.kt + 3))), dim3(kBlock), 0, st, n,
+                       as<double>(ctx->d_th), d.dt,
+                       static_cast<int>(d.kt), T.off_t, as<double>(ctx->d_trig), as<double>(ctx->d_tdd));
     CVG_HIP(ctx, hipMemsetAsync(scal + S_VALID, 0, (kScalarBufWords - S_VALID) * sizeof(unsigned), st));
     if (pairs > 0) {
         hipLaunchKernelGGL(k_pair_prep, dim3(blocks_for(pairs)), dim3(kBlock), 0, st, pairs, n,
                            as<int>(ctx->d_pair_anchor), as<int>(ctx->d_ids), as<double>(ctx->d_x),
                            as<double>(ctx->d_y), as<double>(ctx->d_trig), d.close_thr,
                            as<unsigned char>(ctx->d_cls), as<double>(ctx->d_prec));
-        hipLaunchKernelGGL(k_pair_trig, dim3(blocks_for(pairs)), dim3(kBlock), 0, st,
-                           pairs, static_cast<int>(d.kt), as<int>(ctx->d_pair_anchor), as<int>(ctx->d_ids),
-                           as<double>(ctx->d_x), as<double>(ctx->d_y), as<double>(ctx->d_th), T.off_t, d.dt,
-                           d.close_thr, as<double>(ctx->d_psin));
+        const size_t trig_lds = d.kt <= static_cast<unsigned>(kPairTrigMaxKt) ? kBlock * d.kt * sizeof(double2) : 0;
+        hipLaunchKernelGGL(d.kt == 7 ? k_pair_trig<7> : k_pair_trig<0>, dim3(blocks_for(pairs)), dim3(kBlock),
+                           trig_lds, st,
+                           pairs, n, static_cast<int>(d.kt), as<int>(ctx->d_pair_anchor), as<int>(ctx->d_ids),
+                           as<unsigned char>(ctx->d_cls), as<double>(ctx->d_th), T.off_t, d.dt,
+                           as<double>(ctx->d_tdd), as<double>(ctx->d_psin));
     }
     hipLaunchKernelGGL(k_bucket, dim3(blocks_for(n, kBucketBlock)), dim3(kBucketBlock), 0, st, n, as<int>(ctx->d_sorted),
                        as<unsigned char>(ctx->d_cls), as<unsigned>(ctx->d_off), nb, T.caps, as<int4>(ctx->d_lists),
@@ -2726,7 +2804,7 @@ void cvg_destroy(cvg_context *ctx)
                       &ctx->d_cy,       &ctx->d_scal,  &ctx->d_grid,    &ctx->d_tmp_ids, &ctx->d_sorted,
                       &ctx->d_sxy,      &ctx->d_scell,   &ctx->d_part,    &ctx->d_deg,     &ctx->d_off,     &ctx->d_tiles,
                       &ctx->d_cand,     &ctx->d_ids,   &ctx->d_d2,      &ctx->d_pair_anchor, &ctx->d_trig,
-                      &ctx->d_cls,      &ctx->d_prec,  &ctx->d_psin,  &ctx->d_lists,   &ctx->d_tables,  &ctx->d_st_ids,
+                      &ctx->d_tdd,      &ctx->d_cls,      &ctx->d_prec,  &ctx->d_psin,  &ctx->d_lists,   &ctx->d_tables,  &ctx->d_st_ids,
                       &ctx->d_st_meta,  &ctx->d_st_k,  &ctx->d_st_rank, &ctx->d_acc,     &ctx->d_rowoff,  &ctx->d_gbundle,
                       &ctx->d_chains,   &ctx->d_info,  &ctx->d_dbg,     &ctx->d_map,     &ctx->d_eoff,
                       &ctx->d_mapbox,   &ctx->d_maprow};
@@ -3127,7 +3205,7 @@ int cvg_debug_anchor_bundles(cvg_context *ctx, const cvg_params *params, int anc
 
 int cvg_math_eval(cvg_context *ctx, int op, int n, const double *x, const double *y, double *out)
 {
-    if (!x || !out || n < 0 || op < 0 || op > 7) return CVG_ERR_INVALID_ARGUMENT;
+    if (!x || !out || n < 0 || op < 0 || op > 13) return CVG_ERR_INVALID_ARGUMENT;
     if ((op == 2 || op == 5) && !y) return CVG_ERR_INVALID_ARGUMENT;
     if (!ctx) {
         for (int i = 0; i < n; ++i)

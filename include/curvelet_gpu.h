@@ -314,7 +314,11 @@ int cvg_debug_anchor_bundles(cvg_context *ctx, const cvg_params *params, int anc
                              unsigned char *hyp, double *bmin, double *bmax, int *deg, int *cells);
 /* Evaluate the library's correctly rounded math on n arguments.
  * op: 0 sin(x), 1 cos(x), 2 atan2(y, x), 3 sqrt(x), 4 To2Pi(x), 5 fmod(x, y),
- * 6 and 7 the sine and cosine of the joint sincos(x).
+ * 6 and 7 the sine and cosine of the joint sincos(x), 8 sin(x - y) as the
+ * per-pair transport sines compute it (subtraction formula on double-double
+ * sin/cos of x and y, else the direct evaluation), 9 whether that formula
+ * settled the rounding (1.0) or not (0.0), 10-13 the double-double sin(x)
+ * high and low parts and cos(x) high and low parts.
  * ctx == NULL evaluates the host build of the same code; otherwise the device. */
 int cvg_math_eval(cvg_context *ctx, int op, int n, const double *x, const double *y, double *out);
 

@@ -43,10 +43,18 @@ def test_device_math_matches_host_build(ctx):
                         np.ldexp(rng.uniform(-1, 1, 2000), -rng.integers(0, 60, 2000)),
                         [0.0, -0.0, 1.5707963267948966, 3.141592653589793, 6.283185307179586]])
     y = rng.uniform(-8, 8, x.size)
-    for op in ("sin", "cos", "atan2", "to2pi"):
+    for op in ("sin", "cos", "atan2", "to2pi", "sincos_dd_sin_hi", "sincos_dd_sin_lo", "sincos_dd_cos_hi",
+               "sincos_dd_cos_lo"):
         d = math_eval(op, x, y, ctx=ctx)
         h = math_eval(op, x, y, device=False)
         assert np.array_equal(bits(d), bits(h)), op
+    # the transport's sine of a difference, on operands of its range
+    a = rng.uniform(-0.6, 3.8, 200000)
+    b = rng.uniform(-0.2, 3.4, 200000)
+    a[:4000] = b[:4000] + rng.uniform(-1e-6, 1e-6, 4000)
+    a[4000:4050] = b[4000:4050]
+    for op in ("sin_difference", "sin_difference_settled"):
+        assert np.array_equal(bits(math_eval(op, a, b, ctx=ctx)), bits(math_eval(op, a, b, device=False))), op
 
 
 def test_device_sqrt_fmod_ieee(ctx):

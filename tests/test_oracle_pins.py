@@ -18,7 +18,9 @@ term, a flipped sign, a swapped operand, an off-by-one window) fails it:
   before the scan reaches the seed's own rank never lists it;
 * the library's math (the same code the GPU runs) against high-precision
   decimal evaluation: sin, cos and atan2 correctly rounded, sqrt and fmod
-  exact.
+  exact; the double-double sin/cos within 2^-86 of the true values, and the
+  transport's sine of a difference (subtraction formula with a rounding test)
+  correctly rounded and equal to the direct evaluation.
 """
 import math
 from decimal import Decimal, getcontext
@@ -319,3 +321,52 @@ def test_host_sin_cos_agree_with_joint_sincos():
                           math_eval("sincos_sin", x, device=False).view(np.uint64))
     assert np.array_equal(math_eval("cos", x, device=False).view(np.uint64),
                           math_eval("sincos_cos", x, device=False).view(np.uint64))
+
+
+def test_host_sincos_dd_hi_bits_and_accuracy():
+    """The double-double sin/cos carry cr_sincos's bits in the high part and
+    are within 2^-86 of the true values (the bound the subtraction formula's
+    rounding test assumes)."""
+    rng = np.random.default_rng(9)
+    x = np.concatenate([rng.uniform(-1.0, 4.5, 200000), rng.uniform(-1e-6, 1e-6, 2000),
+                        np.array([0.0, -0.0, 1e-9, math.pi / 2, math.pi, 2.0 ** -26.5])])
+    assert np.array_equal(math_eval("sincos_dd_sin_hi", x, device=False).view(np.uint64),
+                          math_eval("sincos_sin", x, device=False).view(np.uint64))
+    assert np.array_equal(math_eval("sincos_dd_cos_hi", x, device=False).view(np.uint64),
+                          math_eval("sincos_cos", x, device=False).view(np.uint64))
+    xs = np.concatenate([x[:1500], x[-2006:]])
+    sh, sl = math_eval("sincos_dd_sin_hi", xs, device=False), math_eval("sincos_dd_sin_lo", xs, device=False)
+    ch, cl = math_eval("sincos_dd_cos_hi", xs, device=False), math_eval("sincos_dd_cos_lo", xs, device=False)
+    bound = Decimal(2) ** -86
+    for i, xi in enumerate(xs):
+        X = Decimal(float(xi))
+        assert abs(Decimal(float(sh[i])) + Decimal(float(sl[i])) - d_sin(X)) <= bound, xi
+        assert abs(Decimal(float(ch[i])) + Decimal(float(cl[i])) - d_cos(X)) <= bound, xi
+
+
+def transport_angle_pairs(rng, n):
+    """(T2t, theta -+ dt)-like operands: both in the orientation range plus
+    grid offsets, with near-equal and equal pairs."""
+    a = rng.uniform(-0.6, 3.8, n)
+    b = rng.uniform(-0.2, 3.4, n)
+    k = n // 50
+    a[:k] = b[:k] + rng.uniform(-1e-6, 1e-6, k)
+    a[k:k + 50] = b[k:k + 50]
+    a[k + 50:k + 100] = b[k + 50:k + 100] + np.ldexp(1.0, -rng.integers(20, 60, 50))
+    return a, b
+
+
+def test_host_sin_difference_correctly_rounded():
+    rng = np.random.default_rng(10)
+    a, b = transport_angle_pairs(rng, 400000)
+    got = math_eval("sin_difference", a, b, device=False)
+    settled = math_eval("sin_difference_settled", a, b, device=False)
+    assert settled.mean() > 0.99
+    # against decimal evaluation: settled and unsettled lanes, and the edge cases
+    idx = np.concatenate([rng.choice(np.nonzero(settled == 1)[0], 1500, replace=False),
+                          np.nonzero(settled == 0)[0][:300], np.arange(8000, 8100)])
+    for i in idx:
+        d = float(a[i] - b[i])
+        assert got[i] == float(d_sin(Decimal(d))), (a[i], b[i])
+    # the direct evaluation of the same argument agrees everywhere
+    assert np.array_equal(got.view(np.uint64), math_eval("sin", a - b, device=False).view(np.uint64))
