@@ -401,7 +401,9 @@ constexpr int kScalarWords = S_COUNT + 2;  // the words read back to the host
 constexpr int kMaxBuckets = 14;
 constexpr int kBucketStride = 32;  // words
 constexpr int kBucketBase = 32;    // bucket b's list length at [kBucketBase + b * kBucketStride]
-constexpr int kScalarBufWords = kBucketBase + kMaxBuckets * kBucketStride;
+// heavy-bucket anchor streams that have started (own 128-byte line)
+constexpr int kStartedWord = kBucketBase + kMaxBuckets * kBucketStride;
+constexpr int kScalarBufWords = kStartedWord + 32;
 
 __global__ void k_init_scalars(unsigned *s)
 {
@@ -1009,6 +1011,8 @@ __global__ __launch_bounds__(kBlock) void k_pair_prep(int pairs, int n, const in
 // almost always empty), which size the fused kernel's LDS slots and staging.
 // Each list entry carries (anchor, CSR base, degree).
 constexpr int kBucketBlock = 1024;  // few workgroups: one list-length atomic each per bucket
+constexpr int kHeavyCap = 48;       // buckets from this cap start ahead of the light bucket 0
+constexpr int kGateMaxNaps = 40;    // the start gate's bound: 40 sleeps of 127 x 64 clocks
 
 __global__ __launch_bounds__(kBucketBlock) void k_bucket(int n, const int *__restrict__ sorted,
                                                    const unsigned char *__restrict__ pflag,
@@ -1058,6 +1062,30 @@ __global__ __launch_bounds__(kBucketBlock) void k_bucket(int n, const int *__res
     if (b >= 0)
         lists[static_cast<long long>(b) * n + s_base[b] + my_off] =
             make_int4(a, static_cast<int>(o0), static_cast<int>(o1 - o0), 0);
+}
+
+// Start gate for the light buckets' first launch: the heavy buckets' few
+// anchors are the fused stage's critical path, and their workgroups need far
+// more LDS than light ones, so they are placed best before light workgroups
+// fill the CUs.  One wave waits (bounded: at most max_naps sleeps of ~3 us)
+// until every heavy-bucket stream with work has started (each counts itself
+// in scal[kStartedWord]); the light launch queued behind it then starts.
+struct GateArgs {
+    int nheavy;
+    int bucket[kMaxBuckets];
+    int streams[kMaxBuckets];
+};
+
+__global__ void k_gate(GateArgs G, const unsigned *__restrict__ scal, int max_naps)
+{
+    if (threadIdx.x != 0) return;
+    unsigned want = 0;
+    for (int i = 0; i < G.nheavy; ++i)
+        want += min(scal[kBucketBase + G.bucket[i] * kBucketStride], static_cast<unsigned>(G.streams[i]));
+    for (int k = 0; k < max_naps; ++k) {
+        if (__hip_atomic_load(scal + kStartedWord, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= want) return;
+        __builtin_amdgcn_s_sleep(127);
+    }
 }
 
 // Per-pair sines of the transport (compute_curve_bundle, cpu_curvelet.hpp:728-729):
@@ -1173,6 +1201,7 @@ struct FusedArgs {
     unsigned *valid_pairs;
     unsigned long long *intersections;
     double *g_bundle;  // global-bundle kernels: nb_cap*K*2 doubles per anchor stream
+    unsigned *started; // heavy buckets: each stream with work counts itself here at its start
 };
 
 // One anchor stream's LDS slice (byte offsets): the staging buffer (the
@@ -1320,6 +1349,7 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
     const int streams = static_cast<int>(gridDim.x) * apb;
     const int g = static_cast<int>(blockIdx.x) * apb + grp;
     const int len_list = static_cast<int>(uload(A.list_len));
+    if (A.started && g < len_list && lane == 0 && t == 0) atomicAdd(A.started, 1u);
     if (g >= len_list) return;  // COOP: the whole workgroup (same g)
 
     const int n = A.n;
@@ -2446,6 +2476,7 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     for (int i = 0; i < 3; ++i) CVG_HIP(ctx, hipStreamWaitEvent(ctx->aux[i], ctx->ev[3], 0));
     bool aux_used[3] = {false, false, false};
     int rr = 0;
+    GateArgs gate{};
     for (int b = nb - 1; b >= 0; --b) {
         const int slot = b == 0 ? 0 : 1 + (rr++ % 3);
         hipStream_t ls = slot == 0 ? st : ctx->aux[slot - 1];
@@ -2454,6 +2485,18 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
         A.list = as<int4>(ctx->d_lists) + static_cast<size_t>(b) * n;
         A.list_len = scal + kBucketBase + b * kBucketStride;
         A.nb_cap = caps[b];
+        const bool heavy = b > 0 && caps[b] >= kHeavyCap;  // on an auxiliary stream
+        A.started = heavy ? scal + kStartedWord : nullptr;
+        if (heavy) {
+            gate.bucket[gate.nheavy] = b;
+            gate.streams[gate.nheavy] = l.grid * l.apb;
+            ++gate.nheavy;
+        }
+        if (b == 0 && gate.nheavy > 0) {
+            hipLaunchKernelGGL(k_gate, dim3(1), dim3(64), 0, st, gate, static_cast<const unsigned *>(scal),
+                               kGateMaxNaps);
+            CVG_HIP(ctx, hipGetLastError());
+        }
         A.deg_cap = l.dcap;
         A.g_bundle = l.gb ? as<double>(ctx->d_gbundle) + gb_off[b] : nullptr;
         hipLaunchKernelGGL(l.fn, dim3(l.grid), dim3(64 * l.wpb), l.shmem, ls, A);
