@@ -1011,7 +1011,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_prep(int pairs, int n, const in
 // almost always empty), which size the fused kernel's LDS slots and staging.
 // Each list entry carries (anchor, CSR base, degree).
 constexpr int kBucketBlock = 1024;  // few workgroups: one list-length atomic each per bucket
-constexpr int kHeavyCap = 48;       // buckets from this cap start ahead of the light bucket 0
+constexpr int kHeavyCap = 32;       // buckets from this cap start ahead of the light bucket 0
 constexpr int kGateMaxNaps = 40;    // the start gate's bound: 40 sleeps of 127 x 64 clocks
 
 __global__ __launch_bounds__(kBucketBlock) void k_bucket(int n, const int *__restrict__ sorted,
