@@ -348,13 +348,15 @@ class Context:
         self._batch_keep = arrs
         return {k: getattr(st, k) for k, _ in _RunStats._fields_}
 
-    def run_batch_device(self, dptr: int, counts, sz: int, params: CurveletParams) -> dict:
+    def run_batch_device(self, dptr: int, counts, sz: int, params: CurveletParams,
+                         stats: bool = True) -> dict | None:
+        """stats=False: return once the work is queued (results complete at their first query)."""
         c = np.ascontiguousarray(counts, dtype=np.int32)
         st = _RunStats()
         p = params._c()
         self.check(self._lib.cvg_run_batch_device(self._h, len(c), _ptr(c, C.c_int), sz, C.c_void_p(dptr),
-                                                  C.byref(p), C.byref(st)), "run_batch_device")
-        return {k: getattr(st, k) for k, _ in _RunStats._fields_}
+                                                  C.byref(p), C.byref(st) if stats else None), "run_batch_device")
+        return {k: getattr(st, k) for k, _ in _RunStats._fields_} if stats else None
 
     @property
     def tuning(self) -> dict:
