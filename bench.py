@@ -371,6 +371,12 @@ def main(argv=None) -> int:
                      "kernel": "k_curvelets", "kernel_ms": fused_ms,
                      "flop_per_launch": flops},
     }
+    traffic = fused_traffic()
+    if traffic:
+        # the same kernel against the HBM roofline: rocprof bytes per step over its time
+        gbps = traffic / (fused_ms * 1e-3) / 1e9
+        line["roofline_hbm"] = {"achieved": gbps, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                "frac": gbps / PEAK_HBM_GBS, "traffic": traffic}
     if not args.no_cpu_baseline:
         cb = cpu_baseline(edges, params)
         if cb is not None:
