@@ -63,3 +63,26 @@ def test_view_and_release(ctx):
             assert np.array_equal(ch, a) and np.array_equal(bits(inf), bits(b))
         for i in ids:
             srv.release(i)
+
+
+def test_bursts_reuse_page_locked_blocks(ctx):
+    """Later bursts reuse the pool's page-locked blocks (size classes 1.25x
+    apart, holding earlier requests' data): every result must still be the
+    map's lone run."""
+    p = CurveletParams(group_max_sz=5)
+    maps = [synth_edges("C1", 1300 + s) for s in range(6)] + [
+        np.array(edges_for("synth:C2:1002")[: 2000 * (k + 1)]) for k in range(4)]
+    alone = []
+    for m in maps:
+        ctx.run(m, p)
+        alone.append(ctx.curvelets())
+    rng = np.random.default_rng(11)
+    with Server(p, workers=2, max_batch_maps=8) as srv:
+        for _ in range(3):
+            order = rng.permutation(len(maps))
+            ids = [srv.submit(maps[k]) for k in order]
+            for k, i in zip(order, ids):
+                ch, inf, r = srv.view(i)
+                assert r["status"] == 0
+                assert np.array_equal(ch, alone[k][0]) and np.array_equal(bits(inf), bits(alone[k][1]))
+                srv.release(i)
