@@ -2636,12 +2636,13 @@ struct Segment {
     size_t bytes;
 };
 
-bool page_locked(const void *p)
+// Memory the runtime knows (page-locked host, device or managed): one DMA.
+bool dma_direct(const void *p)
 {
     hipPointerAttribute_t a;
     const hipError_t e = hipPointerGetAttributes(&a, p);
     (void)hipGetLastError();  // an unregistered pointer reports an error: it must not stay pending
-    return e == hipSuccess && a.type == hipMemoryTypeHost;
+    return e == hipSuccess && a.type != hipMemoryTypeUnregistered;
 }
 
 int download(cvg_context *ctx, std::initializer_list<Segment> segs)
@@ -2655,8 +2656,8 @@ int download(cvg_context *ctx, std::initializer_list<Segment> segs)
     std::vector<Chunk> chunks;
     for (const Segment &g : segs) {
         if (g.bytes == 0 || !g.dst) continue;
-        if (g.bytes < kStageMinBytes || page_locked(g.dst)) {
-            CVG_HIP(ctx, hipMemcpyAsync(g.dst, g.src, g.bytes, hipMemcpyDeviceToHost, st));
+        if (g.bytes < kStageMinBytes || dma_direct(g.dst)) {
+            CVG_HIP(ctx, hipMemcpyAsync(g.dst, g.src, g.bytes, hipMemcpyDefault, st));
             continue;
         }
         for (size_t o = 0; o < g.bytes; o += kStageBytes)

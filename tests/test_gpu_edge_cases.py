@@ -269,3 +269,30 @@ def test_runs_without_stats_return_queued_and_match(ctx):
     assert np.array_equal(ca, ca2) and np.array_equal(ia.view(np.uint64), ia2.view(np.uint64))
     ms = ctx.recent_fused_ms(6)
     assert len(ms) == 6 and all(m > 0 for m in ms)
+
+
+def test_downloads_into_device_memory(ctx):
+    """A destination the runtime knows as device memory takes the direct copy
+    (no host staging): the rows arrive intact."""
+    import ctypes as C
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:
+        hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+    e = synthetic("C3", 1003)
+    ctx.run(e, CurveletParams(group_max_sz=7))
+    chains, info = ctx.curvelets()
+    dc, di = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(dc), C.c_size_t(chains.nbytes)) == 0
+    assert hip.hipMalloc(C.byref(di), C.c_size_t(info.nbytes)) == 0
+    try:
+        lib = ctx._lib
+        ctx.check(lib.cvg_get_curvelets(ctx.handle, C.cast(dc, C.POINTER(C.c_uint)),
+                                        C.cast(di, C.POINTER(C.c_double))), "get_curvelets")
+        hc, hi = np.empty_like(chains), np.empty_like(info)
+        assert hip.hipMemcpy(C.c_void_p(hc.ctypes.data), dc, C.c_size_t(chains.nbytes), 2) == 0  # device to host
+        assert hip.hipMemcpy(C.c_void_p(hi.ctypes.data), di, C.c_size_t(info.nbytes), 2) == 0
+        assert np.array_equal(hc, chains) and np.array_equal(hi.view(np.uint64), info.view(np.uint64))
+    finally:
+        hip.hipFree(dc)
+        hip.hipFree(di)
