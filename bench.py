@@ -127,12 +127,14 @@ def bench_batch(args) -> int:
         ctx.run_batch_device(dall, counts, 4, params, stats=False)
     ctx.synchronize()
     t_batch = (time.perf_counter() - t0) / reps
+    ctx.tuning = default_tuning  # the plain loop runs one map at a time: its own (automatic) launch shape
     t0 = time.perf_counter()
     for _ in range(reps):
         for m in range(len(maps)):
             ctx.run_device(dall + int(eoff[m]) * step, counts[m], 4, params, stats=False)
     ctx.synchronize()
     t_plain = (time.perf_counter() - t0) / reps
+    ctx.tuning = tuning
     st = ctx.run_batch_device(dall, counts, 4, params)  # counts for the report (untimed)
     # the batch's rows of a few maps against their plain-path rows
     same = True

@@ -75,7 +75,8 @@ class _ServerResult(C.Structure):
 
 
 class _Tuning(C.Structure):
-    _fields_ = [("coop_min_key", C.c_int), ("coop_waves", C.c_int), ("window_lanes", C.c_int)]
+    _fields_ = [("coop_min_key", C.c_int), ("coop_waves", C.c_int), ("window_lanes", C.c_int),
+                ("grid_pct", C.c_int)]
 
 
 class _TuningTrial(C.Structure):
@@ -360,14 +361,16 @@ class Context:
 
     @property
     def tuning(self) -> dict:
-        """Launch shape (results never depend on it): coop_min_key, coop_waves, window_lanes."""
+        """Launch shape (results never depend on it): coop_min_key, coop_waves, window_lanes, grid_pct."""
         t = _Tuning()
         self.check(self._lib.cvg_get_tuning(self._h, C.byref(t)), "get_tuning")
         return {k: getattr(t, k) for k, _ in _Tuning._fields_}
 
     @tuning.setter
     def tuning(self, value: dict) -> None:
-        t = _Tuning(**{k: int(value[k]) for k, _ in _Tuning._fields_})
+        cur = self.tuning
+        cur.update(value)  # fields not given keep their current values
+        t = _Tuning(**{k: int(cur[k]) for k, _ in _Tuning._fields_})
         self.check(self._lib.cvg_set_tuning(self._h, C.byref(t)), "set_tuning")
 
     def autotune_batch(self, dptr: int, counts, sz: int, params: CurveletParams, reps: int = 2):

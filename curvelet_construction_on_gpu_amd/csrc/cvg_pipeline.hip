@@ -95,6 +95,9 @@ struct cvg_context {
     // lanes per anchor in the neighbour-window scans (8, 16, 32 or 64;
     // cvg_set_tuning, or CVG_FILL_LANES at creation; results never depend on it)
     int fill_lanes = 16;
+    // grid of the light buckets' launches as a percentage of their resident
+    // capacity (25..100), or 0: automatic (cvg_set_tuning, or CVG_GRID_PCT)
+    int grid_pct = 0;
 
     unsigned *mailbox = nullptr;  // pinned host scalars read back after syncs
     // page-locked staging blocks for downloads into pageable caller memory
@@ -1013,6 +1016,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_prep(int pairs, int n, const in
 constexpr int kBucketBlock = 1024;  // few workgroups: one list-length atomic each per bucket
 constexpr int kHeavyCap = 32;       // buckets from this cap start ahead of the light bucket 0
 constexpr int kGateMaxNaps = 40;    // the start gate's bound: 40 sleeps of 127 x 64 clocks
+constexpr int kGridAutoEdges = 4 << 20;  // automatic light-bucket grids: 75% below this many edges, else 100%
 
 __global__ __launch_bounds__(kBucketBlock) void k_bucket(int n, const int *__restrict__ sorted,
                                                    const unsigned char *__restrict__ pflag,
@@ -2456,7 +2460,16 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
         // the bucket sizes stay on the device: every launch gets the resident
         // grid (streams beyond the bucket's length exit at once), at most one
         // stream per anchor of the map
-        const long long resident = static_cast<long long>(std::max(occ, 1)) * ctx->num_cu;
+        long long resident = static_cast<long long>(std::max(occ, 1)) * ctx->num_cu;
+        // Launches run four at a time: a light bucket's persistent grid sized
+        // for the whole GPU has workgroups that are dispatched late and still
+        // own a full share of the list, which stretches the stage's tail.  A
+        // smaller grid fits one map's lists better; batches, whose launches
+        // run longer, keep the full one.
+        if (caps[b] < kHeavyCap) {
+            const int pct = ctx->grid_pct > 0 ? ctx->grid_pct : (n < kGridAutoEdges ? 75 : 100);
+            resident = std::max(1LL, resident * pct / 100);
+        }
         const long long max_blocks = (static_cast<long long>(n) + l.apb - 1) / l.apb;
         l.grid = static_cast<int>(std::min(resident, max_blocks));
         if (l.gb) {
@@ -2830,6 +2843,10 @@ int cvg_create(int device, cvg_context **out_ctx)
             const int f = std::atoi(v);
             ctx->fill_lanes = (f == 8 || f == 16 || f == 32 || f == 64) ? f : 16;
         }
+        if (const char *v = std::getenv("CVG_GRID_PCT")) {
+            const int g = std::atoi(v);
+            ctx->grid_pct = (g >= 25 && g <= 100) ? g : 0;
+        }
     } while (false);
     if (rc != CVG_OK) {
         cvg_destroy(ctx);
@@ -3029,6 +3046,7 @@ int cvg_get_tuning(cvg_context *ctx, cvg_tuning *tuning)
     tuning->coop_min_key = ctx->coop_min_cap;
     tuning->coop_waves = ctx->coop_waves;
     tuning->window_lanes = ctx->fill_lanes;
+    tuning->grid_pct = ctx->grid_pct;
     return CVG_OK;
 }
 
@@ -3036,13 +3054,16 @@ int cvg_set_tuning(cvg_context *ctx, const cvg_tuning *tuning)
 {
     CVG_TRY(check_ctx(ctx));
     if (!tuning) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "cvg_set_tuning: null tuning");
-    const int w = tuning->coop_waves, f = tuning->window_lanes;
-    if (tuning->coop_min_key < 1 || !(w == 1 || w == 2 || w == 4) || !(f == 8 || f == 16 || f == 32 || f == 64))
+    const int w = tuning->coop_waves, f = tuning->window_lanes, g = tuning->grid_pct;
+    if (tuning->coop_min_key < 1 || !(w == 1 || w == 2 || w == 4) || !(f == 8 || f == 16 || f == 32 || f == 64) ||
+        !(g == 0 || (g >= 25 && g <= 100)))
         return fail(ctx, CVG_ERR_INVALID_ARGUMENT,
-                    "cvg_set_tuning: need coop_min_key >= 1, coop_waves in {1, 2, 4}, window_lanes in {8, 16, 32, 64}");
+                    "cvg_set_tuning: need coop_min_key >= 1, coop_waves in {1, 2, 4}, window_lanes in {8, 16, 32, 64}, "
+                    "grid_pct 0 or 25..100");
     ctx->coop_min_cap = tuning->coop_min_key;
     ctx->coop_waves = w;
     ctx->fill_lanes = f;
+    ctx->grid_pct = g;
     return CVG_OK;
 }
 
@@ -3074,18 +3095,22 @@ int cvg_autotune_batch(cvg_context *ctx, int num_maps, const int *num_edges, int
     CVG_TRY(cvg_get_tuning(ctx, &cur));
     double cur_ms = 0.0;
     CVG_TRY(time_of(cur, &cur_ms));
-    // coordinate descent: the cooperative threshold, its waves, the scan lanes
+    // coordinate descent: the cooperative threshold, its waves, the scan
+    // lanes, the light buckets' grid
     const int keys[] = {4, 6, 8, 11, 16, 1 << 30};
     const int waves[] = {2, 4};
     const int lanes[] = {8, 16, 32};
-    for (int field = 0; field < 3; ++field) {
-        const int *vals = field == 0 ? keys : (field == 1 ? waves : lanes);
+    const int grids[] = {50, 75, 100};
+    for (int field = 0; field < 4; ++field) {
+        const int *vals = field == 0 ? keys : (field == 1 ? waves : (field == 2 ? lanes : grids));
         const int nv = field == 0 ? 6 : (field == 1 ? 2 : 3);
         cvg_tuning base = cur;
         for (int k = 0; k < nv; ++k) {
             cvg_tuning t = base;
-            int &slot = field == 0 ? t.coop_min_key : (field == 1 ? t.coop_waves : t.window_lanes);
-            const int old = field == 0 ? base.coop_min_key : (field == 1 ? base.coop_waves : base.window_lanes);
+            int &slot = field == 0 ? t.coop_min_key
+                                   : (field == 1 ? t.coop_waves : (field == 2 ? t.window_lanes : t.grid_pct));
+            const int old = field == 0 ? base.coop_min_key
+                                       : (field == 1 ? base.coop_waves : (field == 2 ? base.window_lanes : base.grid_pct));
             if (vals[k] == old) continue;
             slot = vals[k];
             double ms = 0.0;

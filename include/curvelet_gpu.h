@@ -207,6 +207,9 @@ typedef struct cvg_tuning {
                           coop_waves waves; lighter ones one anchor per wave (>= 1) */
     int coop_waves;    /* 1, 2 or 4 */
     int window_lanes;  /* lanes per anchor in the neighbour-window scans: 8, 16, 32 or 64 */
+    int grid_pct;      /* grid of the light buckets' curvelet launches as a percentage of
+                          their resident capacity, 25..100; 0 = automatic (75 below 4M
+                          edges in the run, else 100) */
 } cvg_tuning;
 typedef struct cvg_tuning_trial {
     cvg_tuning tuning;
@@ -215,7 +218,7 @@ typedef struct cvg_tuning_trial {
 int cvg_get_tuning(cvg_context *ctx, cvg_tuning *tuning);
 int cvg_set_tuning(cvg_context *ctx, const cvg_tuning *tuning);
 /* Autotuner for the batched form: times cvg_run_batch_device on the caller's
- * maps under candidate tunings (coordinate descent over the three fields,
+ * maps under candidate tunings (coordinate descent over the four fields,
  * one warm-up and `reps` timed runs each) and keeps the fastest on the
  * context.  best/best_ms receive the result; trials (capacity max_trials, may
  * be null) every candidate tried, in order; num_trials their count.  The last

@@ -85,14 +85,16 @@ def test_tuning_set_get_and_rejects(ctx):
     from curvelet_construction_on_gpu_amd import CurveletError
     old = ctx.tuning
     try:
-        ctx.tuning = {"coop_min_key": 11, "coop_waves": 2, "window_lanes": 32}
-        assert ctx.tuning == {"coop_min_key": 11, "coop_waves": 2, "window_lanes": 32}
-        for bad in ({"coop_min_key": 0, "coop_waves": 4, "window_lanes": 16},
-                    {"coop_min_key": 8, "coop_waves": 3, "window_lanes": 16},
-                    {"coop_min_key": 8, "coop_waves": 4, "window_lanes": 12}):
+        want = {"coop_min_key": 11, "coop_waves": 2, "window_lanes": 32, "grid_pct": 60}
+        ctx.tuning = want
+        assert ctx.tuning == want
+        for bad in ({"coop_min_key": 0}, {"coop_waves": 3}, {"window_lanes": 12}, {"grid_pct": 20},
+                    {"grid_pct": 101}):
             with pytest.raises(CurveletError):
-                ctx.tuning = bad
-        assert ctx.tuning == {"coop_min_key": 11, "coop_waves": 2, "window_lanes": 32}
+                ctx.tuning = {**want, **bad}
+        assert ctx.tuning == want
+        ctx.tuning = {"grid_pct": 0}  # automatic; the other fields keep their values
+        assert ctx.tuning == {**want, "grid_pct": 0}
     finally:
         ctx.tuning = old
 
@@ -114,6 +116,7 @@ def test_autotune_batch_keeps_results(ctx):
         assert min(ms for _, ms in trials) == pytest.approx(best_ms)
         assert ctx.tuning == best
         assert best["coop_waves"] in (1, 2, 4) and best["window_lanes"] in (8, 16, 32, 64)
+        assert best["grid_pct"] == 0 or 25 <= best["grid_pct"] <= 100
         ctx.run_batch(maps, p)
         for i in range(len(maps)):
             got = ctx.batch_curvelets(i)

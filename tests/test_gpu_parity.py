@@ -153,13 +153,15 @@ def test_full_size_map_matches_reference(ctx):
     np.testing.assert_allclose(np.abs(info).sum(axis=0), meta["info_colabssum"], rtol=1e-9)
 
 
-@pytest.mark.parametrize("min_cap,waves", [(1, 4), (1, 2), (1, 1), (1000, 4)])
-def test_launch_shapes_same_results(oracle_cr, monkeypatch, min_cap, waves):
+@pytest.mark.parametrize("min_cap,waves,grid", [(1, 4, 0), (1, 2, 100), (1, 1, 25), (1000, 4, 50), (8, 4, 100)])
+def test_launch_shapes_same_results(oracle_cr, monkeypatch, min_cap, waves, grid):
     """Every launch shape of the fused kernel (one anchor per wave, or one
-    anchor per workgroup of 1, 2 or 4 waves) gives the same bits."""
+    anchor per workgroup of 1, 2 or 4 waves; light-bucket grids from a quarter
+    to all of their resident capacity) gives the same bits."""
     from curvelet_construction_on_gpu_amd import Context
     monkeypatch.setenv("CVG_COOP_MIN_CAP", str(min_cap))
     monkeypatch.setenv("CVG_COOP_WAVES", str(waves))
+    monkeypatch.setenv("CVG_GRID_PCT", str(grid))
     with Context(0) as c:
         for key, G in (("file:TO_edges", 7), ("synth:C2:1002", 5)):
             e = edges_for(key)
