@@ -4,18 +4,24 @@
 Workload: one synthetic 1920x1080 third-order edge map (~207k edges; 1500
 curves, 40 T-junction patches, 30% clutter; seed 1003), group_max_sz = 7,
 other parameters at the reference defaults.  A step is one pass of the whole
-path over that map: spatial hash + CSR neighbour graph + pairwise bundles +
-chain growth + dedup + record, edges already resident in HBM, chains and info
-left in HBM.
+path over that map on the reference's timed span (cpu_curvelet/main.cpp:43-66):
+edges in host memory -> spatial hash + CSR neighbour graph + pairwise bundles +
+chain growth + dedup + record -> compacted chains and info in host memory.
 
-Prints one JSON line.  `value` is edges/s over the timed steps (host clock,
-synchronised on both sides).  `roofline` is for the dominant kernel
-(k_curvelets, the fused bundle + growth kernel), timed with HIP events on the
-context's stream; its algorithmic FP64 work is the bundle transport (1,900
-FLOP per neighbour pair) plus the growth intersections (3 FLOP per cell per
-intersection).  `cpu_baseline` is the reference's own cpu_curvelet (built in
-place under oracle/_ref) on this host's cores, timed on the same map with
-span (ii): CSR build + build_curvelets_greedy.
+Prints one JSON line.  `value` is edges / the median step time over the timed
+steps, each step timed with HIP events on the context's stream from before
+the edges' upload to after the results' download (page-locked host buffers on
+both sides).  Extra keys: the rate with edges resident in HBM and results left
+there, the rate when consecutive maps overlap their transfers, and the text
+end-to-end time (reader + path + writers).  `roofline` is for the dominant
+kernel (k_curvelets, the fused bundle + growth kernel), timed with HIP events
+on the context's stream; its algorithmic FP64 work is what the kernel
+executes: the transport cells (1,621 FLOP per neighbour pair; the 14 sines per
+pair come precomputed) and the growth intersections (189 FLOP each).
+`cpu_baseline` is the reference's own cpu_curvelet (built in place under
+oracle/_ref) on this host's cores, timed on the same map with span (ii):
+CSR build + build_curvelets_greedy; without oracle/_ref, the repository's
+restatement of it (kind "port").
 """
 from __future__ import annotations
 
@@ -31,7 +37,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 WORKLOAD = {"preset": "C3", "seed": 1003, "group_max_sz": 7}
-F_PAIR = 1900.0           # FP64 FLOP per bundle pair (transport of 63 cells incl. 14 sines)
+F_CELLS = 1621.0          # FP64 FLOP per bundle pair in the fused kernel: 63 cells x ~25 + per-row/pair terms
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector peak
 PEAK_HBM_GBS = 8000.0
 
@@ -58,23 +64,55 @@ def fused_traffic():
         return None
 
 
-def cpu_baseline(edges, params):
-    """The reference binary's code (oracle/_ref) on host cores, span (ii)."""
+def host_info() -> dict:
+    """CPU model and glibc of this host (recorded beside every CPU timing)."""
+    import platform
+    model = platform.processor() or ""
     try:
-        from oracle import Ref
-        if not Ref.available():
-            return None
-        ref = Ref()
-        cores = host_cores()
-        times = []
-        for _ in range(3):
-            r = ref.run(edges, params, nthreads=cores, valid_pairs=False)
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "glibc": "-".join(platform.libc_ver())}
+
+
+def cpu_time(edges, params, threads, reps=3):
+    """Median seconds of span (ii) (CSR build + build_curvelets_greedy) on
+    `threads` host threads: the reference's own code built in place
+    (oracle/_ref) when it is there, else the repository's restatement of it
+    (oracle/liboracle.so, bit-exact with it on every fixture)."""
+    from oracle import Oracle, Ref
+    times, rows = [], 0
+    if Ref.available():
+        ref, kind = Ref(), "reference"
+        for _ in range(reps):
+            r = ref.run(edges, params, nthreads=threads, valid_pairs=False)
             times.append(r["times"]["csr"] + r["times"]["greedy"])
-        t = statistics.median(times)
-        return {"value": len(edges) / t, "unit": "edges/s", "cores": cores, "kind": "reference",
+            rows = len(r["chains"])
+    else:
+        orc, kind = Oracle(), "port"
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            orc.build_csr(edges, params, nthreads=threads)
+            chains, _ = orc.greedy(params, nthreads=threads)
+            times.append(time.perf_counter() - t0)
+            rows = len(chains)
+    return {"seconds": statistics.median(times), "kind": kind, "rows": int(rows)}
+
+
+def cpu_baseline(edges, params):
+    """The CPU baseline on this host's cores, span (ii), on the whole map."""
+    try:
+        cores = host_cores()
+        t = cpu_time(edges, params, cores)
+        what = ("the reference's code (oracle/_ref, built in place from cpu_curvelet)" if t["kind"] == "reference"
+                else "the repository's restatement of cpu_curvelet (oracle/liboracle.so; oracle/_ref absent)")
+        return {"value": len(edges) / t["seconds"], "unit": "edges/s", "cores": cores, "kind": t["kind"],
                 "sample": f"the full {len(edges)}-edge map, CSR + build_curvelets_greedy, median of 3 runs "
-                          f"({t:.3f} s each on {cores} threads)",
-                "seconds": t, "num_curvelets": int(len(r["chains"]))}
+                          f"({t['seconds']:.3f} s each on {cores} threads) of {what}",
+                "seconds": t["seconds"], "num_curvelets": t["rows"], **host_info()}
     except Exception as exc:  # the baseline is reported, never required
         return {"error": repr(exc)}
 
@@ -317,40 +355,39 @@ def main(argv=None) -> int:
     edges = synth_edges(WORKLOAD["preset"], WORKLOAD["seed"])
     params = CurveletParams(group_max_sz=WORKLOAD["group_max_sz"])
     n, sz = edges.shape
-
     ctx = Context(0)
-    dptr = ctx.upload(edges)  # inputs resident in HBM before the timed region
+
+    # --- the protocol span: host edges in -> host chains and info out ---------
+    # (the reference's timed span, main.cpp:43-66): H2D of the edges, every
+    # kernel, D2H of the compacted chains and info, timed with HIP events on
+    # the context's stream; page-locked host buffers on both sides
+    proto_ms, proto_wall_ms, counts = protocol_span(ctx, edges, params, args.warmup, args.steps)
+    ms_step = statistics.median(proto_ms)
+
+    # --- resident in HBM: edges already on the device, results left there ----
+    dptr = ctx.upload(edges)
     for _ in range(args.warmup):
-        st = ctx.run_device(dptr, n, sz, params)
+        ctx.run_device(dptr, n, sz, params)
     ctx.synchronize()
-    # timed steps: each run is queued without waiting for its end (no stats),
-    # so the host prepares step k + 1 while step k runs; the fused kernel's
-    # duration per step comes from HIP events read after the loop
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ctx.run_device(dptr, n, sz, params, stats=False)
     ctx.synchronize()
-    elapsed = time.perf_counter() - t0
+    resident_s = (time.perf_counter() - t0) / args.steps
     fused = ctx.recent_fused_ms(args.steps)
     st = ctx.run_device(dptr, n, sz, params)  # counts and stage times (untimed)
-
-    # the same map from host memory to host memory (PCIe both ways): reported
-    # beside `value`, never as it
-    pcie = []
-    for _ in range(5):
-        s0 = time.perf_counter()
-        ctx.run(edges, params)
-        ctx.curvelets()
-        pcie.append(time.perf_counter() - s0)
+    pipelined = pipelined_pcie(edges, params, max(args.steps, 20))
 
     pairs = st["total_neighbor_pairs"]
-    ms_step = elapsed / args.steps * 1e3
     fused_ms = statistics.mean(fused)
-    flops = pairs * F_PAIR + st["intersections"] * 63 * 3
+    # FP64 work executed inside k_curvelets: the transport cells (F_CELLS per
+    # pair; its sines come precomputed from k_edge_trig / k_pair_trig) and the
+    # growth intersections (3 FLOP per cell per intersection)
+    flops = pairs * F_CELLS + st["intersections"] * 63 * 3
     achieved = flops / (fused_ms * 1e-3) / 1e12
     line = {
         "metric": "edges/s per map at 1080p (~200k edges)",
-        "value": n * args.steps / elapsed,
+        "value": n / (ms_step * 1e-3),
         "unit": "edges/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -364,17 +401,26 @@ def main(argv=None) -> int:
         "config": {"workload": "1920x1080 synthetic third-order edge map, ~207k edges, group_max_sz=7 "
                                "(preset C3, seed 1003)",
                    "edges": n, "neighbor_pairs": pairs, "valid_pairs": st["valid_pairs"],
-                   "curvelets": st["num_curvelets"], "group_max_sz": WORKLOAD["group_max_sz"]},
-        "bundle_pairs_per_s": pairs * args.steps / elapsed,
-        "pcie_inclusive_edges_per_s": n / statistics.median(pcie),
+                   "curvelets": st["num_curvelets"], "group_max_sz": WORKLOAD["group_max_sz"],
+                   "span": "host edges in -> host chains and info out (H2D + all kernels + D2H), page-locked "
+                           "host buffers, HIP events on the context stream, median of the timed runs after "
+                           "the warm-up runs",
+                   "d2h_bytes": counts["d2h_bytes"], "h2d_bytes": counts["h2d_bytes"]},
+        "ms_per_step_spread": {"min": min(proto_ms), "max": max(proto_ms), "host_wall_median": statistics.median(proto_wall_ms)},
+        "bundle_pairs_per_s": pairs / (ms_step * 1e-3),
+        "resident_edges_per_s": n / resident_s,
+        "resident_ms_per_step": resident_s * 1e3,
+        "pipelined_pcie_edges_per_s": pipelined,
         "fused_kernel_ms_median": statistics.median(fused),
         "stage_ms": {"graph": st["ms_graph"], "curvelets": st["ms_curvelets"], "fused_kernel": fused_ms},
         "lds_bytes_per_workgroup": st["lds_bytes_per_workgroup"],
-        "roofline": {"bound": "mfma", "pipe": "fp64 vector ALU (no MFMA in this path)",
+        "roofline": {"bound": "fp64_valu", "pipe": "fp64 vector ALU (no MFMA in this path)",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": fused_traffic(),
                      "kernel": "k_curvelets", "kernel_ms": fused_ms,
-                     "flop_per_launch": flops},
+                     "flop_per_launch": flops,
+                     "flop_model": f"{F_CELLS:.0f} per neighbour pair + 189 per growth intersection, "
+                                   "executed inside k_curvelets"},
     }
     traffic = fused_traffic()
     if traffic:
@@ -388,9 +434,123 @@ def main(argv=None) -> int:
             line["cpu_baseline"] = cb
             if "value" in cb:
                 line["speedup_vs_cpu_baseline"] = line["value"] / cb["value"]
+        line["c1_single_thread"] = c1_single_thread(ctx, args.warmup, args.steps)
+    line["end_to_end_with_text_io"] = end_to_end_text(ctx, edges, params)
     ctx.close()
     print(json.dumps(line))
     return 0
+
+
+def protocol_span(ctx, edges, params, warmup, runs):
+    """H2D + kernels + D2H per run (HIP events on the context stream, ms),
+    host wall time of the same runs, and the transfer sizes."""
+    import numpy as np
+    from curvelet_construction_on_gpu_amd import pinned_empty
+    from curvelet_construction_on_gpu_amd.hip import Event
+    n, sz = edges.shape
+    h_in = pinned_empty((n, sz), np.float64)
+    h_in[:] = edges
+    ctx.run(h_in, params)
+    nc, w = ctx.shape()
+    h_chains = pinned_empty((nc, w), np.uint32)
+    h_info = pinned_empty((nc, 10), np.float64)
+    stream = ctx.stream
+    e0, e1 = Event(), Event()
+    dev_ms, wall_ms = [], []
+    for k in range(warmup + runs):
+        t0 = time.perf_counter()
+        e0.record(stream)
+        ctx.run(h_in, params, stats=False)
+        got = ctx.curvelets_into(h_chains, h_info)
+        e1.record(stream)
+        e1.synchronize()
+        t1 = time.perf_counter()
+        if got != nc:
+            raise RuntimeError(f"row count changed between runs: {got} != {nc}")
+        if k >= warmup:
+            dev_ms.append(e0.elapsed_ms(e1))
+            wall_ms.append((t1 - t0) * 1e3)
+    return dev_ms, wall_ms, {"h2d_bytes": int(h_in.nbytes), "d2h_bytes": int(h_chains.nbytes + h_info.nbytes)}
+
+
+def pipelined_pcie(edges, params, maps):
+    """Host-to-host edges/s when consecutive maps overlap: two contexts (own
+    streams) on two host threads, each running map after map from page-locked
+    input to page-locked outputs, so one map's transfers run beside another's
+    kernels (the served form's pattern)."""
+    import threading
+    import numpy as np
+    from curvelet_construction_on_gpu_amd import Context, pinned_empty
+    n, sz = edges.shape
+    ctxs = [Context(0) for _ in range(2)]
+    bufs = []
+    for c in ctxs:
+        h_in = pinned_empty((n, sz), np.float64)
+        h_in[:] = edges
+        c.run(h_in, params)
+        nc, w = c.shape()
+        bufs.append((h_in, pinned_empty((nc, w), np.uint32), pinned_empty((nc, 10), np.float64)))
+    per = maps // 2
+
+    def work(i):
+        c, (h_in, hc, hi) = ctxs[i], bufs[i]
+        for _ in range(per):
+            c.run(h_in, params, stats=False)
+            c.curvelets_into(hc, hi)
+
+    for i in range(2):  # warm-up
+        work(i)
+    ths = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    t0 = time.perf_counter()
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    dt = time.perf_counter() - t0
+    for c in ctxs:
+        c.close()
+    return 2 * per * n / dt
+
+
+def c1_single_thread(ctx, warmup, runs):
+    """The CPU-runnable config (320x240, ~6k edges, defaults): the reference's
+    code on one host thread against the GPU's protocol span on the same map."""
+    from curvelet_construction_on_gpu_amd import CurveletParams, synth_edges
+    try:
+        e = synth_edges("C1", 1001)
+        p = CurveletParams()
+        gpu_ms, _, _ = protocol_span(ctx, e, p, warmup, runs)
+        gpu = len(e) / (statistics.median(gpu_ms) * 1e-3)
+        cpu = cpu_time(e, p, 1, reps=3)
+        return {"edges": int(len(e)), "gpu_edges_per_s": gpu, "cpu_1_thread_edges_per_s": len(e) / cpu["seconds"],
+                "cpu_kind": cpu["kind"], "speedup": gpu / (len(e) / cpu["seconds"])}
+    except Exception as exc:  # reported, never required
+        return {"error": repr(exc)}
+
+
+def end_to_end_text(ctx, edges, params, reps=3):
+    """Edge file text in -> chain and info files out: the library's reader,
+    the host-to-host path and its writers (the reference parses and writes
+    outside its timer, main.cpp:37-44 and :68-91); median of reps."""
+    import tempfile
+    import numpy as np
+    from curvelet_construction_on_gpu_amd import read_TO_edges_from_file, write_chain_file, write_info_file
+    with tempfile.TemporaryDirectory() as d:
+        src = Path(d) / "edges.txt"
+        np.savetxt(src, edges, fmt="%.17g", delimiter=" ")
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            e = read_TO_edges_from_file(src)
+            ctx.run(e, params, stats=False)
+            chains, info = ctx.curvelets()
+            write_chain_file(Path(d) / "chain.txt", chains)
+            write_info_file(Path(d) / "info.txt", info)
+            times.append(time.perf_counter() - t0)
+        t = statistics.median(times)
+        return {"edges_per_s": len(edges) / t, "ms": t * 1e3, "input_bytes": src.stat().st_size,
+                "output_bytes": (Path(d) / "chain.txt").stat().st_size + (Path(d) / "info.txt").stat().st_size,
+                "runs": reps, "statistic": "median"}
 
 
 if __name__ == "__main__":

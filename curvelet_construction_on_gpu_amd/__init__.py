@@ -29,7 +29,7 @@ import numpy as np
 __all__ = [
     "CurveletError", "CurveletParams", "NeighborGraph", "CurveletGPU", "Context", "lib_path", "load_library",
     "read_TO_edges_from_file", "build_neighbor_csr_graph", "run_curvelet", "synth_edges", "synth_preset",
-    "math_eval", "write_chain_file", "write_info_file", "chain_to_info_filename", "dt_radians",
+    "math_eval", "write_chain_file", "write_info_file", "chain_to_info_filename", "dt_radians", "pinned_empty",
 ]
 
 _PKG = Path(__file__).resolve().parent
@@ -145,6 +145,7 @@ _SIGNATURES = {
     "cvg_synth_generate": (C.c_int, [_P(_SynthConfig), C.c_ulonglong, _dp, C.c_int, _ip]),
     "cvg_debug_anchor_bundles": (C.c_int, [C.c_void_p, _P(_Params), C.c_int, _ucp, _ucp, _dp, _dp, _ip, _ip]),
     "cvg_math_eval": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]),
+    "cvg_debug_cell_order": (C.c_int, [C.c_int, _dp, _dp, _ip]),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)
@@ -297,14 +298,43 @@ class Context:
                    "get_curvelets")
         return chains, info
 
+    def shape(self) -> tuple[int, int]:
+        """(num_curvelets(), chain_width()) of the last run."""
+        nc, w = C.c_uint(), C.c_uint()
+        self.check(self._lib.cvg_curvelet_shape(self._h, C.byref(nc), C.byref(w)), "curvelet_shape")
+        return nc.value, w.value
+
+    def curvelets_into(self, chains: np.ndarray, info: np.ndarray) -> int:
+        """cvg_get_curvelets into caller arrays (C-contiguous, at least
+        num_curvelets rows; page-locked ones from pinned_empty take one DMA
+        each).  Returns num_curvelets."""
+        nc, w = self.shape()
+        if (chains.dtype != np.uint32 or info.dtype != np.float64 or not chains.flags.c_contiguous
+                or not info.flags.c_contiguous or chains.shape[0] < nc or info.shape[0] < nc
+                or chains.shape[1] != w or info.shape[1] != 10):
+            raise CurveletError(f"curvelets_into: need uint32 ({nc}, {w}) and float64 ({nc}, 10) C arrays")
+        self.check(self._lib.cvg_get_curvelets(self._h, _ptr(chains, C.c_uint), _ptr(info, C.c_double)),
+                   "get_curvelets")
+        return nc
+
+    @property
+    def stream(self) -> int:
+        """The context's hipStream_t as an integer (for torch.cuda.ExternalStream)."""
+        s = C.c_void_p()
+        self.check(self._lib.cvg_get_stream(self._h, C.byref(s)), "get_stream")
+        return s.value or 0
+
     # -- whole path ----------------------------------------------------------
-    def run(self, edges: np.ndarray, params: CurveletParams) -> dict:
+    def run(self, edges: np.ndarray, params: CurveletParams, stats: bool = True) -> dict | None:
+        """cvg_run from host memory.  stats=False returns once the work is
+        queued; results are complete at their first query."""
         e = np.ascontiguousarray(edges, dtype=np.float64)
         n, sz = e.shape
         st = _RunStats()
         p = params._c()
-        self.check(self._lib.cvg_run(self._h, n, sz, _ptr(e, C.c_double), C.byref(p), C.byref(st)), "run")
-        return {k: getattr(st, k) for k, _ in _RunStats._fields_}
+        self.check(self._lib.cvg_run(self._h, n, sz, _ptr(e, C.c_double), C.byref(p),
+                                     C.byref(st) if stats else None), "run")
+        return {k: getattr(st, k) for k, _ in _RunStats._fields_} if stats else None
 
     def upload(self, edges: np.ndarray) -> int:
         e = np.ascontiguousarray(edges, dtype=np.float64)
@@ -499,6 +529,35 @@ class Server:
         self.close()
 
 
+class _Pinned:
+    """Page-locked host block from cvg_host_alloc, freed with the last array over it."""
+
+    def __init__(self, nbytes: int):
+        self._lib = load_library()
+        p = C.c_void_p()
+        rc = self._lib.cvg_host_alloc(max(int(nbytes), 1), C.byref(p))
+        if rc != 0:
+            raise CurveletError(f"cvg_host_alloc({nbytes}) failed", rc)
+        self.ptr = p.value
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self._lib.cvg_host_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+
+def pinned_empty(shape, dtype=np.float64) -> np.ndarray:
+    """A numpy array in page-locked host memory (DMA at full link speed for
+    uploads and downloads)."""
+    dt = np.dtype(dtype)
+    shape = tuple(int(v) for v in np.atleast_1d(shape))
+    count = int(np.prod(shape)) if shape else 1
+    holder = _Pinned(count * dt.itemsize)
+    buf = (C.c_char * max(count * dt.itemsize, 1)).from_address(holder.ptr)
+    buf._holder = holder  # the block lives as long as the array's buffer
+    return np.frombuffer(buf, dtype=dt, count=count).reshape(shape)
+
+
 _default_ctx: Context | None = None
 
 
@@ -654,3 +713,17 @@ def math_eval(op: str, x: np.ndarray, y: np.ndarray | None = None, ctx: Context 
     if rc != 0:
         raise CurveletError(f"math_eval({op}) failed", rc)
     return out
+
+
+def cell_order_ranks(x: np.ndarray, y: np.ndarray) -> np.ndarray:
+    """Each edge's position inside its pixel cell after the reference's
+    spatial-index sort (preprocess.hpp:78-92), as the library replays it
+    (host code; no device needed)."""
+    lib = load_library()
+    xx = np.ascontiguousarray(x, dtype=np.float64)
+    yy = np.ascontiguousarray(y, dtype=np.float64)
+    r = np.zeros(len(xx), np.int32)
+    rc = lib.cvg_debug_cell_order(len(xx), _ptr(xx, C.c_double), _ptr(yy, C.c_double), _ptr(r, C.c_int))
+    if rc != 0:
+        raise CurveletError("cvg_debug_cell_order failed", rc)
+    return r

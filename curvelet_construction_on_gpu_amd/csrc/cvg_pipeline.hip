@@ -118,6 +118,11 @@ struct cvg_context {
     int gx0 = 0, gy0 = 0, gw = 0, gh = 0;
     DevBuf d_edges_up, d_x, d_y, d_th, d_cx, d_cy, d_scal, d_grid, d_tmp_ids, d_sorted, d_sxy, d_scell, d_part, d_deg,
         d_off, d_tiles, d_cand, d_ids, d_d2, d_pair_anchor;
+    // the last graph's edge array (device) and, once the reference's
+    // within-cell order has been replayed for it, each edge's rank in its cell
+    const double *last_edges = nullptr;
+    bool use_cellrank = false;
+    DevBuf d_cellrank;
 
     // the maps of a batch (a single map outside batches): edge id ranges,
     // per-edge map ids, per-map cell boxes and shifts, output row ranges
@@ -141,6 +146,19 @@ struct cvg_context {
 
     // diagnostics
     DevBuf d_dbg;
+
+    // every device pool of the context
+    template <typename F>
+    void for_each_pool(F f)
+    {
+        DevBuf *bufs[] = {&d_edges_up, &d_x,      &d_y,      &d_th,     &d_cx,     &d_cy,          &d_scal,
+                          &d_grid,     &d_tmp_ids, &d_sorted, &d_sxy,    &d_scell,  &d_part,        &d_deg,
+                          &d_off,      &d_tiles,  &d_cand,   &d_ids,    &d_d2,     &d_pair_anchor, &d_trig,
+                          &d_tdd,      &d_cls,    &d_prec,   &d_psin,   &d_lists,  &d_tables,      &d_st_ids,
+                          &d_st_meta,  &d_st_k,   &d_st_rank, &d_acc,   &d_rowoff, &d_gbundle,     &d_chains,
+                          &d_info,     &d_dbg,    &d_map,    &d_eoff,   &d_mapbox, &d_maprow,       &d_cellrank};
+        for (DevBuf *b : bufs) f(*b);
+    }
 };
 
 namespace {
@@ -622,24 +640,32 @@ __global__ __launch_bounds__(kBlock) void k_cell_scatter(int n, const int *__res
     }
 }
 
-// Within-cell order: ascending edge id (the reference's order is whatever its
-// unstable std::sort of the index permutation yields, preprocess.hpp:90-92;
-// the two agree for up to 16 edges and only matter for the cases that
-// k_fill_csr counts as order-ambiguous).  The cell-sorted arrays are the edge
+// Within-cell order: ascending edge id, or the given per-edge rank.  The
+// reference's order is whatever its unstable std::sort of the index
+// permutation yields (preprocess.hpp:90-92); the two agree for maps of up to
+// 16 edges, and the order only matters for the anchors k_fill_csr counts as
+// order-ambiguous, for which the host replays that sort and passes the
+// resulting ranks.  The cell-sorted arrays are the edge
 // id, its cell index and its (x, y) pair, so one window candidate costs one
 // int, one int and one 16-byte load, all independent.
 __global__ __launch_bounds__(kBlock) void k_cell_order(int n, const int *__restrict__ cx, const int *__restrict__ cy,
                                                        GridDesc g, const unsigned *__restrict__ start,
-                                                       const int *__restrict__ tmp, const double *__restrict__ x,
-                                                       const double *__restrict__ y, int *__restrict__ sorted,
-                                                       int *__restrict__ scell, double2 *__restrict__ sxy)
+                                                       const int *__restrict__ tmp, const int *__restrict__ rank,
+                                                       const double *__restrict__ x, const double *__restrict__ y,
+                                                       int *__restrict__ sorted, int *__restrict__ scell,
+                                                       double2 *__restrict__ sxy)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int c = cell_index(g, cx[i], cy[i]);
     const unsigned s = start[c], e = start[c + 1];
     unsigned r = 0;
-    for (unsigned k = s; k < e; ++k) r += (tmp[k] < i) ? 1u : 0u;
+    if (rank) {
+        // the reference's own within-cell order, replayed on the host
+        r = min(static_cast<unsigned>(rank[i]), e - s - 1u);
+    } else {
+        for (unsigned k = s; k < e; ++k) r += (tmp[k] < i) ? 1u : 0u;
+    }
     const unsigned p = s + r;
     sorted[p] = i;
     scell[p] = c;
@@ -797,6 +823,8 @@ __global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const doubl
                                                               int *__restrict__ ids, double *__restrict__ d2,
                                                               int *__restrict__ pair_anchor, int *__restrict__ tie_list,
                                                               unsigned max_cand, int cap,
+                                                              const int *__restrict__ map_of,
+                                                              const long long *__restrict__ map_eoff,
                                                               unsigned *__restrict__ scal)
 {
     // per segment: cap candidates (d2, id, cell) in LDS for the ranking
@@ -872,8 +900,10 @@ __global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const doubl
         }
     }
     wave_sync();
-    // rank sort by d2 with ties by scan position; detect ties and same-cell ties
+    // rank sort by d2 with ties by scan position; detect ties, same-cell ties
+    // and cells that give two or more candidates
     bool tie = false;
+    bool shared = false;
     const int dmax = wave_max_over_segments<G>(dg);
     for (int i0 = 0; i0 < dmax; i0 += G) {
         const int i = i0 + sl;
@@ -895,8 +925,10 @@ __global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const doubl
             const bool in = j < dg;
             rank += (in && (dj < di || (dj == di && j < i))) ? 1 : 0;
             const bool eq = in && active && dj == di && j != i;
+            const bool same = in && active && cellj == celli && j != i;
             tie = tie || eq;
-            ambiguous = ambiguous || (eq && cellj == celli);
+            shared = shared || same;
+            ambiguous = ambiguous || (eq && same);
         }
         if (active) {
             ids[o + rank] = idi;
@@ -904,13 +936,23 @@ __global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const doubl
         }
     }
     const bool any_tie = (__ballot(tie) & segmask) != 0ULL;
+    const bool any_shared = (__ballot(shared) & segmask) != 0ULL;
     const bool any_amb = (__ballot(ambiguous) & segmask) != 0ULL;
     for (int i = sl; i < dg; i += G) pair_anchor[o + i] = a;
     if (any_tie && dg > 16) {
         for (int i = sl; i < dg; i += G) cand[o + i] = Cand{ld[i], lid[i]};  // scan order
         if (sl == 0) tie_list[atomicAdd(&scal[S_TIES], 1u)] = a;
     }
-    if (sl == 0 && any_amb && n > 16) atomicAdd(&scal[S_AMBIG], 1u);
+    // The list depends on the within-cell order when two candidates of one
+    // cell tie (any length), when a max_candidates cut falls inside a cell, or
+    // when a list of more than 16 (introsort, not insertion sort) has ties
+    // and any cell gives it two candidates: introsort places tied entries by
+    // the positions of all entries.  The map's own spatial-index sort leaves
+    // cells in edge-id order (as here) when the map has at most 16 edges.
+    if (valid && sl == 0 && (any_amb || (dg > 16 && any_tie && any_shared))) {
+        const long long map_n = map_of ? map_eoff[map_of[a] + 1] - map_eoff[map_of[a]] : n;
+        if (map_n > 16) atomicAdd(&scal[S_AMBIG], 1u);
+    }
 }
 
 // Anchors whose (more than 16) candidates contain exact distance ties: replay
@@ -2001,6 +2043,7 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     }
     ctx->n = n;
     ctx->sz = sz;
+    ctx->last_edges = d_edges;
     ctx->rad = rad;
     ctx->nbr_radius = R;
     ctx->max_cand = max_cand;
@@ -2118,7 +2161,8 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
                        as<int>(ctx->d_cy), g, start, cnt, as<int>(ctx->d_tmp_ids));
     hipLaunchKernelGGL(k_cell_order, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
-                       as<int>(ctx->d_cy), g, start, as<int>(ctx->d_tmp_ids), as<double>(ctx->d_x),
+                       as<int>(ctx->d_cy), g, start, as<int>(ctx->d_tmp_ids),
+                       ctx->use_cellrank ? as<int>(ctx->d_cellrank) : nullptr, as<double>(ctx->d_x),
                        as<double>(ctx->d_y), as<int>(ctx->d_sorted), as<int>(ctx->d_scell),
                        as<double2>(ctx->d_sxy));
     CVG_HIP(ctx, hipGetLastError());
@@ -2166,7 +2210,8 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     // one G-lane segment per anchor; each segment keeps its candidates in LDS
     using FillFn = void (*)(int, const double *, const double *, const int *, const int *, GridDesc, int, double,
                             const unsigned *, const int *, const int *, const double2 *, const unsigned *, Cand *,
-                            int *, double *, int *, int *, unsigned, int, unsigned *);
+                            int *, double *, int *, int *, unsigned, int, const int *, const long long *,
+                            unsigned *);
     const int fill_cap = static_cast<int>(std::max(ctx->max_deg, 1u));
     const size_t per_seg = static_cast<size_t>(fill_cap) * (2 * sizeof(double));
     int G = ctx->fill_lanes;
@@ -2186,7 +2231,8 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
                        static_cast<int>(R), rad2, start, as<int>(ctx->d_sorted), as<int>(ctx->d_scell),
                        as<double2>(ctx->d_sxy), as<unsigned>(ctx->d_off), as<Cand>(ctx->d_cand),
                        as<int>(ctx->d_ids), as<double>(ctx->d_d2), as<int>(ctx->d_pair_anchor),
-                       as<int>(ctx->d_tmp_ids), max_cand, fill_cap, scal);
+                       as<int>(ctx->d_tmp_ids), max_cand, fill_cap, num_maps > 1 ? as<int>(ctx->d_map) : nullptr,
+                       as<long long>(ctx->d_eoff), scal);
     hipLaunchKernelGGL(k_tie_sort, dim3(std::min(blocks_for(n), 64)), dim3(kBlock), 0, st, as<int>(ctx->d_tmp_ids),
                        scal, as<unsigned>(ctx->d_off), as<Cand>(ctx->d_cand), as<int>(ctx->d_ids),
                        as<double>(ctx->d_d2));
@@ -2276,12 +2322,85 @@ int ensure_host(cvg_context *ctx, size_t bytes)
     return CVG_OK;
 }
 
-// The counts build_curvelets left in flight: waits for the stream once.
+// Comparator of the reference's spatial-index sort (preprocess.hpp:90-92):
+// edge ids by packed cell key.
+struct CellKeyLess {
+    const long long *key;
+    __host__ __device__ bool operator()(int a, int b) const { return key[a] < key[b]; }
+};
+
+// The reference's within-cell order (build_spatial_index_cpu,
+// preprocess.hpp:78-92) for one map: its edge ids sorted by
+// pack_cell_key(lrint(x), lrint(y)) (:41-45) with libstdc++'s introsort; each
+// edge's position inside its cell's run goes to rank[i].
+void cell_ranks(int n, const double *x, const double *y, int *rank)
+{
+    std::vector<long long> key(static_cast<size_t>(std::max(n, 0)));
+    std::vector<int> perm(key.size());
+    for (int i = 0; i < n; ++i) {
+        const int cx = static_cast<int>(std::lrint(x[i]));
+        const int cy = static_cast<int>(std::lrint(y[i]));
+        key[i] = static_cast<long long>((static_cast<unsigned long long>(static_cast<long long>(cy)) << 32) |
+                                        static_cast<unsigned long long>(static_cast<unsigned>(cx)));
+        perm[i] = i;
+    }
+    cvg::stl_sort(perm.data(), n, CellKeyLess{key.data()});
+    int run = 0;
+    for (int q = 0; q < n; ++q) {
+        if (q > 0 && key[perm[q]] != key[perm[q - 1]]) run = q;
+        rank[perm[q]] = q - run;
+    }
+}
+
+// Replays that order for every map of the last graph into d_cellrank; the
+// next graph build orders cells by it.
+int replay_cell_order(cvg_context *ctx)
+{
+    const int n = ctx->n;
+    hipStream_t st = ctx->stream;
+    std::vector<double> hx(static_cast<size_t>(n)), hy(static_cast<size_t>(n));
+    CVG_HIP(ctx, hipMemcpyAsync(hx.data(), ctx->d_x.ptr, hx.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    CVG_HIP(ctx, hipMemcpyAsync(hy.data(), ctx->d_y.ptr, hy.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    CVG_HIP(ctx, hipStreamSynchronize(st));
+    std::vector<int> rank(static_cast<size_t>(n), 0);
+    for (int m = 0; m < ctx->num_maps; ++m) {
+        const long long lo = ctx->map_eoff[m], hi = ctx->map_eoff[m + 1];
+        cell_ranks(static_cast<int>(hi - lo), hx.data() + lo, hy.data() + lo, rank.data() + lo);
+    }
+    CVG_TRY(ensure(ctx, ctx->d_cellrank, rank.size() * sizeof(int)));
+    CVG_HIP(ctx, hipMemcpyAsync(ctx->d_cellrank.ptr, rank.data(), rank.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    CVG_HIP(ctx, hipStreamSynchronize(st));
+    ctx->use_cellrank = true;
+    return CVG_OK;
+}
+
+// Rebuild the last graph with the reference's within-cell order.
+int rebuild_graph_exact(cvg_context *ctx)
+{
+    CVG_TRY(replay_cell_order(ctx));
+    const std::vector<long long> eoff = ctx->map_eoff;
+    return build_graph(ctx, ctx->n, ctx->sz, ctx->last_edges, ctx->rad, ctx->nbr_radius, ctx->max_cand,
+                       ctx->num_maps, ctx->num_maps > 1 ? eoff.data() : nullptr);
+}
+
+int build_curvelets(cvg_context *ctx, const cvg_params &p);
+
+// The counts build_curvelets left in flight: waits for the stream once.  When
+// some anchor's neighbour list depended on the within-cell order, the graph
+// and curvelets are built again with the reference's order first.
 int finish_curvelets(cvg_context *ctx)
 {
     if (!ctx->counts_pending) return CVG_OK;
     CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->counts_pending = false;
+    if (ctx->mailbox[S_AMBIG] > 0 && !ctx->use_cellrank) {
+        const cvg_params p = ctx->cur_params;
+        CVG_TRY(rebuild_graph_exact(ctx));
+        CVG_TRY(build_curvelets(ctx, p));
+        CVG_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+        CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->counts_pending = false;
+    }
     const unsigned C = ctx->mailbox[32];
     ctx->valid_pairs = ctx->mailbox[S_VALID];
     ctx->intersections = static_cast<unsigned long long>(ctx->mailbox[S_COUNT]) |
@@ -2592,6 +2711,7 @@ int run_common(cvg_context *ctx, int n, int sz, const double *d_edges, const cvg
                int num_maps = 1, const long long *eoff = nullptr)
 {
     CVG_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    ctx->use_cellrank = false;
     CVG_TRY(build_graph(ctx, n, sz, d_edges, p->nrad, 3u, p->max_candidates, num_maps, eoff));  // main.cpp:48: 3
     CVG_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     CVG_TRY(build_curvelets(ctx, *p));
@@ -2861,15 +2981,7 @@ void cvg_destroy(cvg_context *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    DevBuf *bufs[] = {&ctx->d_edges_up, &ctx->d_x,     &ctx->d_y,       &ctx->d_th,      &ctx->d_cx,
-                      &ctx->d_cy,       &ctx->d_scal,  &ctx->d_grid,    &ctx->d_tmp_ids, &ctx->d_sorted,
-                      &ctx->d_sxy,      &ctx->d_scell,   &ctx->d_part,    &ctx->d_deg,     &ctx->d_off,     &ctx->d_tiles,
-                      &ctx->d_cand,     &ctx->d_ids,   &ctx->d_d2,      &ctx->d_pair_anchor, &ctx->d_trig,
-                      &ctx->d_tdd,      &ctx->d_cls,      &ctx->d_prec,  &ctx->d_psin,  &ctx->d_lists,   &ctx->d_tables,  &ctx->d_st_ids,
-                      &ctx->d_st_meta,  &ctx->d_st_k,  &ctx->d_st_rank, &ctx->d_acc,     &ctx->d_rowoff,  &ctx->d_gbundle,
-                      &ctx->d_chains,   &ctx->d_info,  &ctx->d_dbg,     &ctx->d_map,     &ctx->d_eoff,
-                      &ctx->d_mapbox,   &ctx->d_maprow};
-    for (DevBuf *b : bufs) release(*b);
+    ctx->for_each_pool([](DevBuf &b) { release(b); });
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &pr : ctx->fused_ev)
@@ -2900,8 +3012,13 @@ int cvg_build_neighbor_csr_graph(cvg_context *ctx, int num_edges, int sz_edge_da
         return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "build_neighbor_csr_graph: need num_edges > 0, sz_edge_data >= 4");
     const double *d = nullptr;
     CVG_TRY(upload(ctx, num_edges, sz_edge_data, edges, &d));
+    ctx->use_cellrank = false;
     CVG_TRY(build_graph(ctx, num_edges, sz_edge_data, d, rad, neighbor_radius, max_candidates));
-    CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CVG_TRY(read_scalars(ctx));
+    if (ctx->mailbox[S_AMBIG] > 0) {
+        CVG_TRY(rebuild_graph_exact(ctx));
+        CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     return CVG_OK;
 }
 
@@ -2910,6 +3027,7 @@ int cvg_get_graph_info(cvg_context *ctx, cvg_graph_info *info)
     CVG_TRY(check_ctx(ctx));
     if (!info) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "null info");
     if (!ctx->graph_ready) return fail(ctx, CVG_ERR_STATE, "no neighbour graph");
+    CVG_TRY(finish_curvelets(ctx));
     info->num_edges = ctx->n;
     info->total_neighbor_pairs = ctx->pairs;
     info->max_num_of_neighbors = ctx->max_deg;
@@ -2925,6 +3043,7 @@ int cvg_get_csr(cvg_context *ctx, int *offsets, int *ids, double *dist2)
 {
     CVG_TRY(check_ctx(ctx));
     if (!ctx->graph_ready) return fail(ctx, CVG_ERR_STATE, "no neighbour graph");
+    CVG_TRY(finish_curvelets(ctx));
     const size_t P = ctx->pairs > 0 ? static_cast<size_t>(ctx->pairs) : 0;
     return download(ctx, {{offsets, ctx->d_off.ptr, static_cast<size_t>(ctx->n + 1) * sizeof(int)},
                           {ids, ctx->d_ids.ptr, P * sizeof(int)},
@@ -3268,6 +3387,17 @@ int cvg_debug_anchor_bundles(cvg_context *ctx, const cvg_params *params, int anc
         const bool close = h[static_cast<size_t>(2) * deg * K + deg + r] != 0.0;
         if (dir) dir[r] = static_cast<unsigned char>(h[static_cast<size_t>(2) * deg * K + r]);
         if (hyp) hyp[r] = (close || valid) ? 1 : 0;
+    }
+    return CVG_OK;
+}
+
+int cvg_debug_cell_order(int num_edges, const double *x, const double *y, int *rank)
+{
+    if (num_edges < 0 || (num_edges > 0 && (!x || !y || !rank))) return CVG_ERR_INVALID_ARGUMENT;
+    try {
+        cell_ranks(num_edges, x, y, rank);
+    } catch (...) {
+        return CVG_ERR_OUT_OF_MEMORY;
     }
     return CVG_OK;
 }

@@ -1,6 +1,7 @@
 // Tie-exact replay of libstdc++'s std::sort (GCC 11.4, bits/stl_algo.h and
-// bits/stl_heap.h) for (distance, id) candidate lists, usable on the host and
-// on the device.
+// bits/stl_heap.h), usable on the host and on the device, for any element
+// type and comparator: the neighbour lists' (distance, id) candidates and the
+// spatial index's permutation of edge ids by cell key.
 //
 // cpu_curvelet orders each anchor's neighbours with an unstable std::sort by
 // squared distance (preprocess.hpp:172-176).  When two candidates have exactly
@@ -10,7 +11,9 @@
 // partition, depth limit 2*floor(log2 n), heap sort once the limit is spent,
 // and a final insertion sort with threshold 16.  Distinct distances need none
 // of this (any sort gives the unique order); the caller uses it only for lists
-// that contain ties.
+// that contain ties.  The spatial index (preprocess.hpp:90-92) sorts the edge
+// ids by their packed cell key the same way, which fixes the order of the
+// edges inside one pixel cell; the host replays it when that order matters.
 #ifndef CVG_STLSORT_HPP
 #define CVG_STLSORT_HPP
 
@@ -25,11 +28,15 @@ struct Cand {
     int id;
 };
 
-CVG_SORT_HD bool cand_less(const Cand &a, const Cand &b) { return a.d < b.d; }
+// The comparator of the neighbour sort, preprocess.hpp:172-176.
+struct CandLess {
+    CVG_SORT_HD bool operator()(const Cand &a, const Cand &b) const { return a.d < b.d; }
+};
 
-CVG_SORT_HD void cand_swap(Cand *a, Cand *b)
+template <class T>
+CVG_SORT_HD void elem_swap(T *a, T *b)
 {
-    const Cand t = *a;
+    const T t = *a;
     *a = *b;
     *b = t;
 }
@@ -45,10 +52,11 @@ CVG_SORT_HD int floor_log2(int n)
 }
 
 // __push_heap
-CVG_SORT_HD void heap_push(Cand *first, int hole, int top, Cand value)
+template <class T, class Less>
+CVG_SORT_HD void heap_push(T *first, int hole, int top, T value, const Less &less)
 {
     int parent = (hole - 1) / 2;
-    while (hole > top && cand_less(first[parent], value)) {
+    while (hole > top && less(first[parent], value)) {
         first[hole] = first[parent];
         hole = parent;
         parent = (hole - 1) / 2;
@@ -57,13 +65,14 @@ CVG_SORT_HD void heap_push(Cand *first, int hole, int top, Cand value)
 }
 
 // __adjust_heap
-CVG_SORT_HD void heap_adjust(Cand *first, int hole, int len, Cand value)
+template <class T, class Less>
+CVG_SORT_HD void heap_adjust(T *first, int hole, int len, T value, const Less &less)
 {
     const int top = hole;
     int child = hole;
     while (child < (len - 1) / 2) {
         child = 2 * (child + 1);
-        if (cand_less(first[child], first[child - 1])) child--;
+        if (less(first[child], first[child - 1])) child--;
         first[hole] = first[child];
         hole = child;
     }
@@ -72,65 +81,69 @@ CVG_SORT_HD void heap_adjust(Cand *first, int hole, int len, Cand value)
         first[hole] = first[child - 1];
         hole = child - 1;
     }
-    heap_push(first, hole, top, value);
+    heap_push(first, hole, top, value, less);
 }
 
 // __make_heap followed by __sort_heap (what __partial_sort(first, last, last) does)
-CVG_SORT_HD void heap_sort_range(Cand *first, int len)
+template <class T, class Less>
+CVG_SORT_HD void heap_sort_range(T *first, int len, const Less &less)
 {
     if (len >= 2) {
         int parent = (len - 2) / 2;
         while (true) {
-            heap_adjust(first, parent, len, first[parent]);
+            heap_adjust(first, parent, len, first[parent], less);
             if (parent == 0) break;
             parent--;
         }
     }
     for (int last = len; last > 1;) {
         --last;
-        const Cand v = first[last];
+        const T v = first[last];
         first[last] = first[0];
-        heap_adjust(first, 0, last, v);
+        heap_adjust(first, 0, last, v, less);
     }
 }
 
 // __move_median_to_first
-CVG_SORT_HD void median_to_first(Cand *result, Cand *a, Cand *b, Cand *c)
+template <class T, class Less>
+CVG_SORT_HD void median_to_first(T *result, T *a, T *b, T *c, const Less &less)
 {
-    if (cand_less(*a, *b)) {
-        if (cand_less(*b, *c))
-            cand_swap(result, b);
-        else if (cand_less(*a, *c))
-            cand_swap(result, c);
+    if (less(*a, *b)) {
+        if (less(*b, *c))
+            elem_swap(result, b);
+        else if (less(*a, *c))
+            elem_swap(result, c);
         else
-            cand_swap(result, a);
-    } else if (cand_less(*a, *c))
-        cand_swap(result, a);
-    else if (cand_less(*b, *c))
-        cand_swap(result, c);
+            elem_swap(result, a);
+    } else if (less(*a, *c))
+        elem_swap(result, a);
+    else if (less(*b, *c))
+        elem_swap(result, c);
     else
-        cand_swap(result, b);
+        elem_swap(result, b);
 }
 
 // __unguarded_partition
-CVG_SORT_HD Cand *unguarded_partition(Cand *first, Cand *last, const Cand *pivot)
+template <class T, class Less>
+CVG_SORT_HD T *unguarded_partition(T *first, T *last, const T *pivot, const Less &less)
 {
     while (true) {
-        while (cand_less(*first, *pivot)) ++first;
+        while (less(*first, *pivot)) ++first;
         --last;
-        while (cand_less(*pivot, *last)) --last;
+        while (less(*pivot, *last)) --last;
         if (!(first < last)) return first;
-        cand_swap(first, last);
+        elem_swap(first, last);
         ++first;
     }
 }
 
 // __unguarded_linear_insert
-CVG_SORT_HD void linear_insert(Cand *last)
+template <class T, class Less>
+CVG_SORT_HD void linear_insert(T *last, const Less &less)
 {
-    const Cand val = *last;
-    Cand *next = last - 1;
-    while (cand_less(val, *next)) {
+    const T val = *last;
+    T *next = last - 1;
+    while (less(val, *next)) {
         *last = *next;
         last = next;
         --next;
@@ -139,47 +152,49 @@ CVG_SORT_HD void linear_insert(Cand *last)
 }
 
 // __insertion_sort
-CVG_SORT_HD void insertion_sort(Cand *first, Cand *last)
+template <class T, class Less>
+CVG_SORT_HD void insertion_sort(T *first, T *last, const Less &less)
 {
     if (first == last) return;
-    for (Cand *i = first + 1; i != last; ++i) {
-        if (cand_less(*i, *first)) {
-            const Cand val = *i;
-            for (Cand *m = i; m != first; --m) *m = *(m - 1);
+    for (T *i = first + 1; i != last; ++i) {
+        if (less(*i, *first)) {
+            const T val = *i;
+            for (T *m = i; m != first; --m) *m = *(m - 1);
             *first = val;
         } else {
-            linear_insert(i);
+            linear_insert(i, less);
         }
     }
 }
 
-// std::sort(first, first + n, [](a, b) { return a.d < b.d; }).  The recursion
-// of __introsort_loop is replaced by an explicit stack: segments are disjoint
-// and each one's processing depends only on its own contents and depth budget,
-// so the visiting order does not change the result.
-CVG_SORT_HD void stl_sort(Cand *first, int n)
+// std::sort(first, first + n, less).  The recursion of __introsort_loop is
+// replaced by an explicit stack: segments are disjoint and each one's
+// processing depends only on its own contents and depth budget, so the
+// visiting order does not change the result.
+template <class T, class Less>
+CVG_SORT_HD void stl_sort(T *first, int n, const Less &less)
 {
     if (n <= 0) return;
     const int threshold = 16;
     struct Seg {
         int lo, hi, depth;
     };
-    Seg stack[48];  // depth limit 2*floor(log2 n) + 1 entries; n < 2^23
+    Seg stack[64];  // at most depth limit 2*floor(log2 n) + 1 <= 61 entries
     int sp = 0;
     stack[sp++] = {0, n, 2 * floor_log2(n)};
     while (sp > 0) {
         Seg s = stack[--sp];
         while (s.hi - s.lo > threshold) {
             if (s.depth == 0) {
-                heap_sort_range(first + s.lo, s.hi - s.lo);
+                heap_sort_range(first + s.lo, s.hi - s.lo, less);
                 break;
             }
             --s.depth;
-            Cand *f = first + s.lo;
-            Cand *l = first + s.hi;
-            Cand *mid = f + (l - f) / 2;
-            median_to_first(f, f + 1, mid, l - 1);
-            Cand *cut = unguarded_partition(f + 1, l, f);
+            T *f = first + s.lo;
+            T *l = first + s.hi;
+            T *mid = f + (l - f) / 2;
+            median_to_first(f, f + 1, mid, l - 1, less);
+            T *cut = unguarded_partition(f + 1, l, f, less);
             const int c = static_cast<int>(cut - first);
             stack[sp++] = {c, s.hi, s.depth};
             s.hi = c;
@@ -187,12 +202,15 @@ CVG_SORT_HD void stl_sort(Cand *first, int n)
     }
     // __final_insertion_sort
     if (n > threshold) {
-        insertion_sort(first, first + threshold);
-        for (Cand *i = first + threshold; i != first + n; ++i) linear_insert(i);
+        insertion_sort(first, first + threshold, less);
+        for (T *i = first + threshold; i != first + n; ++i) linear_insert(i, less);
     } else {
-        insertion_sort(first, first + n);
+        insertion_sort(first, first + n, less);
     }
 }
+
+// The neighbour sort by squared distance (preprocess.hpp:172-176).
+CVG_SORT_HD void stl_sort(Cand *first, int n) { stl_sort(first, n, CandLess{}); }
 
 }  // namespace cvg
 

@@ -66,8 +66,11 @@ typedef struct cvg_graph_info {
     unsigned truncated_anchors;
     /* anchors whose neighbour order depends on the within-cell order that
      * libstdc++'s unstable std::sort gives the reference's spatial index
-     * (exact distance ties inside one pixel cell, or a max_candidates cut
-     * inside one cell).  The library orders such cells by edge id. */
+     * (exact distance ties inside one pixel cell, a max_candidates cut inside
+     * one cell, or ties in a list of more than 16 that holds two candidates
+     * of one cell).  When there are any, the library replays that sort on the
+     * host and builds the graph again in the reference's order, so the
+     * results stay exact; the count is diagnostic. */
     unsigned order_ambiguous_anchors;
 } cvg_graph_info;
 
@@ -315,6 +318,11 @@ int cvg_synth_generate(const cvg_synth_config *config, unsigned long long seed, 
  * the direction filter (:459-461). */
 int cvg_debug_anchor_bundles(cvg_context *ctx, const cvg_params *params, int anchor, unsigned char *dir,
                              unsigned char *hyp, double *bmin, double *bmax, int *deg, int *cells);
+/* The within-cell order the library replays for order-ambiguous maps: for
+ * each edge, its position inside its pixel cell after the reference's
+ * std::sort of the edge ids by packed cell key (build_spatial_index_cpu,
+ * preprocess.hpp:78-92), restated as libstdc++'s introsort.  Host only. */
+int cvg_debug_cell_order(int num_edges, const double *x, const double *y, int *rank);
 /* Evaluate the library's correctly rounded math on n arguments.
  * op: 0 sin(x), 1 cos(x), 2 atan2(y, x), 3 sqrt(x), 4 To2Pi(x), 5 fmod(x, y),
  * 6 and 7 the sine and cosine of the joint sincos(x), 8 sin(x - y) as the

@@ -69,6 +69,7 @@ class Oracle:
         sig = {
             "orc_dt_from_deg": (C.c_double, [C.c_double]),
             "orc_create": (C.c_void_p, []),
+            "orc_spatial_order": (None, [C.c_int, _dp, _dp, _ip]),
             "orc_destroy": (None, [C.c_void_p]),
             "orc_build_csr": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _P(_OParams), C.c_int]),
             "orc_csr_counts": (None, [C.c_void_p, _ip, _up, _up]),
@@ -96,6 +97,14 @@ class Oracle:
             self.lib.orc_destroy(self.h)
         except Exception:
             pass
+
+    def spatial_order(self, x: np.ndarray, y: np.ndarray) -> np.ndarray:
+        """The reference's sorted_edge_ids (std::sort of ids by packed cell key)."""
+        xx = np.ascontiguousarray(x, dtype=np.float64)
+        yy = np.ascontiguousarray(y, dtype=np.float64)
+        out = np.zeros(len(xx), np.int32)
+        self.lib.orc_spatial_order(len(xx), _f(xx, C.c_double), _f(yy, C.c_double), _f(out, C.c_int))
+        return out
 
     def params(self, p) -> _OParams:
         """From a CurveletParams-like object (dt in degrees)."""

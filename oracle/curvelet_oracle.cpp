@@ -436,6 +436,17 @@ extern "C" {
 
 double orc_dt_from_deg(double dt_deg) { return (dt_deg / 180.0) * kPi; }
 
+// build_spatial_index_cpu's sorted_edge_ids (preprocess.hpp:78-96): the edge
+// ids by packed cell key under libstdc++'s std::sort, the same call as above.
+void orc_spatial_order(int n, const double *x, const double *y, int *sorted_ids)
+{
+    std::vector<int64_t> keys(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i)
+        keys[i] = cell_key(static_cast<int>(std::lrint(x[i])), static_cast<int>(std::lrint(y[i])));
+    std::iota(sorted_ids, sorted_ids + n, 0);
+    std::sort(sorted_ids, sorted_ids + n, [&](int a, int b) { return keys[a] < keys[b]; });
+}
+
 void *orc_create(void) { return new Handle(); }
 void orc_destroy(void *h) { delete static_cast<Handle *>(h); }
 

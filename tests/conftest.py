@@ -16,6 +16,21 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: longer CPU test")
 
 
+def pytest_report_header(config):
+    """The host's C library and CPU beside every run: the oracle's glibc sin,
+    cos and atan2 are what the reference's rounding depends on."""
+    import platform
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return [f"glibc: {' '.join(platform.libc_ver())}; cpu: {model}"]
+
+
 def gpu_available() -> bool:
     try:
         import torch  # noqa: F401  (device count without initialising HIP)

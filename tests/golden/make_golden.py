@@ -20,6 +20,12 @@ Writes, under tests/golden/:
                                sampled neighbour pairs and constructed geometries
 The inputs are the reference's shipped edge files (inputs/*.txt.gz) and
 synthetic maps from the package's generator (inputs/<case>.npy, written here).
+
+    python tests/golden/make_golden.py --scale
+
+adds the full-size fixtures (scale_fixtures): correctly rounded oracle hashes
+and sampled reference rows for the 1080p map, and the reference's counts,
+chain hash and sampled rows for a 3840x2160 map.
 """
 from __future__ import annotations
 
@@ -42,7 +48,7 @@ sys.path.insert(0, str(ROOT))
 
 from oracle import REF_CLI, Ref  # noqa: E402
 from curvelet_construction_on_gpu_amd import CurveletParams, synth_edges  # noqa: E402
-from tests._inputs import shipped_edges, shipped_text  # noqa: E402
+from tests._inputs import UHD_MAP, shipped_edges, shipped_text, uhd_edges  # noqa: E402
 
 # (case name, input, group_max_sz, keep full text, keep full-precision arrays)
 CASES = [
@@ -128,6 +134,51 @@ def constructed_geometries():
     return g
 
 
+def scale_fixtures() -> None:
+    """Fixtures for parity at full size, where whole outputs are too large to
+    keep: for the 1080p map (G = 7) the SHA-256 of the chains and info that
+    the oracle with correctly rounded math gives (bit-exact target of the GPU
+    path) and 4,096 sampled rows of the reference's own full-precision info;
+    for the 3840x2160 map (G = 4) the reference's counts and chain SHA-256 and
+    the same correctly rounded oracle hashes."""
+    from oracle import Oracle
+    ref = Ref()
+    cr = Oracle(cr=True)
+    rng = np.random.default_rng(20261203)
+    # 1080p map
+    meta_f = HERE / "ref_cli" / "c3s1003_G7.json"
+    meta = json.loads(meta_f.read_text())
+    e = edges_of(meta["input"])
+    p = CurveletParams(group_max_sz=meta["group_max_sz"])
+    r = ref.run(e, p, nthreads=8, valid_pairs=False)
+    assert sha(np.ascontiguousarray(r["chains"]).tobytes()) == meta["full_chain_sha256"]
+    w = cr.run(e, p, nthreads=8, valid_pairs=False)
+    assert np.array_equal(w["chains"], r["chains"])
+    meta["cr_chain_sha256"] = sha(np.ascontiguousarray(w["chains"]).tobytes())
+    meta["cr_info_sha256"] = sha(np.ascontiguousarray(w["info"]).tobytes())
+    rows = np.unique(np.concatenate([[0, len(r["info"]) - 1], rng.choice(len(r["info"]), 4094, replace=False)]))
+    np.savez_compressed(HERE / "full" / "c3s1003_G7_rows.npz", rows=rows, chains=r["chains"][rows],
+                        info=r["info"][rows])
+    meta_f.write_text(json.dumps(meta, indent=1) + "\n")
+    print("c3s1003_G7: cr hashes and", len(rows), "sampled rows")
+    # 3840x2160 map
+    e = uhd_edges()
+    p = CurveletParams(group_max_sz=4)
+    r = ref.run(e, p, nthreads=8, valid_pairs=True)
+    w = cr.run(e, p, nthreads=8, valid_pairs=False)
+    assert np.array_equal(w["chains"], r["chains"])
+    rec = {"map": UHD_MAP, "group_max_sz": 4, "edges": int(len(e)), "pairs": int(r["csr"]["pairs"]),
+           "max_deg": int(r["csr"]["max_deg"]), "valid_pairs": int(r["valid_pairs"]), "out_h": int(len(r["chains"])),
+           "out_w": int(r["chains"].shape[1]), "input_sha256": sha(np.ascontiguousarray(e).tobytes()),
+           "full_chain_sha256": sha(np.ascontiguousarray(r["chains"]).tobytes()),
+           "cr_info_sha256": sha(np.ascontiguousarray(w["info"]).tobytes())}
+    rows = np.unique(np.concatenate([[0, len(r["info"]) - 1], rng.choice(len(r["info"]), 4094, replace=False)]))
+    np.savez_compressed(HERE / "full" / "uhd7001_G4_rows.npz", rows=rows, chains=r["chains"][rows],
+                        info=r["info"][rows])
+    (HERE / "ref_cli" / "uhd7001_G4.json").write_text(json.dumps(rec, indent=1) + "\n")
+    print("uhd7001_G4:", rec["edges"], "edges,", rec["out_h"], "rows")
+
+
 def main() -> int:
     if not REF_CLI.exists() or not Ref.available():
         print("oracle/_ref is missing: run `make -C oracle ref` where the reference sources exist", file=sys.stderr)
@@ -191,4 +242,11 @@ def main() -> int:
 
 
 if __name__ == "__main__":
+    if sys.argv[1:] == ["--scale"]:
+        if not Ref.available():
+            print("oracle/_ref is missing: run `make -C oracle ref` where the reference sources exist",
+                  file=sys.stderr)
+            sys.exit(1)
+        scale_fixtures()
+        sys.exit(0)
     sys.exit(main())

@@ -33,6 +33,15 @@ def test_served_results_equal_alone(ctx, workers):
         assert np.array_equal(ch, ach)
         assert np.array_equal(bits(inf), bits(ainf))
     assert counters["maps"] == len(maps)
+    # a sample of the served results against the oracle (the reference's
+    # semantics with glibc math): chains bit-exact, info within 1e-9
+    from oracle import Oracle
+    orc = Oracle()
+    for k in (1, 9, 11):
+        want = orc.run(maps[k], p)
+        ch, inf, _ = res[k]
+        assert np.array_equal(ch, want["chains"]), k
+        np.testing.assert_allclose(inf, want["info"], rtol=1e-9, atol=1e-12)
     assert max(r["batch_maps"] for _, _, r in res) > 1  # some requests really were served in a mix
     assert counters["batches"] < len(maps)
 

@@ -423,3 +423,34 @@ def test_server_rejects_bad_config_without_a_device():
     assert lib.cvg_server_create(0, C.byref(p), C.byref(bad), C.byref(out)) == 1
     assert lib.cvg_server_submit(None, 1, 4, None, None) == 1
     lib.cvg_server_destroy(None)
+
+
+@pytest.mark.parametrize("n,per_px,seed", [(5, 3, 1), (16, 4, 2), (17, 4, 3), (300, 2, 4), (50000, 3, 5),
+                                           (200000, 1, 6)])
+def test_cell_order_replay_matches_std_sort(n, per_px, seed):
+    """The within-cell order the library replays for order-ambiguous maps is
+    the one libstdc++'s std::sort gives the reference's spatial index
+    (preprocess.hpp:78-92): ranks from cvg_debug_cell_order against ranks
+    from the oracle's own std::sort call, on maps with many shared cells,
+    negative coordinates and half-integer points (lrint ties)."""
+    from curvelet_construction_on_gpu_amd import cell_order_ranks
+    from oracle import Oracle
+    rng = np.random.default_rng(seed)
+    side = max(2, int(np.sqrt(n / per_px)))
+    x = rng.uniform(-side / 2, side / 2, n)
+    y = rng.uniform(-side / 3, side, n)
+    x[::5] = np.round(x[::5]) + 0.5
+    y[::7] = np.round(y[::7]) - 0.5
+    order = Oracle().spatial_order(x, y)
+    key = (np.rint(y).astype(np.int64) << 32) | (np.rint(x).astype(np.int64) & 0xFFFFFFFF)
+    want = np.zeros(n, np.int32)
+    k = key[order]
+    start = np.concatenate([[0], np.flatnonzero(k[1:] != k[:-1]) + 1])
+    run_of = np.repeat(start, np.diff(np.concatenate([start, [n]])))
+    want[order] = np.arange(n) - run_of
+    got = cell_order_ranks(x, y)
+    assert np.array_equal(got, want)
+    if n > 16:
+        # the reference's order really differs from edge-id order here
+        ids_order = np.argsort(key, kind="stable")
+        assert not np.array_equal(order, ids_order)

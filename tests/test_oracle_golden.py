@@ -19,8 +19,8 @@ import pytest
 from curvelet_construction_on_gpu_amd import CurveletParams, write_chain_file, write_info_file
 from tests._inputs import GOLDEN, edges_for
 
-CASES = sorted(p.stem for p in (GOLDEN / "ref_cli").glob("*.json"))
-FULL = sorted(p.stem for p in (GOLDEN / "full").glob("*.npz"))
+CASES = sorted(p.stem for p in (GOLDEN / "ref_cli").glob("*.json") if "input" in json.loads(p.read_text()))
+FULL = sorted(p.stem for p in (GOLDEN / "full").glob("*.npz") if not p.stem.endswith("_rows"))
 TEXT = sorted(p.name[:-len("_chain.txt.gz")] for p in (GOLDEN / "ref_cli").glob("*_chain.txt.gz"))
 
 
@@ -175,3 +175,32 @@ def test_live_reference(oracle, case):
     assert got["valid_pairs"] == want["valid_pairs"]
     assert np.array_equal(got["chains"], want["chains"])
     assert np.array_equal(got["info"].view(np.uint64), want["info"].view(np.uint64))
+
+
+def test_full_size_fixtures(oracle, oracle_cr):
+    """The full-size fixtures the GPU tests check against: the 1080p map's
+    correctly rounded oracle hashes and sampled reference rows, and the
+    3840x2160 map's reference chain hash, counts and sampled rows (the glibc
+    oracle reproduces the reference's rows bit for bit)."""
+    from tests._inputs import uhd_edges
+    meta = case_meta("c3s1003_G7")
+    e = case_edges(meta)
+    p = CurveletParams(group_max_sz=meta["group_max_sz"])
+    w = oracle_cr.run(e, p, nthreads=0, valid_pairs=False)
+    assert sha(np.ascontiguousarray(w["chains"]).tobytes()) == meta["full_chain_sha256"] == meta["cr_chain_sha256"]
+    assert sha(np.ascontiguousarray(w["info"]).tobytes()) == meta["cr_info_sha256"]
+    rows = np.load(GOLDEN / "full" / "c3s1003_G7_rows.npz")
+    np.testing.assert_allclose(w["info"][rows["rows"]], rows["info"], rtol=1e-9, atol=1e-12)
+    um = json.loads((GOLDEN / "ref_cli" / "uhd7001_G4.json").read_text())
+    e = uhd_edges()
+    assert sha(np.ascontiguousarray(e).tobytes()) == um["input_sha256"]
+    p = CurveletParams(group_max_sz=um["group_max_sz"])
+    r = oracle.run(e, p, nthreads=0)
+    assert (r["csr"]["pairs"], r["csr"]["max_deg"], r["valid_pairs"]) == (um["pairs"], um["max_deg"],
+                                                                          um["valid_pairs"])
+    assert sha(np.ascontiguousarray(r["chains"]).tobytes()) == um["full_chain_sha256"]
+    rows = np.load(GOLDEN / "full" / "uhd7001_G4_rows.npz")
+    assert np.array_equal(r["chains"][rows["rows"]], rows["chains"])
+    assert np.array_equal(r["info"][rows["rows"]].view(np.uint64), rows["info"].view(np.uint64))
+    w = oracle_cr.run(e, p, nthreads=0, valid_pairs=False)
+    assert sha(np.ascontiguousarray(w["info"]).tobytes()) == um["cr_info_sha256"]
