@@ -116,6 +116,8 @@ _SIGNATURES = {
     "cvg_get_curvelets": (C.c_int, [C.c_void_p, _up, _dp]),
     "cvg_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _P(_Params), _P(_RunStats)]),
     "cvg_run_device": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, _P(_Params), _P(_RunStats)]),
+    "cvg_run_device_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, _P(_Params), C.c_void_p,
+                                         C.c_void_p, C.c_uint, _up, _up]),
     "cvg_upload_edges": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _P(C.c_void_p)]),
     "cvg_device_curvelets": (C.c_int, [C.c_void_p, _P(C.c_void_p), _P(C.c_void_p)]),
     "cvg_get_stream": (C.c_int, [C.c_void_p, _P(C.c_void_p)]),
@@ -351,6 +353,18 @@ class Context:
         self.check(self._lib.cvg_run_device(self._h, n, sz, C.c_void_p(dptr), C.byref(p),
                                             C.byref(st) if stats else None), "run_device")
         return {k: getattr(st, k) for k, _ in _RunStats._fields_} if stats else None
+
+    def run_device_noalloc(self, dptr: int, n: int, sz: int, params: CurveletParams, chains_dptr: int,
+                           info_dptr: int, max_rows: int) -> tuple[int, int]:
+        """cvg_run_device_noalloc: intermediate buffers in the library's static
+        device scratch array, chains and info into caller device buffers of
+        max_rows rows.  Returns (num_curvelets, valid_pairs)."""
+        p = params._c()
+        nc, vp = C.c_uint(), C.c_uint()
+        self.check(self._lib.cvg_run_device_noalloc(self._h, n, sz, C.c_void_p(dptr), C.byref(p),
+                                                    C.c_void_p(chains_dptr), C.c_void_p(info_dptr), max_rows,
+                                                    C.byref(nc), C.byref(vp)), "run_device_noalloc")
+        return nc.value, vp.value
 
     def recent_fused_ms(self, count: int) -> list[float]:
         """Durations (ms, HIP events) of the fused kernel in the last runs, oldest first (at most 64)."""

@@ -40,6 +40,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <initializer_list>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -56,11 +57,19 @@ using cvg::PairRec;
 __constant__ double c_sincos_tab[4 * CVG_SIN_ROWS] = CVG_SINCOS_TABLE_INIT;
 __constant__ double c_atan_tab[2 * CVG_ATAN_ROWS] = CVG_ATAN_TABLE_INIT;
 
+// Device memory of cvg_run_device_noalloc: every intermediate buffer of a
+// no-alloc run is carved out of this array.  Host code never uses the name
+// itself as a device address: each device's copy is resolved once with
+// hipGetSymbolAddress (scratch_address) and that pointer is what kernels get.
+static __device__ __align__(256) unsigned char scratch[static_cast<size_t>(CVG_NOALLOC_SCRATCH_MIB) << 20];
+
 namespace {
 
 constexpr int kBlock = 256;
 constexpr int kInfoWidth = 10;
 constexpr long long kMaxGridCells = 1LL << 28;
+// the static scratch array behind cvg_run_device_noalloc (one per device)
+constexpr size_t kScratchBytes = static_cast<size_t>(CVG_NOALLOC_SCRATCH_MIB) << 20;
 
 // ---------------------------------------------------------------------------
 // Context
@@ -147,7 +156,18 @@ struct cvg_context {
     // diagnostics
     DevBuf d_dbg;
 
-    // every device pool of the context
+    // this device's copy of the static scratch array (cvg_run_device_noalloc);
+    // while `arena` is set, ensure() carves buffers out of it instead of
+    // allocating
+    char *scratch_base = nullptr;
+    bool arena = false;
+    size_t arena_used = 0;
+    // caller-owned device outputs of a no-alloc run (null otherwise)
+    unsigned *caller_chains = nullptr;
+    double *caller_info = nullptr;
+    unsigned caller_rows = 0;
+
+    // every device pool of the context, for release and for the no-alloc swap
     template <typename F>
     void for_each_pool(F f)
     {
@@ -186,6 +206,19 @@ int ensure(cvg_context *ctx, DevBuf &b, size_t bytes)
 {
     if (bytes == 0) bytes = 8;
     if (b.bytes >= bytes) return CVG_OK;
+    if (ctx->arena) {
+        // no-alloc mode: carve from the static scratch array, 256-byte aligned
+        const size_t need = (bytes + 255) & ~static_cast<size_t>(255);
+        if (ctx->arena_used + need > kScratchBytes)
+            return fail(ctx, CVG_ERR_UNSUPPORTED,
+                        "cvg_run_device_noalloc: this map needs more than the " +
+                            std::to_string(kScratchBytes >> 20) +
+                            " MiB static scratch array (use cvg_run_device for larger maps)");
+        b.ptr = ctx->scratch_base + ctx->arena_used;
+        b.bytes = bytes;
+        ctx->arena_used += need;
+        return CVG_OK;
+    }
     if (b.ptr) {
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipFree(b.ptr);
@@ -1836,6 +1869,7 @@ struct RecordArgs {
     const double *off_x, *off_t;
     unsigned *chains;
     double *info;
+    unsigned max_rows;  // rows the output buffers hold; rows beyond are not written
 };
 
 // record_curvelet_chain + fill_curvelet_info (cpu_curvelet.hpp:197-280) for
@@ -1848,6 +1882,7 @@ __device__ inline void record_slot(const RecordArgs &R, int q, int rk)
 {
     const int a = R.pair_anchor[q];
     const unsigned o = R.row_off[a] + static_cast<unsigned>(rk);
+    if (o >= R.max_rows) return;  // caller buffers too small: reported by the host
     const int id0 = R.map_of ? static_cast<int>(R.map_eoff[R.map_of[a]]) : 0;
     const double ax = R.x[a], ay = R.y[a], at = R.th[a];
     const int W = R.G + 1;
@@ -2649,8 +2684,14 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     // record runs without a host round trip; the counts come back with the
     // stream (finish_curvelets)
     ctx->width = d.G + 1;
-    CVG_TRY(ensure(ctx, ctx->d_chains, np * ctx->width * sizeof(unsigned)));
-    CVG_TRY(ensure(ctx, ctx->d_info, np * kInfoWidth * sizeof(double)));
+    unsigned *out_chains = ctx->caller_chains;
+    double *out_info = ctx->caller_info;
+    if (!out_chains) {
+        CVG_TRY(ensure(ctx, ctx->d_chains, np * ctx->width * sizeof(unsigned)));
+        CVG_TRY(ensure(ctx, ctx->d_info, np * kInfoWidth * sizeof(double)));
+        out_chains = as<unsigned>(ctx->d_chains);
+        out_info = as<double>(ctx->d_info);
+    }
     RecordArgs R{};
     R.pairs = pairs;
     R.pair_anchor = as<int>(ctx->d_pair_anchor);
@@ -2670,8 +2711,9 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     R.nrad = d.nrad;
     R.off_x = T.off_x;
     R.off_t = T.off_t;
-    R.chains = as<unsigned>(ctx->d_chains);
-    R.info = as<double>(ctx->d_info);
+    R.chains = out_chains;
+    R.info = out_info;
+    R.max_rows = ctx->caller_chains ? ctx->caller_rows : UINT_MAX;
     if (pairs > 0)
         hipLaunchKernelGGL(k_record, dim3(blocks_for(pairs, kRecordBlock)), dim3(kRecordBlock), 0, st, R);
     CVG_HIP(ctx, hipGetLastError());
@@ -2751,6 +2793,26 @@ int upload(cvg_context *ctx, int n, int sz, const double *host, const double **d
     CVG_TRY(ensure(ctx, ctx->d_edges_up, bytes));
     CVG_HIP(ctx, hipMemcpyAsync(ctx->d_edges_up.ptr, host, bytes, hipMemcpyHostToDevice, ctx->stream));
     *dptr = as<double>(ctx->d_edges_up);
+    return CVG_OK;
+}
+
+// This device's copy of `scratch`, resolved once per device with that device
+// current (each GPU holds its own copy at its own address).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(cvg_context *ctx)
+{
+    if (ctx->device < 0 || ctx->device >= kMaxDevices) return CVG_OK;  // no-alloc runs unavailable there
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[ctx->device]) {
+        void *ptr = nullptr;
+        CVG_HIP(ctx, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(ctx, CVG_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned null");
+        g_scratch_ptr[ctx->device] = static_cast<char *>(ptr);
+    }
+    ctx->scratch_base = g_scratch_ptr[ctx->device];
     return CVG_OK;
 }
 
@@ -2946,6 +3008,8 @@ int cvg_create(int device, cvg_context **out_ctx)
             rc = CVG_ERR_OUT_OF_MEMORY;
             break;
         }
+        rc = scratch_address(ctx);
+        if (rc != CVG_OK) break;
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
             size_t lim = prop.sharedMemPerBlock;
@@ -3110,6 +3174,58 @@ int cvg_run_device(cvg_context *ctx, int num_edges, int sz_edge_data, const doub
     if (!params || !device_edges || num_edges <= 0 || sz_edge_data < 4)
         return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "cvg_run_device: bad arguments");
     return run_common(ctx, num_edges, sz_edge_data, device_edges, params, stats);
+}
+
+int cvg_run_device_noalloc(cvg_context *ctx, int num_edges, int sz_edge_data, const double *device_edges,
+                           const cvg_params *params, unsigned *device_chains, double *device_info, unsigned max_rows,
+                           unsigned *num_curvelets, unsigned *valid_pairs)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (num_curvelets) *num_curvelets = 0;
+    if (!params || !device_edges || num_edges <= 0 || sz_edge_data < 4 || !device_chains || !device_info ||
+        !num_curvelets)
+        return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "cvg_run_device_noalloc: bad arguments");
+    if (!ctx->scratch_base) return fail(ctx, CVG_ERR_DEVICE, "cvg_run_device_noalloc: no scratch array on this device");
+    // counts of an earlier run still in flight come back before the mailbox is reused
+    CVG_TRY(finish_curvelets(ctx));
+    // the context's own pools step aside (nothing is freed) and every buffer of
+    // this run is carved from the scratch array; the outputs are the caller's
+    constexpr int kMaxPools = 64;
+    DevBuf saved[kMaxPools];
+    int npools = 0;
+    ctx->for_each_pool([&](DevBuf &b) {
+        if (npools < kMaxPools) {
+            saved[npools++] = b;
+            b = DevBuf{};
+        }
+    });
+    ctx->arena = true;
+    ctx->arena_used = 0;
+    ctx->caller_chains = device_chains;
+    ctx->caller_info = device_info;
+    ctx->caller_rows = max_rows;
+    int rc = run_common(ctx, num_edges, sz_edge_data, device_edges, params, nullptr);
+    if (rc == CVG_OK) rc = finish_curvelets(ctx);
+    else (void)hipStreamSynchronize(ctx->stream);
+    ctx->arena = false;
+    ctx->caller_chains = nullptr;
+    ctx->caller_info = nullptr;
+    ctx->caller_rows = 0;
+    npools = 0;
+    ctx->for_each_pool([&](DevBuf &b) {
+        if (npools < kMaxPools) b = saved[npools++];
+    });
+    // the results live in the caller's buffers, not in the context
+    ctx->graph_ready = false;
+    ctx->curvelets_ready = false;
+    if (rc != CVG_OK) return rc;
+    *num_curvelets = ctx->num_curvelets;
+    if (valid_pairs) *valid_pairs = ctx->valid_pairs;
+    if (ctx->num_curvelets > max_rows)
+        return fail(ctx, CVG_ERR_INVALID_ARGUMENT,
+                    "cvg_run_device_noalloc: the output buffers hold " + std::to_string(max_rows) +
+                        " rows; this map has " + std::to_string(ctx->num_curvelets) + " curvelets");
+    return CVG_OK;
 }
 
 // Batches: maps concatenated in edge order; eoff[m] = first edge of map m.

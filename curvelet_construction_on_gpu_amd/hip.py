@@ -1,6 +1,6 @@
 """Device buffers and events through the HIP runtime that libcurvelet_gpu.so
 itself links (ctypes), for callers of the device-pointer entry points
-(cvg_run_device) and for timing on a context's
+(cvg_run_device, cvg_run_device_noalloc) and for timing on a context's
 stream.  The same runtime matters: a second HIP runtime in the process (for
 instance the one a PyTorch wheel bundles) does not share device state with it.
 """

@@ -161,6 +161,30 @@ int cvg_run(cvg_context *ctx, int num_edges, int sz_edge_data, const double *edg
  * cvg_upload_edges).  No host<->device copy of edges or results. */
 int cvg_run_device(cvg_context *ctx, int num_edges, int sz_edge_data, const double *device_edges,
                    const cvg_params *params, cvg_run_stats *stats);
+/* Size of the static device scratch array behind cvg_run_device_noalloc, in
+ * MiB.  Every device the library runs on holds one copy of it until the
+ * process ends. */
+#define CVG_NOALLOC_SCRATCH_MIB 64
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): cvg_run_device
+ * with every intermediate buffer (cell grid, neighbour graph, pair records and
+ * sines, staging rows, counters) carved from the library's static device
+ * scratch array of CVG_NOALLOC_SCRATCH_MIB MiB instead of the context's pools,
+ * and the results written straight to caller device buffers: device_chains
+ * (max_rows x (group_max_sz + 1) unsigned, 1-based ids) and device_info
+ * (max_rows x 10 double, row-major).  Calls no hipMalloc, hipFree or other
+ * allocating routine.  Waits for the run; *num_curvelets receives the row
+ * count (also when it exceeds max_rows, which fails with
+ * CVG_ERR_INVALID_ARGUMENT and leaves the rows beyond max_rows unwritten).  A
+ * map whose buffers do not fit the array fails with CVG_ERR_UNSUPPORTED (about
+ * 10k edges fit at the default parameters and G = 7).  The array is shared by every
+ * context on a device and keeps the last call's contents (the run clears what
+ * must start at zero): calls that overlap on one device race.  The context's
+ * own results of earlier runs are no longer available afterwards. */
+int cvg_run_device_noalloc(cvg_context *ctx, int num_edges, int sz_edge_data, const double *device_edges,
+                           const cvg_params *params, unsigned *device_chains, double *device_info, unsigned max_rows,
+                           unsigned *num_curvelets, unsigned *valid_pairs);
 /* Copy a host edge array into a context-owned device buffer (valid until the
  * next upload or cvg_destroy). */
 int cvg_upload_edges(cvg_context *ctx, int num_edges, int sz_edge_data, const double *edges,
