@@ -30,6 +30,7 @@ __all__ = [
     "CurveletError", "CurveletParams", "NeighborGraph", "CurveletGPU", "Context", "lib_path", "load_library",
     "read_TO_edges_from_file", "build_neighbor_csr_graph", "run_curvelet", "synth_edges", "synth_preset",
     "math_eval", "write_chain_file", "write_info_file", "chain_to_info_filename", "dt_radians", "pinned_empty",
+    "read_TO_edges_from_file_f32",
 ]
 
 _PKG = Path(__file__).resolve().parent
@@ -116,6 +117,10 @@ _SIGNATURES = {
     "cvg_get_curvelets": (C.c_int, [C.c_void_p, _up, _dp]),
     "cvg_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _P(_Params), _P(_RunStats)]),
     "cvg_run_device": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, _P(_Params), _P(_RunStats)]),
+    "cvg_run_f32": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _P(C.c_float), _P(_Params), _P(_RunStats)]),
+    "cvg_build_neighbor_csr_graph_f32": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _P(C.c_float), C.c_double,
+                                                   C.c_uint, C.c_uint]),
+    "cvg_read_edge_file_f32": (C.c_int, [C.c_char_p, C.c_int, _P(_P(C.c_float)), _ip, C.c_char_p, C.c_size_t]),
     "cvg_run_device_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, _P(_Params), C.c_void_p,
                                          C.c_void_p, C.c_uint, _up, _up]),
     "cvg_upload_edges": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _P(C.c_void_p)]),
@@ -337,6 +342,25 @@ class Context:
         self.check(self._lib.cvg_run(self._h, n, sz, _ptr(e, C.c_double), C.byref(p),
                                      C.byref(st) if stats else None), "run")
         return {k: getattr(st, k) for k, _ in _RunStats._fields_} if stats else None
+
+    def run_f32(self, edges: np.ndarray, params: CurveletParams, stats: bool = True) -> dict | None:
+        """run_curvelet<float> (--float): edges as float32, the reference's float arithmetic."""
+        e = np.ascontiguousarray(edges, dtype=np.float32)
+        n, sz = e.shape
+        st = _RunStats()
+        p = params._c()
+        self.check(self._lib.cvg_run_f32(self._h, n, sz, _ptr(e, C.c_float), C.byref(p),
+                                         C.byref(st) if stats else None), "run_f32")
+        return {k: getattr(st, k) for k, _ in _RunStats._fields_} if stats else None
+
+    def build_graph_f32(self, edges: np.ndarray, rad: float, neighbor_radius: int = 3,
+                        max_candidates: int = 128) -> NeighborGraph:
+        e = np.ascontiguousarray(edges, dtype=np.float32)
+        n, sz = e.shape
+        self.check(self._lib.cvg_build_neighbor_csr_graph_f32(self._h, n, sz, _ptr(e, C.c_float), rad,
+                                                              neighbor_radius, max_candidates),
+                   "build_neighbor_csr_graph_f32")
+        return self.graph(edges=e)
 
     def upload(self, edges: np.ndarray) -> int:
         e = np.ascontiguousarray(edges, dtype=np.float64)
@@ -657,6 +681,22 @@ def read_TO_edges_from_file(path: str | os.PathLike, values_per_edge: int = 4) -
     return out.reshape(n.value, values_per_edge)
 
 
+def read_TO_edges_from_file_f32(path: str | os.PathLike, values_per_edge: int = 4) -> np.ndarray:
+    """data_io.hpp:20-78 on T = float: decimal text straight to float32."""
+    lib = load_library()
+    buf = _P(C.c_float)()
+    n = C.c_int()
+    err = C.create_string_buffer(1024)
+    rc = lib.cvg_read_edge_file_f32(os.fsencode(str(path)), values_per_edge, C.byref(buf), C.byref(n), err, 1024)
+    if rc != 0:
+        raise CurveletError(err.value.decode(), rc)
+    try:
+        out = np.ctypeslib.as_array(buf, shape=(n.value * values_per_edge,)).copy()
+    finally:
+        lib.cvg_free(buf)
+    return out.reshape(n.value, values_per_edge)
+
+
 def write_chain_file(path, chains: np.ndarray) -> None:
     lib = load_library()
     c = np.ascontiguousarray(chains, dtype=np.uint32)
@@ -711,7 +751,8 @@ def synth_edges(preset: str | _SynthConfig, seed: int, **overrides) -> np.ndarra
 
 MATH_OPS = {"sin": 0, "cos": 1, "atan2": 2, "sqrt": 3, "to2pi": 4, "fmod": 5, "sincos_sin": 6, "sincos_cos": 7,
             "sin_difference": 8, "sin_difference_settled": 9, "sincos_dd_sin_hi": 10, "sincos_dd_sin_lo": 11,
-            "sincos_dd_cos_hi": 12, "sincos_dd_cos_lo": 13}
+            "sincos_dd_cos_hi": 12, "sincos_dd_cos_lo": 13, "sinf": 14, "cosf": 15, "atan2f": 16, "to2pi_f": 17,
+            "sqrtf": 18}
 
 
 def math_eval(op: str, x: np.ndarray, y: np.ndarray | None = None, ctx: Context | None = None,

@@ -7,7 +7,9 @@
 // (data_io.hpp:10-13); chain and info files written to "../outputs/" in the
 // reference's text format (data_io.hpp:80-145).  Additive: --device <N>
 // selects the HIP device.  --nthreads is accepted and has no effect (there is
-// no CPU path).  --float is rejected: the GPU path computes in double only.
+// no CPU path).  --float runs the reference's float instantiation
+// (run_curvelet<float>): float edges, float arithmetic and its double
+// promotions, info widened to double.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -232,7 +234,7 @@ int fail_ctx(cvg_context *ctx, const char *what)
 
 int run(const Options &o)
 {
-    std::cout << "Using scalar type: double" << std::endl;
+    std::cout << "Using scalar type: " << (o.use_double ? "double" : "float") << std::endl;
     const std::string path = "../test_files/" + o.edge_file;  // data_io.hpp:10-13
     if (FILE *f = std::fopen(path.c_str(), "rb")) {
         std::fclose(f);
@@ -242,9 +244,13 @@ int run(const Options &o)
     }
     std::cout << "reading data from file " << o.edge_file << std::endl;
     double *edges = nullptr;
+    float *edges_f = nullptr;
     int n = 0;
     char err[1024];
-    if (cvg_read_edge_file(path.c_str(), o.p.edge_data_sz, &edges, &n, err, sizeof(err)) != CVG_OK) {
+    const int rrc = o.use_double ? cvg_read_edge_file(path.c_str(), o.p.edge_data_sz, &edges, &n, err, sizeof(err))
+                                 : cvg_read_edge_file_f32(path.c_str(), o.p.edge_data_sz, &edges_f, &n, err,
+                                                          sizeof(err));
+    if (rrc != CVG_OK) {
         std::cerr << err << std::endl;
         return 1;
     }
@@ -255,24 +261,23 @@ int run(const Options &o)
     if (rc != CVG_OK) {
         std::cerr << "Error: cannot open HIP device " << o.device << ": " << cvg_status_string(rc) << std::endl;
         cvg_free(edges);
+        cvg_free(edges_f);
         return 1;
     }
     Laps timer;
     timer.start();
-    if (cvg_build_neighbor_csr_graph(ctx, n, o.p.edge_data_sz, edges, o.p.nrad, 3u, o.p.max_candidates) != CVG_OK) {
-        cvg_free(edges);
-        return fail_ctx(ctx, "build_neighbor_csr_graph");
-    }
+    const int grc = o.use_double
+                        ? cvg_build_neighbor_csr_graph(ctx, n, o.p.edge_data_sz, edges, o.p.nrad, 3u, o.p.max_candidates)
+                        : cvg_build_neighbor_csr_graph_f32(ctx, n, o.p.edge_data_sz, edges_f, o.p.nrad, 3u,
+                                                           o.p.max_candidates);
     cvg_free(edges);
+    cvg_free(edges_f);
+    if (grc != CVG_OK) return fail_ctx(ctx, "build_neighbor_csr_graph");
     cvg_graph_info gi{};
     cvg_get_graph_info(ctx, &gi);
     if (gi.truncated_anchors > 0)
         std::cerr << "Warning: " << gi.truncated_anchors << " anchors hit max_candidates=" << o.p.max_candidates
                   << "; neighbor lists may be truncated\n";
-    if (gi.order_ambiguous_anchors > 0)
-        std::cerr << "Warning: " << gi.order_ambiguous_anchors
-                  << " anchors have exact distance ties or a max_candidates cut inside one pixel cell; their "
-                     "neighbour order follows edge ids within the cell\n";
     std::cout << "CSR neighbor graph: " << gi.num_edges << " anchors, " << gi.total_neighbor_pairs
               << " total pairs, max number of neighbors per anchor: " << gi.max_num_of_neighbors << std::endl;
     timer.lap("build CSR neighbor graph");
@@ -308,9 +313,5 @@ int main(int argc, char **argv)
     cvg_params_default(&o.p);
     bool help = false;
     if (!parse(argc, argv, o, help)) return help ? 0 : 1;
-    if (!o.use_double) {
-        std::cerr << "Error: --float is not supported: the GPU path computes in double precision" << std::endl;
-        return 1;
-    }
     return run(o);
 }

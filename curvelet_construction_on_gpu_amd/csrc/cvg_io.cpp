@@ -191,9 +191,11 @@ bool parse_fast(const char *s, const char *e, double *out)
     return eisel_lemire(w, q, neg, out);
 }
 
-// One istream >> double on [*p, e).  Advances *p past what the extraction
-// consumed.  Returns false on failure (as the stream's failbit).
-bool extract_double(const char **p, const char *e, double *out)
+// One istream >> double (f32: istream >> float) on [*p, e).  Advances *p past
+// what the extraction consumed.  Returns false on failure (as the stream's
+// failbit).  A float is converted from the decimal text directly (strtof),
+// never through a double, whose rounding could land on a float midpoint.
+bool extract_double(const char **p, const char *e, double *out, bool f32 = false)
 {
     const char *s = *p;
     while (s < e && is_stream_space(*s)) ++s;
@@ -222,7 +224,7 @@ bool extract_double(const char **p, const char *e, double *out)
     *p = q;
     if (q == start) return false;
     double fast;
-    if (parse_fast(start, q, &fast)) {
+    if (!f32 && parse_fast(start, q, &fast)) {
         if (std::isinf(fast)) return false;  // overflow: the stream's failbit
         *out = fast;
         return true;
@@ -242,7 +244,7 @@ bool extract_double(const char **p, const char *e, double *out)
         tok = big.c_str();
     }
     char *end = nullptr;
-    const double v = strtod_l(tok, &end, c_locale());
+    const double v = f32 ? static_cast<double>(strtof_l(tok, &end, c_locale())) : strtod_l(tok, &end, c_locale());
     if (end == tok || *end != '\0') return false;
     if (std::isinf(v)) return false;
     *out = v;
@@ -346,7 +348,7 @@ struct Piece {
     int found = 0;    // values read on the failing line
 };
 
-void parse_piece(const char *b, const char *e, int values_per_edge, Piece &out)
+void parse_piece(const char *b, const char *e, int values_per_edge, Piece &out, bool f32 = false)
 {
     out.values.reserve(static_cast<size_t>(e - b) / 16 + 16);
     const char *p = b;
@@ -361,7 +363,7 @@ void parse_piece(const char *b, const char *e, int values_per_edge, Piece &out)
         const char *q = p;
         for (int j = 0; j < values_per_edge; ++j) {
             double v;
-            if (!extract_double(&q, le, &v)) {
+            if (!extract_double(&q, le, &v, f32)) {
                 out.error = kShort;
                 out.found = j;
                 return;
@@ -423,8 +425,12 @@ void set_err(char *err, size_t err_len, const std::string &msg)
 
 extern "C" {
 
-int cvg_read_edge_file(const char *path, int values_per_edge, double **edges, int *num_edges, char *err,
-                       size_t err_len)
+namespace {
+
+// read_TO_edges_from_file<T> (data_io.hpp:20-78): the values as doubles
+// (for T = float, the float values), malloc'd.
+int read_edges(const char *path, int values_per_edge, double **edges, int *num_edges, char *err, size_t err_len,
+               bool f32)
 {
     if (!path || !edges || !num_edges || values_per_edge <= 0) {
         set_err(err, err_len, "Error: invalid arguments to cvg_read_edge_file");
@@ -467,7 +473,7 @@ int cvg_read_edge_file(const char *path, int values_per_edge, double **edges, in
         cut[t] = c;
     }
     std::vector<Piece> pieces(nt);
-    run_parallel(nt, [&](int t) { parse_piece(cut[t], cut[t + 1], values_per_edge, pieces[t]); });
+    run_parallel(nt, [&](int t) { parse_piece(cut[t], cut[t + 1], values_per_edge, pieces[t], f32); });
 
     long line_num = 0;
     long edge_count = 0;
@@ -512,6 +518,38 @@ int cvg_read_edge_file(const char *path, int values_per_edge, double **edges, in
     }
     *edges = out;
     *num_edges = static_cast<int>(edge_count);
+    return CVG_OK;
+}
+
+}  // namespace
+
+int cvg_read_edge_file(const char *path, int values_per_edge, double **edges, int *num_edges, char *err,
+                       size_t err_len)
+{
+    return read_edges(path, values_per_edge, edges, num_edges, err, err_len, false);
+}
+
+int cvg_read_edge_file_f32(const char *path, int values_per_edge, float **edges, int *num_edges, char *err,
+                           size_t err_len)
+{
+    if (!edges) {
+        set_err(err, err_len, "Error: invalid arguments to cvg_read_edge_file_f32");
+        return CVG_ERR_INVALID_ARGUMENT;
+    }
+    *edges = nullptr;
+    double *wide = nullptr;
+    const int rc = read_edges(path, values_per_edge, &wide, num_edges, err, err_len, true);
+    if (rc != CVG_OK) return rc;
+    const size_t total = static_cast<size_t>(*num_edges) * values_per_edge;
+    float *out = static_cast<float *>(std::malloc(total * sizeof(float)));
+    if (!out) {
+        std::free(wide);
+        set_err(err, err_len, "Error: out of host memory");
+        return CVG_ERR_OUT_OF_MEMORY;
+    }
+    for (size_t i = 0; i < total; ++i) out[i] = static_cast<float>(wide[i]);  // exact: float values
+    std::free(wide);
+    *edges = out;
     return CVG_OK;
 }
 

@@ -131,6 +131,9 @@ struct cvg_context {
     // within-cell order has been replayed for it, each edge's rank in its cell
     const double *last_edges = nullptr;
     bool use_cellrank = false;
+    // the run's map came in as float (the reference's float instantiation):
+    // distances, bundles and info in its float arithmetic
+    bool f32 = false;
     DevBuf d_cellrank;
 
     // the maps of a batch (a single map outside batches): edge id ranges,
@@ -705,6 +708,19 @@ __global__ __launch_bounds__(kBlock) void k_cell_order(int n, const int *__restr
     sxy[p] = make_double2(x[i], y[i]);
 }
 
+// sq_dist<T> (curvelet_utils.hpp:13-17) of edge coordinates held as doubles:
+// in double, or for maps read as float (the float instantiation, --float) in
+// float arithmetic on the (exact) float values.
+__device__ inline double pair_dist2(double ax, double ay, double px, double py, bool f32)
+{
+    if (f32) {
+        const float dx = static_cast<float>(ax) - static_cast<float>(px);
+        const float dy = static_cast<float>(ay) - static_cast<float>(py);
+        return static_cast<double>(dx * dx + dy * dy);
+    }
+    return (ax - px) * (ax - px) + (ay - py) * (ay - py);
+}
+
 // LDS or global writes by some lanes made visible to the other lanes of the
 // same wave.
 __device__ inline void wave_sync()
@@ -796,7 +812,7 @@ __global__ __launch_bounds__(kBlock) void k_discover_count(int n, const double *
                                                            unsigned max_cand, const unsigned *__restrict__ start,
                                                            const int *__restrict__ sorted,
                                                            const double2 *__restrict__ sxy,
-                                                           unsigned *__restrict__ deg)
+                                                           unsigned *__restrict__ deg, bool f32)
 {
     const int lane = threadIdx.x & 63;
     const int sl = lane & (G - 1);
@@ -822,7 +838,7 @@ __global__ __launch_bounds__(kBlock) void k_discover_count(int n, const double *
             bool pass = false;
             if (k >= 0) {
                 const double2 p = sxy[k];
-                const double d = (ax - p.x) * (ax - p.x) + (ay - p.y) * (ay - p.y);
+                const double d = pair_dist2(ax, ay, p.x, p.y, f32);
                 pass = sorted[k] != a && !(d > rad2);
             }
             cnt += static_cast<unsigned>(__popcll(__ballot(pass) & segmask));
@@ -858,7 +874,7 @@ __global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const doubl
                                                               unsigned max_cand, int cap,
                                                               const int *__restrict__ map_of,
                                                               const long long *__restrict__ map_eoff,
-                                                              unsigned *__restrict__ scal)
+                                                              unsigned *__restrict__ scal, bool f32)
 {
     // per segment: cap candidates (d2, id, cell) in LDS for the ranking
     extern __shared__ double fill_lds[];
@@ -902,7 +918,7 @@ __global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const doubl
                 b = sorted[k];
                 cell = scell[k];
                 const double2 p = sxy[k];
-                d = (ax - p.x) * (ax - p.x) + (ay - p.y) * (ay - p.y);
+                d = pair_dist2(ax, ay, p.x, p.y, f32);
                 pass = b != a && !(d > rad2);
             }
             const unsigned long long m = __ballot(pass) & segmask;
@@ -1870,6 +1886,11 @@ struct RecordArgs {
     unsigned *chains;
     double *info;
     unsigned max_rows;  // rows the output buffers hold; rows beyond are not written
+    // float instantiation: info in float arithmetic from the float grid
+    // offsets of fill_curvelet_info (cpu_curvelet.hpp:209-210, 221-222)
+    bool f32;
+    const float *inf_x, *inf_t;
+    float nrad_f;
 };
 
 // record_curvelet_chain + fill_curvelet_info (cpu_curvelet.hpp:197-280) for
@@ -1895,6 +1916,43 @@ __device__ inline void record_slot(const RecordArgs &R, int q, int rk)
     row[0] = static_cast<unsigned>(a - id0) + 1;
     for (int l = 0; l < R.G; ++l) row[l + 1] = l < len ? static_cast<unsigned>(chain[l] - id0) + 1 : 0u;
 
+    if (R.f32) {
+        // fill_curvelet_info<float> (:197-256): every value a float, std::sin,
+        // std::cos and std::sqrt the float functions
+        const float fx = static_cast<float>(ax), fy = static_cast<float>(ay), ft = static_cast<float>(at);
+        const float ox = R.inf_x[c / R.kt];
+        const float ot = R.inf_t[c % R.kt];
+        const float k_max = static_cast<float>(R.st_k[2 * static_cast<long long>(q)]);
+        const float k_min = static_cast<float>(R.st_k[2 * static_cast<long long>(q) + 1]);
+        const float kk = (k_max + k_min) / 2.0f;
+        const float theta = cvg::wrap_two_pi_f(ft + ot);
+        float s, cth;
+        cvg::cr_sincosf(theta, c_sincos_tab, &s, &cth);
+        const float px = fx - ox * s;
+        const float py = fy + ox * cth;
+        float length = 0.0f;
+        for (int l = 0; l + 1 < len; ++l) {
+            const int e1 = chain[l], e2 = chain[l + 1];
+            const float x1 = static_cast<float>(R.x[e1]), y1 = static_cast<float>(R.y[e1]);
+            const float x2 = static_cast<float>(R.x[e2]), y2 = static_cast<float>(R.y[e2]);
+            length += cvg::sqrt_f((x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2));
+        }
+        const float quality = (length > 0.0f && len > 0)
+                                  ? 2.0f / (1.0f * R.nrad_f / length + 1.0f * length / static_cast<float>(len))
+                                  : 0.0f;
+        double *inf = R.info + static_cast<long long>(o) * kInfoWidth;
+        inf[0] = fwd ? 1.0 : 0.0;
+        inf[1] = k_max;
+        inf[2] = k_min;
+        inf[3] = ft;
+        inf[4] = px;
+        inf[5] = py;
+        inf[6] = theta;
+        inf[7] = kk;
+        inf[8] = length;
+        inf[9] = quality;
+        return;
+    }
     const double ox = R.off_x[c / R.kt];
     const double ot = R.off_t[c % R.kt];
     const double k_max = R.st_k[2 * static_cast<long long>(q)];
@@ -1954,6 +2012,281 @@ __global__ __launch_bounds__(kRecordBlock) void k_record(RecordArgs R)
     __syncthreads();
     const int cnt = scount;
     for (int i = threadIdx.x; i < cnt; i += blockDim.x) record_slot(R, sq[i], srk[i]);
+}
+
+// ---------------------------------------------------------------------------
+// The float instantiation (run_curvelet<float>, main.cpp:110 with --float)
+// ---------------------------------------------------------------------------
+
+// Direction class of a neighbour in float: dot(To2Pi(atan2f(dy, dx)), theta_a)
+// with atan2f, cosf and sinf (curvelet_utils.hpp:58-69 on T = float).  Its
+// evaluation errors (atan2f, To2Pi's 2 pi, cosf, sinf, two products and a
+// sum in float) stay below 2e-6, so where the projection of the neighbour on
+// the anchor's direction exceeds 4e-6 |v| its sign is the class; elsewhere
+// the float expression itself decides.
+__device__ inline int direction_class_f(float ax, float ay, float at, float nx, float ny, float cos_at,
+                                        float sin_at)
+{
+    const float vx = nx - ax, vy = ny - ay;
+    const double p = static_cast<double>(vx) * cos_at + static_cast<double>(vy) * sin_at;
+    const double r = sqrt(static_cast<double>(vx) * vx + static_cast<double>(vy) * vy);
+    if (fabs(p) > 4e-6 * r) return p > 0 ? 1 : 2;
+    const float dir = cvg::wrap_two_pi_f(cvg::cr_atan2f(vy, vx, c_atan_tab));
+    float sd, cd;
+    cvg::cr_sincosf(dir, c_sincos_tab, &sd, &cd);
+    const float dot = cd * cos_at + sd * sin_at;
+    return dot > 0 ? 1 : (dot < 0 ? 2 : 0);
+}
+
+struct F32Args {
+    int n;
+    const double *x, *y, *th;   // the float edge values, held exactly as doubles
+    const unsigned *off;
+    const int *ids;
+    int kx, kt, K, G, st_stride, max_deg;
+    float dx, dt, max_k, dx2;
+    const float *off_x, *off_t;  // delta_x2, delta_t2 (:697-698)
+    const float *cell_d;         // fill_curvelet_info's d per cell, 3.0e38 where d >= 100
+    float *sc_bundle;            // per wave: max_deg x K lo, then hi
+    int *sc_flag;                // per wave: max_deg neighbour flags
+    int *sc_acc;                 // per wave: max_deg x (G + 1) accepted chains of the run
+    int *st_ids;
+    unsigned *st_meta;
+    double *st_k;
+    int *st_rank;
+    unsigned *acc_count;
+    unsigned *valid_pairs;
+};
+
+constexpr int kF32Waves = 4;  // waves per workgroup of k_curvelets_f32
+
+// build_curvelets_greedy<float> (cpu_curvelet.hpp:437-591) with the
+// form_pairwise_bundles<float> count (:305-361), one wave per anchor (a
+// persistent grid: wave w takes anchors w, w + waves, ...), lane = bundle
+// cell (up to 4 per lane).  compute_curve_bundle<float> (:636-830) takes the
+// float sin and cos (main.cpp includes <math.h>, whose std:: names resolve its
+// unqualified calls), and the double literals of :697-698, 711-715 and 774
+// promote those expressions to double until they are stored in T.  Each
+// neighbour's bundle goes to the wave's scratch once (it does not depend on
+// the run); the runs' hypotheses are the neighbours of their direction class
+// with a valid (or close) bundle.  Accepted rows are staged as
+// k_curvelets stages them, at the anchor's slots base, base + 1, ...
+__global__ __launch_bounds__(64 * kF32Waves) void k_curvelets_f32(F32Args A)
+{
+    const int lane = threadIdx.x & 63;
+    const int w = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int nwaves = static_cast<int>((gridDim.x * blockDim.x) >> 6);
+    const int K = A.K;
+    const int G = A.G;
+    const int cw = G + 1;
+    float *blo = A.sc_bundle + static_cast<size_t>(w) * A.max_deg * K * 2;
+    float *bhi = blo + static_cast<size_t>(A.max_deg) * K;
+    int *flag = A.sc_flag + static_cast<size_t>(w) * A.max_deg;
+    int *acc = A.sc_acc + static_cast<size_t>(w) * A.max_deg * cw;
+    const unsigned long long lt = (1ULL << lane) - 1ULL;
+    // per-lane cells
+    bool live[4];
+    int cell[4];
+    float ox[4], ot[4], cd[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const int c = lane + 64 * m;
+        live[m] = c < K;
+        cell[m] = live[m] ? c : 0;
+        ox[m] = A.off_x[cell[m] / A.kt];
+        ot[m] = A.off_t[cell[m] % A.kt];
+        cd[m] = live[m] ? A.cell_d[cell[m]] : 3.0e38f;
+    }
+    unsigned vpw = 0;
+    for (int a = w; a < A.n; a += nwaves) {
+        const unsigned base = A.off[a];
+        const int deg = static_cast<int>(A.off[a + 1] - base);
+        const float ax = static_cast<float>(A.x[a]), ay = static_cast<float>(A.y[a]);
+        const float at = static_cast<float>(A.th[a]);
+        float sat, cat;
+        cvg::cr_sincosf(at, c_sincos_tab, &sat, &cat);
+        float T2t[4], sT2t[4], cT2t[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            T2t[m] = at + ot[m];  // :702
+            cvg::cr_sincosf(T2t[m], c_sincos_tab, &sT2t[m], &cT2t[m]);
+        }
+        // every neighbour: direction class and bundle (compute_curve_bundle<float>)
+        for (int r = 0; r < deg; ++r) {
+            const int b = A.ids[base + r];
+            const float nx = static_cast<float>(A.x[b]), ny = static_cast<float>(A.y[b]);
+            const float nt = static_cast<float>(A.th[b]);
+            const int cls = direction_class_f(ax, ay, at, nx, ny, cat, sat);
+            const float DX = ax - nx, DY = ay - ny;
+            const float DX2 = DX * DX, DY2 = DY * DY;
+            const float T0p = nt + A.dt, T0m = nt - A.dt;
+            const bool close = DX2 + DY2 < 4 * A.dx * A.dx;  // :682
+            float lo[4], hi[4];
+            unsigned long long anym = 0ULL;
+            if (close) {
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    lo[m] = -A.max_k;
+                    hi[m] = A.max_k;
+                }
+            } else {
+                const bool branch_a = (DX * cat + DY * sat) < 0;  // :690
+                float sp0, cp0, sm0, cm0;
+                cvg::cr_sincosf(T0p, c_sincos_tab, &sp0, &cp0);
+                cvg::cr_sincosf(T0m, c_sincos_tab, &sm0, &cm0);
+                const float sT0p_DY = DY * sp0;  // :730-733
+                const float cT0p_DX = DX * cp0;
+                const float sT0m_DY = DY * sm0;
+                const float cT0m_DX = DX * cm0;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    lo[m] = -A.max_k;
+                    hi[m] = A.max_k;
+                    if (!live[m]) continue;
+                    const float o = ox[m];
+                    const float tt = sT2t[m] * DX - cT2t[m] * DY;  // :705
+                    const float ttm = tt - o;
+                    const float sm = cvg::cr_sinf(T2t[m] - T0m, c_sincos_tab);  // :728-729
+                    const float sp = cvg::cr_sinf(T2t[m] - T0p, c_sincos_tab);
+                    if (branch_a) {
+                        const float den = static_cast<float>(-2.0 * o * tt + DX2 + DY2 - A.dx2 + o * o);  // :711
+                        const float kx_min = static_cast<float>(2.0 * (ttm - A.dx) / den);
+                        const float kx_max = static_cast<float>(2.0 * (ttm + A.dx) / den);
+                        if (lo[m] < kx_min) lo[m] = kx_min;
+                        if (hi[m] > kx_max) hi[m] = kx_max;
+                        const float kt_min = sm / (-sm * o + cT0m_DX + sT0m_DY);
+                        const float kt_max = sp / (-sp * o + cT0p_DX + sT0p_DY);
+                        if (lo[m] < kt_min) lo[m] = kt_min;
+                        if (hi[m] > kt_max) hi[m] = kt_max;
+                    } else {
+                        const float den = static_cast<float>(-(2.0 * o * tt - DX2 - DY2 + A.dx2 - o * o) / 2.0);  // :774
+                        const float kx_min = (ttm - A.dx) / den;
+                        const float kx_max = (ttm + A.dx) / den;
+                        if (lo[m] < kx_min) lo[m] = kx_min;
+                        if (hi[m] > kx_max) hi[m] = kx_max;
+                        const float kt_max = sm / (-sm * o + cT0m_DX + sT0m_DY);
+                        if (hi[m] > kt_max) hi[m] = kt_max;
+                        const float kt_min = sp / (-sp * o + cT0p_DX + sT0p_DY);
+                        if (lo[m] < kt_min) lo[m] = kt_min;
+                    }
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                anym |= __ballot(live[m] && hi[m] > lo[m]);
+                if (live[m]) {
+                    blo[static_cast<size_t>(r) * K + cell[m]] = lo[m];
+                    bhi[static_cast<size_t>(r) * K + cell[m]] = hi[m];
+                }
+            }
+            const bool valid = close || anym != 0ULL;  // bundle_valid_check, :821-830
+            vpw += valid ? 1u : 0u;
+            if (lane == 0) flag[r] = cls | (valid ? 4 : 0);
+        }
+        wave_sync();  // flags written by lane 0 are read by every lane
+        unsigned nacc = 0;  // accepted rows of the anchor so far
+        for (int run = 0; run < 2; ++run) {
+            const int want = run == 0 ? 1 : 2;
+            unsigned nrun = 0;  // accepted rows of this run (the dedup table)
+            for (int sidx = 0; sidx < deg; ++sidx) {
+                const int fs = flag[sidx];
+                if ((fs & 3) != want || !(fs & 4)) continue;
+                float lo[4], hi[4];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    lo[m] = live[m] ? blo[static_cast<size_t>(sidx) * K + cell[m]] : 0.0f;
+                    hi[m] = live[m] ? bhi[static_cast<size_t>(sidx) * K + cell[m]] : 0.0f;
+                }
+                int len = 1;
+                int cur = lane == 0 ? a : 0;
+                for (int r = 0; r < deg; ++r) {  // :479-536
+                    const int fr = flag[r];
+                    if ((fr & 3) != want || !(fr & 4)) continue;
+                    const int nb = A.ids[base + r];
+                    if (r == sidx) {
+                        cur = lane == len ? nb : cur;
+                        ++len;
+                        if (len >= G) break;
+                        continue;
+                    }
+                    float nlo[4], nhi[4];
+                    unsigned long long okm = 0ULL;
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) {
+                        const float bl = live[m] ? blo[static_cast<size_t>(r) * K + cell[m]] : 0.0f;
+                        const float bh = live[m] ? bhi[static_cast<size_t>(r) * K + cell[m]] : 0.0f;
+                        nlo[m] = (lo[m] < bl) ? bl : lo[m];  // :838-841
+                        nhi[m] = (hi[m] > bh) ? bh : hi[m];
+                        okm |= __ballot(live[m] && nhi[m] > nlo[m]);
+                    }
+                    if (okm != 0ULL) {
+                        cur = lane == len ? nb : cur;
+                        ++len;
+#pragma unroll
+                        for (int m = 0; m < 4; ++m) {
+                            lo[m] = nlo[m];
+                            hi[m] = nhi[m];
+                        }
+                    }
+                    if (len >= G) break;
+                }
+                if (len <= 2) continue;  // :549
+                // check_curvelet_exist (:876-916) against this run's accepted chains
+                bool dup = false;
+                for (unsigned k = 0; k < nrun && !dup; ++k) {
+                    const int *row = acc + static_cast<size_t>(k) * cw;
+                    if (row[0] != len) continue;
+                    const bool ne = lane < len && row[1 + lane] != cur;
+                    dup = __ballot(ne) == 0ULL;
+                }
+                if (dup) continue;
+                int *row = acc + static_cast<size_t>(nrun) * cw;
+                if (lane == 0) row[0] = len;
+                if (lane < len) row[1 + lane] = cur;
+                wave_sync();
+                ++nrun;
+                // fill_curvelet_info's cell: smallest d, first in row-major order
+                unsigned long long key = ~0ULL;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const bool ok = live[m] && hi[m] > lo[m] && cd[m] < 1.0e38f;
+                    const unsigned long long k2 =
+                        ok ? ((static_cast<unsigned long long>(__float_as_uint(cd[m])) << 32) |
+                              static_cast<unsigned>(cell[m]))
+                           : ~0ULL;
+                    key = k2 < key ? k2 : key;
+                }
+                for (int o = 32; o > 0; o >>= 1) {
+                    const unsigned long long v = __shfl_xor(key, o, 64);
+                    key = v < key ? v : key;
+                }
+                const int best = key == ~0ULL ? 0 : static_cast<int>(key & 0xffffffffULL);
+                const int bl = best & 63, bm = best >> 6;
+                float kmax = 0.0f, kmin = 0.0f;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const float h = __shfl(hi[m], bl, 64), l = __shfl(lo[m], bl, 64);
+                    if (m == bm) {
+                        kmax = h;
+                        kmin = l;
+                    }
+                }
+                const long long q = static_cast<long long>(base) + nacc;
+                if (lane < len) A.st_ids[q * A.st_stride + lane] = cur;
+                if (lane == 0) {
+                    A.st_meta[q] = static_cast<unsigned>(len) | (run == 0 ? 0x100u : 0u) |
+                                   (static_cast<unsigned>(best) << 16);
+                    A.st_k[2 * q] = kmax;
+                    A.st_k[2 * q + 1] = kmin;
+                }
+                ++nacc;
+            }
+        }
+        for (int k = lane; k < deg; k += 64) A.st_rank[base + k] = k < static_cast<int>(nacc) ? k : -1;
+        if (lane == 0) A.acc_count[a] = nacc;
+        wave_sync();  // the scratch is rewritten by the next anchor
+    }
+    if (lane == 0 && vpw) atomicAdd(A.valid_pairs, vpw);
 }
 
 // ---------------------------------------------------------------------------
@@ -2035,6 +2368,12 @@ __host__ __device__ inline double math_op(int op, double x, double y, const doub
         cvg::cr_sincos_dd(x, sc, &sv, &cv);
         return op == 10 ? sv.hi : op == 11 ? sv.lo : op == 12 ? cv.hi : cv.lo;
     }
+    // binary32: arguments rounded to float first, results widened (exactly)
+    case 14: return cvg::cr_sinf(static_cast<float>(x), sc);
+    case 15: return cvg::cr_cosf(static_cast<float>(x), sc);
+    case 16: return cvg::cr_atan2f(static_cast<float>(y), static_cast<float>(x), atb);
+    case 17: return cvg::wrap_two_pi_f(static_cast<float>(x));
+    case 18: return cvg::sqrt_f(static_cast<float>(x));
     default: return 0.0;
     }
 }
@@ -2205,10 +2544,11 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     // neighbour counts and CSR offsets
     CVG_TRY(ensure(ctx, ctx->d_deg, ni));
     CVG_TRY(ensure(ctx, ctx->d_off, static_cast<size_t>(n + 1) * sizeof(unsigned)));
-    const double rad2 = rad * rad;  // preprocess.hpp:221
+    // preprocess.hpp:221: rad * rad in T
+    const double rad2 = ctx->f32 ? static_cast<double>(static_cast<float>(rad) * static_cast<float>(rad)) : rad * rad;
     {
         using CountFn = void (*)(int, const double *, const double *, const int *, const int *, GridDesc, int, double,
-                                 unsigned, const unsigned *, const int *, const double2 *, unsigned *);
+                                 unsigned, const unsigned *, const int *, const double2 *, unsigned *, bool);
         const int G = ctx->fill_lanes;
         const CountFn fn = G == 8    ? k_discover_count<8>
                            : G == 32 ? k_discover_count<32>
@@ -2217,7 +2557,7 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
         hipLaunchKernelGGL(fn, dim3(blocks_for(static_cast<long long>(n) * G)), dim3(kBlock), 0, st, n,
                            as<double>(ctx->d_x), as<double>(ctx->d_y), as<int>(ctx->d_cx), as<int>(ctx->d_cy), g,
                            static_cast<int>(R), rad2, max_cand, start, as<int>(ctx->d_sorted),
-                           as<double2>(ctx->d_sxy), as<unsigned>(ctx->d_deg));
+                           as<double2>(ctx->d_sxy), as<unsigned>(ctx->d_deg), ctx->f32);
     }
     CVG_HIP(ctx, hipGetLastError());
     CVG_TRY(exclusive_scan(ctx, as<unsigned>(ctx->d_deg), n, as<unsigned>(ctx->d_off), ctx->d_tiles,
@@ -2246,7 +2586,7 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     using FillFn = void (*)(int, const double *, const double *, const int *, const int *, GridDesc, int, double,
                             const unsigned *, const int *, const int *, const double2 *, const unsigned *, Cand *,
                             int *, double *, int *, int *, unsigned, int, const int *, const long long *,
-                            unsigned *);
+                            unsigned *, bool);
     const int fill_cap = static_cast<int>(std::max(ctx->max_deg, 1u));
     const size_t per_seg = static_cast<size_t>(fill_cap) * (2 * sizeof(double));
     int G = ctx->fill_lanes;
@@ -2267,7 +2607,7 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
                        as<double2>(ctx->d_sxy), as<unsigned>(ctx->d_off), as<Cand>(ctx->d_cand),
                        as<int>(ctx->d_ids), as<double>(ctx->d_d2), as<int>(ctx->d_pair_anchor),
                        as<int>(ctx->d_tmp_ids), max_cand, fill_cap, num_maps > 1 ? as<int>(ctx->d_map) : nullptr,
-                       as<long long>(ctx->d_eoff), scal);
+                       as<long long>(ctx->d_eoff), scal, ctx->f32);
     hipLaunchKernelGGL(k_tie_sort, dim3(std::min(blocks_for(n), 64)), dim3(kBlock), 0, st, as<int>(ctx->d_tmp_ids),
                        scal, as<unsigned>(ctx->d_off), as<Cand>(ctx->d_cand), as<int>(ctx->d_ids),
                        as<double>(ctx->d_d2));
@@ -2451,8 +2791,186 @@ int finish_curvelets(cvg_context *ctx)
     return CVG_OK;
 }
 
+// compact_curvelet_output + record_curvelet_chain (cpu_curvelet.hpp:258-303):
+// row offsets from the per-anchor accepted counts, then every accepted
+// staging row to its row (R carries the grid and info parameters), and the
+// counts back with the stream.  Outputs are sized by their bound (one row per
+// CSR pair at most), so the record runs without a host round trip; the
+// counts are read at the first query (finish_curvelets).
+int record_rows(cvg_context *ctx, const cvg_params &p, RecordArgs R, int st_stride)
+{
+    hipStream_t st = ctx->stream;
+    const int n = ctx->n;
+    const int pairs = ctx->pairs;
+    const size_t np = static_cast<size_t>(std::max(pairs, 1));
+    CVG_TRY(exclusive_scan(ctx, as<unsigned>(ctx->d_acc), n, as<unsigned>(ctx->d_rowoff), ctx->d_tiles));
+    ctx->width = static_cast<unsigned>(R.G) + 1;
+    unsigned *out_chains = ctx->caller_chains;
+    double *out_info = ctx->caller_info;
+    if (!out_chains) {
+        CVG_TRY(ensure(ctx, ctx->d_chains, np * ctx->width * sizeof(unsigned)));
+        CVG_TRY(ensure(ctx, ctx->d_info, np * kInfoWidth * sizeof(double)));
+        out_chains = as<unsigned>(ctx->d_chains);
+        out_info = as<double>(ctx->d_info);
+    }
+    R.pairs = pairs;
+    R.pair_anchor = as<int>(ctx->d_pair_anchor);
+    R.map_of = ctx->num_maps > 1 ? as<int>(ctx->d_map) : nullptr;
+    R.map_eoff = as<long long>(ctx->d_eoff);
+    R.x = as<double>(ctx->d_x);
+    R.y = as<double>(ctx->d_y);
+    R.th = as<double>(ctx->d_th);
+    R.st_rank = as<int>(ctx->d_st_rank);
+    R.row_off = as<unsigned>(ctx->d_rowoff);
+    R.st_ids = as<int>(ctx->d_st_ids);
+    R.st_meta = as<unsigned>(ctx->d_st_meta);
+    R.st_k = as<double>(ctx->d_st_k);
+    R.st_stride = st_stride;
+    R.chains = out_chains;
+    R.info = out_info;
+    R.max_rows = ctx->caller_chains ? ctx->caller_rows : UINT_MAX;
+    if (pairs > 0)
+        hipLaunchKernelGGL(k_record, dim3(blocks_for(pairs, kRecordBlock)), dim3(kRecordBlock), 0, st, R);
+    CVG_HIP(ctx, hipGetLastError());
+    CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox + 32, as<unsigned>(ctx->d_rowoff) + n, sizeof(unsigned),
+                                hipMemcpyDeviceToHost, st));
+    CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox, ctx->d_scal.ptr, kScalarWords * sizeof(unsigned),
+                                hipMemcpyDeviceToHost, st));
+    if (ctx->num_maps > 1) {
+        const int M = ctx->num_maps;
+        CVG_TRY(ensure(ctx, ctx->d_maprow, (static_cast<size_t>(M) + 1) * sizeof(unsigned)));
+        CVG_TRY(ensure_host(ctx, (static_cast<size_t>(M) + 1) * sizeof(unsigned)));
+        hipLaunchKernelGGL(k_map_rows, dim3(blocks_for(M + 1LL)), dim3(kBlock), 0, st, M, as<long long>(ctx->d_eoff),
+                           as<unsigned>(ctx->d_rowoff), as<unsigned>(ctx->d_maprow));
+        CVG_HIP(ctx, hipMemcpyAsync(ctx->h_maprow, ctx->d_maprow.ptr, (static_cast<size_t>(M) + 1) * sizeof(unsigned),
+                                    hipMemcpyDeviceToHost, st));
+    }
+    ctx->counts_pending = true;
+    ctx->cur_params = p;
+    ctx->curvelets_ready = true;
+    return CVG_OK;
+}
+
+// The curvelet stage of the float instantiation (run_curvelet<float>): the
+// parameters cast to float as main.cpp:22-30 casts them, the bundle grid of
+// init_curvelet_storage (:73-74) from float dx / sx and dt / st, one
+// k_curvelets_f32 pass, then the common record with float info.
+int build_curvelets_f32(cvg_context *ctx, const cvg_params &p)
+{
+    if (!ctx->graph_ready) return fail(ctx, CVG_ERR_STATE, "build the neighbour graph first");
+    ctx->curvelets_ready = false;
+    if (!(p.sx > 0) || !(p.st > 0)) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "sx and st must be positive");
+    if (p.group_max_sz > 64) return fail(ctx, CVG_ERR_UNSUPPORTED, "group_max_sz above 64");
+    hipStream_t st = ctx->stream;
+    const int n = ctx->n;
+    const int pairs = ctx->pairs;
+    const float nrad = static_cast<float>(p.nrad), dx = static_cast<float>(p.dx);
+    const float dt = static_cast<float>(p.dt_deg / 180.0) * static_cast<float>(3.14159265358979323846);
+    const float max_k = static_cast<float>(p.max_k), sx = static_cast<float>(p.sx), st_ = static_cast<float>(p.st);
+    const double fx = 2 * std::floor(dx / sx + 0.5) + 1;
+    const double ft = 2 * std::floor(dt / st_ + 0.5) + 1;
+    if (!(fx >= 1 && ft >= 1) || fx * ft > 256)
+        return fail(ctx, CVG_ERR_UNSUPPORTED, "bundle grid must have between 1 and 256 cells (dx/sx, dt/st)");
+    const int kx = static_cast<int>(fx), kt = static_cast<int>(ft), K = kx * kt;
+    const int G = static_cast<int>(p.group_max_sz);
+    std::vector<float> off_x(kx), off_t(kt), inf_x(kx), inf_t(kt), cell_d(K);
+    for (int i = 0; i < kx; ++i) {
+        off_x[i] = static_cast<float>(sx * (static_cast<float>(i) - (static_cast<float>(kx) - 1.0) / 2.0));  // :697
+        inf_x[i] = sx * (static_cast<float>(i) - (static_cast<float>(kx) - 1.0f) / 2.0f);                   // :209
+    }
+    for (int j = 0; j < kt; ++j) {
+        off_t[j] = static_cast<float>(st_ * (static_cast<float>(j) - (static_cast<float>(kt) - 1.0) / 2.0));  // :698
+        inf_t[j] = st_ * (static_cast<float>(j) - (static_cast<float>(kt) - 1.0f) / 2.0f);                     // :210
+    }
+    for (int c = 0; c < K; ++c) {
+        const float dd = inf_x[c / kt] * inf_x[c / kt] + inf_t[c % kt] * inf_t[c % kt];  // :211
+        cell_d[c] = dd < 100.0f ? dd : 3.0e38f;
+    }
+    std::vector<char> blob;
+    const size_t o_off_x = put(blob, off_x), o_off_t = put(blob, off_t), o_inf_x = put(blob, inf_x),
+                 o_inf_t = put(blob, inf_t), o_cell_d = put(blob, cell_d);
+    CVG_TRY(ensure(ctx, ctx->d_tables, blob.size()));
+    CVG_HIP(ctx, hipMemcpyAsync(ctx->d_tables.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, st));
+    char *tb = static_cast<char *>(ctx->d_tables.ptr);
+
+    const int st_stride = std::max(G, 1);
+    const size_t np = static_cast<size_t>(std::max(pairs, 1));
+    CVG_TRY(ensure(ctx, ctx->d_st_ids, np * st_stride * sizeof(int)));
+    CVG_TRY(ensure(ctx, ctx->d_st_meta, np * sizeof(unsigned)));
+    CVG_TRY(ensure(ctx, ctx->d_st_k, np * 2 * sizeof(double)));
+    CVG_TRY(ensure(ctx, ctx->d_st_rank, np * sizeof(int)));
+    CVG_TRY(ensure(ctx, ctx->d_acc, static_cast<size_t>(n) * sizeof(unsigned)));
+    CVG_TRY(ensure(ctx, ctx->d_rowoff, static_cast<size_t>(n + 1) * sizeof(unsigned)));
+    unsigned *scal = as<unsigned>(ctx->d_scal);
+    CVG_HIP(ctx, hipMemsetAsync(scal + S_VALID, 0, (kScalarBufWords - S_VALID) * sizeof(unsigned), st));
+
+    // one wave per anchor, as many waves as the per-wave scratch allows
+    const int max_deg = static_cast<int>(std::max(ctx->max_deg, 1u));
+    const size_t per_wave = static_cast<size_t>(max_deg) * K * 2 * sizeof(float) +
+                            static_cast<size_t>(max_deg) * sizeof(int) +
+                            static_cast<size_t>(max_deg) * (G + 1) * sizeof(int) + 256;
+    long long waves = std::min<long long>(n, static_cast<long long>(ctx->num_cu) * 16);
+    waves = std::max(1LL, std::min<long long>(waves, static_cast<long long>((size_t(512) << 20) / per_wave)));
+    const int blocks = static_cast<int>((waves + kF32Waves - 1) / kF32Waves);
+    const size_t nwaves = static_cast<size_t>(blocks) * kF32Waves;
+    const size_t sb_floats = nwaves * max_deg * K * 2;
+    const size_t sf_ints = nwaves * max_deg;
+    const size_t sa_ints = nwaves * max_deg * (G + 1);
+    CVG_TRY(ensure(ctx, ctx->d_gbundle, sb_floats * sizeof(float) + (sf_ints + sa_ints) * sizeof(int) + 512));
+    F32Args F{};
+    F.n = n;
+    F.x = as<double>(ctx->d_x);
+    F.y = as<double>(ctx->d_y);
+    F.th = as<double>(ctx->d_th);
+    F.off = as<unsigned>(ctx->d_off);
+    F.ids = as<int>(ctx->d_ids);
+    F.kx = kx;
+    F.kt = kt;
+    F.K = K;
+    F.G = G;
+    F.st_stride = st_stride;
+    F.max_deg = max_deg;
+    F.dx = dx;
+    F.dt = dt;
+    F.max_k = max_k;
+    F.dx2 = dx * dx;
+    F.off_x = reinterpret_cast<const float *>(tb + o_off_x);
+    F.off_t = reinterpret_cast<const float *>(tb + o_off_t);
+    F.cell_d = reinterpret_cast<const float *>(tb + o_cell_d);
+    F.sc_bundle = as<float>(ctx->d_gbundle);
+    F.sc_flag = reinterpret_cast<int *>(F.sc_bundle + sb_floats);
+    F.sc_acc = F.sc_flag + sf_ints;
+    F.st_ids = as<int>(ctx->d_st_ids);
+    F.st_meta = as<unsigned>(ctx->d_st_meta);
+    F.st_k = as<double>(ctx->d_st_k);
+    F.st_rank = as<int>(ctx->d_st_rank);
+    F.acc_count = as<unsigned>(ctx->d_acc);
+    F.valid_pairs = scal + S_VALID;
+    const int ring = static_cast<int>(ctx->fused_runs % cvg_context::kFusedRing);
+    CVG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+    CVG_HIP(ctx, hipEventRecord(ctx->fused_ev[ring][0], st));
+    hipLaunchKernelGGL(k_curvelets_f32, dim3(blocks), dim3(64 * kF32Waves), 0, st, F);
+    CVG_HIP(ctx, hipGetLastError());
+    CVG_HIP(ctx, hipEventRecord(ctx->ev[4], st));
+    CVG_HIP(ctx, hipEventRecord(ctx->fused_ev[ring][1], st));
+    ++ctx->fused_runs;
+
+    RecordArgs R{};
+    R.G = G;
+    R.kt = kt;
+    R.nrad = p.nrad;
+    R.off_x = nullptr;
+    R.off_t = nullptr;
+    R.f32 = true;
+    R.inf_x = reinterpret_cast<const float *>(tb + o_inf_x);
+    R.inf_t = reinterpret_cast<const float *>(tb + o_inf_t);
+    R.nrad_f = nrad;
+    return record_rows(ctx, p, R, st_stride);
+}
+
 int build_curvelets(cvg_context *ctx, const cvg_params &p)
 {
+    if (ctx->f32) return build_curvelets_f32(ctx, p);
     if (!ctx->graph_ready) return fail(ctx, CVG_ERR_STATE, "build the neighbour graph first");
     ctx->curvelets_ready = false;
     Derived d;
@@ -2678,62 +3196,13 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     CVG_HIP(ctx, hipEventRecord(ctx->fused_ev[ring][1], st));
     ++ctx->fused_runs;
     // every anchor is in exactly one bucket, so every acc_count entry is written
-
-    CVG_TRY(exclusive_scan(ctx, as<unsigned>(ctx->d_acc), n, as<unsigned>(ctx->d_rowoff), ctx->d_tiles));
-    // outputs sized by their bound (one row per CSR pair at most), so the
-    // record runs without a host round trip; the counts come back with the
-    // stream (finish_curvelets)
-    ctx->width = d.G + 1;
-    unsigned *out_chains = ctx->caller_chains;
-    double *out_info = ctx->caller_info;
-    if (!out_chains) {
-        CVG_TRY(ensure(ctx, ctx->d_chains, np * ctx->width * sizeof(unsigned)));
-        CVG_TRY(ensure(ctx, ctx->d_info, np * kInfoWidth * sizeof(double)));
-        out_chains = as<unsigned>(ctx->d_chains);
-        out_info = as<double>(ctx->d_info);
-    }
     RecordArgs R{};
-    R.pairs = pairs;
-    R.pair_anchor = as<int>(ctx->d_pair_anchor);
-    R.map_of = ctx->num_maps > 1 ? as<int>(ctx->d_map) : nullptr;
-    R.map_eoff = as<long long>(ctx->d_eoff);
-    R.x = as<double>(ctx->d_x);
-    R.y = as<double>(ctx->d_y);
-    R.th = as<double>(ctx->d_th);
-    R.st_rank = A.st_rank;
-    R.row_off = as<unsigned>(ctx->d_rowoff);
-    R.st_ids = A.st_ids;
-    R.st_meta = A.st_meta;
-    R.st_k = A.st_k;
-    R.st_stride = st_stride;
     R.G = static_cast<int>(d.G);
     R.kt = static_cast<int>(d.kt);
     R.nrad = d.nrad;
     R.off_x = T.off_x;
     R.off_t = T.off_t;
-    R.chains = out_chains;
-    R.info = out_info;
-    R.max_rows = ctx->caller_chains ? ctx->caller_rows : UINT_MAX;
-    if (pairs > 0)
-        hipLaunchKernelGGL(k_record, dim3(blocks_for(pairs, kRecordBlock)), dim3(kRecordBlock), 0, st, R);
-    CVG_HIP(ctx, hipGetLastError());
-    CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox + 32, as<unsigned>(ctx->d_rowoff) + n, sizeof(unsigned),
-                                hipMemcpyDeviceToHost, st));
-    CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox, ctx->d_scal.ptr, kScalarWords * sizeof(unsigned),
-                                hipMemcpyDeviceToHost, st));
-    if (ctx->num_maps > 1) {
-        const int M = ctx->num_maps;
-        CVG_TRY(ensure(ctx, ctx->d_maprow, (static_cast<size_t>(M) + 1) * sizeof(unsigned)));
-        CVG_TRY(ensure_host(ctx, (static_cast<size_t>(M) + 1) * sizeof(unsigned)));
-        hipLaunchKernelGGL(k_map_rows, dim3(blocks_for(M + 1LL)), dim3(kBlock), 0, st, M, as<long long>(ctx->d_eoff),
-                           as<unsigned>(ctx->d_rowoff), as<unsigned>(ctx->d_maprow));
-        CVG_HIP(ctx, hipMemcpyAsync(ctx->h_maprow, ctx->d_maprow.ptr, (static_cast<size_t>(M) + 1) * sizeof(unsigned),
-                                    hipMemcpyDeviceToHost, st));
-    }
-    ctx->counts_pending = true;
-    ctx->cur_params = p;
-    ctx->curvelets_ready = true;
-    return CVG_OK;
+    return record_rows(ctx, p, R, st_stride);
 }
 
 bool same_params(const cvg_params &a, const cvg_params &b)
@@ -2750,8 +3219,9 @@ int check_ctx(cvg_context *ctx)
 }
 
 int run_common(cvg_context *ctx, int n, int sz, const double *d_edges, const cvg_params *p, cvg_run_stats *stats,
-               int num_maps = 1, const long long *eoff = nullptr)
+               int num_maps = 1, const long long *eoff = nullptr, bool f32 = false)
 {
+    ctx->f32 = f32;
     CVG_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     ctx->use_cellrank = false;
     CVG_TRY(build_graph(ctx, n, sz, d_edges, p->nrad, 3u, p->max_candidates, num_maps, eoff));  // main.cpp:48: 3
@@ -3077,6 +3547,32 @@ int cvg_build_neighbor_csr_graph(cvg_context *ctx, int num_edges, int sz_edge_da
     const double *d = nullptr;
     CVG_TRY(upload(ctx, num_edges, sz_edge_data, edges, &d));
     ctx->use_cellrank = false;
+    ctx->f32 = false;
+    CVG_TRY(build_graph(ctx, num_edges, sz_edge_data, d, rad, neighbor_radius, max_candidates));
+    CVG_TRY(read_scalars(ctx));
+    if (ctx->mailbox[S_AMBIG] > 0) {
+        CVG_TRY(rebuild_graph_exact(ctx));
+        CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return CVG_OK;
+}
+
+int cvg_build_neighbor_csr_graph_f32(cvg_context *ctx, int num_edges, int sz_edge_data, const float *edges,
+                                     double rad, unsigned neighbor_radius, unsigned max_candidates)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!edges || num_edges <= 0 || sz_edge_data < 4)
+        return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "build_neighbor_csr_graph: need num_edges > 0, sz_edge_data >= 4");
+    std::vector<double> wide;
+    try {
+        wide.assign(edges, edges + static_cast<size_t>(num_edges) * sz_edge_data);
+    } catch (...) {
+        return fail(ctx, CVG_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+    const double *d = nullptr;
+    CVG_TRY(upload(ctx, num_edges, sz_edge_data, wide.data(), &d));
+    ctx->use_cellrank = false;
+    ctx->f32 = true;
     CVG_TRY(build_graph(ctx, num_edges, sz_edge_data, d, rad, neighbor_radius, max_candidates));
     CVG_TRY(read_scalars(ctx));
     if (ctx->mailbox[S_AMBIG] > 0) {
@@ -3165,6 +3661,26 @@ int cvg_run(cvg_context *ctx, int num_edges, int sz_edge_data, const double *edg
     const double *d = nullptr;
     CVG_TRY(upload(ctx, num_edges, sz_edge_data, edges, &d));
     return run_common(ctx, num_edges, sz_edge_data, d, params, stats);
+}
+
+int cvg_run_f32(cvg_context *ctx, int num_edges, int sz_edge_data, const float *edges, const cvg_params *params,
+                cvg_run_stats *stats)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!params || !edges || num_edges <= 0 || sz_edge_data < 4)
+        return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "cvg_run_f32: bad arguments");
+    // the float values held exactly as doubles on the device
+    std::vector<double> wide;
+    try {
+        wide.assign(edges, edges + static_cast<size_t>(num_edges) * sz_edge_data);
+    } catch (...) {
+        return fail(ctx, CVG_ERR_OUT_OF_MEMORY, "cvg_run_f32: host allocation failed");
+    }
+    const double *d = nullptr;
+    CVG_TRY(upload(ctx, num_edges, sz_edge_data, wide.data(), &d));
+    // the upload must have left the host vector before it goes away
+    CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return run_common(ctx, num_edges, sz_edge_data, d, params, stats, 1, nullptr, true);
 }
 
 int cvg_run_device(cvg_context *ctx, int num_edges, int sz_edge_data, const double *device_edges,
@@ -3520,8 +4036,8 @@ int cvg_debug_cell_order(int num_edges, const double *x, const double *y, int *r
 
 int cvg_math_eval(cvg_context *ctx, int op, int n, const double *x, const double *y, double *out)
 {
-    if (!x || !out || n < 0 || op < 0 || op > 13) return CVG_ERR_INVALID_ARGUMENT;
-    if ((op == 2 || op == 5) && !y) return CVG_ERR_INVALID_ARGUMENT;
+    if (!x || !out || n < 0 || op < 0 || op > 18) return CVG_ERR_INVALID_ARGUMENT;
+    if ((op == 2 || op == 5 || op == 8 || op == 9 || op == 16) && !y) return CVG_ERR_INVALID_ARGUMENT;
     if (!ctx) {
         for (int i = 0; i < n; ++i)
             out[i] = math_op(op, x[i], y ? y[i] : 0.0, cvg::kSinCosTableHost, cvg::kAtanTableHost);

@@ -111,6 +111,11 @@ const char *cvg_last_error(const cvg_context *ctx);
  * *edges is allocated by the library (release with cvg_free). */
 int cvg_read_edge_file(const char *path, int values_per_edge, double **edges, int *num_edges, char *err,
                        size_t err_len);
+/* read_TO_edges_from_file<float> (data_io.hpp:20-78 on T = float, the --float
+ * mode of main.cpp:110): the same grammar and messages, each value converted
+ * from its decimal text directly to float (istream >> float). */
+int cvg_read_edge_file_f32(const char *path, int values_per_edge, float **edges, int *num_edges, char *err,
+                           size_t err_len);
 void cvg_free(void *ptr);
 /* write_int_array_to_file / write_double_array_to_file (data_io.hpp:80-133):
  * tab after every value, newline per row, doubles in default ostream format
@@ -125,6 +130,12 @@ int cvg_write_info_file(const char *path, const double *info, unsigned rows, uns
  * host array, row-major num_edges x sz_edge_data (sz_edge_data >= 4). */
 int cvg_build_neighbor_csr_graph(cvg_context *ctx, int num_edges, int sz_edge_data, const double *edges,
                                  double rad, unsigned neighbor_radius, unsigned max_candidates);
+/* build_neighbor_csr_graph<float>: the same on a float edge array; the
+ * context then runs the float instantiation (run_curvelet<float>,
+ * main.cpp:110) in the stage-wise calls below: float distances, bundles in
+ * compute_curve_bundle<float>'s arithmetic, info in fill_curvelet_info<float>'s. */
+int cvg_build_neighbor_csr_graph_f32(cvg_context *ctx, int num_edges, int sz_edge_data, const float *edges,
+                                     double rad, unsigned neighbor_radius, unsigned max_candidates);
 int cvg_get_graph_info(cvg_context *ctx, cvg_graph_info *info);
 /* neighbor_offsets (num_edges + 1), neighbor_ids and neighbor_dist2
  * (total_neighbor_pairs each) of CPUNeighborGraph<T>. */
@@ -157,6 +168,11 @@ int cvg_get_curvelets(cvg_context *ctx, unsigned *chains, double *info);
  * and fills them. */
 int cvg_run(cvg_context *ctx, int num_edges, int sz_edge_data, const double *edges, const cvg_params *params,
             cvg_run_stats *stats);
+/* run_curvelet<float> (main.cpp:110 with --float): the whole path on a float
+ * edge array (row-major num_edges x sz_edge_data), in the reference's float
+ * arithmetic; results as for cvg_run (chains, and info widened to double). */
+int cvg_run_f32(cvg_context *ctx, int num_edges, int sz_edge_data, const float *edges, const cvg_params *params,
+                cvg_run_stats *stats);
 /* Same, with the edge array already in device memory (for instance from
  * cvg_upload_edges).  No host<->device copy of edges or results. */
 int cvg_run_device(cvg_context *ctx, int num_edges, int sz_edge_data, const double *device_edges,
@@ -353,7 +369,9 @@ int cvg_debug_cell_order(int num_edges, const double *x, const double *y, int *r
  * per-pair transport sines compute it (subtraction formula on double-double
  * sin/cos of x and y, else the direct evaluation), 9 whether that formula
  * settled the rounding (1.0) or not (0.0), 10-13 the double-double sin(x)
- * high and low parts and cos(x) high and low parts.
+ * high and low parts and cos(x) high and low parts; 14-18 the binary32
+ * functions of the float mode on float(x) (and float(y)): sinf, cosf,
+ * atan2f(y, x), To2Pi, sqrtf.
  * ctx == NULL evaluates the host build of the same code; otherwise the device. */
 int cvg_math_eval(cvg_context *ctx, int op, int n, const double *x, const double *y, double *out);
 

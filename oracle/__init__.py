@@ -6,6 +6,9 @@ package never does.
 
   Oracle  - oracle/liboracle.so, the from-scratch CPU restatement of
             cpu_curvelet (curvelet_oracle.cpp).
+  OracleF32 - oracle/liboracle_f32.so (glibc) and liboracle_f32_cr.so
+            (correctly rounded float math), the restatement of the reference's
+            float instantiation (--float, curvelet_oracle_f32.cpp).
   Ref     - oracle/_ref/libcurvelet_ref.so, the reference's own cpu_curvelet
             headers compiled in place by `make -C oracle ref` (absent where the
             reference tree is not available; callers skip then).
@@ -22,6 +25,8 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 ORACLE_LIB = HERE / "liboracle.so"
 ORACLE_CR_LIB = HERE / "liboracle_cr.so"
+ORACLE_F32_LIB = HERE / "liboracle_f32.so"
+ORACLE_F32_CR_LIB = HERE / "liboracle_f32_cr.so"
 REF_LIB = HERE / "_ref" / "libcurvelet_ref.so"
 REF_CLI = HERE / "_ref" / "CURVELET"
 REFERENCE_SRC = Path("/root/reference/cpu_curvelet")
@@ -227,6 +232,31 @@ class Ref:
                 "valid_pairs": vp.value if valid_pairs else None, "chains": chains, "info": info,
                 "times": {"csr": t[0], "pairwise": t[1], "greedy": t[2]}}
 
+    def run_f32(self, edges: np.ndarray, p, nthreads: int = 1, valid_pairs: bool = True):
+        """run_curvelet<float> on a float32 edge array."""
+        e = np.ascontiguousarray(edges, dtype=np.float32)
+        n, sz = e.shape
+        rp = self.params(p)
+        fn = self.lib.ref_run_f32
+        fn.restype = C.c_int
+        fn.argtypes = [C.c_int, C.c_int, _P(C.c_float), _P(_RParams), C.c_int, C.c_int, C.c_int]
+        if fn(n, sz, _f(e, C.c_float), C.byref(rp), nthreads, 1, 1 if valid_pairs else 0) != 0:
+            raise ValueError("reference float run failed")
+        pairs, maxdeg, vp, nc, w = C.c_int(), C.c_uint(), C.c_uint(), C.c_uint(), C.c_uint()
+        self.lib.ref_counts_f32.argtypes = [_ip, _up, _up, _up, _up]
+        self.lib.ref_counts_f32(C.byref(pairs), C.byref(maxdeg), C.byref(vp), C.byref(nc), C.byref(w))
+        off = np.zeros(n + 1, np.int32)
+        ids = np.zeros(pairs.value, np.int32)
+        d2 = np.zeros(pairs.value)
+        self.lib.ref_get_csr_f32.argtypes = [_ip, _ip, _dp]
+        self.lib.ref_get_csr_f32(_f(off, C.c_int), _f(ids, C.c_int), _f(d2, C.c_double))
+        chains = np.zeros((nc.value, w.value), np.uint32)
+        info = np.zeros((nc.value, 10))
+        self.lib.ref_get_curvelets_f32.argtypes = [_up, _dp]
+        self.lib.ref_get_curvelets_f32(_f(chains, C.c_uint), _f(info, C.c_double))
+        return {"csr": {"offsets": off, "ids": ids, "dist2": d2, "pairs": pairs.value, "max_deg": maxdeg.value},
+                "valid_pairs": vp.value if valid_pairs else None, "chains": chains, "info": info}
+
     def pair_bundle(self, p, a, n):
         rp = self.params(p)
         lo = np.zeros(256)
@@ -238,6 +268,77 @@ class Ref:
 
     def dir_dot(self, a, n) -> float:
         return self.lib.ref_dir_dot(a[0], a[1], a[2], n[0], n[1])
+
+
+class _FParams(C.Structure):
+    _fields_ = [("nrad", C.c_double), ("dx", C.c_double), ("dt_deg", C.c_double), ("max_k", C.c_double),
+                ("sx", C.c_double), ("st", C.c_double), ("group_max_sz", C.c_uint),
+                ("max_candidates", C.c_uint), ("neighbor_radius", C.c_uint)]
+
+
+class OracleF32:
+    """The float instantiation's restatement; edges are float32 arrays."""
+
+    def __init__(self, cr: bool = False):
+        lib = C.CDLL(str(ORACLE_F32_CR_LIB if cr else ORACLE_F32_LIB))
+        _fp = _P(C.c_float)
+        sig = {
+            "orcf_create": (C.c_void_p, []),
+            "orcf_destroy": (None, [C.c_void_p]),
+            "orcf_build_csr": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _fp, _P(_FParams), C.c_int]),
+            "orcf_csr_counts": (None, [C.c_void_p, _ip, _up, _up]),
+            "orcf_get_csr": (None, [C.c_void_p, _ip, _ip, _dp]),
+            "orcf_valid_pairs": (C.c_uint, [C.c_void_p, _P(_FParams), C.c_int]),
+            "orcf_greedy": (C.c_uint, [C.c_void_p, _P(_FParams), C.c_int]),
+            "orcf_chain_width": (C.c_uint, [C.c_void_p]),
+            "orcf_get_curvelets": (None, [C.c_void_p, _up, _dp]),
+            "orcf_pair_bundle": (C.c_int, [_P(_FParams)] + [C.c_float] * 6 + [_fp, _fp]),
+        }
+        for k, (r, a) in sig.items():
+            fn = getattr(lib, k)
+            fn.restype = r
+            fn.argtypes = a
+        self.lib = lib
+        self.h = lib.orcf_create()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.orcf_destroy(self.h)
+            self.h = None
+
+    @staticmethod
+    def params(p) -> _FParams:
+        return _FParams(p.nrad, p.dx, p.dt_deg, p.max_k, p.sx, p.st, p.group_max_sz, p.max_candidates, 3)
+
+    def run(self, edges: np.ndarray, p, nthreads: int = 0, valid_pairs: bool = True):
+        e = np.ascontiguousarray(edges, dtype=np.float32)
+        n, sz = e.shape
+        fp = self.params(p)
+        if self.lib.orcf_build_csr(self.h, n, sz, _f(e, C.c_float), C.byref(fp), nthreads) != 0:
+            raise ValueError("orcf_build_csr failed")
+        pairs, maxdeg, trunc = C.c_int(), C.c_uint(), C.c_uint()
+        self.lib.orcf_csr_counts(self.h, C.byref(pairs), C.byref(maxdeg), C.byref(trunc))
+        off = np.zeros(n + 1, np.int32)
+        ids = np.zeros(pairs.value, np.int32)
+        d2 = np.zeros(pairs.value)
+        self.lib.orcf_get_csr(self.h, _f(off, C.c_int), _f(ids, C.c_int), _f(d2, C.c_double))
+        vp = int(self.lib.orcf_valid_pairs(self.h, C.byref(fp), nthreads)) if valid_pairs else None
+        nc = self.lib.orcf_greedy(self.h, C.byref(fp), nthreads)
+        w = self.lib.orcf_chain_width(self.h)
+        chains = np.zeros((nc, w), np.uint32)
+        info = np.zeros((nc, 10))
+        self.lib.orcf_get_curvelets(self.h, _f(chains, C.c_uint), _f(info, C.c_double))
+        return {"csr": {"offsets": off, "ids": ids, "dist2": d2, "pairs": pairs.value, "max_deg": maxdeg.value,
+                        "truncated": trunc.value},
+                "valid_pairs": vp, "chains": chains, "info": info}
+
+    def pair_bundle(self, p, a, n):
+        fp = self.params(p)
+        lo = np.zeros(256, np.float32)
+        hi = np.zeros(256, np.float32)
+        hyp = self.lib.orcf_pair_bundle(C.byref(fp), *[float(v) for v in a[:3]], *[float(v) for v in n[:3]],
+                                        _f(lo, C.c_float), _f(hi, C.c_float))
+        return bool(hyp), lo, hi
 
 
 def read_edges_text(path) -> np.ndarray:

@@ -26,6 +26,12 @@ synthetic maps from the package's generator (inputs/<case>.npy, written here).
 adds the full-size fixtures (scale_fixtures): correctly rounded oracle hashes
 and sampled reference rows for the 1080p map, and the reference's counts,
 chain hash and sampled rows for a 3840x2160 map.
+
+    python tests/golden/make_golden.py --float
+
+adds the --float fixtures (float_fixtures): the reference CLI's float-mode
+counts and file hashes (small cases' files kept) and its full-precision
+hashes.
 """
 from __future__ import annotations
 
@@ -88,16 +94,18 @@ def edges_of(key: str) -> np.ndarray:
     return np.load(f) if f.exists() else synth_edges(preset, int(seed))
 
 
-def run_cli(fname: str, text: bytes, G: int) -> tuple[dict, bytes, bytes]:
+def run_cli(fname: str, text: bytes, G: int, f32: bool = False) -> tuple[dict, bytes, bytes]:
     with tempfile.TemporaryDirectory() as td:
         td = Path(td)
         (td / "test_files").mkdir()
         (td / "run").mkdir()
         (td / "test_files" / fname).write_bytes(text)
-        out = subprocess.run([str(REF_CLI), "--edge-file", fname, "--group-max-sz", str(G), "--nthreads", "4"],
+        extra = ["--float"] if f32 else []
+        out = subprocess.run([str(REF_CLI), *extra, "--edge-file", fname, "--group-max-sz", str(G), "--nthreads", "4"],
                              cwd=td / "run", capture_output=True, text=True, check=True).stdout
-        chain = (td / "outputs" / "chain_cpu.txt").read_bytes()
-        info = (td / "outputs" / "info_cpu.txt").read_bytes()
+        suffix = "_float" if f32 else ""
+        chain = (td / "outputs" / f"chain_cpu{suffix}.txt").read_bytes()
+        info = (td / "outputs" / f"info_cpu{suffix}.txt").read_bytes()
     m = re.search(r"CSR neighbor graph: (\d+) anchors, (\d+) total pairs, max number of neighbors per anchor: (\d+)",
                   out)
     v = re.search(r"Pairwise curve bundles formed: (\d+) valid pairs", out)
@@ -179,6 +187,46 @@ def scale_fixtures() -> None:
     print("uhd7001_G4:", rec["edges"], "edges,", rec["out_h"], "rows")
 
 
+# --float cases: (case name, input, group_max_sz, keep the CLI's text files)
+FLOAT_CASES = [
+    ("digit1_G4_f32", "file:TO_edges_digit1", 4, True),
+    ("abc_G4_f32", "file:TO_edges_ABC_0006_thresh1", 4, True),
+    ("abc_G7_f32", "file:TO_edges_ABC_0006_thresh1", 7, False),
+    ("toedges_G7_f32", "file:TO_edges", 7, False),
+    ("c1s1001_G4_f32", "synth:C1:1001", 4, False),
+]
+
+
+def float_fixtures() -> None:
+    """The reference's --float runs (run_curvelet<float>): its CLI's counts and
+    file hashes (the small cases' files kept), and the SHA-256 of its
+    full-precision chains and info from the harness, on the float edges its
+    reader produces (decimal text straight to float)."""
+    from curvelet_construction_on_gpu_amd import read_TO_edges_from_file_f32
+    ref = Ref()
+    for name, key, G, keep_text in FLOAT_CASES:
+        fname, text = edge_text(key, False)
+        counts, chain, info = run_cli(fname, text, G, f32=True)
+        rec = dict(counts, input=key, group_max_sz=G, chain_sha256=sha(chain), info_sha256=sha(info),
+                   input_sha256=sha(text), scalar="float")
+        if keep_text:
+            for kind, data in (("chain", chain), ("info", info)):
+                with gzip.GzipFile(HERE / "ref_cli" / f"{name}_{kind}.txt.gz", "wb", mtime=0) as f:
+                    f.write(data)
+        with tempfile.TemporaryDirectory() as td:
+            src = Path(td) / fname
+            src.write_bytes(text)
+            e = read_TO_edges_from_file_f32(src)
+        r = ref.run_f32(e, CurveletParams(group_max_sz=G), nthreads=8)
+        assert r["csr"]["pairs"] == counts["pairs"] and r["valid_pairs"] == counts["valid_pairs"]
+        assert len(r["chains"]) == counts["out_h"]
+        rec["edges_f32_sha256"] = sha(np.ascontiguousarray(e).tobytes())
+        rec["full_chain_sha256"] = sha(np.ascontiguousarray(r["chains"]).tobytes())
+        rec["full_info_sha256"] = sha(np.ascontiguousarray(r["info"]).tobytes())
+        (HERE / "ref_cli" / f"{name}.json").write_text(json.dumps(rec, indent=1) + "\n")
+        print(name, counts)
+
+
 def main() -> int:
     if not REF_CLI.exists() or not Ref.available():
         print("oracle/_ref is missing: run `make -C oracle ref` where the reference sources exist", file=sys.stderr)
@@ -242,6 +290,13 @@ def main() -> int:
 
 
 if __name__ == "__main__":
+    if sys.argv[1:] == ["--float"]:
+        if not Ref.available():
+            print("oracle/_ref is missing: run `make -C oracle ref` where the reference sources exist",
+                  file=sys.stderr)
+            sys.exit(1)
+        float_fixtures()
+        sys.exit(0)
     if sys.argv[1:] == ["--scale"]:
         if not Ref.available():
             print("oracle/_ref is missing: run `make -C oracle ref` where the reference sources exist",

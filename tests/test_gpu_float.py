@@ -1,0 +1,206 @@
+"""The float instantiation (the reference CLI's --float, run_curvelet<float>)
+on the GPU against the float oracle.
+
+Bars:
+  OracleF32(cr=True) - the float restatement with correctly rounded sinf, cosf
+                       and atan2f (the device's math): CSR, valid pairs,
+                       chains and info bit-exact.
+  OracleF32()        - glibc's float math, bit-exact with the reference build:
+                       chains and counts bit-exact, info within 1e-4 relative
+                       (1e-6 absolute), since glibc's sinf/cosf/atan2f are not
+                       correctly rounded and move a few rows by float ulps.
+  reference fixtures - counts and the SHA-256 of every chain row of the
+                       reference's own --float runs.
+"""
+import gzip
+import hashlib
+import json
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from curvelet_construction_on_gpu_amd import CurveletParams, math_eval
+from tests._inputs import GOLDEN, edges_for, shipped_text
+from tests.test_oracle_golden import FLOAT_CASES, _META, float_edges
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parents[1]
+CLI = ROOT / "curvelet_construction_on_gpu_amd" / "bin" / "CURVELET"
+
+INFO_RTOL = 1e-4
+INFO_ATOL = 1e-6
+
+
+def sha(b):
+    return hashlib.sha256(b).hexdigest()
+
+
+@pytest.fixture(scope="module")
+def oracle_f32():
+    from oracle import OracleF32
+    return OracleF32()
+
+
+@pytest.fixture(scope="module")
+def oracle_f32_cr():
+    from oracle import OracleF32
+    return OracleF32(cr=True)
+
+
+def test_device_float_math_matches_host_build(ctx):
+    rng = np.random.default_rng(14)
+    x = np.concatenate([rng.uniform(-8, 8, 40000), rng.uniform(-1e4, 1e4, 4000),
+                        np.ldexp(rng.uniform(-1, 1, 4000), -rng.integers(0, 40, 4000)),
+                        [0.0, -0.0, 1.5707963, 3.1415927, 6.2831855]]).astype(np.float32).astype(np.float64)
+    y = rng.uniform(-8, 8, x.size).astype(np.float32).astype(np.float64)
+    for op in ("sinf", "cosf", "atan2f", "to2pi_f"):
+        d = math_eval(op, x, y, ctx=ctx)
+        h = math_eval(op, x, y, device=False)
+        assert np.array_equal(d.view(np.uint64), h.view(np.uint64)), op
+        assert np.array_equal(d.astype(np.float32).astype(np.float64), d), op
+    s = np.abs(x)
+    assert np.array_equal(math_eval("sqrtf", s, ctx=ctx), np.sqrt(s.astype(np.float32)).astype(np.float64))
+
+
+def _run(ctx, e, p):
+    stats = ctx.run_f32(e, p)
+    chains, info = ctx.curvelets()
+    return stats, chains, info
+
+
+@pytest.mark.parametrize("G", [4, 7])
+@pytest.mark.parametrize("key", ["file:TO_edges_digit1", "file:TO_edges_ABC_0006_thresh1", "file:TO_edges",
+                                 "synth:C1:1001", "synth:C2:1002"])
+def test_float_curvelets_parity(ctx, oracle_f32, oracle_f32_cr, key, G):
+    e = edges_for(key).astype(np.float32)
+    p = CurveletParams(group_max_sz=G)
+    stats, chains, info = _run(ctx, e, p)
+    want = oracle_f32_cr.run(e, p, nthreads=0)
+    g = ctx.graph()
+    assert stats["total_neighbor_pairs"] == want["csr"]["pairs"]
+    assert np.array_equal(g.neighbor_offsets, want["csr"]["offsets"])
+    assert np.array_equal(g.neighbor_ids, want["csr"]["ids"])
+    assert stats["valid_pairs"] == want["valid_pairs"]
+    assert np.array_equal(chains, want["chains"])
+    assert np.array_equal(info.view(np.uint64), want["info"].view(np.uint64))
+    glibc = oracle_f32.run(e, p, nthreads=0)
+    assert stats["valid_pairs"] == glibc["valid_pairs"]
+    assert np.array_equal(chains, glibc["chains"])
+    np.testing.assert_allclose(info, glibc["info"], rtol=INFO_RTOL, atol=INFO_ATOL)
+
+
+@pytest.mark.parametrize("name", FLOAT_CASES)
+def test_float_against_reference_runs(ctx, name):
+    """The reference's --float runs on the same text inputs: counts and the
+    hash of every full-precision chain row."""
+    meta = _META[name]
+    e = float_edges(meta)
+    stats, chains, info = _run(ctx, e, CurveletParams(group_max_sz=meta["group_max_sz"]))
+    g = ctx.graph()
+    assert (g.num_edges, g.total_neighbor_pairs, g.max_num_of_neighbors) == (meta["edges"], meta["pairs"],
+                                                                             meta["max_deg"])
+    assert stats["valid_pairs"] == meta["valid_pairs"]
+    assert chains.shape == (meta["out_h"], meta["out_w"])
+    assert sha(np.ascontiguousarray(chains).tobytes()) == meta["full_chain_sha256"]
+
+
+def test_float_full_size_map(ctx, oracle_f32_cr):
+    """The 1920x1080 benchmark map in float: every chain and info bit against
+    the correctly rounded float oracle."""
+    e = edges_for("synth:C3:1003").astype(np.float32)
+    p = CurveletParams(group_max_sz=7)
+    stats, chains, info = _run(ctx, e, p)
+    assert ctx.graph().order_ambiguous_anchors == 0
+    want = oracle_f32_cr.run(e, p, nthreads=0)
+    assert stats["valid_pairs"] == want["valid_pairs"]
+    assert sha(np.ascontiguousarray(chains).tobytes()) == sha(np.ascontiguousarray(want["chains"]).tobytes())
+    assert sha(np.ascontiguousarray(info).tobytes()) == sha(np.ascontiguousarray(want["info"]).tobytes())
+
+
+def _check_cr(ctx, oracle_f32_cr, e, p):
+    stats, chains, info = _run(ctx, e, p)
+    want = oracle_f32_cr.run(e, p, nthreads=0)
+    assert stats["valid_pairs"] == want["valid_pairs"]
+    assert np.array_equal(chains, want["chains"])
+    assert np.array_equal(info.view(np.uint64), want["info"].view(np.uint64))
+    return stats
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 17])
+def test_float_tiny_maps(ctx, oracle_f32_cr, n):
+    _check_cr(ctx, oracle_f32_cr, edges_for("synth:C1:1001")[:n].astype(np.float32), CurveletParams())
+
+
+def test_float_empty_rejected(ctx):
+    from curvelet_construction_on_gpu_amd import CurveletError
+    with pytest.raises(CurveletError):
+        ctx.run_f32(np.zeros((0, 4), np.float32), CurveletParams())
+
+
+@pytest.mark.parametrize("G", [0, 1, 2, 12, 64])
+def test_float_group_sizes(ctx, oracle_f32_cr, G):
+    stats = _check_cr(ctx, oracle_f32_cr, edges_for("synth:C1:1002").astype(np.float32),
+                      CurveletParams(group_max_sz=G))
+    if G < 3:
+        assert stats["num_curvelets"] == 0
+    assert stats["chain_width"] == G + 1
+
+
+@pytest.mark.parametrize("sx,st,nrad,dx,max_k", [(0.05, 0.08, 3.5, 0.4, 0.3), (0.2, 0.3, 2.0, 0.25, 0.5),
+                                                 (0.1, 0.1, 5.0, 0.8, 0.1)])
+def test_float_other_parameters(ctx, oracle_f32_cr, sx, st, nrad, dx, max_k):
+    e = edges_for("synth:C1:1003")[:4000].astype(np.float32)
+    _check_cr(ctx, oracle_f32_cr, e, CurveletParams(sx=sx, st=st, nrad=nrad, dx=dx, max_k=max_k, group_max_sz=6))
+
+
+def test_float_wider_rows(ctx, oracle_f32_cr):
+    e = edges_for("synth:C2:1002")[:6000].copy()
+    e[:, 0] -= 250.0
+    e6 = np.concatenate([e, np.full((len(e), 2), 7.0)], axis=1).astype(np.float32)
+    _check_cr(ctx, oracle_f32_cr, e6, CurveletParams(edge_data_sz=6))
+
+
+def test_float_then_double_same_context(ctx, oracle_cr):
+    """Switching scalar type between calls on one context leaves no state
+    behind: a double run after a float run is the double run."""
+    e = edges_for("file:TO_edges_digit1")
+    p = CurveletParams(group_max_sz=5)
+    ctx.run_f32(e.astype(np.float32), p)
+    ctx.run(e, p)
+    chains, info = ctx.curvelets()
+    want = oracle_cr.run(e, p, nthreads=0)
+    assert np.array_equal(chains, want["chains"])
+    assert np.array_equal(info.view(np.uint64), want["info"].view(np.uint64))
+
+
+@pytest.mark.parametrize("case,inp", [("digit1_G4_f32", "TO_edges_digit1"),
+                                      ("abc_G4_f32", "TO_edges_ABC_0006_thresh1")])
+def test_cli_float(tmp_path, case, inp):
+    """CURVELET --float: the reference's printed counts and chain file byte
+    for byte; the info file's values within the float bar above."""
+    assert CLI.exists(), "CURVELET is not built"
+    (tmp_path / "test_files").mkdir()
+    (tmp_path / "run").mkdir()
+    (tmp_path / "test_files" / f"{inp}.txt").write_bytes(shipped_text(inp))
+    meta = json.loads((GOLDEN / "ref_cli" / f"{case}.json").read_text())
+    args = [str(CLI), "--edge-file", f"{inp}.txt", "--group-max-sz", str(meta["group_max_sz"]), "--float"]
+    r = subprocess.run(args, cwd=tmp_path / "run", capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Using scalar type: float" in r.stdout
+    m = re.search(r"CSR neighbor graph: (\d+) anchors, (\d+) total pairs, max number of neighbors per anchor: (\d+)",
+                  r.stdout)
+    assert m and (int(m.group(1)), int(m.group(2)), int(m.group(3))) == (meta["edges"], meta["pairs"],
+                                                                          meta["max_deg"])
+    assert f"Pairwise curve bundles formed: {meta['valid_pairs']} valid pairs" in r.stdout
+    out = tmp_path / "outputs"
+    chain = (out / "chain_cpu_float.txt").read_bytes()
+    assert sha(chain) == meta["chain_sha256"]
+    with gzip.open(GOLDEN / "ref_cli" / f"{case}_info.txt.gz") as f:
+        want = np.loadtxt(f, ndmin=2)
+    got = np.loadtxt(out / "info_cpu_float.txt", ndmin=2)
+    assert got.shape == want.shape
+    np.testing.assert_allclose(got, want, rtol=INFO_RTOL, atol=INFO_ATOL)

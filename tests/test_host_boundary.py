@@ -405,9 +405,16 @@ def test_cli_errors(workdir, args, code, needle):
 
 
 @needs_cli
-def test_cli_float_rejected(workdir):
-    r = run_cli(CLI, ["--float", "--edge-file", "bad.txt"], workdir)
-    assert r.returncode == 1 and "--float" in r.stderr
+def test_cli_float_reader_errors(workdir):
+    """--float reads through the float reader, which reports a malformed
+    line as the double reader does (and as the reference's --float does)."""
+    args = ["--float", "--edge-file", "bad.txt"]
+    r = run_cli(CLI, args, workdir)
+    assert r.returncode == 1 and "expected 4 numeric values per line, found 3." in r.stderr
+    assert "Using scalar type: float" in r.stdout
+    if REF_CLI.exists():
+        ref = run_cli(REF_CLI, args, workdir)
+        assert ref.returncode == 1 and "expected 4 numeric values per line, found 3." in ref.stderr
 
 
 def test_server_rejects_bad_config_without_a_device():

@@ -19,9 +19,12 @@ import pytest
 from curvelet_construction_on_gpu_amd import CurveletParams, write_chain_file, write_info_file
 from tests._inputs import GOLDEN, edges_for
 
-CASES = sorted(p.stem for p in (GOLDEN / "ref_cli").glob("*.json") if "input" in json.loads(p.read_text()))
+_META = {p.stem: json.loads(p.read_text()) for p in (GOLDEN / "ref_cli").glob("*.json")}
+CASES = sorted(k for k, m in _META.items() if "input" in m and m.get("scalar", "double") == "double")
+FLOAT_CASES = sorted(k for k, m in _META.items() if m.get("scalar") == "float")
 FULL = sorted(p.stem for p in (GOLDEN / "full").glob("*.npz") if not p.stem.endswith("_rows"))
-TEXT = sorted(p.name[:-len("_chain.txt.gz")] for p in (GOLDEN / "ref_cli").glob("*_chain.txt.gz"))
+TEXT = sorted(p.name[:-len("_chain.txt.gz")] for p in (GOLDEN / "ref_cli").glob("*_chain.txt.gz")
+              if not p.name.startswith(tuple(FLOAT_CASES)))
 
 
 def case_meta(name):
@@ -204,3 +207,94 @@ def test_full_size_fixtures(oracle, oracle_cr):
     assert np.array_equal(r["info"][rows["rows"]].view(np.uint64), rows["info"].view(np.uint64))
     w = oracle_cr.run(e, p, nthreads=0, valid_pairs=False)
     assert sha(np.ascontiguousarray(w["info"]).tobytes()) == um["cr_info_sha256"]
+
+
+def float_edges(meta):
+    """The float edges the reference's --float reader produces from the case's text."""
+    import tempfile
+    from pathlib import Path
+    from curvelet_construction_on_gpu_amd import read_TO_edges_from_file_f32
+    from tests._inputs import shipped_text
+    key = meta["input"]
+    if key.startswith("file:"):
+        text = shipped_text(key[5:])
+    else:
+        e = case_edges({"input": key})
+        text = ("\n".join("\t".join(repr(float(v)) for v in row) for row in e) + "\n").encode()
+    assert sha(text) == meta["input_sha256"]
+    with tempfile.TemporaryDirectory() as td:
+        f = Path(td) / "edges.txt"
+        f.write_bytes(text)
+        e32 = read_TO_edges_from_file_f32(f)
+    assert sha(np.ascontiguousarray(e32).tobytes()) == meta["edges_f32_sha256"]
+    return e32
+
+
+@pytest.mark.parametrize("name", FLOAT_CASES)
+def test_float_oracle_against_reference(name, tmp_path):
+    """The float-instantiation restatement (glibc) reproduces the reference's
+    --float run bit for bit (counts, full-precision chains and info, the CLI's
+    text files); the correctly rounded variant (the GPU path's math) gives the
+    same chains and valid pairs on these inputs; glibc's sinf and cosf are not
+    correctly rounded (about 1.3% of arguments differ by an ulp), which moves
+    the info of 3-7% of rows by a few float ulps (at most 3.3e-7 absolute
+    here)."""
+    from oracle import OracleF32
+    meta = _META[name]
+    e = float_edges(meta)
+    p = CurveletParams(group_max_sz=meta["group_max_sz"])
+    r = OracleF32().run(e, p, nthreads=0)
+    assert (r["csr"]["pairs"], r["csr"]["max_deg"], r["valid_pairs"]) == (meta["pairs"], meta["max_deg"],
+                                                                          meta["valid_pairs"])
+    assert r["chains"].shape == (meta["out_h"], meta["out_w"])
+    assert sha(np.ascontiguousarray(r["chains"]).tobytes()) == meta["full_chain_sha256"]
+    assert sha(np.ascontiguousarray(r["info"]).tobytes()) == meta["full_info_sha256"]
+    write_chain_file(tmp_path / "c.txt", r["chains"])
+    write_info_file(tmp_path / "i.txt", r["info"])
+    assert sha((tmp_path / "c.txt").read_bytes()) == meta["chain_sha256"]
+    assert sha((tmp_path / "i.txt").read_bytes()) == meta["info_sha256"]
+    c = OracleF32(cr=True).run(e, p, nthreads=0)
+    assert c["valid_pairs"] == r["valid_pairs"]
+    assert np.array_equal(c["chains"], r["chains"])
+    np.testing.assert_allclose(c["info"], r["info"], rtol=1e-4, atol=1e-6)
+
+
+def test_float_reader_is_strtof():
+    """The --float reader converts each decimal token straight to float, as
+    istream >> float does (libc strtof), not through a double."""
+    import ctypes as C
+    import tempfile
+    from pathlib import Path
+    from curvelet_construction_on_gpu_amd import read_TO_edges_from_file_f32
+    libc = C.CDLL("libc.so.6")
+    libc.strtof.restype = C.c_float
+    libc.strtof.argtypes = [C.c_char_p, C.c_void_p]
+    rng = np.random.default_rng(8)
+    # decimal strings that sit just beside float midpoints: the double nearest
+    # to them rounds to the other float
+    from decimal import Decimal, getcontext
+    getcontext().prec = 60
+    toks = []
+    for _ in range(400):
+        f = np.float32(rng.uniform(0.5, 1000))
+        g = np.nextafter(f, np.float32(np.inf))
+        mid = (np.float64(f) + np.float64(g)) / 2
+        ulp = np.nextafter(mid, np.inf) - mid
+        # a quarter of a double ulp above the midpoint: the nearest double is
+        # the midpoint itself, which then rounds to the even float
+        toks.append(str(Decimal(float(mid)) + Decimal(float(ulp)) / 4))
+        toks.append(f"{mid:.25g}")
+        toks.append(repr(float(rng.uniform(-500, 500))))
+    toks += ["0.1", "1e-50", "-0", "3.4028234e38", "1e-38", "7", "-2.5e-3"]
+    while len(toks) % 4:
+        toks.append("1")
+    text = "\n".join(" ".join(toks[i:i + 4]) for i in range(0, len(toks), 4)) + "\n"
+    with tempfile.TemporaryDirectory() as td:
+        f = Path(td) / "e.txt"
+        f.write_text(text)
+        got = read_TO_edges_from_file_f32(f).ravel()
+    want = np.array([libc.strtof(t.encode(), None) for t in toks], dtype=np.float32)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    # the double route would differ on some of these
+    via_double = np.array([np.float32(float(t)) for t in toks], dtype=np.float32)
+    assert not np.array_equal(via_double.view(np.uint32), want.view(np.uint32))

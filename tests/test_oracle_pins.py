@@ -20,7 +20,8 @@ term, a flipped sign, a swapped operand, an off-by-one window) fails it:
   decimal evaluation: sin, cos and atan2 correctly rounded, sqrt and fmod
   exact; the double-double sin/cos within 2^-86 of the true values, and the
   transport's sine of a difference (subtraction formula with a rounding test)
-  correctly rounded and equal to the direct evaluation.
+  correctly rounded and equal to the direct evaluation; the float
+  instantiation's sinf, cosf and atan2f correctly rounded to binary32.
 """
 import math
 from decimal import Decimal, getcontext
@@ -309,6 +310,52 @@ def test_host_sqrt_fmod_exact():
     t = math_eval("to2pi", a, device=False)
     assert np.array_equal(t, np.array([to2pi(v) for v in a]))
 
+
+
+def d_to_f32(v: Decimal) -> float:
+    """The binary32 value nearest v (ties to even), by comparing v with the
+    neighbours of a first guess."""
+    f = np.float32(float(v))
+    cands = [np.nextafter(f, np.float32(-np.inf)), f, np.nextafter(f, np.float32(np.inf))]
+    best = min(cands, key=lambda c: (abs(Decimal(float(c)) - v), int(np.float32(c).view(np.uint32)) & 1))
+    return float(best)
+
+
+def test_host_float_sin_cos_atan2_correctly_rounded():
+    """The float instantiation's sinf, cosf and atan2f (binary32 in and out)
+    are the correctly rounded values of sin, cos and atan2 at the float
+    argument."""
+    rng = np.random.default_rng(15)
+    x = np.concatenate([rng.uniform(-7.0, 7.0, 1200), rng.uniform(-3e3, 3e3, 150), rng.uniform(-1e-3, 1e-3, 100),
+                        np.ldexp(rng.uniform(-1, 1, 50), -rng.integers(10, 120, 50)),
+                        np.array([0.5, 1.0, math.pi / 2, math.pi, 2 * math.pi, 3.0, -2.75, 1e-9])])
+    x = x.astype(np.float32).astype(np.float64)
+    s = math_eval("sinf", x, device=False)
+    c = math_eval("cosf", x, device=False)
+    for xi, si, ci in zip(x, s, c):
+        assert si == d_to_f32(d_sin(Decimal(xi))), xi
+        assert ci == d_to_f32(d_cos(Decimal(xi))), xi
+    ys = np.concatenate([rng.uniform(-4, 4, 1200), [0.0, 1.0, -1.0, 3.0, 1e-12, -0.0]]).astype(np.float32)
+    xs = np.concatenate([rng.uniform(-4, 4, 1200), [1.0, 0.0, 0.0, -3.0, -1.0, -2.0]]).astype(np.float32)
+    got = math_eval("atan2f", xs.astype(np.float64), ys.astype(np.float64), device=False)
+    for yi, xi, gi in zip(ys, xs, got):
+        if yi == 0 and xi < 0:
+            assert gi == float(np.float32(math.copysign(math.pi, yi))), (yi, xi)
+            continue
+        assert gi == d_to_f32(d_atan2(float(yi), float(xi))), (yi, xi)
+
+
+def test_host_float_to2pi_sqrt():
+    """To2Pi on T = float (curvelet_utils.hpp:26-37): float(2) * float(M_PI),
+    an exact fmod, and the float sum; sqrtf is the IEEE square root."""
+    rng = np.random.default_rng(16)
+    a = rng.uniform(-30, 30, 4000).astype(np.float32)
+    two_pi = np.float32(2) * np.float32(math.pi)
+    want = np.where(a >= two_pi, np.fmod(a, two_pi), np.where(a < 0, two_pi + np.fmod(a, two_pi), a))
+    want = np.where((want >= 0) & (want < two_pi), want, np.float32(0)).astype(np.float32)
+    assert np.array_equal(math_eval("to2pi_f", a.astype(np.float64), device=False), want.astype(np.float64))
+    x = rng.uniform(0, 100, 4000).astype(np.float32)
+    assert np.array_equal(math_eval("sqrtf", x.astype(np.float64), device=False), np.sqrt(x).astype(np.float64))
 
 def test_host_sin_cos_agree_with_joint_sincos():
     """sin and cos are evaluated by one branch-free expression (the cosine as the
