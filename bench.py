@@ -324,6 +324,10 @@ def main(argv=None) -> int:
                     help="map: the 1080p map (the headline line); batch: 64 mixed-size maps, batched "
                          "form against the plain per-map loop; serve: a Poisson request stream through "
                          "the served form (continuous batching), latency and maps/s")
+    ap.add_argument("--span-only", action="store_true",
+                    help="map workload: only the protocol-span and resident steps of the 1080p map on one "
+                         "context (no CPU baseline, two-context pipelined run, C1 map or text end-to-end), "
+                         "so that a profiler's trace splits into whole steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -376,7 +380,7 @@ def main(argv=None) -> int:
     resident_s = (time.perf_counter() - t0) / args.steps
     fused = ctx.recent_fused_ms(args.steps)
     st = ctx.run_device(dptr, n, sz, params)  # counts and stage times (untimed)
-    pipelined = pipelined_pcie(edges, params, max(args.steps, 20))
+    pipelined = None if args.span_only else pipelined_pcie(edges, params, max(args.steps, 20))
 
     pairs = st["total_neighbor_pairs"]
     fused_ms = statistics.mean(fused)
@@ -428,6 +432,11 @@ def main(argv=None) -> int:
         gbps = traffic / (fused_ms * 1e-3) / 1e9
         line["roofline_hbm"] = {"achieved": gbps, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                                 "frac": gbps / PEAK_HBM_GBS, "traffic": traffic}
+    if args.span_only:
+        line["passes"] = {"protocol_span": 1 + args.warmup + args.steps, "resident": args.warmup + args.steps + 1}
+        ctx.close()
+        print(json.dumps(line))
+        return 0
     if not args.no_cpu_baseline:
         cb = cpu_baseline(edges, params)
         if cb is not None:
