@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <atomic>
 #include <chrono>
 #include <climits>
@@ -45,6 +46,8 @@
 #include <string>
 #include <thread>
 #include <vector>
+
+#include <hipcub/hipcub.hpp>
 
 #include "../../include/curvelet_gpu.h"
 #include "cvg_device.hpp"
@@ -67,7 +70,8 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kInfoWidth = 10;
-constexpr long long kMaxGridCells = 1LL << 28;
+constexpr long long kMaxGridCells = 1LL << 28;  // larger spans take the sorted-key cells
+constexpr double kMaxCoord = 2147475456.0;        // 2^31 - 2^13
 // the static scratch array behind cvg_run_device_noalloc (one per device)
 constexpr size_t kScratchBytes = static_cast<size_t>(CVG_NOALLOC_SCRATCH_MIB) << 20;
 
@@ -107,6 +111,12 @@ struct cvg_context {
     // grid of the light buckets' launches as a percentage of their resident
     // capacity (25..100), or 0: automatic (cvg_set_tuning, or CVG_GRID_PCT)
     int grid_pct = 0;
+    // use the sorted-key cell layout even where the dense grid fits
+    // (CVG_SPARSE_CELLS=1 at creation; results never depend on it)
+    bool force_sparse = false;
+    // run double maps on the general curvelet kernel even where the fused one
+    // applies (CVG_GENERIC_KERNEL=1 at creation; same results)
+    bool force_generic = false;
 
     unsigned *mailbox = nullptr;  // pinned host scalars read back after syncs
     // page-locked staging blocks for downloads into pageable caller memory
@@ -134,7 +144,7 @@ struct cvg_context {
     // the run's map came in as float (the reference's float instantiation):
     // distances, bundles and info in its float arithmetic
     bool f32 = false;
-    DevBuf d_cellrank;
+    DevBuf d_cellrank, d_skey, d_sorttmp;
 
     // the maps of a batch (a single map outside batches): edge id ranges,
     // per-edge map ids, per-map cell boxes and shifts, output row ranges
@@ -179,7 +189,8 @@ struct cvg_context {
                           &d_off,      &d_tiles,  &d_cand,   &d_ids,    &d_d2,     &d_pair_anchor, &d_trig,
                           &d_tdd,      &d_cls,    &d_prec,   &d_psin,   &d_lists,  &d_tables,      &d_st_ids,
                           &d_st_meta,  &d_st_k,   &d_st_rank, &d_acc,   &d_rowoff, &d_gbundle,     &d_chains,
-                          &d_info,     &d_dbg,    &d_map,    &d_eoff,   &d_mapbox, &d_maprow,       &d_cellrank};
+                          &d_info,     &d_dbg,    &d_map,    &d_eoff,   &d_mapbox, &d_maprow,       &d_cellrank,
+                          &d_skey,     &d_sorttmp};
         for (DevBuf *b : bufs) f(*b);
     }
 };
@@ -435,6 +446,50 @@ struct GridDesc {
     int x0, y0, w, h;
 };
 
+// Where the edges of a window row lie in the cell-sorted arrays.  The dense
+// grid keeps every cell's start (one load per row end); maps whose cells span
+// more than the dense grid allows keep the edges' sorted linear cell keys
+// instead, and a row's ends are binary searches over them.  Both give the
+// same sorted positions, so the scans below are the same for either.
+struct DenseCells {
+    const unsigned *start;
+    int x0, y0, w;
+    // edges of cells (cx - R .. cx + R, py), py absolute
+    __device__ void row(int cx, int py, int R, int &rs, int &len) const
+    {
+        const int rr = (py - y0) * w + (cx - x0);
+        rs = static_cast<int>(start[rr - R]);
+        len = static_cast<int>(start[rr + R + 1]) - rs;
+    }
+};
+
+struct SparseCells {
+    // cell key ((cy - y0) << 32) + (cx - x0) of each sorted position, ascending
+    // (row-major; both offsets are below 2^32)
+    const unsigned long long *key;
+    int n;
+    int x0, y0;
+    __device__ int lower(unsigned long long k) const
+    {
+        int lo = 0, hi = n;
+        while (lo < hi) {
+            const int mid = lo + ((hi - lo) >> 1);
+            if (key[mid] < k) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    }
+    __device__ void row(int cx, int py, int R, int &rs, int &len) const
+    {
+        // cx - R - x0 >= 0; cx + R + 1 - x0 <= 2^32 carries into the next
+        // row's first key, which is still this row's end
+        const unsigned long long b = (static_cast<unsigned long long>(static_cast<long long>(py) - y0) << 32) +
+                                     static_cast<unsigned long long>(static_cast<long long>(cx) - x0);
+        rs = lower(b - static_cast<unsigned>(R));
+        len = lower(b + static_cast<unsigned>(R) + 1ULL) - rs;
+    }
+};
+
 // Scalars block layout (d_scal, uint32): [0..3] bbox (minx, maxx, miny, maxy as
 // int), [4] bad-input flag, [5] max degree, [6] truncated anchors,
 // [7] order-ambiguous anchors, [8] valid pairs, [9] anchors needing the
@@ -475,8 +530,9 @@ __global__ void k_init_scalars(unsigned *s)
 }
 
 // Cells are (lrint(x), lrint(y)) (preprocess.hpp:85-86).  Coordinates must be
-// finite with |x|, |y| < 2^30 and |theta| < 2^20 (beyond that the reference's
-// int casts and the trig reduction leave their domain); others raise S_BAD.
+// finite with |x|, |y| < 2^31 - 2^13 (so that every window cell cx +- R of the
+// reference's int arithmetic exists) and |theta| < 2^20 (the trig
+// reduction's domain); others raise S_BAD.
 // Each workgroup leaves its cell box and bad flag in part[5 * block]
 // (k_bbox_finish reduces them).
 __global__ __launch_bounds__(kBlock) void k_prep_edges(int n, int sz, const double *__restrict__ e,
@@ -498,7 +554,7 @@ __global__ __launch_bounds__(kBlock) void k_prep_edges(int n, int sz, const doub
         const double xv = e[static_cast<long long>(i) * sz];
         const double yv = e[static_cast<long long>(i) * sz + 1];
         const double tv = e[static_cast<long long>(i) * sz + 2];
-        const bool ok = fabs(xv) < 0x1p30 && fabs(yv) < 0x1p30 && fabs(tv) < 0x1p20;
+        const bool ok = fabs(xv) < kMaxCoord && fabs(yv) < kMaxCoord && fabs(tv) < 0x1p20;
         if (!ok) atomicOr(&sbad, 1);
         const int ix = ok ? static_cast<int>(rint(xv)) : 0;
         const int iy = ok ? static_cast<int>(rint(yv)) : 0;
@@ -708,6 +764,54 @@ __global__ __launch_bounds__(kBlock) void k_cell_order(int n, const int *__restr
     sxy[p] = make_double2(x[i], y[i]);
 }
 
+// Maps whose cells span more than the dense grid allows: every edge's cell
+// key in the bounding box (row offset in the high 32 bits, column offset in
+// the low ones, so keys order row-major), and its id (or, when the
+// reference's within-cell order is replayed, its rank in its cell) for the
+// stable sorts that follow.
+__global__ __launch_bounds__(kBlock) void k_sparse_keys(int n, const int *__restrict__ cx, const int *__restrict__ cy,
+                                                        int x0, int y0, const int *__restrict__ rank,
+                                                        unsigned long long *__restrict__ key, int *__restrict__ rkey,
+                                                        int *__restrict__ id)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    key[i] = (static_cast<unsigned long long>(static_cast<long long>(cy[i]) - y0) << 32) +
+             static_cast<unsigned long long>(static_cast<long long>(cx[i]) - x0);
+    if (rank) rkey[i] = rank[i];
+    id[i] = i;
+}
+
+// The keys of edges already ordered by their within-cell rank.
+__global__ __launch_bounds__(kBlock) void k_gather_keys(int n, const unsigned long long *__restrict__ key,
+                                                        const int *__restrict__ id, unsigned long long *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = key[id[i]];
+}
+
+// The cell-sorted arrays from the sorted (key, id) pairs; a cell's identity
+// is its first sorted position.
+__global__ __launch_bounds__(kBlock) void k_sparse_fill(int n, const unsigned long long *__restrict__ skey,
+                                                        const int *__restrict__ sid, const double *__restrict__ x,
+                                                        const double *__restrict__ y, int *__restrict__ sorted,
+                                                        int *__restrict__ scell, double2 *__restrict__ sxy)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const unsigned long long k = skey[p];
+    int lo = 0, hi = p;
+    while (lo < hi) {
+        const int mid = lo + ((hi - lo) >> 1);
+        if (skey[mid] < k) lo = mid + 1;
+        else hi = mid;
+    }
+    const int i = sid[p];
+    sorted[p] = i;
+    scell[p] = lo;
+    sxy[p] = make_double2(x[i], y[i]);
+}
+
 // sq_dist<T> (curvelet_utils.hpp:13-17) of edge coordinates held as doubles:
 // in double, or for maps read as float (the float instantiation, --float) in
 // float arithmetic on the (exact) float values.
@@ -748,18 +852,14 @@ __device__ inline int bcast(int v, int lane) { return __builtin_amdgcn_readlane(
 // the segment holding row rb+s: its first sorted position `rs` and the
 // exclusive prefix `pre` of the row lengths; `tot` is the rows' total.  Row
 // loads are independent, so the whole window costs one load latency.
-template <int G>
-__device__ inline void window_rows(const unsigned *__restrict__ start, GridDesc g, int ix, int iy, int R, int rb,
-                                   int sl, bool valid, int &rs, int &pre, int &tot)
+template <int G, class Cells>
+__device__ inline void window_rows(const Cells &cells, int cx, int cy, int R, int rb, int sl, bool valid, int &rs,
+                                   int &pre, int &tot)
 {
     const int r = rb + sl;
     int len = 0;
     rs = 0;
-    if (valid && r <= 2 * R) {
-        const int rr = (iy - R + r) * g.w;
-        rs = static_cast<int>(start[rr + ix - R]);
-        len = static_cast<int>(start[rr + ix + R + 1]) - rs;
-    }
+    if (valid && r <= 2 * R) cells.row(cx, cy - R + r, R, rs, len);
     int inc = len;
 #pragma unroll
     for (int o = 1; o < G; o <<= 1) {
@@ -805,12 +905,11 @@ __device__ inline int wave_max_over_segments(int v)
 // discover_neighbors_csr counting pass (preprocess.hpp:114-184): in-radius
 // candidates in the reference's form `!(d2 > rad2)`, the anchor itself
 // excluded, at most max_candidates kept.  One G-lane segment per anchor.
-template <int G>
+template <int G, class Cells>
 __global__ __launch_bounds__(kBlock) void k_discover_count(int n, const double *__restrict__ x,
                                                            const double *__restrict__ y, const int *__restrict__ cx,
-                                                           const int *__restrict__ cy, GridDesc g, int R, double rad2,
-                                                           unsigned max_cand, const unsigned *__restrict__ start,
-                                                           const int *__restrict__ sorted,
+                                                           const int *__restrict__ cy, Cells cells, int R, double rad2,
+                                                           unsigned max_cand, const int *__restrict__ sorted,
                                                            const double2 *__restrict__ sxy,
                                                            unsigned *__restrict__ deg, bool f32)
 {
@@ -825,13 +924,13 @@ __global__ __launch_bounds__(kBlock) void k_discover_count(int n, const double *
     if (valid) {
         ax = x[a];
         ay = y[a];
-        ix = cx[a] - g.x0;
-        iy = cy[a] - g.y0;
+        ix = cx[a];
+        iy = cy[a];
     }
     unsigned cnt = 0;
     for (int rb = 0; rb <= 2 * R; rb += G) {
         int rs, pre, tot;
-        window_rows<G>(start, g, ix, iy, R, rb, sl, valid, rs, pre, tot);
+        window_rows<G>(cells, ix, iy, R, rb, sl, valid, rs, pre, tot);
         const int nrows = min(G, 2 * R + 1 - rb);
         for (int t0 = 0; __ballot(t0 < tot) != 0ULL; t0 += G) {
             const int k = window_pos<G>(rs, pre, tot, nrows, t0 + sl);
@@ -860,12 +959,11 @@ constexpr int kFillWaves = 4;  // waves per workgroup in k_fill_csr
 // position, which is the insertion-sort order std::sort yields up to 16
 // entries); lists with ties and more than 16 entries get their scan-order
 // copy in `cand` and are queued for the std::sort replay (k_tie_sort).
-template <int G>
+template <int G, class Cells>
 __global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const double *__restrict__ x,
                                                               const double *__restrict__ y, const int *__restrict__ cx,
-                                                              const int *__restrict__ cy, GridDesc g, int R, double rad2,
-                                                              const unsigned *__restrict__ start,
-                                                              const int *__restrict__ sorted,
+                                                              const int *__restrict__ cy, Cells cells, int R,
+                                                              double rad2, const int *__restrict__ sorted,
                                                               const int *__restrict__ scell,
                                                               const double2 *__restrict__ sxy,
                                                               const unsigned *__restrict__ off, Cand *__restrict__ cand,
@@ -895,8 +993,8 @@ __global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const doubl
     if (valid) {
         ax = x[a];
         ay = y[a];
-        ix = cx[a] - g.x0;
-        iy = cy[a] - g.y0;
+        ix = cx[a];
+        iy = cy[a];
         o = off[a];
         dg = static_cast<int>(off[a + 1] - o);
     }
@@ -906,7 +1004,7 @@ __global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const doubl
     bool done = !valid;
     for (int rb = 0; rb <= 2 * R && __ballot(!done) != 0ULL; rb += G) {
         int rs, pre, tot;
-        window_rows<G>(start, g, ix, iy, R, rb, sl, valid, rs, pre, tot);
+        window_rows<G>(cells, ix, iy, R, rb, sl, valid, rs, pre, tot);
         const int nrows = min(G, 2 * R + 1 - rb);
         for (int t0 = 0; __ballot(!done && t0 < tot) != 0ULL; t0 += G) {
             int k = window_pos<G>(rs, pre, tot, nrows, t0 + sl);
@@ -1773,7 +1871,7 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
                     const long long q = static_cast<long long>(base) + nseed + j;
                     if (lane < len) A.st_ids[q * A.st_stride + lane] = cur;
                     if (lane == 0) {
-                        A.st_meta[q] = static_cast<unsigned>(len) | (run == 0 ? 0x100u : 0u) |
+                        A.st_meta[q] = static_cast<unsigned>(len) | (run == 0 ? 0x8000u : 0u) |
                                        (static_cast<unsigned>(best_cell) << 16);
                         A.st_k[2 * q] = kmax;
                         A.st_k[2 * q + 1] = kmin;
@@ -1908,8 +2006,8 @@ __device__ inline void record_slot(const RecordArgs &R, int q, int rk)
     const double ax = R.x[a], ay = R.y[a], at = R.th[a];
     const int W = R.G + 1;
     const unsigned meta = R.st_meta[q];
-    const int len = static_cast<int>(meta & 0xffu);
-    const bool fwd = (meta >> 8) & 1u;
+    const int len = static_cast<int>(meta & 0x7fffu);
+    const bool fwd = (meta >> 15) & 1u;
     const int c = static_cast<int>(meta >> 16);
     const int *chain = R.st_ids + static_cast<long long>(q) * R.st_stride;
     unsigned *row = R.chains + static_cast<long long>(o) * W;
@@ -2015,7 +2113,10 @@ __global__ __launch_bounds__(kRecordBlock) void k_record(RecordArgs R)
 }
 
 // ---------------------------------------------------------------------------
-// The float instantiation (run_curvelet<float>, main.cpp:110 with --float)
+// The general curvelet kernel: the float instantiation (run_curvelet<float>,
+// main.cpp:110 with --float), and double maps outside the fused kernel's range
+// (more than 256 neighbours per anchor, group_max_sz above 64, neighbour lists
+// too long for one workgroup's LDS)
 // ---------------------------------------------------------------------------
 
 // Direction class of a neighbour in float: dot(To2Pi(atan2f(dy, dx)), theta_a)
@@ -2038,56 +2139,170 @@ __device__ inline int direction_class_f(float ax, float ay, float at, float nx, 
     return dot > 0 ? 1 : (dot < 0 ? 2 : 0);
 }
 
-struct F32Args {
+// The arithmetic of one instantiation: T = double (the reference's default,
+// and the same device functions as the fused kernel's pre-passes) or T =
+// float (main.cpp includes <math.h>, whose float overloads the unqualified
+// sin and cos calls of cpu_curvelet.hpp resolve to; the double literals of
+// :697-698, 711-715 and 774 promote those expressions to double until they
+// are stored in T).
+template <typename T>
+struct GenMath;
+
+template <>
+struct GenMath<double> {
+    __device__ static void sincos(double a, double *s, double *c) { cvg::cr_sincos(a, c_sincos_tab, s, c); }
+    __device__ static int direction(double ax, double ay, double at, double nx, double ny, double cat, double sat)
+    {
+        (void)at;
+        return cvg::direction_class(ax, ay, cat, sat, nx, ny, c_sincos_tab, c_atan_tab);
+    }
+    // compute_curve_bundle<double> (:636-830) for the lane's cells; false
+    // when the pair is close (its bundle stays the full default)
+    template <int M>
+    __device__ static bool bundle(double ax, double ay, double nx, double ny, double nt, double cat, double sat,
+                                  const bool *live, const double *ox, const double *T2t, const double *sT2t,
+                                  const double *cT2t, double dt, double dx, double dx2, double close_thr,
+                                  double max_k, double *lo, double *hi)
+    {
+        double sp0, cp0, sm0, cm0;
+        cvg::cr_sincos(nt + dt, c_sincos_tab, &sp0, &cp0);
+        cvg::cr_sincos(nt - dt, c_sincos_tab, &sm0, &cm0);
+        const PairRec pr = cvg::pair_record(ax, ay, nx, ny, sp0, cp0, sm0, cm0);
+        const bool close = pr.DX2 + pr.DY2 < close_thr;  // :682
+        const bool branch_a = (pr.DX * cat + pr.DY * sat) < 0;  // :690
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            lo[m] = -max_k;
+            hi[m] = max_k;
+            if (close || !live[m]) continue;
+            const double sin_m = cvg::cr_sin(T2t[m] - (nt - dt), c_sincos_tab);
+            const double sin_p = cvg::cr_sin(T2t[m] - (nt + dt), c_sincos_tab);
+            cvg::bundle_cell(pr, branch_a, ox[m], sT2t[m], cT2t[m], sin_m, sin_p, dx, dx2, max_k, lo[m], hi[m]);
+        }
+        return !close;
+    }
+};
+
+template <>
+struct GenMath<float> {
+    __device__ static void sincos(float a, float *s, float *c) { cvg::cr_sincosf(a, c_sincos_tab, s, c); }
+    __device__ static int direction(float ax, float ay, float at, float nx, float ny, float cat, float sat)
+    {
+        return direction_class_f(ax, ay, at, nx, ny, cat, sat);
+    }
+    template <int M>
+    __device__ static bool bundle(float ax, float ay, float nx, float ny, float nt, float cat, float sat,
+                                  const bool *live, const float *ox, const float *T2t, const float *sT2t,
+                                  const float *cT2t, float dt, float dx, float dx2, float close_thr, float max_k,
+                                  float *lo, float *hi)
+    {
+        (void)close_thr;
+        const float DX = ax - nx, DY = ay - ny;
+        const float DX2 = DX * DX, DY2 = DY * DY;
+        const float T0p = nt + dt, T0m = nt - dt;
+        const bool close = DX2 + DY2 < 4 * dx * dx;  // :682
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            lo[m] = -max_k;
+            hi[m] = max_k;
+        }
+        if (close) return false;
+        const bool branch_a = (DX * cat + DY * sat) < 0;  // :690
+        float sp0, cp0, sm0, cm0;
+        cvg::cr_sincosf(T0p, c_sincos_tab, &sp0, &cp0);
+        cvg::cr_sincosf(T0m, c_sincos_tab, &sm0, &cm0);
+        const float sT0p_DY = DY * sp0;  // :730-733
+        const float cT0p_DX = DX * cp0;
+        const float sT0m_DY = DY * sm0;
+        const float cT0m_DX = DX * cm0;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            if (!live[m]) continue;
+            const float o = ox[m];
+            const float tt = sT2t[m] * DX - cT2t[m] * DY;  // :705
+            const float ttm = tt - o;
+            const float sm = cvg::cr_sinf(T2t[m] - T0m, c_sincos_tab);  // :728-729
+            const float sp = cvg::cr_sinf(T2t[m] - T0p, c_sincos_tab);
+            if (branch_a) {
+                const float den = static_cast<float>(-2.0 * o * tt + DX2 + DY2 - dx2 + o * o);  // :711
+                const float kx_min = static_cast<float>(2.0 * (ttm - dx) / den);
+                const float kx_max = static_cast<float>(2.0 * (ttm + dx) / den);
+                if (lo[m] < kx_min) lo[m] = kx_min;
+                if (hi[m] > kx_max) hi[m] = kx_max;
+                const float kt_min = sm / (-sm * o + cT0m_DX + sT0m_DY);
+                const float kt_max = sp / (-sp * o + cT0p_DX + sT0p_DY);
+                if (lo[m] < kt_min) lo[m] = kt_min;
+                if (hi[m] > kt_max) hi[m] = kt_max;
+            } else {
+                const float den = static_cast<float>(-(2.0 * o * tt - DX2 - DY2 + dx2 - o * o) / 2.0);  // :774
+                const float kx_min = (ttm - dx) / den;
+                const float kx_max = (ttm + dx) / den;
+                if (lo[m] < kx_min) lo[m] = kx_min;
+                if (hi[m] > kx_max) hi[m] = kx_max;
+                const float kt_max = sm / (-sm * o + cT0m_DX + sT0m_DY);
+                if (hi[m] > kt_max) hi[m] = kt_max;
+                const float kt_min = sp / (-sp * o + cT0p_DX + sT0p_DY);
+                if (lo[m] < kt_min) lo[m] = kt_min;
+            }
+        }
+        return true;
+    }
+};
+
+template <typename T>
+struct GenArgs {
     int n;
-    const double *x, *y, *th;   // the float edge values, held exactly as doubles
+    const double *x, *y, *th;   // edge values (a float map's values, held exactly as doubles)
     const unsigned *off;
     const int *ids;
-    int kx, kt, K, G, st_stride, max_deg;
-    float dx, dt, max_k, dx2;
-    const float *off_x, *off_t;  // delta_x2, delta_t2 (:697-698)
-    const float *cell_d;         // fill_curvelet_info's d per cell, 3.0e38 where d >= 100
-    float *sc_bundle;            // per wave: max_deg x K lo, then hi
+    int kt, K, G, cap, max_deg;  // cap: longest chain the map allows, min(G, max_deg + 1)
+    T dx, dt, max_k, dx2, close_thr;
+    const T *off_x, *off_t;      // delta_x2, delta_t2 (:697-698)
+    const int *cell_rank;        // fill_curvelet_info's order of a cell (d, then index), K where d >= 100
+    const int *rank_cell;        // the cell of each rank
+    T *sc_bundle;                // per wave: max_deg x K lo, then hi
     int *sc_flag;                // per wave: max_deg neighbour flags
-    int *sc_acc;                 // per wave: max_deg x (G + 1) accepted chains of the run
+    int *sc_acc;                 // per wave: max_deg x (cap + 1) accepted chains of the run
+    int *sc_chain;               // per wave: cap, the chain being grown
     int *st_ids;
     unsigned *st_meta;
     double *st_k;
     int *st_rank;
     unsigned *acc_count;
     unsigned *valid_pairs;
+    unsigned long long *intersections;
 };
 
-constexpr int kF32Waves = 4;  // waves per workgroup of k_curvelets_f32
+constexpr int kGenWaves = 4;  // waves per workgroup of k_curvelets_generic
 
-// build_curvelets_greedy<float> (cpu_curvelet.hpp:437-591) with the
-// form_pairwise_bundles<float> count (:305-361), one wave per anchor (a
+// build_curvelets_greedy<T> (cpu_curvelet.hpp:437-591) with the
+// form_pairwise_bundles<T> count (:305-361), one wave per anchor (a
 // persistent grid: wave w takes anchors w, w + waves, ...), lane = bundle
-// cell (up to 4 per lane).  compute_curve_bundle<float> (:636-830) takes the
-// float sin and cos (main.cpp includes <math.h>, whose std:: names resolve its
-// unqualified calls), and the double literals of :697-698, 711-715 and 774
-// promote those expressions to double until they are stored in T.  Each
-// neighbour's bundle goes to the wave's scratch once (it does not depend on
-// the run); the runs' hypotheses are the neighbours of their direction class
-// with a valid (or close) bundle.  Accepted rows are staged as
-// k_curvelets stages them, at the anchor's slots base, base + 1, ...
-__global__ __launch_bounds__(64 * kF32Waves) void k_curvelets_f32(F32Args A)
+// cell (up to 4 per lane).  Each neighbour's bundle goes to the wave's slice
+// of device memory once (it does not depend on the run); the runs'
+// hypotheses are the neighbours of their direction class with a valid (or
+// close) bundle.  Chains grow in the wave's chain slice (any length the map
+// allows), and accepted rows are staged as k_curvelets stages them, at the
+// anchor's slots base, base + 1, ...
+template <typename T>
+__global__ __launch_bounds__(64 * kGenWaves) void k_curvelets_generic(GenArgs<T> A)
 {
+    using M = GenMath<T>;
     const int lane = threadIdx.x & 63;
     const int w = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int nwaves = static_cast<int>((gridDim.x * blockDim.x) >> 6);
     const int K = A.K;
     const int G = A.G;
-    const int cw = G + 1;
-    float *blo = A.sc_bundle + static_cast<size_t>(w) * A.max_deg * K * 2;
-    float *bhi = blo + static_cast<size_t>(A.max_deg) * K;
+    const int cw = A.cap + 1;
+    T *blo = A.sc_bundle + static_cast<size_t>(w) * A.max_deg * K * 2;
+    T *bhi = blo + static_cast<size_t>(A.max_deg) * K;
     int *flag = A.sc_flag + static_cast<size_t>(w) * A.max_deg;
     int *acc = A.sc_acc + static_cast<size_t>(w) * A.max_deg * cw;
-    const unsigned long long lt = (1ULL << lane) - 1ULL;
+    int *chn = A.sc_chain + static_cast<size_t>(w) * A.cap;
     // per-lane cells
     bool live[4];
-    int cell[4];
-    float ox[4], ot[4], cd[4];
+    int cell[4], crank[4];
+    T ox[4], ot[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
         const int c = lane + 64 * m;
@@ -2095,82 +2310,33 @@ __global__ __launch_bounds__(64 * kF32Waves) void k_curvelets_f32(F32Args A)
         cell[m] = live[m] ? c : 0;
         ox[m] = A.off_x[cell[m] / A.kt];
         ot[m] = A.off_t[cell[m] % A.kt];
-        cd[m] = live[m] ? A.cell_d[cell[m]] : 3.0e38f;
+        crank[m] = live[m] ? A.cell_rank[cell[m]] : K;
     }
     unsigned vpw = 0;
+    unsigned long long isect = 0;
     for (int a = w; a < A.n; a += nwaves) {
         const unsigned base = A.off[a];
         const int deg = static_cast<int>(A.off[a + 1] - base);
-        const float ax = static_cast<float>(A.x[a]), ay = static_cast<float>(A.y[a]);
-        const float at = static_cast<float>(A.th[a]);
-        float sat, cat;
-        cvg::cr_sincosf(at, c_sincos_tab, &sat, &cat);
-        float T2t[4], sT2t[4], cT2t[4];
+        const T ax = static_cast<T>(A.x[a]), ay = static_cast<T>(A.y[a]);
+        const T at = static_cast<T>(A.th[a]);
+        T sat, cat;
+        M::sincos(at, &sat, &cat);
+        T T2t[4], sT2t[4], cT2t[4];
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
             T2t[m] = at + ot[m];  // :702
-            cvg::cr_sincosf(T2t[m], c_sincos_tab, &sT2t[m], &cT2t[m]);
+            M::sincos(T2t[m], &sT2t[m], &cT2t[m]);
         }
-        // every neighbour: direction class and bundle (compute_curve_bundle<float>)
+        // every neighbour: direction class and bundle (compute_curve_bundle<T>)
         for (int r = 0; r < deg; ++r) {
             const int b = A.ids[base + r];
-            const float nx = static_cast<float>(A.x[b]), ny = static_cast<float>(A.y[b]);
-            const float nt = static_cast<float>(A.th[b]);
-            const int cls = direction_class_f(ax, ay, at, nx, ny, cat, sat);
-            const float DX = ax - nx, DY = ay - ny;
-            const float DX2 = DX * DX, DY2 = DY * DY;
-            const float T0p = nt + A.dt, T0m = nt - A.dt;
-            const bool close = DX2 + DY2 < 4 * A.dx * A.dx;  // :682
-            float lo[4], hi[4];
+            const T nx = static_cast<T>(A.x[b]), ny = static_cast<T>(A.y[b]);
+            const T nt = static_cast<T>(A.th[b]);
+            const int cls = M::direction(ax, ay, at, nx, ny, cat, sat);
+            T lo[4], hi[4];
+            const bool far = M::template bundle<4>(ax, ay, nx, ny, nt, cat, sat, live, ox, T2t, sT2t, cT2t, A.dt,
+                                                   A.dx, A.dx2, A.close_thr, A.max_k, lo, hi);
             unsigned long long anym = 0ULL;
-            if (close) {
-#pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    lo[m] = -A.max_k;
-                    hi[m] = A.max_k;
-                }
-            } else {
-                const bool branch_a = (DX * cat + DY * sat) < 0;  // :690
-                float sp0, cp0, sm0, cm0;
-                cvg::cr_sincosf(T0p, c_sincos_tab, &sp0, &cp0);
-                cvg::cr_sincosf(T0m, c_sincos_tab, &sm0, &cm0);
-                const float sT0p_DY = DY * sp0;  // :730-733
-                const float cT0p_DX = DX * cp0;
-                const float sT0m_DY = DY * sm0;
-                const float cT0m_DX = DX * cm0;
-#pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    lo[m] = -A.max_k;
-                    hi[m] = A.max_k;
-                    if (!live[m]) continue;
-                    const float o = ox[m];
-                    const float tt = sT2t[m] * DX - cT2t[m] * DY;  // :705
-                    const float ttm = tt - o;
-                    const float sm = cvg::cr_sinf(T2t[m] - T0m, c_sincos_tab);  // :728-729
-                    const float sp = cvg::cr_sinf(T2t[m] - T0p, c_sincos_tab);
-                    if (branch_a) {
-                        const float den = static_cast<float>(-2.0 * o * tt + DX2 + DY2 - A.dx2 + o * o);  // :711
-                        const float kx_min = static_cast<float>(2.0 * (ttm - A.dx) / den);
-                        const float kx_max = static_cast<float>(2.0 * (ttm + A.dx) / den);
-                        if (lo[m] < kx_min) lo[m] = kx_min;
-                        if (hi[m] > kx_max) hi[m] = kx_max;
-                        const float kt_min = sm / (-sm * o + cT0m_DX + sT0m_DY);
-                        const float kt_max = sp / (-sp * o + cT0p_DX + sT0p_DY);
-                        if (lo[m] < kt_min) lo[m] = kt_min;
-                        if (hi[m] > kt_max) hi[m] = kt_max;
-                    } else {
-                        const float den = static_cast<float>(-(2.0 * o * tt - DX2 - DY2 + A.dx2 - o * o) / 2.0);  // :774
-                        const float kx_min = (ttm - A.dx) / den;
-                        const float kx_max = (ttm + A.dx) / den;
-                        if (lo[m] < kx_min) lo[m] = kx_min;
-                        if (hi[m] > kx_max) hi[m] = kx_max;
-                        const float kt_max = sm / (-sm * o + cT0m_DX + sT0m_DY);
-                        if (hi[m] > kt_max) hi[m] = kt_max;
-                        const float kt_min = sp / (-sp * o + cT0p_DX + sT0p_DY);
-                        if (lo[m] < kt_min) lo[m] = kt_min;
-                    }
-                }
-            }
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
                 anym |= __ballot(live[m] && hi[m] > lo[m]);
@@ -2179,7 +2345,7 @@ __global__ __launch_bounds__(64 * kF32Waves) void k_curvelets_f32(F32Args A)
                     bhi[static_cast<size_t>(r) * K + cell[m]] = hi[m];
                 }
             }
-            const bool valid = close || anym != 0ULL;  // bundle_valid_check, :821-830
+            const bool valid = !far || anym != 0ULL;  // bundle_valid_check, :821-830
             vpw += valid ? 1u : 0u;
             if (lane == 0) flag[r] = cls | (valid ? 4 : 0);
         }
@@ -2191,36 +2357,37 @@ __global__ __launch_bounds__(64 * kF32Waves) void k_curvelets_f32(F32Args A)
             for (int sidx = 0; sidx < deg; ++sidx) {
                 const int fs = flag[sidx];
                 if ((fs & 3) != want || !(fs & 4)) continue;
-                float lo[4], hi[4];
+                T lo[4], hi[4];
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
-                    lo[m] = live[m] ? blo[static_cast<size_t>(sidx) * K + cell[m]] : 0.0f;
-                    hi[m] = live[m] ? bhi[static_cast<size_t>(sidx) * K + cell[m]] : 0.0f;
+                    lo[m] = live[m] ? blo[static_cast<size_t>(sidx) * K + cell[m]] : T(0);
+                    hi[m] = live[m] ? bhi[static_cast<size_t>(sidx) * K + cell[m]] : T(0);
                 }
                 int len = 1;
-                int cur = lane == 0 ? a : 0;
+                if (lane == 0) chn[0] = a;
                 for (int r = 0; r < deg; ++r) {  // :479-536
                     const int fr = flag[r];
                     if ((fr & 3) != want || !(fr & 4)) continue;
                     const int nb = A.ids[base + r];
                     if (r == sidx) {
-                        cur = lane == len ? nb : cur;
+                        if (lane == 0) chn[len] = nb;
                         ++len;
                         if (len >= G) break;
                         continue;
                     }
-                    float nlo[4], nhi[4];
+                    ++isect;
+                    T nlo[4], nhi[4];
                     unsigned long long okm = 0ULL;
 #pragma unroll
                     for (int m = 0; m < 4; ++m) {
-                        const float bl = live[m] ? blo[static_cast<size_t>(r) * K + cell[m]] : 0.0f;
-                        const float bh = live[m] ? bhi[static_cast<size_t>(r) * K + cell[m]] : 0.0f;
+                        const T bl = live[m] ? blo[static_cast<size_t>(r) * K + cell[m]] : T(0);
+                        const T bh = live[m] ? bhi[static_cast<size_t>(r) * K + cell[m]] : T(0);
                         nlo[m] = (lo[m] < bl) ? bl : lo[m];  // :838-841
                         nhi[m] = (hi[m] > bh) ? bh : hi[m];
                         okm |= __ballot(live[m] && nhi[m] > nlo[m]);
                     }
                     if (okm != 0ULL) {
-                        cur = lane == len ? nb : cur;
+                        if (lane == 0) chn[len] = nb;
                         ++len;
 #pragma unroll
                         for (int m = 0; m < 4; ++m) {
@@ -2231,55 +2398,54 @@ __global__ __launch_bounds__(64 * kF32Waves) void k_curvelets_f32(F32Args A)
                     if (len >= G) break;
                 }
                 if (len <= 2) continue;  // :549
+                wave_sync();  // the chain written by lane 0 is read by every lane
                 // check_curvelet_exist (:876-916) against this run's accepted chains
                 bool dup = false;
                 for (unsigned k = 0; k < nrun && !dup; ++k) {
                     const int *row = acc + static_cast<size_t>(k) * cw;
                     if (row[0] != len) continue;
-                    const bool ne = lane < len && row[1 + lane] != cur;
+                    bool ne = false;
+                    for (int j = lane; j < len; j += 64) ne = ne || row[1 + j] != chn[j];
                     dup = __ballot(ne) == 0ULL;
                 }
                 if (dup) continue;
                 int *row = acc + static_cast<size_t>(nrun) * cw;
                 if (lane == 0) row[0] = len;
-                if (lane < len) row[1 + lane] = cur;
-                wave_sync();
+                for (int j = lane; j < len; j += 64) row[1 + j] = chn[j];
                 ++nrun;
-                // fill_curvelet_info's cell: smallest d, first in row-major order
-                unsigned long long key = ~0ULL;
+                // fill_curvelet_info's cell (:202-226): smallest d among the
+                // running bundle's valid cells, first in row-major order
+                int key = K;
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
-                    const bool ok = live[m] && hi[m] > lo[m] && cd[m] < 1.0e38f;
-                    const unsigned long long k2 =
-                        ok ? ((static_cast<unsigned long long>(__float_as_uint(cd[m])) << 32) |
-                              static_cast<unsigned>(cell[m]))
-                           : ~0ULL;
+                    const int k2 = (live[m] && hi[m] > lo[m]) ? crank[m] : K;
                     key = k2 < key ? k2 : key;
                 }
                 for (int o = 32; o > 0; o >>= 1) {
-                    const unsigned long long v = __shfl_xor(key, o, 64);
+                    const int v = __shfl_xor(key, o, 64);
                     key = v < key ? v : key;
                 }
-                const int best = key == ~0ULL ? 0 : static_cast<int>(key & 0xffffffffULL);
+                const int best = key < K ? A.rank_cell[key] : 0;
                 const int bl = best & 63, bm = best >> 6;
-                float kmax = 0.0f, kmin = 0.0f;
+                T kmax = T(0), kmin = T(0);
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
-                    const float h = __shfl(hi[m], bl, 64), l = __shfl(lo[m], bl, 64);
+                    const T h = __shfl(hi[m], bl, 64), l = __shfl(lo[m], bl, 64);
                     if (m == bm) {
                         kmax = h;
                         kmin = l;
                     }
                 }
                 const long long q = static_cast<long long>(base) + nacc;
-                if (lane < len) A.st_ids[q * A.st_stride + lane] = cur;
+                for (int j = lane; j < len; j += 64) A.st_ids[q * A.cap + j] = chn[j];
                 if (lane == 0) {
-                    A.st_meta[q] = static_cast<unsigned>(len) | (run == 0 ? 0x100u : 0u) |
+                    A.st_meta[q] = static_cast<unsigned>(len) | (run == 0 ? 0x8000u : 0u) |
                                    (static_cast<unsigned>(best) << 16);
-                    A.st_k[2 * q] = kmax;
-                    A.st_k[2 * q + 1] = kmin;
+                    A.st_k[2 * q] = static_cast<double>(kmax);
+                    A.st_k[2 * q + 1] = static_cast<double>(kmin);
                 }
                 ++nacc;
+                wave_sync();  // the dedup row is read by every lane
             }
         }
         for (int k = lane; k < deg; k += 64) A.st_rank[base + k] = k < static_cast<int>(nacc) ? k : -1;
@@ -2287,6 +2453,7 @@ __global__ __launch_bounds__(64 * kF32Waves) void k_curvelets_f32(F32Args A)
         wave_sync();  // the scratch is rewritten by the next anchor
     }
     if (lane == 0 && vpw) atomicAdd(A.valid_pairs, vpw);
+    if (lane == 0 && isect) atomicAdd(A.intersections, isect);
 }
 
 // ---------------------------------------------------------------------------
@@ -2397,6 +2564,86 @@ int read_scalars(cvg_context *ctx)
     return CVG_OK;
 }
 
+// Neighbour counts, CSR offsets, then the gathering pass, the distance sort
+// and the CSR fill (preprocess.hpp:186-274) over either cell layout.
+template <class Cells>
+int graph_scans(cvg_context *ctx, const Cells &cells, int n, unsigned R, unsigned max_cand, double rad2, int num_maps)
+{
+    hipStream_t st = ctx->stream;
+    unsigned *scal = as<unsigned>(ctx->d_scal);
+    const size_t ni = static_cast<size_t>(n) * sizeof(int);
+    CVG_TRY(ensure(ctx, ctx->d_deg, ni));
+    CVG_TRY(ensure(ctx, ctx->d_off, static_cast<size_t>(n + 1) * sizeof(unsigned)));
+    {
+        using CountFn = void (*)(int, const double *, const double *, const int *, const int *, Cells, int, double,
+                                 unsigned, const int *, const double2 *, unsigned *, bool);
+        const int G = ctx->fill_lanes;
+        const CountFn fn = G == 8    ? k_discover_count<8, Cells>
+                           : G == 32 ? k_discover_count<32, Cells>
+                           : G == 64 ? k_discover_count<64, Cells>
+                                     : k_discover_count<16, Cells>;
+        hipLaunchKernelGGL(fn, dim3(blocks_for(static_cast<long long>(n) * G)), dim3(kBlock), 0, st, n,
+                           as<double>(ctx->d_x), as<double>(ctx->d_y), as<int>(ctx->d_cx), as<int>(ctx->d_cy), cells,
+                           static_cast<int>(R), rad2, max_cand, as<int>(ctx->d_sorted), as<double2>(ctx->d_sxy),
+                           as<unsigned>(ctx->d_deg), ctx->f32);
+    }
+    CVG_HIP(ctx, hipGetLastError());
+    CVG_TRY(exclusive_scan(ctx, as<unsigned>(ctx->d_deg), n, as<unsigned>(ctx->d_off), ctx->d_tiles,
+                           scal + S_MAXDEG, scal + S_TRUNC, max_cand));
+    CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox + 32, as<unsigned>(ctx->d_off) + n, sizeof(unsigned),
+                                hipMemcpyDeviceToHost, st));
+    CVG_TRY(read_scalars(ctx));
+    const unsigned pairs_u = ctx->mailbox[32];
+    if (pairs_u > static_cast<unsigned>(INT_MAX))
+        return fail(ctx, CVG_ERR_UNSUPPORTED, "more than 2^31-1 neighbour pairs");
+    const int pairs = static_cast<int>(pairs_u);
+    ctx->pairs = pairs;
+    ctx->max_deg = ctx->mailbox[S_MAXDEG];
+    if (ctx->max_deg >= (1u << 23))
+        return fail(ctx, CVG_ERR_UNSUPPORTED, "an anchor has 2^23 or more neighbours");
+    ctx->truncated = ctx->mailbox[S_TRUNC];
+
+    // gather, sort, fill
+    const size_t np = static_cast<size_t>(pairs);
+    CVG_TRY(ensure(ctx, ctx->d_cand, np * sizeof(Cand)));
+    CVG_TRY(ensure(ctx, ctx->d_ids, np * sizeof(int)));
+    CVG_TRY(ensure(ctx, ctx->d_d2, np * sizeof(double)));
+    CVG_TRY(ensure(ctx, ctx->d_pair_anchor, np * sizeof(int)));
+    // one G-lane segment per anchor; each segment keeps its candidates in LDS
+    using FillFn = void (*)(int, const double *, const double *, const int *, const int *, Cells, int, double,
+                            const int *, const int *, const double2 *, const unsigned *, Cand *, int *, double *,
+                            int *, int *, unsigned, int, const int *, const long long *, unsigned *, bool);
+    const int fill_cap = static_cast<int>(std::max(ctx->max_deg, 1u));
+    const size_t per_seg = static_cast<size_t>(fill_cap) * (2 * sizeof(double));
+    int G = ctx->fill_lanes;
+    // fewer anchors per wave when the lists are long
+    while (G < 64 && per_seg * (64 / G) * kFillWaves > ctx->lds_limit) G *= 2;
+    int fill_waves = kFillWaves;
+    while (fill_waves > 1 && per_seg * (64 / G) * fill_waves > ctx->lds_limit) fill_waves /= 2;
+    if (per_seg * (64 / G) * fill_waves > ctx->lds_limit)
+        return fail(ctx, CVG_ERR_UNSUPPORTED, "an anchor has too many neighbours for one workgroup's LDS");
+    const FillFn fill = G == 8    ? k_fill_csr<8, Cells>
+                        : G == 16 ? k_fill_csr<16, Cells>
+                        : G == 32 ? k_fill_csr<32, Cells>
+                                  : k_fill_csr<64, Cells>;
+    const int anchors_per_block = (64 / G) * fill_waves;
+    const unsigned fill_shmem = static_cast<unsigned>(per_seg * anchors_per_block);
+    CVG_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(fill), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     static_cast<int>(fill_shmem)));
+    hipLaunchKernelGGL(fill, dim3((n + anchors_per_block - 1) / anchors_per_block), dim3(64 * fill_waves), fill_shmem,
+                       st, n, as<double>(ctx->d_x), as<double>(ctx->d_y), as<int>(ctx->d_cx), as<int>(ctx->d_cy),
+                       cells, static_cast<int>(R), rad2, as<int>(ctx->d_sorted), as<int>(ctx->d_scell),
+                       as<double2>(ctx->d_sxy), as<unsigned>(ctx->d_off), as<Cand>(ctx->d_cand),
+                       as<int>(ctx->d_ids), as<double>(ctx->d_d2), as<int>(ctx->d_pair_anchor),
+                       as<int>(ctx->d_tmp_ids), max_cand, fill_cap, num_maps > 1 ? as<int>(ctx->d_map) : nullptr,
+                       as<long long>(ctx->d_eoff), scal, ctx->f32);
+    hipLaunchKernelGGL(k_tie_sort, dim3(std::min(blocks_for(n), 64)), dim3(kBlock), 0, st, as<int>(ctx->d_tmp_ids),
+                       scal, as<unsigned>(ctx->d_off), as<Cand>(ctx->d_cand), as<int>(ctx->d_ids),
+                       as<double>(ctx->d_d2));
+    CVG_HIP(ctx, hipGetLastError());
+    return CVG_OK;
+}
+
 int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double rad, unsigned R, unsigned max_cand,
                 int num_maps = 1, const long long *eoff = nullptr)
 {
@@ -2442,7 +2689,7 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     const unsigned *mb = ctx->mailbox;
     if (mb[S_BAD])
         return fail(ctx, CVG_ERR_BAD_INPUT,
-                    "edge values out of range: need finite x, y with |x|,|y| < 2^30 and |theta| < 2^20");
+                    "edge values out of range: need finite x, y with |x|,|y| < 2^31 - 2^13 and |theta| < 2^20");
     int minx = static_cast<int>(mb[S_MINX]), maxx = static_cast<int>(mb[S_MAXX]);
     int miny = static_cast<int>(mb[S_MINY]), maxy = static_cast<int>(mb[S_MAXY]);
     CVG_TRY(ensure(ctx, ctx->d_eoff, (static_cast<size_t>(num_maps) + 1) * sizeof(long long)));
@@ -2487,8 +2734,11 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
             }
             if (cy0 + h_m > (1LL << 30) || cx0 + w_m > (1LL << 30))
                 return fail(ctx, CVG_ERR_UNSUPPORTED, "batch too large for one cell grid");
-            shift[2 * m] = static_cast<int>(cx0 - box[4 * m]);
-            shift[2 * m + 1] = static_cast<int>(cy0 - box[4 * m + 2]);
+            const long long sx_m = cx0 - box[4 * m], sy_m = cy0 - box[4 * m + 2];
+            if (sx_m > INT_MAX || sx_m < INT_MIN || sy_m > INT_MAX || sy_m < INT_MIN)
+                return fail(ctx, CVG_ERR_UNSUPPORTED, "batch too large for one cell grid");
+            shift[2 * m] = static_cast<int>(sx_m);
+            shift[2 * m + 1] = static_cast<int>(sy_m);
             used_w = std::max(used_w, cx0 + w_m);
             shelf_h = std::max(shelf_h, h_m);
             cx0 += w_m + gap;
@@ -2503,115 +2753,90 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
                            as<int>(ctx->d_mapbox), as<int>(ctx->d_cx), as<int>(ctx->d_cy));
         CVG_HIP(ctx, hipGetLastError());
     }
-    GridDesc g;
-    g.x0 = minx - static_cast<int>(R);
-    g.y0 = miny - static_cast<int>(R);
+    const int x0 = minx - static_cast<int>(R);
+    const int y0 = miny - static_cast<int>(R);
     const long long w = static_cast<long long>(maxx) - minx + 1 + 2LL * R;
     const long long h = static_cast<long long>(maxy) - miny + 1 + 2LL * R;
-    if (w * h > kMaxGridCells)
-        return fail(ctx, CVG_ERR_UNSUPPORTED,
-                    "edge coordinates span " + std::to_string(w) + " x " + std::to_string(h) +
-                        " pixel cells; the dense cell grid supports up to 2^28 cells");
-    g.w = static_cast<int>(w);
-    g.h = static_cast<int>(h);
-    ctx->gx0 = g.x0;
-    ctx->gy0 = g.y0;
-    ctx->gw = g.w;
-    ctx->gh = g.h;
-    const long long cells = w * h;
-
-    // counting sort into cells
-    CVG_TRY(ensure(ctx, ctx->d_grid, static_cast<size_t>(2 * cells + 1) * sizeof(unsigned)));
-    unsigned *cnt = as<unsigned>(ctx->d_grid);
-    unsigned *start = cnt + cells;
-    CVG_HIP(ctx, hipMemsetAsync(cnt, 0, static_cast<size_t>(cells) * sizeof(unsigned), st));
-    hipLaunchKernelGGL(k_cell_count, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
-                       as<int>(ctx->d_cy), g, cnt);
-    CVG_TRY(exclusive_scan(ctx, cnt, cells, start, ctx->d_tiles));
+    ctx->gx0 = x0;
+    ctx->gy0 = y0;
     CVG_TRY(ensure(ctx, ctx->d_tmp_ids, ni));
     CVG_TRY(ensure(ctx, ctx->d_sorted, ni));
     CVG_TRY(ensure(ctx, ctx->d_sxy, 2 * nd));
     CVG_TRY(ensure(ctx, ctx->d_scell, ni));
-    hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
-                       as<int>(ctx->d_cy), g, start, cnt, as<int>(ctx->d_tmp_ids));
-    hipLaunchKernelGGL(k_cell_order, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
-                       as<int>(ctx->d_cy), g, start, as<int>(ctx->d_tmp_ids),
-                       ctx->use_cellrank ? as<int>(ctx->d_cellrank) : nullptr, as<double>(ctx->d_x),
-                       as<double>(ctx->d_y), as<int>(ctx->d_sorted), as<int>(ctx->d_scell),
-                       as<double2>(ctx->d_sxy));
-    CVG_HIP(ctx, hipGetLastError());
-
-    // neighbour counts and CSR offsets
-    CVG_TRY(ensure(ctx, ctx->d_deg, ni));
-    CVG_TRY(ensure(ctx, ctx->d_off, static_cast<size_t>(n + 1) * sizeof(unsigned)));
+    const int *rank = ctx->use_cellrank ? as<int>(ctx->d_cellrank) : nullptr;
     // preprocess.hpp:221: rad * rad in T
     const double rad2 = ctx->f32 ? static_cast<double>(static_cast<float>(rad) * static_cast<float>(rad)) : rad * rad;
-    {
-        using CountFn = void (*)(int, const double *, const double *, const int *, const int *, GridDesc, int, double,
-                                 unsigned, const unsigned *, const int *, const double2 *, unsigned *, bool);
-        const int G = ctx->fill_lanes;
-        const CountFn fn = G == 8    ? k_discover_count<8>
-                           : G == 32 ? k_discover_count<32>
-                           : G == 64 ? k_discover_count<64>
-                                     : k_discover_count<16>;
-        hipLaunchKernelGGL(fn, dim3(blocks_for(static_cast<long long>(n) * G)), dim3(kBlock), 0, st, n,
-                           as<double>(ctx->d_x), as<double>(ctx->d_y), as<int>(ctx->d_cx), as<int>(ctx->d_cy), g,
-                           static_cast<int>(R), rad2, max_cand, start, as<int>(ctx->d_sorted),
-                           as<double2>(ctx->d_sxy), as<unsigned>(ctx->d_deg), ctx->f32);
+    if (w <= kMaxGridCells && h <= kMaxGridCells && w * h <= kMaxGridCells && !ctx->force_sparse) {
+        // dense grid: counting sort into cells
+        GridDesc g;
+        g.x0 = x0;
+        g.y0 = y0;
+        g.w = static_cast<int>(w);
+        g.h = static_cast<int>(h);
+        ctx->gw = g.w;
+        ctx->gh = g.h;
+        const long long cells = w * h;
+        CVG_TRY(ensure(ctx, ctx->d_grid, static_cast<size_t>(2 * cells + 1) * sizeof(unsigned)));
+        unsigned *cnt = as<unsigned>(ctx->d_grid);
+        unsigned *start = cnt + cells;
+        CVG_HIP(ctx, hipMemsetAsync(cnt, 0, static_cast<size_t>(cells) * sizeof(unsigned), st));
+        hipLaunchKernelGGL(k_cell_count, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
+                           as<int>(ctx->d_cy), g, cnt);
+        CVG_TRY(exclusive_scan(ctx, cnt, cells, start, ctx->d_tiles));
+        hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
+                           as<int>(ctx->d_cy), g, start, cnt, as<int>(ctx->d_tmp_ids));
+        hipLaunchKernelGGL(k_cell_order, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
+                           as<int>(ctx->d_cy), g, start, as<int>(ctx->d_tmp_ids), rank, as<double>(ctx->d_x),
+                           as<double>(ctx->d_y), as<int>(ctx->d_sorted), as<int>(ctx->d_scell),
+                           as<double2>(ctx->d_sxy));
+        CVG_HIP(ctx, hipGetLastError());
+        DenseCells cells_d{start, x0, y0, g.w};
+        CVG_TRY(graph_scans(ctx, cells_d, n, R, max_cand, rad2, num_maps));
+    } else {
+        // sorted linear cell keys: stable radix sorts of (key, id), by the
+        // within-cell rank first when the reference's order is replayed, so
+        // each cell's edges follow edge id (or rank) order as in the dense grid
+        ctx->gw = 0;
+        ctx->gh = 0;
+        CVG_TRY(ensure(ctx, ctx->d_skey, 4 * static_cast<size_t>(n) * sizeof(unsigned long long)));
+        unsigned long long *key = as<unsigned long long>(ctx->d_skey);
+        unsigned long long *key2 = key + n;
+        int *rkey = reinterpret_cast<int *>(key + 2 * n);
+        int *rkey2 = rkey + n;
+        int *id = reinterpret_cast<int *>(key + 3 * n);
+        int *id2 = as<int>(ctx->d_tmp_ids);
+        hipLaunchKernelGGL(k_sparse_keys, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
+                           as<int>(ctx->d_cy), x0, y0, rank, key, rkey, id);
+        CVG_HIP(ctx, hipGetLastError());
+        int key_bits = 33;  // the column offsets, then as many row bits as the rows need
+        while (key_bits < 64 && (1LL << (key_bits - 32)) < h) ++key_bits;
+        size_t tmp_bytes = 0, t2 = 0;
+        CVG_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key, key2, id, id2, n, 0, key_bits, st));
+        CVG_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, t2, rkey, rkey2, id, id2, n, 0, 32, st));
+        tmp_bytes = std::max(tmp_bytes, t2);
+        CVG_TRY(ensure(ctx, ctx->d_sorttmp, tmp_bytes));
+        if (rank) {
+            CVG_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->d_sorttmp.ptr, tmp_bytes, rkey, rkey2, id, id2, n, 0,
+                                                            32, st));
+            hipLaunchKernelGGL(k_gather_keys, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, key, id2, key2);
+            CVG_HIP(ctx, hipGetLastError());
+            // (key2, id2) in rank order -> (key, id) sorted by key, stable
+            CVG_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->d_sorttmp.ptr, tmp_bytes, key2, key, id2, id, n, 0,
+                                                            key_bits, st));
+        } else {
+            CVG_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->d_sorttmp.ptr, tmp_bytes, key, key2, id, id2, n, 0,
+                                                            key_bits, st));
+            CVG_HIP(ctx, hipMemcpyAsync(key, key2, static_cast<size_t>(n) * sizeof(unsigned long long),
+                                        hipMemcpyDeviceToDevice, st));
+            CVG_HIP(ctx, hipMemcpyAsync(id, id2, ni, hipMemcpyDeviceToDevice, st));
+        }
+        hipLaunchKernelGGL(k_sparse_fill, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, key, id, as<double>(ctx->d_x),
+                           as<double>(ctx->d_y), as<int>(ctx->d_sorted), as<int>(ctx->d_scell),
+                           as<double2>(ctx->d_sxy));
+        CVG_HIP(ctx, hipGetLastError());
+        SparseCells cells_s{key, n, x0, y0};
+        CVG_TRY(graph_scans(ctx, cells_s, n, R, max_cand, rad2, num_maps));
     }
-    CVG_HIP(ctx, hipGetLastError());
-    CVG_TRY(exclusive_scan(ctx, as<unsigned>(ctx->d_deg), n, as<unsigned>(ctx->d_off), ctx->d_tiles,
-                           scal + S_MAXDEG, scal + S_TRUNC, max_cand));
-    unsigned pairs_u = 0;
-    CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox + 32, as<unsigned>(ctx->d_off) + n, sizeof(unsigned),
-                                hipMemcpyDeviceToHost, st));
-    CVG_TRY(read_scalars(ctx));
-    pairs_u = ctx->mailbox[32];
-    if (pairs_u > static_cast<unsigned>(INT_MAX))
-        return fail(ctx, CVG_ERR_UNSUPPORTED, "more than 2^31-1 neighbour pairs");
-    const int pairs = static_cast<int>(pairs_u);
-    ctx->pairs = pairs;
-    ctx->max_deg = ctx->mailbox[S_MAXDEG];
-    if (ctx->max_deg >= (1u << 23))
-        return fail(ctx, CVG_ERR_UNSUPPORTED, "an anchor has 2^23 or more neighbours");
-    ctx->truncated = ctx->mailbox[S_TRUNC];
-
-    // gather, sort, fill
-    const size_t np = static_cast<size_t>(pairs);
-    CVG_TRY(ensure(ctx, ctx->d_cand, np * sizeof(Cand)));
-    CVG_TRY(ensure(ctx, ctx->d_ids, np * sizeof(int)));
-    CVG_TRY(ensure(ctx, ctx->d_d2, np * sizeof(double)));
-    CVG_TRY(ensure(ctx, ctx->d_pair_anchor, np * sizeof(int)));
-    // one G-lane segment per anchor; each segment keeps its candidates in LDS
-    using FillFn = void (*)(int, const double *, const double *, const int *, const int *, GridDesc, int, double,
-                            const unsigned *, const int *, const int *, const double2 *, const unsigned *, Cand *,
-                            int *, double *, int *, int *, unsigned, int, const int *, const long long *,
-                            unsigned *, bool);
-    const int fill_cap = static_cast<int>(std::max(ctx->max_deg, 1u));
-    const size_t per_seg = static_cast<size_t>(fill_cap) * (2 * sizeof(double));
-    int G = ctx->fill_lanes;
-    // fewer anchors per wave when the lists are long
-    while (G < 64 && per_seg * (64 / G) * kFillWaves > ctx->lds_limit) G *= 2;
-    int fill_waves = kFillWaves;
-    while (fill_waves > 1 && per_seg * (64 / G) * fill_waves > ctx->lds_limit) fill_waves /= 2;
-    if (per_seg * (64 / G) * fill_waves > ctx->lds_limit)
-        return fail(ctx, CVG_ERR_UNSUPPORTED, "an anchor has too many neighbours for one workgroup's LDS");
-    const FillFn fill = G == 8 ? k_fill_csr<8> : G == 16 ? k_fill_csr<16> : G == 32 ? k_fill_csr<32> : k_fill_csr<64>;
-    const int anchors_per_block = (64 / G) * fill_waves;
-    const unsigned fill_shmem = static_cast<unsigned>(per_seg * anchors_per_block);
-    CVG_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(fill), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     static_cast<int>(fill_shmem)));
-    hipLaunchKernelGGL(fill, dim3((n + anchors_per_block - 1) / anchors_per_block), dim3(64 * fill_waves), fill_shmem,
-                       st, n, as<double>(ctx->d_x), as<double>(ctx->d_y), as<int>(ctx->d_cx), as<int>(ctx->d_cy), g,
-                       static_cast<int>(R), rad2, start, as<int>(ctx->d_sorted), as<int>(ctx->d_scell),
-                       as<double2>(ctx->d_sxy), as<unsigned>(ctx->d_off), as<Cand>(ctx->d_cand),
-                       as<int>(ctx->d_ids), as<double>(ctx->d_d2), as<int>(ctx->d_pair_anchor),
-                       as<int>(ctx->d_tmp_ids), max_cand, fill_cap, num_maps > 1 ? as<int>(ctx->d_map) : nullptr,
-                       as<long long>(ctx->d_eoff), scal, ctx->f32);
-    hipLaunchKernelGGL(k_tie_sort, dim3(std::min(blocks_for(n), 64)), dim3(kBlock), 0, st, as<int>(ctx->d_tmp_ids),
-                       scal, as<unsigned>(ctx->d_off), as<Cand>(ctx->d_cand), as<int>(ctx->d_ids),
-                       as<double>(ctx->d_d2));
-    CVG_HIP(ctx, hipGetLastError());
     ctx->graph_ready = true;
     return CVG_OK;
 }
@@ -2643,7 +2868,6 @@ int derive(cvg_context *ctx, const cvg_params &p, Derived &d)
     d.kt = static_cast<unsigned>(ft);
     d.K = d.kx * d.kt;
     d.cpl = d.K <= 64 ? 1 : (d.K <= 128 ? 2 : 4);
-    if (p.group_max_sz > 64) return fail(ctx, CVG_ERR_UNSUPPORTED, "group_max_sz above 64");
     d.G = p.group_max_sz;
     d.off_x.resize(d.kx);
     d.off_t.resize(d.kt);
@@ -2851,51 +3075,74 @@ int record_rows(cvg_context *ctx, const cvg_params &p, RecordArgs R, int st_stri
     return CVG_OK;
 }
 
-// The curvelet stage of the float instantiation (run_curvelet<float>): the
-// parameters cast to float as main.cpp:22-30 casts them, the bundle grid of
-// init_curvelet_storage (:73-74) from float dx / sx and dt / st, one
-// k_curvelets_f32 pass, then the common record with float info.
-int build_curvelets_f32(cvg_context *ctx, const cvg_params &p)
+// The curvelet stage on k_curvelets_generic: the float instantiation
+// (run_curvelet<float>: the parameters cast to float as main.cpp:22-30 casts
+// them, the bundle grid of init_curvelet_storage (:73-74) from float dx / sx
+// and dt / st), or a double map outside the fused kernel's range; then the
+// common record.
+template <typename T>
+int build_curvelets_generic(cvg_context *ctx, const cvg_params &p)
 {
+    constexpr bool kFloat = std::is_same<T, float>::value;
     if (!ctx->graph_ready) return fail(ctx, CVG_ERR_STATE, "build the neighbour graph first");
     ctx->curvelets_ready = false;
     if (!(p.sx > 0) || !(p.st > 0)) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "sx and st must be positive");
-    if (p.group_max_sz > 64) return fail(ctx, CVG_ERR_UNSUPPORTED, "group_max_sz above 64");
     hipStream_t st = ctx->stream;
     const int n = ctx->n;
     const int pairs = ctx->pairs;
-    const float nrad = static_cast<float>(p.nrad), dx = static_cast<float>(p.dx);
-    const float dt = static_cast<float>(p.dt_deg / 180.0) * static_cast<float>(3.14159265358979323846);
-    const float max_k = static_cast<float>(p.max_k), sx = static_cast<float>(p.sx), st_ = static_cast<float>(p.st);
-    const double fx = 2 * std::floor(dx / sx + 0.5) + 1;
+    const T dx = static_cast<T>(p.dx), max_k = static_cast<T>(p.max_k), sx = static_cast<T>(p.sx),
+            st_ = static_cast<T>(p.st);
+    const T dt = kFloat ? static_cast<T>(static_cast<float>(p.dt_deg / 180.0) * static_cast<float>(3.14159265358979323846))
+                        : static_cast<T>((p.dt_deg / 180.0) * 3.14159265358979323846);  // main.cpp:24
+    const double fx = 2 * std::floor(dx / sx + 0.5) + 1;  // :73-74
     const double ft = 2 * std::floor(dt / st_ + 0.5) + 1;
     if (!(fx >= 1 && ft >= 1) || fx * ft > 256)
         return fail(ctx, CVG_ERR_UNSUPPORTED, "bundle grid must have between 1 and 256 cells (dx/sx, dt/st)");
     const int kx = static_cast<int>(fx), kt = static_cast<int>(ft), K = kx * kt;
     const int G = static_cast<int>(p.group_max_sz);
-    std::vector<float> off_x(kx), off_t(kt), inf_x(kx), inf_t(kt), cell_d(K);
+    const int max_deg = static_cast<int>(std::max(ctx->max_deg, 1u));
+    // no chain is longer than its anchor plus all of its neighbours
+    const int cap = std::max(1, static_cast<int>(std::min<long long>(G, static_cast<long long>(max_deg) + 1)));
+    if (cap > 0x7fff) return fail(ctx, CVG_ERR_UNSUPPORTED, "chains of more than 32767 edges");
+    std::vector<T> off_x(kx), off_t(kt);
+    std::vector<float> inf_x(kx), inf_t(kt);
+    std::vector<double> dist(K);
     for (int i = 0; i < kx; ++i) {
-        off_x[i] = static_cast<float>(sx * (static_cast<float>(i) - (static_cast<float>(kx) - 1.0) / 2.0));  // :697
-        inf_x[i] = sx * (static_cast<float>(i) - (static_cast<float>(kx) - 1.0f) / 2.0f);                   // :209
+        off_x[i] = static_cast<T>(sx * (static_cast<T>(i) - (static_cast<T>(kx) - 1.0) / 2.0));  // :697
+        inf_x[i] = static_cast<float>(sx) * (static_cast<float>(i) - (static_cast<float>(kx) - 1.0f) / 2.0f);  // :209
     }
     for (int j = 0; j < kt; ++j) {
-        off_t[j] = static_cast<float>(st_ * (static_cast<float>(j) - (static_cast<float>(kt) - 1.0) / 2.0));  // :698
-        inf_t[j] = st_ * (static_cast<float>(j) - (static_cast<float>(kt) - 1.0f) / 2.0f);                     // :210
+        off_t[j] = static_cast<T>(st_ * (static_cast<T>(j) - (static_cast<T>(kt) - 1.0) / 2.0));  // :698
+        inf_t[j] = static_cast<float>(st_) * (static_cast<float>(j) - (static_cast<float>(kt) - 1.0f) / 2.0f);  // :210
     }
+    // fill_curvelet_info's d per cell (:211): in T from the info offsets (the
+    // float ones for T = float; for T = double they equal off_x, off_t)
+    std::vector<int> order(K), cell_rank(K), rank_cell(K);
     for (int c = 0; c < K; ++c) {
-        const float dd = inf_x[c / kt] * inf_x[c / kt] + inf_t[c % kt] * inf_t[c % kt];  // :211
-        cell_d[c] = dd < 100.0f ? dd : 3.0e38f;
+        if (kFloat) {
+            const float a = inf_x[c / kt], b = inf_t[c % kt];
+            dist[c] = static_cast<double>(a * a + b * b);
+        } else {
+            const double a = static_cast<double>(off_x[c / kt]), b = static_cast<double>(off_t[c % kt]);
+            dist[c] = a * a + b * b;
+        }
+        order[c] = c;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int u, int v) { return dist[u] < dist[v]; });
+    for (int r = 0; r < K; ++r) {
+        const int c = order[r];
+        rank_cell[r] = c;
+        cell_rank[c] = dist[c] < 100.0 ? r : K;
     }
     std::vector<char> blob;
     const size_t o_off_x = put(blob, off_x), o_off_t = put(blob, off_t), o_inf_x = put(blob, inf_x),
-                 o_inf_t = put(blob, inf_t), o_cell_d = put(blob, cell_d);
+                 o_inf_t = put(blob, inf_t), o_crank = put(blob, cell_rank), o_rcell = put(blob, rank_cell);
     CVG_TRY(ensure(ctx, ctx->d_tables, blob.size()));
     CVG_HIP(ctx, hipMemcpyAsync(ctx->d_tables.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, st));
     char *tb = static_cast<char *>(ctx->d_tables.ptr);
 
-    const int st_stride = std::max(G, 1);
     const size_t np = static_cast<size_t>(std::max(pairs, 1));
-    CVG_TRY(ensure(ctx, ctx->d_st_ids, np * st_stride * sizeof(int)));
+    CVG_TRY(ensure(ctx, ctx->d_st_ids, np * cap * sizeof(int)));
     CVG_TRY(ensure(ctx, ctx->d_st_meta, np * sizeof(unsigned)));
     CVG_TRY(ensure(ctx, ctx->d_st_k, np * 2 * sizeof(double)));
     CVG_TRY(ensure(ctx, ctx->d_st_rank, np * sizeof(int)));
@@ -2905,51 +3152,55 @@ int build_curvelets_f32(cvg_context *ctx, const cvg_params &p)
     CVG_HIP(ctx, hipMemsetAsync(scal + S_VALID, 0, (kScalarBufWords - S_VALID) * sizeof(unsigned), st));
 
     // one wave per anchor, as many waves as the per-wave scratch allows
-    const int max_deg = static_cast<int>(std::max(ctx->max_deg, 1u));
-    const size_t per_wave = static_cast<size_t>(max_deg) * K * 2 * sizeof(float) +
+    const size_t per_wave = static_cast<size_t>(max_deg) * K * 2 * sizeof(T) +
                             static_cast<size_t>(max_deg) * sizeof(int) +
-                            static_cast<size_t>(max_deg) * (G + 1) * sizeof(int) + 256;
+                            static_cast<size_t>(max_deg) * (cap + 1) * sizeof(int) + static_cast<size_t>(cap) * sizeof(int) +
+                            256;
     long long waves = std::min<long long>(n, static_cast<long long>(ctx->num_cu) * 16);
     waves = std::max(1LL, std::min<long long>(waves, static_cast<long long>((size_t(512) << 20) / per_wave)));
-    const int blocks = static_cast<int>((waves + kF32Waves - 1) / kF32Waves);
-    const size_t nwaves = static_cast<size_t>(blocks) * kF32Waves;
-    const size_t sb_floats = nwaves * max_deg * K * 2;
+    const int blocks = static_cast<int>((waves + kGenWaves - 1) / kGenWaves);
+    const size_t nwaves = static_cast<size_t>(blocks) * kGenWaves;
+    const size_t sb_elems = nwaves * max_deg * K * 2;
     const size_t sf_ints = nwaves * max_deg;
-    const size_t sa_ints = nwaves * max_deg * (G + 1);
-    CVG_TRY(ensure(ctx, ctx->d_gbundle, sb_floats * sizeof(float) + (sf_ints + sa_ints) * sizeof(int) + 512));
-    F32Args F{};
-    F.n = n;
-    F.x = as<double>(ctx->d_x);
-    F.y = as<double>(ctx->d_y);
-    F.th = as<double>(ctx->d_th);
-    F.off = as<unsigned>(ctx->d_off);
-    F.ids = as<int>(ctx->d_ids);
-    F.kx = kx;
-    F.kt = kt;
-    F.K = K;
-    F.G = G;
-    F.st_stride = st_stride;
-    F.max_deg = max_deg;
-    F.dx = dx;
-    F.dt = dt;
-    F.max_k = max_k;
-    F.dx2 = dx * dx;
-    F.off_x = reinterpret_cast<const float *>(tb + o_off_x);
-    F.off_t = reinterpret_cast<const float *>(tb + o_off_t);
-    F.cell_d = reinterpret_cast<const float *>(tb + o_cell_d);
-    F.sc_bundle = as<float>(ctx->d_gbundle);
-    F.sc_flag = reinterpret_cast<int *>(F.sc_bundle + sb_floats);
-    F.sc_acc = F.sc_flag + sf_ints;
-    F.st_ids = as<int>(ctx->d_st_ids);
-    F.st_meta = as<unsigned>(ctx->d_st_meta);
-    F.st_k = as<double>(ctx->d_st_k);
-    F.st_rank = as<int>(ctx->d_st_rank);
-    F.acc_count = as<unsigned>(ctx->d_acc);
-    F.valid_pairs = scal + S_VALID;
+    const size_t sa_ints = nwaves * max_deg * (cap + 1);
+    const size_t sc_ints = nwaves * cap;
+    CVG_TRY(ensure(ctx, ctx->d_gbundle, sb_elems * sizeof(T) + (sf_ints + sa_ints + sc_ints) * sizeof(int) + 512));
+    GenArgs<T> A{};
+    A.n = n;
+    A.x = as<double>(ctx->d_x);
+    A.y = as<double>(ctx->d_y);
+    A.th = as<double>(ctx->d_th);
+    A.off = as<unsigned>(ctx->d_off);
+    A.ids = as<int>(ctx->d_ids);
+    A.kt = kt;
+    A.K = K;
+    A.G = G;
+    A.cap = cap;
+    A.max_deg = max_deg;
+    A.dx = dx;
+    A.dt = dt;
+    A.max_k = max_k;
+    A.dx2 = dx * dx;                      // cpu_curvelet.hpp:654
+    A.close_thr = static_cast<T>(4) * dx * dx;  // :682, (4*dx)*dx
+    A.off_x = reinterpret_cast<const T *>(tb + o_off_x);
+    A.off_t = reinterpret_cast<const T *>(tb + o_off_t);
+    A.cell_rank = reinterpret_cast<const int *>(tb + o_crank);
+    A.rank_cell = reinterpret_cast<const int *>(tb + o_rcell);
+    A.sc_bundle = as<T>(ctx->d_gbundle);
+    A.sc_flag = reinterpret_cast<int *>(A.sc_bundle + sb_elems);
+    A.sc_acc = A.sc_flag + sf_ints;
+    A.sc_chain = A.sc_acc + sa_ints;
+    A.st_ids = as<int>(ctx->d_st_ids);
+    A.st_meta = as<unsigned>(ctx->d_st_meta);
+    A.st_k = as<double>(ctx->d_st_k);
+    A.st_rank = as<int>(ctx->d_st_rank);
+    A.acc_count = as<unsigned>(ctx->d_acc);
+    A.valid_pairs = scal + S_VALID;
+    A.intersections = reinterpret_cast<unsigned long long *>(scal + S_COUNT);
     const int ring = static_cast<int>(ctx->fused_runs % cvg_context::kFusedRing);
     CVG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
     CVG_HIP(ctx, hipEventRecord(ctx->fused_ev[ring][0], st));
-    hipLaunchKernelGGL(k_curvelets_f32, dim3(blocks), dim3(64 * kF32Waves), 0, st, F);
+    hipLaunchKernelGGL(k_curvelets_generic<T>, dim3(blocks), dim3(64 * kGenWaves), 0, st, A);
     CVG_HIP(ctx, hipGetLastError());
     CVG_HIP(ctx, hipEventRecord(ctx->ev[4], st));
     CVG_HIP(ctx, hipEventRecord(ctx->fused_ev[ring][1], st));
@@ -2959,22 +3210,28 @@ int build_curvelets_f32(cvg_context *ctx, const cvg_params &p)
     R.G = G;
     R.kt = kt;
     R.nrad = p.nrad;
-    R.off_x = nullptr;
-    R.off_t = nullptr;
-    R.f32 = true;
-    R.inf_x = reinterpret_cast<const float *>(tb + o_inf_x);
-    R.inf_t = reinterpret_cast<const float *>(tb + o_inf_t);
-    R.nrad_f = nrad;
-    return record_rows(ctx, p, R, st_stride);
+    if (kFloat) {
+        R.f32 = true;
+        R.inf_x = reinterpret_cast<const float *>(tb + o_inf_x);
+        R.inf_t = reinterpret_cast<const float *>(tb + o_inf_t);
+        R.nrad_f = static_cast<float>(p.nrad);
+    } else {
+        R.off_x = reinterpret_cast<const double *>(tb + o_off_x);
+        R.off_t = reinterpret_cast<const double *>(tb + o_off_t);
+    }
+    return record_rows(ctx, p, R, cap);
 }
 
 int build_curvelets(cvg_context *ctx, const cvg_params &p)
 {
-    if (ctx->f32) return build_curvelets_f32(ctx, p);
+    if (ctx->f32) return build_curvelets_generic<float>(ctx, p);
     if (!ctx->graph_ready) return fail(ctx, CVG_ERR_STATE, "build the neighbour graph first");
     ctx->curvelets_ready = false;
     Derived d;
     CVG_TRY(derive(ctx, p, d));
+    // the fused kernel holds a chain in the lanes of a wave and up to four
+    // 64-neighbour chunks per anchor; other maps take the general kernel
+    if (d.G > 64 || ctx->max_deg > 256 || ctx->force_generic) return build_curvelets_generic<double>(ctx, p);
     hipStream_t st = ctx->stream;
     const int n = ctx->n;
     const int pairs = ctx->pairs;
@@ -3101,7 +3358,7 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
         Launch &l = plan[b];
         l.dcap = std::min(2 * caps[b], static_cast<int>(std::max(ctx->max_deg, 1u)));
         const int need = (l.dcap + 63) / 64;
-        if (need > 4) return fail(ctx, CVG_ERR_UNSUPPORTED, "more than 256 neighbours per anchor");
+        if (need > 4) return build_curvelets_generic<double>(ctx, p);
         l.nch = need <= 1 ? 1 : (need <= 2 ? 2 : 4);
         l.gb = false;
         l.coop = caps[b] >= ctx->coop_min_cap;
@@ -3117,8 +3374,7 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
             l.coop = false;
             l.wpb = 1;
             l.slice = anchor_carve(caps[b], l.dcap, A.K, A.kt, chain_stride, false, l.nch).bytes;
-            if (l.slice > ctx->lds_limit)
-                return fail(ctx, CVG_ERR_UNSUPPORTED, "neighbour lists too long for the workgroup's LDS");
+            if (l.slice > ctx->lds_limit) return build_curvelets_generic<double>(ctx, p);
         }
         l.apb = l.coop ? 1 : l.wpb;
         l.shmem = static_cast<unsigned>(l.slice * l.apb);
@@ -3501,6 +3757,8 @@ int cvg_create(int device, cvg_context **out_ctx)
             const int g = std::atoi(v);
             ctx->grid_pct = (g >= 25 && g <= 100) ? g : 0;
         }
+        if (const char *v = std::getenv("CVG_SPARSE_CELLS")) ctx->force_sparse = std::atoi(v) != 0;
+        if (const char *v = std::getenv("CVG_GENERIC_KERNEL")) ctx->force_generic = std::atoi(v) != 0;
     } while (false);
     if (rc != CVG_OK) {
         cvg_destroy(ctx);

@@ -164,6 +164,44 @@ def test_float_wider_rows(ctx, oracle_f32_cr):
     _check_cr(ctx, oracle_f32_cr, e6, CurveletParams(edge_data_sz=6))
 
 
+def _lattice(nx, ny, offset=0.0, seed=0):
+    rng = np.random.default_rng(seed)
+    xs, ys = np.meshgrid(np.arange(nx) + offset, np.arange(ny) + offset)
+    th = rng.uniform(0, np.pi, xs.size)
+    return np.stack([xs.ravel(), ys.ravel(), th, np.ones(xs.size)], axis=1).astype(np.float32)
+
+
+def test_float_order_ambiguous_exact(ctx, oracle_f32_cr):
+    """Same-cell distance ties and duplicates in float: the reference's
+    within-cell order (its unstable spatial-index sort) is replayed, so the
+    float results stay bit-exact."""
+    p = CurveletParams(group_max_sz=7)
+    e = _lattice(25, 25, offset=0.5, seed=2)
+    _check_cr(ctx, oracle_f32_cr, e, p)
+    assert ctx.graph().order_ambiguous_anchors > 0
+    d = edges_for("synth:C1:1001")[:800].astype(np.float32)
+    d = np.concatenate([d, d[::7]])
+    _check_cr(ctx, oracle_f32_cr, d, p)
+    assert ctx.graph().order_ambiguous_anchors > 0
+
+
+@pytest.mark.parametrize("max_cand", [5, 17])
+def test_float_max_candidates_cut(ctx, oracle_f32_cr, max_cand):
+    e = edges_for("synth:C2:1002")[:5000].astype(np.float32)
+    _check_cr(ctx, oracle_f32_cr, e, CurveletParams(max_candidates=max_cand))
+
+
+def test_float_long_lists(ctx, oracle_f32_cr):
+    """Lists of up to ~150 neighbours (4 jittered edges per pixel) with
+    max_candidates 256."""
+    rng = np.random.default_rng(31)
+    k = 40 * 30 * 4
+    e = np.stack([rng.uniform(0, 40, k), rng.uniform(0, 30, k), rng.uniform(0, np.pi, k), np.ones(k)],
+                 axis=1).astype(np.float32)
+    _check_cr(ctx, oracle_f32_cr, e, CurveletParams(max_candidates=256, group_max_sz=6))
+    assert ctx.graph().max_num_of_neighbors > 128
+
+
 def test_float_then_double_same_context(ctx, oracle_cr):
     """Switching scalar type between calls on one context leaves no state
     behind: a double run after a float run is the double run."""
