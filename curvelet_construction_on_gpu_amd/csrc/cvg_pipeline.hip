@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <type_traits>
 #include <atomic>
 #include <chrono>
@@ -57,7 +58,7 @@
 using cvg::Cand;
 using cvg::PairRec;
 
-__constant__ double c_sincos_tab[4 * CVG_SIN_ROWS] = CVG_SINCOS_TABLE_INIT;
+__constant__ double c_sincos_tab[CVG_SINCOS_TABLE_SIZE] = CVG_SINCOS_TABLE_INIT;
 __constant__ double c_atan_tab[2 * CVG_ATAN_ROWS] = CVG_ATAN_TABLE_INIT;
 
 // Device memory of cvg_run_device_noalloc: every intermediate buffer of a
@@ -531,8 +532,8 @@ __global__ void k_init_scalars(unsigned *s)
 
 // Cells are (lrint(x), lrint(y)) (preprocess.hpp:85-86).  Coordinates must be
 // finite with |x|, |y| < 2^31 - 2^13 (so that every window cell cx +- R of the
-// reference's int arithmetic exists) and |theta| < 2^20 (the trig
-// reduction's domain); others raise S_BAD.
+// reference's int arithmetic exists), and orientations finite; others raise
+// S_BAD.
 // Each workgroup leaves its cell box and bad flag in part[5 * block]
 // (k_bbox_finish reduces them).
 __global__ __launch_bounds__(kBlock) void k_prep_edges(int n, int sz, const double *__restrict__ e,
@@ -554,7 +555,7 @@ __global__ __launch_bounds__(kBlock) void k_prep_edges(int n, int sz, const doub
         const double xv = e[static_cast<long long>(i) * sz];
         const double yv = e[static_cast<long long>(i) * sz + 1];
         const double tv = e[static_cast<long long>(i) * sz + 2];
-        const bool ok = fabs(xv) < kMaxCoord && fabs(yv) < kMaxCoord && fabs(tv) < 0x1p20;
+        const bool ok = fabs(xv) < kMaxCoord && fabs(yv) < kMaxCoord && fabs(tv) <= DBL_MAX;
         if (!ok) atomicOr(&sbad, 1);
         const int ix = ok ? static_cast<int>(rint(xv)) : 0;
         const int iy = ok ? static_cast<int>(rint(yv)) : 0;
@@ -2689,7 +2690,7 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     const unsigned *mb = ctx->mailbox;
     if (mb[S_BAD])
         return fail(ctx, CVG_ERR_BAD_INPUT,
-                    "edge values out of range: need finite x, y with |x|,|y| < 2^31 - 2^13 and |theta| < 2^20");
+                    "edge values out of range: need finite x, y with |x|,|y| < 2^31 - 2^13 and a finite theta");
     int minx = static_cast<int>(mb[S_MINX]), maxx = static_cast<int>(mb[S_MAXX]);
     int miny = static_cast<int>(mb[S_MINY]), maxy = static_cast<int>(mb[S_MAXY]);
     CVG_TRY(ensure(ctx, ctx->d_eoff, (static_cast<size_t>(num_maps) + 1) * sizeof(long long)));

@@ -41,7 +41,10 @@ def test_device_math_matches_host_build(ctx):
     rng = np.random.default_rng(3)
     x = np.concatenate([rng.uniform(-8, 8, 20000), rng.uniform(-1e4, 1e4, 2000),
                         np.ldexp(rng.uniform(-1, 1, 2000), -rng.integers(0, 60, 2000)),
-                        [0.0, -0.0, 1.5707963267948966, 3.141592653589793, 6.283185307179586]])
+                        [0.0, -0.0, 1.5707963267948966, 3.141592653589793, 6.283185307179586],
+                        # Payne-Hanek range, up to the largest double
+                        np.ldexp(rng.uniform(1, 2, 3000), rng.integers(19, 1024, 3000)) * rng.choice([-1.0, 1.0], 3000),
+                        [6381956970095103 * 2.0 ** 797, 1.7976931348623157e308, 2.0 ** 20, 1e22]])
     y = rng.uniform(-8, 8, x.size)
     for op in ("sin", "cos", "atan2", "to2pi", "sincos_dd_sin_hi", "sincos_dd_sin_lo", "sincos_dd_cos_hi",
                "sincos_dd_cos_lo"):

@@ -151,7 +151,18 @@ def test_float_group_size_above_64(ctx):
     assert np.array_equal(bits(info), bits(want["info"]))
 
 
-@pytest.mark.parametrize("col,value", [(0, 2.2e9), (1, -2.2e9), (2, 2.0e6), (0, np.inf), (2, np.nan)])
+def test_orientations_of_any_magnitude(ctx, oracle, oracle_cr):
+    """Orientations far outside [0, 2 pi) (the reference takes any finite
+    value): the trig of theta, theta +- dt and the grid angles goes through
+    the large-argument reduction."""
+    e = synthetic("C1", 1001)[:2500].copy()
+    rng = np.random.default_rng(44)
+    e[:, 2] += np.ldexp(1.0, rng.integers(10, 60, len(e))) * rng.choice([-1.0, 1.0], len(e))
+    stats, g = check(ctx, oracle, oracle_cr, e, CurveletParams(group_max_sz=6))
+    assert stats["num_curvelets"] > 0
+
+
+@pytest.mark.parametrize("col,value", [(0, 2.2e9), (1, -2.2e9), (0, np.inf), (2, np.inf), (2, np.nan)])
 def test_out_of_domain_rejected(ctx, col, value):
     e = synthetic("C1", 1001)[:200].copy()
     e[17, col] = value

@@ -312,6 +312,38 @@ def test_host_sqrt_fmod_exact():
 
 
 
+
+def test_host_trig_large_arguments_correctly_rounded():
+    """Orientations of any magnitude (Payne-Hanek reduction from 2^20 on):
+    sin, cos and the double-double sincos against 1500-bit evaluation,
+    including the double closest to a multiple of pi/2 (Kahan and McDonald's
+    6381956970095103 * 2^797, |x mod pi/2| ~ 4.7e-19)."""
+    import mpmath
+    from mpmath.libmp import to_float
+    mpmath.mp.prec = 1500
+    rng = np.random.default_rng(17)
+    e = rng.integers(19, 1024, 500)
+    x = np.ldexp(rng.uniform(1, 2, e.size), e) * rng.choice([-1.0, 1.0], e.size)
+    x = np.concatenate([x, [6381956970095103 * 2.0 ** 797, -6381956970095103 * 2.0 ** 797, 1.7976931348623157e308,
+                            1e22, 2.0 ** 20, -2.0 ** 20, np.nextafter(2.0 ** 20, 0), 1048576.5, 5.0e15, 1.0e300,
+                            2.0 ** 60 * np.pi]])
+    s = math_eval("sin", x, device=False)
+    c = math_eval("cos", x, device=False)
+    sh = math_eval("sincos_dd_sin_hi", x, device=False)
+    sl = math_eval("sincos_dd_sin_lo", x, device=False)
+    ch = math_eval("sincos_dd_cos_hi", x, device=False)
+    cl = math_eval("sincos_dd_cos_lo", x, device=False)
+    for i, xi in enumerate(x):
+        X = mpmath.mpf(float(xi))
+        ts, tc = mpmath.sin(X), mpmath.cos(X)
+        assert s[i] == to_float(ts._mpf_, rnd="n"), xi
+        assert c[i] == to_float(tc._mpf_, rnd="n"), xi
+        assert sh[i] == s[i] and ch[i] == c[i], xi
+        # the double-double parts within 2^-80 of the true values
+        assert abs(mpmath.mpf(float(sh[i])) + mpmath.mpf(float(sl[i])) - ts) < mpmath.mpf(2) ** -80, xi
+        assert abs(mpmath.mpf(float(ch[i])) + mpmath.mpf(float(cl[i])) - tc) < mpmath.mpf(2) ** -80, xi
+    assert min(abs(s[-11]), abs(c[-11])) < 1e-18  # the hard case really is that close to a multiple of pi/2
+
 def d_to_f32(v: Decimal) -> float:
     """The binary32 value nearest v (ties to even), by comparing v with the
     neighbours of a first guess."""
