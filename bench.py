@@ -452,7 +452,9 @@ def main(argv=None) -> int:
 
 def protocol_span(ctx, edges, params, warmup, runs):
     """H2D + kernels + D2H per run (HIP events on the context stream, ms),
-    host wall time of the same runs, and the transfer sizes."""
+    host wall time of the same runs, and the transfer sizes.  Each run is one
+    cvg_run_to_host into page-locked outputs; the call returns with every row
+    in host memory."""
     import numpy as np
     from curvelet_construction_on_gpu_amd import pinned_empty
     from curvelet_construction_on_gpu_amd.hip import Event
@@ -469,8 +471,7 @@ def protocol_span(ctx, edges, params, warmup, runs):
     for k in range(warmup + runs):
         t0 = time.perf_counter()
         e0.record(stream)
-        ctx.run(h_in, params, stats=False)
-        got = ctx.curvelets_into(h_chains, h_info)
+        got, _ = ctx.run_to_host(h_in, params, h_chains, h_info)
         e1.record(stream)
         e1.synchronize()
         t1 = time.perf_counter()

@@ -116,6 +116,8 @@ _SIGNATURES = {
     "cvg_curvelet_shape": (C.c_int, [C.c_void_p, _up, _up]),
     "cvg_get_curvelets": (C.c_int, [C.c_void_p, _up, _dp]),
     "cvg_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _P(_Params), _P(_RunStats)]),
+    "cvg_run_to_host": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _P(_Params), _P(C.c_uint), _dp, C.c_uint,
+                                  _P(C.c_uint), _P(_RunStats)]),
     "cvg_run_device": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, _P(_Params), _P(_RunStats)]),
     "cvg_run_f32": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _P(C.c_float), _P(_Params), _P(_RunStats)]),
     "cvg_build_neighbor_csr_graph_f32": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _P(C.c_float), C.c_double,
@@ -332,6 +334,27 @@ class Context:
         return s.value or 0
 
     # -- whole path ----------------------------------------------------------
+    def run_to_host(self, edges: np.ndarray, params: CurveletParams, chains: np.ndarray, info: np.ndarray,
+                    stats: bool = False) -> tuple[int, dict | None]:
+        """cvg_run_to_host: host edges in, rows into the caller's chains
+        (uint32, (rows, group_max_sz + 1)) and info (float64, (rows, 10))
+        C arrays (page-locked ones from pinned_empty take one DMA each).
+        Returns (rows, stats or None)."""
+        e = np.ascontiguousarray(edges, dtype=np.float64)
+        n, sz = e.shape
+        w = int(params.group_max_sz) + 1
+        if (chains.dtype != np.uint32 or info.dtype != np.float64 or not chains.flags.c_contiguous
+                or not info.flags.c_contiguous or chains.ndim != 2 or chains.shape[1] != w
+                or info.shape != (chains.shape[0], 10)):
+            raise CurveletError(f"run_to_host: need uint32 (rows, {w}) and float64 (rows, 10) C arrays")
+        st = _RunStats()
+        rows = C.c_uint()
+        p = params._c()
+        self.check(self._lib.cvg_run_to_host(self._h, n, sz, _ptr(e, C.c_double), C.byref(p), _ptr(chains, C.c_uint),
+                                             _ptr(info, C.c_double), chains.shape[0], C.byref(rows),
+                                             C.byref(st) if stats else None), "run_to_host")
+        return rows.value, ({k: getattr(st, k) for k, _ in _RunStats._fields_} if stats else None)
+
     def run(self, edges: np.ndarray, params: CurveletParams, stats: bool = True) -> dict | None:
         """cvg_run from host memory.  stats=False returns once the work is
         queued; results are complete at their first query."""

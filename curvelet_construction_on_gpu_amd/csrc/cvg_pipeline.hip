@@ -3475,6 +3475,8 @@ int check_ctx(cvg_context *ctx)
     return CVG_OK;
 }
 
+void fill_stats(cvg_context *ctx, cvg_run_stats *stats);
+
 int run_common(cvg_context *ctx, int n, int sz, const double *d_edges, const cvg_params *p, cvg_run_stats *stats,
                int num_maps = 1, const long long *eoff = nullptr, bool f32 = false)
 {
@@ -3490,6 +3492,13 @@ int run_common(cvg_context *ctx, int n, int sz, const double *d_edges, const cvg
     if (!stats) return CVG_OK;
     CVG_HIP(ctx, hipEventSynchronize(ctx->ev[2]));
     CVG_TRY(finish_curvelets(ctx));
+    fill_stats(ctx, stats);
+    return CVG_OK;
+}
+
+// The run's counts and stage times (the stream has passed ev[2]).
+void fill_stats(cvg_context *ctx, cvg_run_stats *stats)
+{
     {
         float t01 = 0, t12 = 0, t02 = 0;
         (void)hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]);
@@ -3510,7 +3519,6 @@ int run_common(cvg_context *ctx, int n, int sz, const double *d_edges, const cvg
         stats->order_ambiguous_anchors = ctx->ambiguous;
         stats->lds_bytes_per_workgroup = static_cast<unsigned>(ctx->lds_limit);
     }
-    return CVG_OK;
 }
 
 int upload(cvg_context *ctx, int n, int sz, const double *host, const double **dptr)
@@ -3940,6 +3948,30 @@ int cvg_run_f32(cvg_context *ctx, int num_edges, int sz_edge_data, const float *
     // the upload must have left the host vector before it goes away
     CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return run_common(ctx, num_edges, sz_edge_data, d, params, stats, 1, nullptr, true);
+}
+
+int cvg_run_to_host(cvg_context *ctx, int num_edges, int sz_edge_data, const double *edges, const cvg_params *params,
+                    unsigned *chains, double *info, unsigned max_rows, unsigned *num_rows, cvg_run_stats *stats)
+{
+    CVG_TRY(check_ctx(ctx));
+    if (!params || !edges || !chains || !info || num_edges <= 0 || sz_edge_data < 4)
+        return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "cvg_run_to_host: bad arguments");
+    const double *d = nullptr;
+    CVG_TRY(upload(ctx, num_edges, sz_edge_data, edges, &d));
+    CVG_TRY(run_common(ctx, num_edges, sz_edge_data, d, params, nullptr));
+    CVG_HIP(ctx, hipEventSynchronize(ctx->ev[2]));
+    CVG_TRY(finish_curvelets(ctx));
+    const unsigned rows = ctx->num_curvelets;
+    if (num_rows) *num_rows = rows;
+    const size_t C = std::min(rows, max_rows);
+    CVG_TRY(download(ctx, {{chains, ctx->d_chains.ptr, C * ctx->width * sizeof(unsigned)},
+                           {info, ctx->d_info.ptr, C * kInfoWidth * sizeof(double)}}));
+    if (stats) fill_stats(ctx, stats);
+    if (rows > max_rows)
+        return fail(ctx, CVG_ERR_INVALID_ARGUMENT,
+                    "cvg_run_to_host: the outputs hold " + std::to_string(max_rows) + " rows, the run has " +
+                        std::to_string(rows));
+    return CVG_OK;
 }
 
 int cvg_run_device(cvg_context *ctx, int num_edges, int sz_edge_data, const double *device_edges,

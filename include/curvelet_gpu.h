@@ -168,6 +168,17 @@ int cvg_get_curvelets(cvg_context *ctx, unsigned *chains, double *info);
  * and fills them. */
 int cvg_run(cvg_context *ctx, int num_edges, int sz_edge_data, const double *edges, const cvg_params *params,
             cvg_run_stats *stats);
+
+/* run_curvelet<double> from host edges to host results (main.cpp:43-88: the
+ * timed span plus the copy of _edge_chain_final and _curvelet_info into the
+ * output arrays) in one call: chains (max_rows x (group_max_sz + 1),
+ * 1-based) and info (max_rows x 10, row-major) in host memory (page-locked
+ * ones from cvg_host_alloc take one DMA each), *num_rows the row count.
+ * Returns when the rows are in the outputs.  More rows than max_rows fails
+ * with CVG_ERR_INVALID_ARGUMENT after writing the first max_rows (*num_rows
+ * still says how many there are). */
+int cvg_run_to_host(cvg_context *ctx, int num_edges, int sz_edge_data, const double *edges, const cvg_params *params,
+                    unsigned *chains, double *info, unsigned max_rows, unsigned *num_rows, cvg_run_stats *stats);
 /* run_curvelet<float> (main.cpp:110 with --float): the whole path on a float
  * edge array (row-major num_edges x sz_edge_data), in the reference's float
  * arithmetic; results as for cvg_run (chains, and info widened to double). */

@@ -356,3 +356,64 @@ def test_downloads_into_device_memory(ctx):
     finally:
         hip.hipFree(dc)
         hip.hipFree(di)
+
+
+def _to_host(c, e, p, pinned=True, rows=None):
+    from curvelet_construction_on_gpu_amd import pinned_empty
+    c.run(e, p)
+    want_chains, want_info = c.curvelets()
+    nr = want_chains.shape[0] if rows is None else rows
+    w = p.group_max_sz + 1
+    if pinned:
+        hc, hi = pinned_empty((nr, w), np.uint32), pinned_empty((nr, 10), np.float64)
+    else:
+        hc, hi = np.empty((nr, w), np.uint32), np.empty((nr, 10), np.float64)
+    hc[:] = 0xFFFFFFFF
+    hi[:] = np.nan
+    got, _ = c.run_to_host(e, p, hc, hi)
+    return got, hc, hi, want_chains, want_info
+
+
+@pytest.mark.parametrize("key,G", [("synth:C3:1003", 7), ("file:TO_edges", 4), ("synth:C1:1001", 5)])
+def test_run_to_host_matches_run(ctx, key, G):
+    """cvg_run_to_host writes the same rows as a run followed by
+    cvg_get_curvelets, into page-locked and pageable arrays."""
+    from tests._inputs import edges_for
+    e = edges_for(key)
+    p = CurveletParams(group_max_sz=G)
+    for pinned in (True, False):
+        got, hc, hi, wc, wi = _to_host(ctx, e, p, pinned)
+        assert got == wc.shape[0]
+        assert np.array_equal(hc, wc), (key, pinned)
+        assert np.array_equal(hi.view(np.uint64), wi.view(np.uint64)), (key, pinned)
+
+
+def test_run_to_host_exact_order_and_general_kernel(ctx, oracle):
+    """A map whose within-cell order is replayed (the stage runs again) and G
+    above 64 (the general kernel) through the same call."""
+    d = synthetic("C1", 1001)[:800].copy()
+    e = np.concatenate([d, d[::7]])
+    p = CurveletParams(group_max_sz=7)
+    got, hc, hi, wc, wi = _to_host(ctx, e, p)
+    want = oracle.run(e, p)
+    assert ctx.graph().order_ambiguous_anchors > 0
+    assert np.array_equal(hc, want["chains"]) and np.array_equal(hc, wc)
+    assert np.array_equal(hi.view(np.uint64), wi.view(np.uint64))
+    p = CurveletParams(group_max_sz=70)
+    e = synthetic("C1", 1001)
+    got, hc, hi, wc, wi = _to_host(ctx, e, p)
+    assert np.array_equal(hc, wc) and np.array_equal(hi.view(np.uint64), wi.view(np.uint64))
+
+
+def test_run_to_host_too_few_rows(ctx):
+    from curvelet_construction_on_gpu_amd import pinned_empty
+    e = synthetic("C1", 1001)
+    p = CurveletParams(group_max_sz=5)
+    ctx.run(e, p)
+    wc, wi = ctx.curvelets()
+    k = wc.shape[0] // 2
+    hc, hi = pinned_empty((k, 6), np.uint32), pinned_empty((k, 10), np.float64)
+    with pytest.raises(CurveletError):
+        ctx.run_to_host(e, p, hc, hi)
+    assert np.array_equal(hc, wc[:k])
+    assert np.array_equal(hi.view(np.uint64), wi[:k].view(np.uint64))
