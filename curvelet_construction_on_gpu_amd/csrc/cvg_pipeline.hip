@@ -136,6 +136,12 @@ struct cvg_context {
     unsigned truncated = 0;
     unsigned ambiguous = 0;
     int gx0 = 0, gy0 = 0, gw = 0, gh = 0;
+    // d_grid's leading cell counts known to be zero (while d_grid is grid_zero_ptr)
+    const void *grid_zero_ptr = nullptr;
+    long long grid_zero_cells = 0;
+    // the bytes last copied to d_tables (while d_tables is tables_ptr)
+    const void *tables_ptr = nullptr;
+    std::vector<char> tables_blob;
     DevBuf d_edges_up, d_x, d_y, d_th, d_cx, d_cy, d_scal, d_grid, d_tmp_ids, d_sorted, d_sxy, d_scell, d_part, d_deg,
         d_off, d_tiles, d_cand, d_ids, d_d2, d_pair_anchor;
     // the last graph's edge array (device) and, once the reference's
@@ -359,7 +365,8 @@ __global__ __launch_bounds__(1024) void k_scan_tile_sums(unsigned *tile_sums, in
                                                          const unsigned *__restrict__ tile_max,
                                                          const unsigned *__restrict__ tile_cnt,
                                                          unsigned *__restrict__ stat_max,
-                                                         unsigned *__restrict__ stat_cnt)
+                                                         unsigned *__restrict__ stat_cnt,
+                                                         unsigned *__restrict__ stat_total)
 {
     __shared__ unsigned lw[1024 / 64];
     __shared__ unsigned lm[1024 / 64], lc[1024 / 64];
@@ -384,6 +391,7 @@ __global__ __launch_bounds__(1024) void k_scan_tile_sums(unsigned *tile_sums, in
         if constexpr (STATS) {
             *stat_max = mx;
             *stat_cnt = c;
+            if (stat_total) *stat_total = carry;
         }
     }
 }
@@ -412,7 +420,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const unsigned *in, long 
 // the maximum of the inputs goes to *stat_max and the number of inputs >= thr
 // to *stat_cnt.
 int exclusive_scan(cvg_context *ctx, const unsigned *in, long long n, unsigned *out, DevBuf &tiles,
-                   unsigned *stat_max = nullptr, unsigned *stat_cnt = nullptr, unsigned thr = 0)
+                   unsigned *stat_max = nullptr, unsigned *stat_cnt = nullptr, unsigned thr = 0,
+                   unsigned *stat_total = nullptr)
 {
     const int ntiles = static_cast<int>((n + kScanTile - 1) / kScanTile);
     CVG_TRY(ensure(ctx, tiles, static_cast<size_t>(3) * (ntiles + 1) * sizeof(unsigned)));
@@ -423,13 +432,13 @@ int exclusive_scan(cvg_context *ctx, const unsigned *in, long long n, unsigned *
             hipLaunchKernelGGL(k_scan_tiles<true>, dim3(ntiles), dim3(kBlock), 0, ctx->stream, in, n, t, thr, tmax,
                                tcnt);
         hipLaunchKernelGGL(k_scan_tile_sums<true>, dim3(1), dim3(1024), 0, ctx->stream, t, ntiles, tmax, tcnt,
-                           stat_max, stat_cnt);
+                           stat_max, stat_cnt, stat_total);
     } else {
         if (ntiles > 0)
             hipLaunchKernelGGL(k_scan_tiles<false>, dim3(ntiles), dim3(kBlock), 0, ctx->stream, in, n, t, thr, tmax,
                                tcnt);
         hipLaunchKernelGGL(k_scan_tile_sums<false>, dim3(1), dim3(1024), 0, ctx->stream, t, ntiles, tmax, tcnt,
-                           stat_max, stat_cnt);
+                           stat_max, stat_cnt, static_cast<unsigned *>(nullptr));
     }
     if (ntiles > 0)
         hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(kBlock), 0, ctx->stream, in, n, t, ntiles, out);
@@ -508,27 +517,16 @@ enum {
     S_AMBIG = 7,
     S_VALID = 8,
     S_TIES = 9,
-    S_COUNT = 10
+    S_COUNT = 10,
+    S_PAIRS = 12
 };
-constexpr int kScalarWords = S_COUNT + 2;  // the words read back to the host
+constexpr int kScalarWords = S_PAIRS + 1;  // the words read back to the host
 constexpr int kMaxBuckets = 14;
 constexpr int kBucketStride = 32;  // words
 constexpr int kBucketBase = 32;    // bucket b's list length at [kBucketBase + b * kBucketStride]
 // heavy-bucket anchor streams that have started (own 128-byte line)
 constexpr int kStartedWord = kBucketBase + kMaxBuckets * kBucketStride;
 constexpr int kScalarBufWords = kStartedWord + 32;
-
-__global__ void k_init_scalars(unsigned *s)
-{
-    const int t = threadIdx.x;
-    if (t >= S_COUNT && t < kScalarWords) s[t] = 0u;
-    if (t < S_COUNT) {
-        int v = 0;
-        if (t == S_MINX || t == S_MINY) v = INT_MAX;
-        if (t == S_MAXX || t == S_MAXY) v = INT_MIN;
-        s[t] = static_cast<unsigned>(v);
-    }
-}
 
 // Cells are (lrint(x), lrint(y)) (preprocess.hpp:85-86).  Coordinates must be
 // finite with |x|, |y| < 2^31 - 2^13 (so that every window cell cx +- R of the
@@ -617,6 +615,8 @@ __global__ __launch_bounds__(1024) void k_bbox_finish(int nparts, const int *__r
     if ((threadIdx.x & 63) == 0)
         for (int k = 0; k < 5; ++k) l[k][w] = v[k];
     __syncthreads();
+    // every other scalar word starts at zero (this is the run's first write to them)
+    if (threadIdx.x > S_BAD && threadIdx.x < kScalarWords) scal[threadIdx.x] = 0u;
     if (threadIdx.x == 0) {
         for (int u = 1; u < static_cast<int>(blockDim.x >> 6); ++u) {
             v[0] = min(v[0], l[0][u]);
@@ -1136,8 +1136,12 @@ __global__ __launch_bounds__(kBlock) void k_tie_sort(const int *__restrict__ tie
 // k = Kt theta + dt, Kt + 1 theta - dt, Kt + 2 theta itself.
 __global__ __launch_bounds__(kBlock) void k_edge_trig(int n, const double *__restrict__ th, double dt, int kt,
                                                       const double *__restrict__ off_t, double *__restrict__ trig,
-                                                      double *__restrict__ tdd)
+                                                      double *__restrict__ tdd, unsigned *__restrict__ zero,
+                                                      int zero_words)
 {
+    // the curvelet stage's counters start at zero (block 0 clears them)
+    if (blockIdx.x == 0)
+        for (int t = static_cast<int>(threadIdx.x); t < zero_words; t += static_cast<int>(blockDim.x)) zero[t] = 0u;
     const unsigned na = static_cast<unsigned>(kt) + 3u;
     const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= static_cast<unsigned>(n) * na) return;
@@ -2590,11 +2594,9 @@ int graph_scans(cvg_context *ctx, const Cells &cells, int n, unsigned R, unsigne
     }
     CVG_HIP(ctx, hipGetLastError());
     CVG_TRY(exclusive_scan(ctx, as<unsigned>(ctx->d_deg), n, as<unsigned>(ctx->d_off), ctx->d_tiles,
-                           scal + S_MAXDEG, scal + S_TRUNC, max_cand));
-    CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox + 32, as<unsigned>(ctx->d_off) + n, sizeof(unsigned),
-                                hipMemcpyDeviceToHost, st));
+                           scal + S_MAXDEG, scal + S_TRUNC, max_cand, scal + S_PAIRS));
     CVG_TRY(read_scalars(ctx));
-    const unsigned pairs_u = ctx->mailbox[32];
+    const unsigned pairs_u = ctx->mailbox[S_PAIRS];
     if (pairs_u > static_cast<unsigned>(INT_MAX))
         return fail(ctx, CVG_ERR_UNSUPPORTED, "more than 2^31-1 neighbour pairs");
     const int pairs = static_cast<int>(pairs_u);
@@ -2678,7 +2680,6 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     CVG_TRY(ensure(ctx, ctx->d_cy, ni));
     CVG_TRY(ensure(ctx, ctx->d_scal, kScalarBufWords * sizeof(unsigned)));
     unsigned *scal = as<unsigned>(ctx->d_scal);
-    hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, st, scal);
     const int prep_blocks = blocks_for(n);
     CVG_TRY(ensure(ctx, ctx->d_part, static_cast<size_t>(prep_blocks) * 5 * sizeof(int)));
     hipLaunchKernelGGL(k_prep_edges, dim3(prep_blocks), dim3(kBlock), 0, st, n, sz, d_edges, as<double>(ctx->d_x),
@@ -2693,9 +2694,11 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
                     "edge values out of range: need finite x, y with |x|,|y| < 2^31 - 2^13 and a finite theta");
     int minx = static_cast<int>(mb[S_MINX]), maxx = static_cast<int>(mb[S_MAXX]);
     int miny = static_cast<int>(mb[S_MINY]), maxy = static_cast<int>(mb[S_MAXY]);
+    // the per-map edge ranges (kernels read them only for batches)
     CVG_TRY(ensure(ctx, ctx->d_eoff, (static_cast<size_t>(num_maps) + 1) * sizeof(long long)));
-    CVG_HIP(ctx, hipMemcpyAsync(ctx->d_eoff.ptr, ctx->map_eoff.data(), (static_cast<size_t>(num_maps) + 1) *
-                                sizeof(long long), hipMemcpyHostToDevice, st));
+    if (num_maps > 1)
+        CVG_HIP(ctx, hipMemcpyAsync(ctx->d_eoff.ptr, ctx->map_eoff.data(), (static_cast<size_t>(num_maps) + 1) *
+                                    sizeof(long long), hipMemcpyHostToDevice, st));
     if (num_maps > 1) {
         // per-map boxes, then a shelf packing of the boxes into one grid
         CVG_TRY(ensure(ctx, ctx->d_map, ni));
@@ -2780,12 +2783,22 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
         CVG_TRY(ensure(ctx, ctx->d_grid, static_cast<size_t>(2 * cells + 1) * sizeof(unsigned)));
         unsigned *cnt = as<unsigned>(ctx->d_grid);
         unsigned *start = cnt + cells;
-        CVG_HIP(ctx, hipMemsetAsync(cnt, 0, static_cast<size_t>(cells) * sizeof(unsigned), st));
+        // k_cell_scatter leaves every count it decrements at zero, so the
+        // counts of a grid no larger than the last one in the same context
+        // buffer start zeroed (not tracked for the no-alloc scratch, whose
+        // regions other buffers reuse)
+        const bool zeroed = !ctx->arena && ctx->d_grid.ptr == ctx->grid_zero_ptr && cells <= ctx->grid_zero_cells;
+        ctx->grid_zero_ptr = nullptr;
+        if (!zeroed) CVG_HIP(ctx, hipMemsetAsync(cnt, 0, static_cast<size_t>(cells) * sizeof(unsigned), st));
         hipLaunchKernelGGL(k_cell_count, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
                            as<int>(ctx->d_cy), g, cnt);
         CVG_TRY(exclusive_scan(ctx, cnt, cells, start, ctx->d_tiles));
         hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
                            as<int>(ctx->d_cy), g, start, cnt, as<int>(ctx->d_tmp_ids));
+        if (!ctx->arena) {
+            ctx->grid_zero_ptr = ctx->d_grid.ptr;
+            ctx->grid_zero_cells = cells;
+        }
         hipLaunchKernelGGL(k_cell_order, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
                            as<int>(ctx->d_cy), g, start, as<int>(ctx->d_tmp_ids), rank, as<double>(ctx->d_x),
                            as<double>(ctx->d_y), as<int>(ctx->d_sorted), as<int>(ctx->d_scell),
@@ -2894,6 +2907,19 @@ int derive(cvg_context *ctx, const cvg_params &p, Derived &d)
         d.rank_elig[r] = dist[c] < 100.0 ? 1 : 0;
         if (c == 0) d.rank_of_cell0 = static_cast<int>(r);
     }
+    return CVG_OK;
+}
+
+// The parameter tables to d_tables, unless the same bytes are already there
+// (repeated runs with the same parameters skip the copy; the no-alloc
+// scratch's regions are reused by other buffers, so there it always copies).
+int upload_tables(cvg_context *ctx, const std::vector<char> &blob)
+{
+    CVG_TRY(ensure(ctx, ctx->d_tables, blob.size()));
+    if (!ctx->arena && ctx->d_tables.ptr == ctx->tables_ptr && blob == ctx->tables_blob) return CVG_OK;
+    CVG_HIP(ctx, hipMemcpyAsync(ctx->d_tables.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, ctx->stream));
+    ctx->tables_ptr = ctx->arena ? nullptr : ctx->d_tables.ptr;
+    ctx->tables_blob = blob;
     return CVG_OK;
 }
 
@@ -3138,8 +3164,7 @@ int build_curvelets_generic(cvg_context *ctx, const cvg_params &p)
     std::vector<char> blob;
     const size_t o_off_x = put(blob, off_x), o_off_t = put(blob, off_t), o_inf_x = put(blob, inf_x),
                  o_inf_t = put(blob, inf_t), o_crank = put(blob, cell_rank), o_rcell = put(blob, rank_cell);
-    CVG_TRY(ensure(ctx, ctx->d_tables, blob.size()));
-    CVG_HIP(ctx, hipMemcpyAsync(ctx->d_tables.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, st));
+    CVG_TRY(upload_tables(ctx, blob));
     char *tb = static_cast<char *>(ctx->d_tables.ptr);
 
     const size_t np = static_cast<size_t>(std::max(pairs, 1));
@@ -3254,8 +3279,7 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     const size_t o_off_x = put(blob, d.off_x);
     const size_t o_off_t = put(blob, d.off_t);
     const size_t o_caps = put(blob, caps);
-    CVG_TRY(ensure(ctx, ctx->d_tables, blob.size()));
-    CVG_HIP(ctx, hipMemcpyAsync(ctx->d_tables.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, st));
+    CVG_TRY(upload_tables(ctx, blob));
     char *tb = static_cast<char *>(ctx->d_tables.ptr);
     Tables T;
     T.rank_ox = reinterpret_cast<double *>(tb + o_rank_ox);
@@ -3280,8 +3304,8 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
         return fail(ctx, CVG_ERR_UNSUPPORTED, "more than 2^32 (edge, angle) items in one run");
     hipLaunchKernelGGL(k_edge_trig, dim3(blocks_for(static_cast<long long>(n) * (d.kt + 3))), dim3(kBlock), 0, st, n,
                        as<double>(ctx->d_th), d.dt,
-                       static_cast<int>(d.kt), T.off_t, as<double>(ctx->d_trig), as<double>(ctx->d_tdd));
-    CVG_HIP(ctx, hipMemsetAsync(scal + S_VALID, 0, (kScalarBufWords - S_VALID) * sizeof(unsigned), st));
+                       static_cast<int>(d.kt), T.off_t, as<double>(ctx->d_trig), as<double>(ctx->d_tdd),
+                       scal + S_VALID, kScalarBufWords - S_VALID);
     if (pairs > 0) {
         hipLaunchKernelGGL(k_pair_prep, dim3(blocks_for(pairs)), dim3(kBlock), 0, st, pairs, n,
                            as<int>(ctx->d_pair_anchor), as<int>(ctx->d_ids), as<double>(ctx->d_x),
