@@ -85,6 +85,11 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// A dense pixel-cell grid: origin and size in cells.
+struct GridDesc {
+    int x0, y0, w, h;
+};
+
 }  // namespace
 
 struct cvg_context {
@@ -139,6 +144,12 @@ struct cvg_context {
     // d_grid's leading cell counts known to be zero (while d_grid is grid_zero_ptr)
     const void *grid_zero_ptr = nullptr;
     long long grid_zero_cells = 0;
+    // the grid, window radius and longest list a single map tries first (the
+    // last single map's; spec_valid once there is one)
+    bool spec_valid = false;
+    GridDesc spec_g{};
+    unsigned spec_R = 0;
+    int spec_fill_cap = 1;
     // the bytes last copied to d_tables (while d_tables is tables_ptr)
     const void *tables_ptr = nullptr;
     std::vector<char> tables_blob;
@@ -452,9 +463,6 @@ int exclusive_scan(cvg_context *ctx, const unsigned *in, long long n, unsigned *
 // Graph stage: build_neighbor_csr_graph (preprocess.hpp:62-274)
 // ---------------------------------------------------------------------------
 
-struct GridDesc {
-    int x0, y0, w, h;
-};
 
 // Where the edges of a window row lie in the cell-sorted arrays.  The dense
 // grid keeps every cell's start (one load per row end); maps whose cells span
@@ -518,9 +526,10 @@ enum {
     S_VALID = 8,
     S_TIES = 9,
     S_COUNT = 10,
-    S_PAIRS = 12
+    S_PAIRS = 12,
+    S_MISS = 13
 };
-constexpr int kScalarWords = S_PAIRS + 1;  // the words read back to the host
+constexpr int kScalarWords = S_MISS + 1;  // the words read back to the host
 constexpr int kMaxBuckets = 14;
 constexpr int kBucketStride = 32;  // words
 constexpr int kBucketBase = 32;    // bucket b's list length at [kBucketBase + b * kBucketStride]
@@ -713,11 +722,30 @@ __global__ void k_map_rows(int num_maps, const long long *__restrict__ eoff, con
     if (m <= num_maps) rows[m] = row_off[eoff[m]];
 }
 
-__global__ __launch_bounds__(kBlock) void k_cell_count(int n, const int *__restrict__ cx, const int *__restrict__ cy,
-                                                       GridDesc g, unsigned *__restrict__ cnt)
+// With miss set, the grid is a guess (the last map's): an edge whose window
+// (its cell +- R) leaves the grid is moved inside it, so that every later
+// access stays in bounds, and *miss is raised; the host then builds the graph
+// again on the map's own grid.
+__global__ __launch_bounds__(kBlock) void k_cell_count(int n, int *__restrict__ cx, int *__restrict__ cy, GridDesc g,
+                                                       unsigned *__restrict__ cnt, int R, unsigned *__restrict__ miss)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(&cnt[cell_index(g, cx[i], cy[i])], 1u);
+    bool out = false;
+    if (i < n) {
+        int x = cx[i], y = cy[i];
+        if (miss) {
+            const int lox = g.x0 + R, hix = g.x0 + g.w - 1 - R, loy = g.y0 + R, hiy = g.y0 + g.h - 1 - R;
+            out = x < lox || x > hix || y < loy || y > hiy;
+            if (out) {
+                x = min(max(x, lox), hix);
+                y = min(max(y, loy), hiy);
+                cx[i] = x;
+                cy[i] = y;
+            }
+        }
+        atomicAdd(&cnt[cell_index(g, x, y)], 1u);
+    }
+    if (__ballot(out) != 0ULL && (threadIdx.x & 63) == 0) atomicOr(miss, 1u);
 }
 
 // cnt holds the per-cell counts on entry and is decremented to zero.
@@ -973,7 +1001,8 @@ __global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const doubl
                                                               unsigned max_cand, int cap,
                                                               const int *__restrict__ map_of,
                                                               const long long *__restrict__ map_eoff,
-                                                              unsigned *__restrict__ scal, bool f32)
+                                                              unsigned *__restrict__ scal, bool f32,
+                                                              unsigned long long pair_cap)
 {
     // per segment: cap candidates (d2, id, cell) in LDS for the ranking
     extern __shared__ double fill_lds[];
@@ -998,6 +1027,12 @@ __global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const doubl
         iy = cy[a];
         o = off[a];
         dg = static_cast<int>(off[a + 1] - o);
+        // buffers sized by a guess (the last map's): a list beyond them is
+        // skipped and S_MISS raised, and the host builds the graph again
+        if (dg > cap || static_cast<unsigned long long>(o) + static_cast<unsigned>(dg) > pair_cap) {
+            if (sl == 0) atomicOr(&scal[S_MISS], 1u);
+            dg = 0;
+        }
     }
     int kept = 0;
     int last_cell = -1;  // cell of the last kept candidate
@@ -2569,10 +2604,17 @@ int read_scalars(cvg_context *ctx)
     return CVG_OK;
 }
 
+constexpr int kSpecMiss = 1000;  // internal: a guessed size did not hold (never returned to callers)
+
 // Neighbour counts, CSR offsets, then the gathering pass, the distance sort
-// and the CSR fill (preprocess.hpp:186-274) over either cell layout.
+// and the CSR fill (preprocess.hpp:186-274) over either cell layout.  The
+// pair count and largest degree size the fill; with spec they are not read
+// first: the fill runs on the buffers and list capacity of the context's last
+// graph (guarded on the device, S_MISS when they do not hold), and the host
+// reads the counts once, after it (kSpecMiss: build again with the real sizes).
 template <class Cells>
-int graph_scans(cvg_context *ctx, const Cells &cells, int n, unsigned R, unsigned max_cand, double rad2, int num_maps)
+int graph_scans(cvg_context *ctx, const Cells &cells, int n, unsigned R, unsigned max_cand, double rad2, int num_maps,
+                bool spec)
 {
     hipStream_t st = ctx->stream;
     unsigned *scal = as<unsigned>(ctx->d_scal);
@@ -2595,28 +2637,41 @@ int graph_scans(cvg_context *ctx, const Cells &cells, int n, unsigned R, unsigne
     CVG_HIP(ctx, hipGetLastError());
     CVG_TRY(exclusive_scan(ctx, as<unsigned>(ctx->d_deg), n, as<unsigned>(ctx->d_off), ctx->d_tiles,
                            scal + S_MAXDEG, scal + S_TRUNC, max_cand, scal + S_PAIRS));
-    CVG_TRY(read_scalars(ctx));
-    const unsigned pairs_u = ctx->mailbox[S_PAIRS];
-    if (pairs_u > static_cast<unsigned>(INT_MAX))
-        return fail(ctx, CVG_ERR_UNSUPPORTED, "more than 2^31-1 neighbour pairs");
-    const int pairs = static_cast<int>(pairs_u);
-    ctx->pairs = pairs;
-    ctx->max_deg = ctx->mailbox[S_MAXDEG];
-    if (ctx->max_deg >= (1u << 23))
-        return fail(ctx, CVG_ERR_UNSUPPORTED, "an anchor has 2^23 or more neighbours");
-    ctx->truncated = ctx->mailbox[S_TRUNC];
+    auto take_counts = [&]() -> int {
+        const unsigned pairs_u = ctx->mailbox[S_PAIRS];
+        if (pairs_u > static_cast<unsigned>(INT_MAX))
+            return fail(ctx, CVG_ERR_UNSUPPORTED, "more than 2^31-1 neighbour pairs");
+        ctx->pairs = static_cast<int>(pairs_u);
+        ctx->max_deg = ctx->mailbox[S_MAXDEG];
+        if (ctx->max_deg >= (1u << 23))
+            return fail(ctx, CVG_ERR_UNSUPPORTED, "an anchor has 2^23 or more neighbours");
+        ctx->truncated = ctx->mailbox[S_TRUNC];
+        return CVG_OK;
+    };
+    int fill_cap;
+    unsigned long long pair_cap;
+    if (spec) {
+        fill_cap = ctx->spec_fill_cap;
+        pair_cap = std::min({ctx->d_cand.bytes / sizeof(Cand), ctx->d_ids.bytes / sizeof(int),
+                             ctx->d_d2.bytes / sizeof(double), ctx->d_pair_anchor.bytes / sizeof(int)});
+    } else {
+        CVG_TRY(read_scalars(ctx));
+        CVG_TRY(take_counts());
+        const size_t np = static_cast<size_t>(ctx->pairs);
+        CVG_TRY(ensure(ctx, ctx->d_cand, np * sizeof(Cand)));
+        CVG_TRY(ensure(ctx, ctx->d_ids, np * sizeof(int)));
+        CVG_TRY(ensure(ctx, ctx->d_d2, np * sizeof(double)));
+        CVG_TRY(ensure(ctx, ctx->d_pair_anchor, np * sizeof(int)));
+        fill_cap = static_cast<int>(std::max(ctx->max_deg, 1u));
+        pair_cap = np;
+    }
 
-    // gather, sort, fill
-    const size_t np = static_cast<size_t>(pairs);
-    CVG_TRY(ensure(ctx, ctx->d_cand, np * sizeof(Cand)));
-    CVG_TRY(ensure(ctx, ctx->d_ids, np * sizeof(int)));
-    CVG_TRY(ensure(ctx, ctx->d_d2, np * sizeof(double)));
-    CVG_TRY(ensure(ctx, ctx->d_pair_anchor, np * sizeof(int)));
-    // one G-lane segment per anchor; each segment keeps its candidates in LDS
+    // gather, sort, fill: one G-lane segment per anchor; each segment keeps
+    // its candidates in LDS
     using FillFn = void (*)(int, const double *, const double *, const int *, const int *, Cells, int, double,
                             const int *, const int *, const double2 *, const unsigned *, Cand *, int *, double *,
-                            int *, int *, unsigned, int, const int *, const long long *, unsigned *, bool);
-    const int fill_cap = static_cast<int>(std::max(ctx->max_deg, 1u));
+                            int *, int *, unsigned, int, const int *, const long long *, unsigned *, bool,
+                            unsigned long long);
     const size_t per_seg = static_cast<size_t>(fill_cap) * (2 * sizeof(double));
     int G = ctx->fill_lanes;
     // fewer anchors per wave when the lists are long
@@ -2639,12 +2694,64 @@ int graph_scans(cvg_context *ctx, const Cells &cells, int n, unsigned R, unsigne
                        as<double2>(ctx->d_sxy), as<unsigned>(ctx->d_off), as<Cand>(ctx->d_cand),
                        as<int>(ctx->d_ids), as<double>(ctx->d_d2), as<int>(ctx->d_pair_anchor),
                        as<int>(ctx->d_tmp_ids), max_cand, fill_cap, num_maps > 1 ? as<int>(ctx->d_map) : nullptr,
-                       as<long long>(ctx->d_eoff), scal, ctx->f32);
+                       as<long long>(ctx->d_eoff), scal, ctx->f32, pair_cap);
     hipLaunchKernelGGL(k_tie_sort, dim3(std::min(blocks_for(n), 64)), dim3(kBlock), 0, st, as<int>(ctx->d_tmp_ids),
                        scal, as<unsigned>(ctx->d_off), as<Cand>(ctx->d_cand), as<int>(ctx->d_ids),
                        as<double>(ctx->d_d2));
     CVG_HIP(ctx, hipGetLastError());
+    if (spec) {
+        CVG_TRY(read_scalars(ctx));
+        if (ctx->mailbox[S_BAD] || ctx->mailbox[S_MISS]) return kSpecMiss;  // the exact path reports bad input
+        CVG_TRY(take_counts());
+    }
     return CVG_OK;
+}
+
+// The dense cell grid g (counting sort into cells), then the scans.  With
+// spec, g is a guess (the last map's grid): k_cell_count raises S_MISS for an
+// edge whose window leaves it, and graph_scans runs on guessed sizes;
+// kSpecMiss then asks for a build on the map's own grid.
+int graph_dense(cvg_context *ctx, int n, GridDesc g, unsigned R, unsigned max_cand, double rad2, int num_maps, bool spec)
+{
+    hipStream_t st = ctx->stream;
+    const size_t nd = static_cast<size_t>(n) * sizeof(double), ni = static_cast<size_t>(n) * sizeof(int);
+    CVG_TRY(ensure(ctx, ctx->d_tmp_ids, ni));
+    CVG_TRY(ensure(ctx, ctx->d_sorted, ni));
+    CVG_TRY(ensure(ctx, ctx->d_sxy, 2 * nd));
+    CVG_TRY(ensure(ctx, ctx->d_scell, ni));
+    const int *rank = ctx->use_cellrank ? as<int>(ctx->d_cellrank) : nullptr;
+    unsigned *scal = as<unsigned>(ctx->d_scal);
+    ctx->gx0 = g.x0;
+    ctx->gy0 = g.y0;
+    ctx->gw = g.w;
+    ctx->gh = g.h;
+    const long long cells = static_cast<long long>(g.w) * g.h;
+    CVG_TRY(ensure(ctx, ctx->d_grid, static_cast<size_t>(2 * cells + 1) * sizeof(unsigned)));
+    unsigned *cnt = as<unsigned>(ctx->d_grid);
+    unsigned *start = cnt + cells;
+    // k_cell_scatter leaves every count it decrements at zero, so the
+    // counts of a grid no larger than the last one in the same context
+    // buffer start zeroed (not tracked for the no-alloc scratch, whose
+    // regions other buffers reuse)
+    const bool zeroed = !ctx->arena && ctx->d_grid.ptr == ctx->grid_zero_ptr && cells <= ctx->grid_zero_cells;
+    ctx->grid_zero_ptr = nullptr;
+    if (!zeroed) CVG_HIP(ctx, hipMemsetAsync(cnt, 0, static_cast<size_t>(cells) * sizeof(unsigned), st));
+    hipLaunchKernelGGL(k_cell_count, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
+                       as<int>(ctx->d_cy), g, cnt, static_cast<int>(R), spec ? scal + S_MISS : nullptr);
+    CVG_TRY(exclusive_scan(ctx, cnt, cells, start, ctx->d_tiles));
+    hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
+                       as<int>(ctx->d_cy), g, start, cnt, as<int>(ctx->d_tmp_ids));
+    if (!ctx->arena) {
+        ctx->grid_zero_ptr = ctx->d_grid.ptr;
+        ctx->grid_zero_cells = cells;
+    }
+    hipLaunchKernelGGL(k_cell_order, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
+                       as<int>(ctx->d_cy), g, start, as<int>(ctx->d_tmp_ids), rank, as<double>(ctx->d_x),
+                       as<double>(ctx->d_y), as<int>(ctx->d_sorted), as<int>(ctx->d_scell),
+                       as<double2>(ctx->d_sxy));
+    CVG_HIP(ctx, hipGetLastError());
+    DenseCells cells_d{start, g.x0, g.y0, g.w};
+    return graph_scans(ctx, cells_d, n, R, max_cand, rad2, num_maps, spec);
 }
 
 int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double rad, unsigned R, unsigned max_cand,
@@ -2682,11 +2789,31 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     unsigned *scal = as<unsigned>(ctx->d_scal);
     const int prep_blocks = blocks_for(n);
     CVG_TRY(ensure(ctx, ctx->d_part, static_cast<size_t>(prep_blocks) * 5 * sizeof(int)));
-    hipLaunchKernelGGL(k_prep_edges, dim3(prep_blocks), dim3(kBlock), 0, st, n, sz, d_edges, as<double>(ctx->d_x),
-                       as<double>(ctx->d_y), as<double>(ctx->d_th), as<int>(ctx->d_cx), as<int>(ctx->d_cy),
-                       as<int>(ctx->d_part));
-    hipLaunchKernelGGL(k_bbox_finish, dim3(1), dim3(1024), 0, st, prep_blocks, as<int>(ctx->d_part), scal);
-    CVG_HIP(ctx, hipGetLastError());
+    auto prep = [&]() -> int {
+        hipLaunchKernelGGL(k_prep_edges, dim3(prep_blocks), dim3(kBlock), 0, st, n, sz, d_edges,
+                           as<double>(ctx->d_x), as<double>(ctx->d_y), as<double>(ctx->d_th), as<int>(ctx->d_cx),
+                           as<int>(ctx->d_cy), as<int>(ctx->d_part));
+        hipLaunchKernelGGL(k_bbox_finish, dim3(1), dim3(1024), 0, st, prep_blocks, as<int>(ctx->d_part), scal);
+        CVG_HIP(ctx, hipGetLastError());
+        return CVG_OK;
+    };
+    CVG_TRY(prep());
+    // preprocess.hpp:221: rad * rad in T
+    const double rad2 = ctx->f32 ? static_cast<double>(static_cast<float>(rad) * static_cast<float>(rad)) : rad * rad;
+    // A single map first runs on the last single map's grid and buffer sizes
+    // with no host read until the fill is done (one round trip instead of
+    // three); when they do not hold, it is built again on its own grid.
+    if (num_maps == 1 && !ctx->force_sparse && !ctx->arena && ctx->spec_valid && ctx->spec_R == R) {
+        CVG_TRY(ensure(ctx, ctx->d_eoff, 2 * sizeof(long long)));
+        const int rc = graph_dense(ctx, n, ctx->spec_g, R, max_cand, rad2, 1, true);
+        if (rc == CVG_OK) {
+            ctx->graph_ready = true;
+            return CVG_OK;
+        }
+        if (rc != kSpecMiss) return rc;
+        ctx->spec_valid = false;
+        CVG_TRY(prep());  // the guess moved some cells: the map's own cells again
+    }
     CVG_TRY(read_scalars(ctx));
     const unsigned *mb = ctx->mailbox;
     if (mb[S_BAD])
@@ -2768,44 +2895,23 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     CVG_TRY(ensure(ctx, ctx->d_sxy, 2 * nd));
     CVG_TRY(ensure(ctx, ctx->d_scell, ni));
     const int *rank = ctx->use_cellrank ? as<int>(ctx->d_cellrank) : nullptr;
-    // preprocess.hpp:221: rad * rad in T
-    const double rad2 = ctx->f32 ? static_cast<double>(static_cast<float>(rad) * static_cast<float>(rad)) : rad * rad;
     if (w <= kMaxGridCells && h <= kMaxGridCells && w * h <= kMaxGridCells && !ctx->force_sparse) {
-        // dense grid: counting sort into cells
         GridDesc g;
         g.x0 = x0;
         g.y0 = y0;
         g.w = static_cast<int>(w);
         g.h = static_cast<int>(h);
-        ctx->gw = g.w;
-        ctx->gh = g.h;
-        const long long cells = w * h;
-        CVG_TRY(ensure(ctx, ctx->d_grid, static_cast<size_t>(2 * cells + 1) * sizeof(unsigned)));
-        unsigned *cnt = as<unsigned>(ctx->d_grid);
-        unsigned *start = cnt + cells;
-        // k_cell_scatter leaves every count it decrements at zero, so the
-        // counts of a grid no larger than the last one in the same context
-        // buffer start zeroed (not tracked for the no-alloc scratch, whose
-        // regions other buffers reuse)
-        const bool zeroed = !ctx->arena && ctx->d_grid.ptr == ctx->grid_zero_ptr && cells <= ctx->grid_zero_cells;
-        ctx->grid_zero_ptr = nullptr;
-        if (!zeroed) CVG_HIP(ctx, hipMemsetAsync(cnt, 0, static_cast<size_t>(cells) * sizeof(unsigned), st));
-        hipLaunchKernelGGL(k_cell_count, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
-                           as<int>(ctx->d_cy), g, cnt);
-        CVG_TRY(exclusive_scan(ctx, cnt, cells, start, ctx->d_tiles));
-        hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
-                           as<int>(ctx->d_cy), g, start, cnt, as<int>(ctx->d_tmp_ids));
-        if (!ctx->arena) {
-            ctx->grid_zero_ptr = ctx->d_grid.ptr;
-            ctx->grid_zero_cells = cells;
+        CVG_TRY(graph_dense(ctx, n, g, R, max_cand, rad2, num_maps, false));
+        if (num_maps == 1 && !ctx->arena) {
+            // the next single map first tries this grid (with a margin) and
+            // these buffers, without reading its sizes first
+            const int pad = 16;
+            GridDesc sg{g.x0 - pad, g.y0 - pad, g.w + 2 * pad, g.h + 2 * pad};
+            ctx->spec_valid = static_cast<long long>(sg.w) * sg.h <= kMaxGridCells;
+            ctx->spec_g = sg;
+            ctx->spec_R = R;
+            ctx->spec_fill_cap = static_cast<int>(std::max(ctx->max_deg, 1u));
         }
-        hipLaunchKernelGGL(k_cell_order, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
-                           as<int>(ctx->d_cy), g, start, as<int>(ctx->d_tmp_ids), rank, as<double>(ctx->d_x),
-                           as<double>(ctx->d_y), as<int>(ctx->d_sorted), as<int>(ctx->d_scell),
-                           as<double2>(ctx->d_sxy));
-        CVG_HIP(ctx, hipGetLastError());
-        DenseCells cells_d{start, x0, y0, g.w};
-        CVG_TRY(graph_scans(ctx, cells_d, n, R, max_cand, rad2, num_maps));
     } else {
         // sorted linear cell keys: stable radix sorts of (key, id), by the
         // within-cell rank first when the reference's order is replayed, so
@@ -2849,7 +2955,7 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
                            as<double2>(ctx->d_sxy));
         CVG_HIP(ctx, hipGetLastError());
         SparseCells cells_s{key, n, x0, y0};
-        CVG_TRY(graph_scans(ctx, cells_s, n, R, max_cand, rad2, num_maps));
+        CVG_TRY(graph_scans(ctx, cells_s, n, R, max_cand, rad2, num_maps, false));
     }
     ctx->graph_ready = true;
     return CVG_OK;

@@ -417,3 +417,31 @@ def test_run_to_host_too_few_rows(ctx):
         ctx.run_to_host(e, p, hc, hi)
     assert np.array_equal(hc, wc[:k])
     assert np.array_equal(hi.view(np.uint64), wi[:k].view(np.uint64))
+
+
+def test_consecutive_maps_of_changing_shape(oracle):
+    """One context, maps one after another whose grids, pair counts and
+    longest lists grow, shift and shrink: the graph first tries the last map's
+    grid and buffer sizes and must fall back whenever they do not hold."""
+    from curvelet_construction_on_gpu_amd import Context
+    base = synthetic("C1", 1001)
+    shifted = base.copy()
+    shifted[:, 0] += 500.0  # outside the last grid
+    dense = np.concatenate([base, base[:, :] + np.array([0.3, 0.2, 0.0, 0.0])])  # longer lists, more pairs
+    small = base[:300].copy()
+    negative = base.copy()
+    negative[:, :2] -= 1000.0
+    p = CurveletParams(group_max_sz=6)
+    with Context(0) as c:
+        for name, e in (("base", base), ("base again", base), ("shifted", shifted), ("dense", dense),
+                        ("small", small), ("base", base), ("negative", negative), ("dense", dense)):
+            print(f"changing shapes: {name}", flush=True)
+            stats = c.run(e, p)
+            g = c.graph()
+            want = oracle.run(e, p)
+            assert np.array_equal(g.neighbor_offsets, want["csr"]["offsets"]), name
+            assert np.array_equal(g.neighbor_ids, want["csr"]["ids"]), name
+            chains, info = c.curvelets()
+            assert stats["valid_pairs"] == want["valid_pairs"], name
+            assert np.array_equal(chains, want["chains"]), name
+            np.testing.assert_allclose(info, want["info"], rtol=1e-9, atol=1e-12)
