@@ -42,6 +42,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <initializer_list>
+#include <map>
+#include <tuple>
 #include <mutex>
 #include <new>
 #include <string>
@@ -150,6 +152,9 @@ struct cvg_context {
     GridDesc spec_g{};
     unsigned spec_R = 0;
     int spec_fill_cap = 1;
+    // launch shapes already set up (prepare_launch)
+    std::map<const void *, unsigned> shmem_set;
+    std::map<std::tuple<const void *, int, unsigned>, int> occupancy;
     // the bytes last copied to d_tables (while d_tables is tables_ptr)
     const void *tables_ptr = nullptr;
     std::vector<char> tables_blob;
@@ -2596,6 +2601,30 @@ __global__ void k_math_eval(int op, int n, const double *__restrict__ x, const d
 // Host orchestration
 // ---------------------------------------------------------------------------
 
+// A kernel's dynamic LDS limit raised to shmem (once per kernel and size) and,
+// with occ, its resident workgroups per CU at that block and LDS size (the
+// runtime's occupancy query, answered once per context and shape): both are
+// host calls that otherwise cost tens of microseconds per run.
+int prepare_launch(cvg_context *ctx, const void *fn, int block, unsigned shmem, int *occ)
+{
+    unsigned &lim = ctx->shmem_set[fn];
+    if (lim < shmem) {
+        CVG_HIP(ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
+        lim = shmem;
+    }
+    if (occ) {
+        const auto key = std::make_tuple(fn, block, shmem);
+        const auto it = ctx->occupancy.find(key);
+        if (it != ctx->occupancy.end()) {
+            *occ = it->second;
+        } else {
+            CVG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, fn, block, shmem));
+            ctx->occupancy[key] = *occ;
+        }
+    }
+    return CVG_OK;
+}
+
 int read_scalars(cvg_context *ctx)
 {
     CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox, ctx->d_scal.ptr, kScalarWords * sizeof(unsigned), hipMemcpyDeviceToHost,
@@ -2686,8 +2715,7 @@ int graph_scans(cvg_context *ctx, const Cells &cells, int n, unsigned R, unsigne
                                   : k_fill_csr<64, Cells>;
     const int anchors_per_block = (64 / G) * fill_waves;
     const unsigned fill_shmem = static_cast<unsigned>(per_seg * anchors_per_block);
-    CVG_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(fill), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     static_cast<int>(fill_shmem)));
+    CVG_TRY(prepare_launch(ctx, reinterpret_cast<const void *>(fill), 64 * fill_waves, fill_shmem, nullptr));
     hipLaunchKernelGGL(fill, dim3((n + anchors_per_block - 1) / anchors_per_block), dim3(64 * fill_waves), fill_shmem,
                        st, n, as<double>(ctx->d_x), as<double>(ctx->d_y), as<int>(ctx->d_cx), as<int>(ctx->d_cy),
                        cells, static_cast<int>(R), rad2, as<int>(ctx->d_sorted), as<int>(ctx->d_scell),
@@ -3511,11 +3539,8 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
         l.shmem = static_cast<unsigned>(l.slice * l.apb);
         l.fn = fused_kernel(d.cpl, l.nch, l.gb, l.coop);
         if (!l.fn) return fail(ctx, CVG_ERR_STATE, "internal: no fused kernel for this grid");
-        CVG_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(l.fn),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(l.shmem)));
         int occ = 0;
-        CVG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(l.fn),
-                                                                  64 * l.wpb, l.shmem));
+        CVG_TRY(prepare_launch(ctx, reinterpret_cast<const void *>(l.fn), 64 * l.wpb, l.shmem, &occ));
         // the bucket sizes stay on the device: every launch gets the resident
         // grid (streams beyond the bucket's length exit at once), at most one
         // stream per anchor of the map
