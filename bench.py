@@ -229,9 +229,12 @@ def bench_serve(args) -> int:
     to their steady-state size: pinning new host memory costs more than
     serving); (2) a Poisson stream at
     200 maps/s (2,000 requests after 100 warm-up ones): p50/p99 latency from
-    submit to results in host memory, overall and for 1080p-class maps."""
+    submit to results in host memory, overall and for 1080p-class maps; every
+    20th served result of the stream (5%) is checked afterwards against the
+    oracle with the device's correctly rounded math (bit for bit, by hash)."""
     import threading
     import numpy as np
+    import xxhash
     from curvelet_construction_on_gpu_amd import CurveletParams, Server
     pool, areas = serve_pool()
     params = CurveletParams()
@@ -279,11 +282,14 @@ def bench_serve(args) -> int:
 
         th = threading.Thread(target=client)
         th.start()
-        lat, lat_1080, batch_sizes = [], [], []
+        lat, lat_1080, batch_sizes, sample = [], [], [], []
         for k in range(nwarm + nreq):
             while ids[k] is None:
                 time.sleep(0.0002)
             ch, inf, r = srv.view(ids[k])
+            if k >= nwarm and (k - nwarm) % 20 == 0:  # the 5% sample checked against the oracle afterwards
+                sample.append((int(picks[k]), xxhash.xxh3_128_hexdigest(np.ascontiguousarray(ch)),
+                               xxhash.xxh3_128_hexdigest(np.ascontiguousarray(inf))))
             srv.release(ids[k])
             if k >= nwarm:
                 lat.append(r["latency_ms"])
@@ -295,6 +301,21 @@ def bench_serve(args) -> int:
         th.join()
         t_end = time.perf_counter()
     stream_rate = nreq / (t_end - t_first)
+    # the sample against the restatement with the device's correctly rounded
+    # math: every chain and info bit (one oracle run per distinct map)
+    oracle_check = {"sampled": len(sample), "maps": 0, "mismatches": 0}
+    try:
+        from oracle import Oracle
+        orc, want = Oracle(cr=True), {}
+        for k, hc, hi in sample:
+            if k not in want:
+                w = orc.run(pool[k], params, nthreads=0, valid_pairs=False)
+                want[k] = (xxhash.xxh3_128_hexdigest(np.ascontiguousarray(w["chains"].astype(np.uint32))),
+                           xxhash.xxh3_128_hexdigest(np.ascontiguousarray(w["info"])))
+            oracle_check["mismatches"] += int((hc, hi) != want[k])
+        oracle_check["maps"] = len(want)
+    except Exception as exc:  # reported, never required
+        oracle_check["error"] = repr(exc)
     line = {"metric": "maps/s (served, continuous batching, saturation)", "value": out["saturation_maps_per_s"],
             "unit": "maps/s", "n_gpus": 1, "steps": nreq, "warmup": nwarm, "ms_per_step": 1e3 / out["saturation_maps_per_s"],
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
@@ -309,7 +330,8 @@ def bench_serve(args) -> int:
                         "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99)),
                         "latency_ms_p50_1080p": float(np.percentile(lat_1080, 50)) if lat_1080 else None,
                         "latency_ms_p99_1080p": float(np.percentile(lat_1080, 99)) if lat_1080 else None,
-                        "requests_1080p": len(lat_1080), "mean_batch_maps": float(np.mean(batch_sizes))}}
+                        "requests_1080p": len(lat_1080), "mean_batch_maps": float(np.mean(batch_sizes)),
+                        "oracle_sample": oracle_check}}
     print(json.dumps(line))
     return 0
 

@@ -605,8 +605,11 @@ __global__ __launch_bounds__(kBlock) void k_prep_edges(int n, int sz, const doub
 
 // One workgroup: the cell box and bad flag over k_prep_edges' nparts partials
 // into the scalars (a tree, not same-address atomics, which serialize).
+// With spec_on, the map is about to be built on the guessed grid `spec`:
+// S_MISS is raised when some edge's window (cell +- R) leaves it, and the
+// speculative kernels then return at once.
 __global__ __launch_bounds__(1024) void k_bbox_finish(int nparts, const int *__restrict__ part,
-                                                      unsigned *__restrict__ scal)
+                                                      unsigned *__restrict__ scal, int spec_on, GridDesc spec, int R)
 {
     __shared__ int l[5][1024 / 64];
     int v[5] = {INT_MAX, INT_MIN, INT_MAX, INT_MIN, 0};
@@ -644,6 +647,13 @@ __global__ __launch_bounds__(1024) void k_bbox_finish(int nparts, const int *__r
         scal[S_MINY] = static_cast<unsigned>(v[2]);
         scal[S_MAXY] = static_cast<unsigned>(v[3]);
         scal[S_BAD] = v[4] ? 1u : 0u;
+        if (spec_on) {
+            const bool fits = static_cast<long long>(v[0]) - R >= spec.x0 &&
+                              static_cast<long long>(v[1]) + R <= static_cast<long long>(spec.x0) + spec.w - 1 &&
+                              static_cast<long long>(v[2]) - R >= spec.y0 &&
+                              static_cast<long long>(v[3]) + R <= static_cast<long long>(spec.y0) + spec.h - 1;
+            if (!fits || v[4]) scal[S_MISS] = 1u;
+        }
     }
 }
 
@@ -734,6 +744,9 @@ __global__ void k_map_rows(int num_maps, const long long *__restrict__ eoff, con
 __global__ __launch_bounds__(kBlock) void k_cell_count(int n, int *__restrict__ cx, int *__restrict__ cy, GridDesc g,
                                                        unsigned *__restrict__ cnt, int R, unsigned *__restrict__ miss)
 {
+    // the bounding box already showed the guessed grid does not hold: count
+    // nothing (k_cell_scatter then moves nothing, and the counts stay zero)
+    if (miss && *miss) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool out = false;
     if (i < n) {
@@ -756,8 +769,10 @@ __global__ __launch_bounds__(kBlock) void k_cell_count(int n, int *__restrict__ 
 // cnt holds the per-cell counts on entry and is decremented to zero.
 __global__ __launch_bounds__(kBlock) void k_cell_scatter(int n, const int *__restrict__ cx, const int *__restrict__ cy,
                                                          GridDesc g, const unsigned *__restrict__ start,
-                                                         unsigned *__restrict__ cnt, int *__restrict__ tmp)
+                                                         unsigned *__restrict__ cnt, int *__restrict__ tmp,
+                                                         const unsigned *__restrict__ abort)
 {
+    if (abort && *abort) return;  // k_cell_count counted nothing either
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         const int c = cell_index(g, cx[i], cy[i]);
@@ -779,8 +794,10 @@ __global__ __launch_bounds__(kBlock) void k_cell_order(int n, const int *__restr
                                                        const int *__restrict__ tmp, const int *__restrict__ rank,
                                                        const double *__restrict__ x, const double *__restrict__ y,
                                                        int *__restrict__ sorted, int *__restrict__ scell,
-                                                       double2 *__restrict__ sxy)
+                                                       double2 *__restrict__ sxy, const unsigned *__restrict__ abort)
 {
+    // a guessed grid that does not hold: the build is repeated, skip the work
+    if (abort && *abort) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int c = cell_index(g, cx[i], cy[i]);
@@ -945,8 +962,10 @@ __global__ __launch_bounds__(kBlock) void k_discover_count(int n, const double *
                                                            const int *__restrict__ cy, Cells cells, int R, double rad2,
                                                            unsigned max_cand, const int *__restrict__ sorted,
                                                            const double2 *__restrict__ sxy,
-                                                           unsigned *__restrict__ deg, bool f32)
+                                                           unsigned *__restrict__ deg, bool f32,
+                                                           const unsigned *__restrict__ abort)
 {
+    if (abort && *abort) return;  // a guessed grid that does not hold: the build is repeated
     const int lane = threadIdx.x & 63;
     const int sl = lane & (G - 1);
     const int a = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / G);
@@ -1009,6 +1028,8 @@ __global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const doubl
                                                               unsigned *__restrict__ scal, bool f32,
                                                               unsigned long long pair_cap)
 {
+    // a guess that did not hold (grid or sizes): the build is repeated
+    if (scal[S_MISS]) return;
     // per segment: cap candidates (d2, id, cell) in LDS for the ranking
     extern __shared__ double fill_lds[];
     const int lane = threadIdx.x & 63;
@@ -2652,7 +2673,7 @@ int graph_scans(cvg_context *ctx, const Cells &cells, int n, unsigned R, unsigne
     CVG_TRY(ensure(ctx, ctx->d_off, static_cast<size_t>(n + 1) * sizeof(unsigned)));
     {
         using CountFn = void (*)(int, const double *, const double *, const int *, const int *, Cells, int, double,
-                                 unsigned, const int *, const double2 *, unsigned *, bool);
+                                 unsigned, const int *, const double2 *, unsigned *, bool, const unsigned *);
         const int G = ctx->fill_lanes;
         const CountFn fn = G == 8    ? k_discover_count<8, Cells>
                            : G == 32 ? k_discover_count<32, Cells>
@@ -2661,7 +2682,7 @@ int graph_scans(cvg_context *ctx, const Cells &cells, int n, unsigned R, unsigne
         hipLaunchKernelGGL(fn, dim3(blocks_for(static_cast<long long>(n) * G)), dim3(kBlock), 0, st, n,
                            as<double>(ctx->d_x), as<double>(ctx->d_y), as<int>(ctx->d_cx), as<int>(ctx->d_cy), cells,
                            static_cast<int>(R), rad2, max_cand, as<int>(ctx->d_sorted), as<double2>(ctx->d_sxy),
-                           as<unsigned>(ctx->d_deg), ctx->f32);
+                           as<unsigned>(ctx->d_deg), ctx->f32, spec ? static_cast<const unsigned *>(scal + S_MISS) : nullptr);
     }
     CVG_HIP(ctx, hipGetLastError());
     CVG_TRY(exclusive_scan(ctx, as<unsigned>(ctx->d_deg), n, as<unsigned>(ctx->d_off), ctx->d_tiles,
@@ -2768,7 +2789,8 @@ int graph_dense(cvg_context *ctx, int n, GridDesc g, unsigned R, unsigned max_ca
                        as<int>(ctx->d_cy), g, cnt, static_cast<int>(R), spec ? scal + S_MISS : nullptr);
     CVG_TRY(exclusive_scan(ctx, cnt, cells, start, ctx->d_tiles));
     hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
-                       as<int>(ctx->d_cy), g, start, cnt, as<int>(ctx->d_tmp_ids));
+                       as<int>(ctx->d_cy), g, start, cnt, as<int>(ctx->d_tmp_ids),
+                       spec ? static_cast<const unsigned *>(scal + S_MISS) : nullptr);
     if (!ctx->arena) {
         ctx->grid_zero_ptr = ctx->d_grid.ptr;
         ctx->grid_zero_cells = cells;
@@ -2776,7 +2798,7 @@ int graph_dense(cvg_context *ctx, int n, GridDesc g, unsigned R, unsigned max_ca
     hipLaunchKernelGGL(k_cell_order, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
                        as<int>(ctx->d_cy), g, start, as<int>(ctx->d_tmp_ids), rank, as<double>(ctx->d_x),
                        as<double>(ctx->d_y), as<int>(ctx->d_sorted), as<int>(ctx->d_scell),
-                       as<double2>(ctx->d_sxy));
+                       as<double2>(ctx->d_sxy), spec ? static_cast<const unsigned *>(scal + S_MISS) : nullptr);
     CVG_HIP(ctx, hipGetLastError());
     DenseCells cells_d{start, g.x0, g.y0, g.w};
     return graph_scans(ctx, cells_d, n, R, max_cand, rad2, num_maps, spec);
@@ -2817,21 +2839,23 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     unsigned *scal = as<unsigned>(ctx->d_scal);
     const int prep_blocks = blocks_for(n);
     CVG_TRY(ensure(ctx, ctx->d_part, static_cast<size_t>(prep_blocks) * 5 * sizeof(int)));
-    auto prep = [&]() -> int {
-        hipLaunchKernelGGL(k_prep_edges, dim3(prep_blocks), dim3(kBlock), 0, st, n, sz, d_edges,
-                           as<double>(ctx->d_x), as<double>(ctx->d_y), as<double>(ctx->d_th), as<int>(ctx->d_cx),
-                           as<int>(ctx->d_cy), as<int>(ctx->d_part));
-        hipLaunchKernelGGL(k_bbox_finish, dim3(1), dim3(1024), 0, st, prep_blocks, as<int>(ctx->d_part), scal);
-        CVG_HIP(ctx, hipGetLastError());
-        return CVG_OK;
-    };
-    CVG_TRY(prep());
-    // preprocess.hpp:221: rad * rad in T
-    const double rad2 = ctx->f32 ? static_cast<double>(static_cast<float>(rad) * static_cast<float>(rad)) : rad * rad;
     // A single map first runs on the last single map's grid and buffer sizes
     // with no host read until the fill is done (one round trip instead of
     // three); when they do not hold, it is built again on its own grid.
-    if (num_maps == 1 && !ctx->force_sparse && !ctx->arena && ctx->spec_valid && ctx->spec_R == R) {
+    const bool spec = num_maps == 1 && !ctx->force_sparse && !ctx->arena && ctx->spec_valid && ctx->spec_R == R;
+    auto prep = [&](bool spec_on) -> int {
+        hipLaunchKernelGGL(k_prep_edges, dim3(prep_blocks), dim3(kBlock), 0, st, n, sz, d_edges,
+                           as<double>(ctx->d_x), as<double>(ctx->d_y), as<double>(ctx->d_th), as<int>(ctx->d_cx),
+                           as<int>(ctx->d_cy), as<int>(ctx->d_part));
+        hipLaunchKernelGGL(k_bbox_finish, dim3(1), dim3(1024), 0, st, prep_blocks, as<int>(ctx->d_part), scal,
+                           spec_on ? 1 : 0, ctx->spec_g, static_cast<int>(R));
+        CVG_HIP(ctx, hipGetLastError());
+        return CVG_OK;
+    };
+    CVG_TRY(prep(spec));
+    // preprocess.hpp:221: rad * rad in T
+    const double rad2 = ctx->f32 ? static_cast<double>(static_cast<float>(rad) * static_cast<float>(rad)) : rad * rad;
+    if (spec) {
         CVG_TRY(ensure(ctx, ctx->d_eoff, 2 * sizeof(long long)));
         const int rc = graph_dense(ctx, n, ctx->spec_g, R, max_cand, rad2, 1, true);
         if (rc == CVG_OK) {
@@ -2840,7 +2864,7 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
         }
         if (rc != kSpecMiss) return rc;
         ctx->spec_valid = false;
-        CVG_TRY(prep());  // the guess moved some cells: the map's own cells again
+        CVG_TRY(prep(false));  // the guess may have moved some cells: the map's own cells again
     }
     CVG_TRY(read_scalars(ctx));
     const unsigned *mb = ctx->mailbox;
