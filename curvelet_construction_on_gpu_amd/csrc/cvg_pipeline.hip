@@ -149,6 +149,7 @@ struct cvg_context {
     // the grid, window radius and longest list a single map tries first (the
     // last single map's; spec_valid once there is one)
     bool spec_valid = false;
+    bool spec_off = false;  // CVG_SPECULATE=0 at creation: every graph reads its sizes first
     GridDesc spec_g{};
     unsigned spec_R = 0;
     int spec_fill_cap = 1;
@@ -2842,7 +2843,8 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     // A single map first runs on the last single map's grid and buffer sizes
     // with no host read until the fill is done (one round trip instead of
     // three); when they do not hold, it is built again on its own grid.
-    const bool spec = num_maps == 1 && !ctx->force_sparse && !ctx->arena && ctx->spec_valid && ctx->spec_R == R;
+    const bool spec = num_maps == 1 && !ctx->force_sparse && !ctx->arena && !ctx->spec_off && ctx->spec_valid &&
+                      ctx->spec_R == R;
     auto prep = [&](bool spec_on) -> int {
         hipLaunchKernelGGL(k_prep_edges, dim3(prep_blocks), dim3(kBlock), 0, st, n, sz, d_edges,
                            as<double>(ctx->d_x), as<double>(ctx->d_y), as<double>(ctx->d_th), as<int>(ctx->d_cx),
@@ -3947,6 +3949,7 @@ int cvg_create(int device, cvg_context **out_ctx)
         }
         if (const char *v = std::getenv("CVG_SPARSE_CELLS")) ctx->force_sparse = std::atoi(v) != 0;
         if (const char *v = std::getenv("CVG_GENERIC_KERNEL")) ctx->force_generic = std::atoi(v) != 0;
+        if (const char *v = std::getenv("CVG_SPECULATE")) ctx->spec_off = std::atoi(v) == 0;
     } while (false);
     if (rc != CVG_OK) {
         cvg_destroy(ctx);
