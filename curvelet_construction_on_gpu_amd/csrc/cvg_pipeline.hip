@@ -150,6 +150,10 @@ struct cvg_context {
     // last single map's; spec_valid once there is one)
     bool spec_valid = false;
     bool spec_off = false;  // CVG_SPECULATE=0 at creation: every graph reads its sizes first
+    // a run's parameters while its graph is built (fused double maps): the
+    // graph stage queues k_edge_trig behind its last host read
+    const cvg_params *early_params = nullptr;
+    bool edge_trig_queued = false;  // k_edge_trig of the current run is queued
     GridDesc spec_g{};
     unsigned spec_R = 0;
     int spec_fill_cap = 1;
@@ -187,7 +191,7 @@ struct cvg_context {
     unsigned long long intersections = 0;
     unsigned num_curvelets = 0;
     unsigned width = 0;
-    DevBuf d_trig, d_tdd, d_cls, d_prec, d_psin, d_lists, d_tables, d_st_ids, d_st_meta, d_st_k, d_st_rank, d_acc, d_rowoff, d_gbundle,
+    DevBuf d_trig, d_tdd, d_cls, d_prec, d_psin, d_lists, d_tables, d_st_ids, d_st_meta, d_st_k, d_st_rank, d_acc, d_rowoff, d_rowslot, d_gbundle,
         d_chains, d_info;
 
     // diagnostics
@@ -212,7 +216,7 @@ struct cvg_context {
                           &d_grid,     &d_tmp_ids, &d_sorted, &d_sxy,    &d_scell,  &d_part,        &d_deg,
                           &d_off,      &d_tiles,  &d_cand,   &d_ids,    &d_d2,     &d_pair_anchor, &d_trig,
                           &d_tdd,      &d_cls,    &d_prec,   &d_psin,   &d_lists,  &d_tables,      &d_st_ids,
-                          &d_st_meta,  &d_st_k,   &d_st_rank, &d_acc,   &d_rowoff, &d_gbundle,     &d_chains,
+                          &d_st_meta,  &d_st_k,   &d_st_rank, &d_acc,   &d_rowoff, &d_rowslot, &d_gbundle, &d_chains,
                           &d_info,     &d_dbg,    &d_map,    &d_eoff,   &d_mapbox, &d_maprow,       &d_cellrank,
                           &d_skey,     &d_sorttmp};
         for (DevBuf *b : bufs) f(*b);
@@ -1197,20 +1201,28 @@ __global__ __launch_bounds__(kBlock) void k_tie_sort(const int *__restrict__ tie
 // One thread per (edge i, angle k), k fastest: k < Kt the grid angle j = k,
 // k = Kt theta + dt, Kt + 1 theta - dt, Kt + 2 theta itself.
 __global__ __launch_bounds__(kBlock) void k_edge_trig(int n, const double *__restrict__ th, double dt, int kt,
-                                                      const double *__restrict__ off_t, double *__restrict__ trig,
-                                                      double *__restrict__ tdd, unsigned *__restrict__ zero,
-                                                      int zero_words)
+                                                      double st, double *__restrict__ trig, double *__restrict__ tdd,
+                                                      unsigned *__restrict__ scal)
 {
-    // the curvelet stage's counters start at zero (block 0 clears them)
-    if (blockIdx.x == 0)
-        for (int t = static_cast<int>(threadIdx.x); t < zero_words; t += static_cast<int>(blockDim.x)) zero[t] = 0u;
+    // the curvelet stage's counters start at zero (block 0 clears them; the
+    // graph stage's words are left alone, so this may run before its last
+    // host read)
+    if (blockIdx.x == 0) {
+        const int t = static_cast<int>(threadIdx.x);
+        if (t == 0) scal[S_VALID] = 0u;
+        if (t == 1) scal[S_COUNT] = 0u;
+        if (t == 2) scal[S_COUNT + 1] = 0u;
+        for (int w = kBucketBase + t; w < kScalarBufWords; w += static_cast<int>(blockDim.x)) scal[w] = 0u;
+    }
     const unsigned na = static_cast<unsigned>(kt) + 3u;
     const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= static_cast<unsigned>(n) * na) return;
     const int i = static_cast<int>(g / na);
     const int k = static_cast<int>(g - static_cast<unsigned>(i) * na);
     const double t = th[i];
-    const double ang = k < kt ? t + off_t[k]      // T2t = T2 + delta_t2, cpu_curvelet.hpp:702
+    // delta_t2 of grid column k (cpu_curvelet.hpp:698), the host table's expression
+    const double off_t = k < kt ? st * (static_cast<double>(k) - (static_cast<double>(kt) - 1.0) / 2.0) : 0.0;
+    const double ang = k < kt ? t + off_t         // T2t = T2 + delta_t2, cpu_curvelet.hpp:702
                        : k == kt ? t + dt         // T0_p_dt, :655
                        : k == kt + 1 ? t - dt     // T0_m_dt, :656
                                      : t;
@@ -2063,12 +2075,15 @@ struct RecordArgs {
 // (compact_curvelet_output, :282-303).  One thread per staging slot q (one per
 // seed of the anchor, the rest unused): an accepted row carries its rank among
 // the anchor's accepted rows.  Chain ids are written 1-based and relative to
-// the anchor's map (batches concatenate maps).
-__device__ inline void record_slot(const RecordArgs &R, int q, int rk)
+// the anchor's map (batches concatenate maps).  F32: the float instantiation
+// (R.f32); each kernel instantiation holds only its own arithmetic.
+constexpr int kRecordChunk = 8;  // chain ids (and their coordinates) loaded together
+
+template <bool F32>
+__device__ inline void record_row(const RecordArgs &R, unsigned o, int q)
 {
+    using T = typename std::conditional<F32, float, double>::type;
     const int a = R.pair_anchor[q];
-    const unsigned o = R.row_off[a] + static_cast<unsigned>(rk);
-    if (o >= R.max_rows) return;  // caller buffers too small: reported by the host
     const int id0 = R.map_of ? static_cast<int>(R.map_eoff[R.map_of[a]]) : 0;
     const double ax = R.x[a], ay = R.y[a], at = R.th[a];
     const int W = R.G + 1;
@@ -2079,9 +2094,42 @@ __device__ inline void record_slot(const RecordArgs &R, int q, int rk)
     const int *chain = R.st_ids + static_cast<long long>(q) * R.st_stride;
     unsigned *row = R.chains + static_cast<long long>(o) * W;
     row[0] = static_cast<unsigned>(a - id0) + 1;
-    for (int l = 0; l < R.G; ++l) row[l + 1] = l < len ? static_cast<unsigned>(chain[l] - id0) + 1 : 0u;
+    // The chain row, and the chain's length: the distances between
+    // consecutive members in chain order (:231-240, sqrt in T).  Ids and
+    // coordinates come kRecordChunk at a time, so their loads overlap.
+    T length = 0;
+    T x1 = 0, y1 = 0;
+    for (int l0 = 0; l0 < R.G; l0 += kRecordChunk) {
+        int id[kRecordChunk];
+#pragma unroll
+        for (int u = 0; u < kRecordChunk; ++u) id[u] = l0 + u < len ? chain[l0 + u] : 0;
+#pragma unroll
+        for (int u = 0; u < kRecordChunk; ++u)
+            if (l0 + u < R.G) row[l0 + u + 1] = l0 + u < len ? static_cast<unsigned>(id[u] - id0) + 1 : 0u;
+        T xs[kRecordChunk], ys[kRecordChunk];
+#pragma unroll
+        for (int u = 0; u < kRecordChunk; ++u) {
+            xs[u] = l0 + u < len ? static_cast<T>(R.x[id[u]]) : T(0);
+            ys[u] = l0 + u < len ? static_cast<T>(R.y[id[u]]) : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kRecordChunk; ++u) {
+            const int l = l0 + u;
+            if (l >= len) continue;
+            if (l > 0) {
+                const T d2 = (x1 - xs[u]) * (x1 - xs[u]) + (y1 - ys[u]) * (y1 - ys[u]);
+                if constexpr (F32)
+                    length += cvg::sqrt_f(d2);
+                else
+                    length += sqrt(d2);
+            }
+            x1 = xs[u];
+            y1 = ys[u];
+        }
+    }
 
-    if (R.f32) {
+    double *inf = R.info + static_cast<long long>(o) * kInfoWidth;
+    if constexpr (F32) {
         // fill_curvelet_info<float> (:197-256): every value a float, std::sin,
         // std::cos and std::sqrt the float functions
         const float fx = static_cast<float>(ax), fy = static_cast<float>(ay), ft = static_cast<float>(at);
@@ -2092,20 +2140,12 @@ __device__ inline void record_slot(const RecordArgs &R, int q, int rk)
         const float kk = (k_max + k_min) / 2.0f;
         const float theta = cvg::wrap_two_pi_f(ft + ot);
         float s, cth;
-        cvg::cr_sincosf(theta, c_sincos_tab, &s, &cth);
+        cvg::cr_sincosf<false>(theta, c_sincos_tab, &s, &cth);  // theta in [0, 2 pi)
         const float px = fx - ox * s;
         const float py = fy + ox * cth;
-        float length = 0.0f;
-        for (int l = 0; l + 1 < len; ++l) {
-            const int e1 = chain[l], e2 = chain[l + 1];
-            const float x1 = static_cast<float>(R.x[e1]), y1 = static_cast<float>(R.y[e1]);
-            const float x2 = static_cast<float>(R.x[e2]), y2 = static_cast<float>(R.y[e2]);
-            length += cvg::sqrt_f((x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2));
-        }
         const float quality = (length > 0.0f && len > 0)
                                   ? 2.0f / (1.0f * R.nrad_f / length + 1.0f * length / static_cast<float>(len))
                                   : 0.0f;
-        double *inf = R.info + static_cast<long long>(o) * kInfoWidth;
         inf[0] = fwd ? 1.0 : 0.0;
         inf[1] = k_max;
         inf[2] = k_min;
@@ -2116,67 +2156,55 @@ __device__ inline void record_slot(const RecordArgs &R, int q, int rk)
         inf[7] = kk;
         inf[8] = length;
         inf[9] = quality;
-        return;
+    } else {
+        const double ox = R.off_x[c / R.kt];
+        const double ot = R.off_t[c % R.kt];
+        const double k_max = R.st_k[2 * static_cast<long long>(q)];
+        const double k_min = R.st_k[2 * static_cast<long long>(q) + 1];
+        const double kk = (k_max + k_min) / 2.0;
+        const double theta = cvg::wrap_two_pi(at + ot);
+        double s, cth;
+        cvg::cr_sincos<false>(theta, c_sincos_tab, &s, &cth);  // theta in [0, 2 pi)
+        const double px = ax - ox * s;
+        const double py = ay + ox * cth;
+        const double quality = (length > 0.0 && len > 0) ? 2.0 / (R.nrad / length + length / double(len)) : 0.0;
+        inf[0] = fwd ? 1.0 : 0.0;
+        inf[1] = k_max;
+        inf[2] = k_min;
+        inf[3] = at;
+        inf[4] = px;
+        inf[5] = py;
+        inf[6] = theta;
+        inf[7] = kk;
+        inf[8] = length;
+        inf[9] = quality;
     }
-    const double ox = R.off_x[c / R.kt];
-    const double ot = R.off_t[c % R.kt];
-    const double k_max = R.st_k[2 * static_cast<long long>(q)];
-    const double k_min = R.st_k[2 * static_cast<long long>(q) + 1];
-    const double kk = (k_max + k_min) / 2.0;
-    const double theta = cvg::wrap_two_pi(at + ot);
-    double s, cth;
-    cvg::cr_sincos(theta, c_sincos_tab, &s, &cth);
-    const double px = ax - ox * s;
-    const double py = ay + ox * cth;
-    double length = 0.0;
-    for (int l = 0; l + 1 < len; ++l) {
-        const int e1 = chain[l], e2 = chain[l + 1];
-        const double x1 = R.x[e1], y1 = R.y[e1], x2 = R.x[e2], y2 = R.y[e2];
-        length += sqrt((x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2));
-    }
-    const double quality = (length > 0.0 && len > 0) ? 2.0 / (R.nrad / length + length / double(len)) : 0.0;
-    double *inf = R.info + static_cast<long long>(o) * kInfoWidth;
-    inf[0] = fwd ? 1.0 : 0.0;
-    inf[1] = k_max;
-    inf[2] = k_min;
-    inf[3] = at;
-    inf[4] = px;
-    inf[5] = py;
-    inf[6] = theta;
-    inf[7] = kk;
-    inf[8] = length;
-    inf[9] = quality;
 }
 
-// One thread per staging slot finds the accepted rows (about one slot in
-// five); a workgroup compacts them in LDS (one ballot and one LDS atomic per
-// wave) and its first threads record them, so the record work runs on full
-// waves.
-constexpr int kRecordBlock = 1024;
-
-__global__ __launch_bounds__(kRecordBlock) void k_record(RecordArgs R)
+// compact_curvelet_output (cpu_curvelet.hpp:282-303) in two passes.  First,
+// one thread per staging slot: an accepted slot (its rank among the anchor's
+// accepted rows, or -1) names its output row in slot_of_row.  Then one thread
+// per output row records it; the row count (the last row offset) stays on
+// the device, and the rows beyond the caller's buffers are left out (the host
+// reports them).
+__global__ __launch_bounds__(kBlock) void k_row_slots(RecordArgs R, int *__restrict__ slot_of_row)
 {
-    __shared__ int sq[kRecordBlock];
-    __shared__ int srk[kRecordBlock];
-    __shared__ int scount;
-    if (threadIdx.x == 0) scount = 0;
-    __syncthreads();
     const int q = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    const int rk = q < R.pairs ? R.st_rank[q] : -1;
-    const bool on = rk >= 0;
-    const unsigned long long m = __ballot(on);
-    const int lane = threadIdx.x & 63;
-    int base = 0;
-    if (lane == 0 && m != 0ULL) base = atomicAdd(&scount, __popcll(m));
-    base = __shfl(base, 0, 64);
-    if (on) {
-        const int pos = base + __popcll(m & ((1ULL << lane) - 1ULL));
-        sq[pos] = q;
-        srk[pos] = rk;
-    }
-    __syncthreads();
-    const int cnt = scount;
-    for (int i = threadIdx.x; i < cnt; i += blockDim.x) record_slot(R, sq[i], srk[i]);
+    if (q >= R.pairs) return;
+    const int rk = R.st_rank[q];
+    if (rk < 0) return;
+    const unsigned o = R.row_off[R.pair_anchor[q]] + static_cast<unsigned>(rk);
+    if (o < R.max_rows) slot_of_row[o] = q;
+}
+
+template <bool F32>
+__global__ __launch_bounds__(kBlock) void k_record(RecordArgs R, const int *__restrict__ slot_of_row,
+                                                   const unsigned *__restrict__ num_rows)
+{
+    const unsigned rows = min(*num_rows, R.max_rows);
+    const unsigned stride = gridDim.x * blockDim.x;
+    for (unsigned o = blockIdx.x * blockDim.x + threadIdx.x; o < rows; o += stride)
+        record_row<F32>(R, o, slot_of_row[o]);
 }
 
 // ---------------------------------------------------------------------------
@@ -2657,6 +2685,8 @@ int read_scalars(cvg_context *ctx)
 
 constexpr int kSpecMiss = 1000;  // internal: a guessed size did not hold (never returned to callers)
 
+int queue_edge_trig(cvg_context *ctx, const cvg_params &p);
+
 // Neighbour counts, CSR offsets, then the gathering pass, the distance sort
 // and the CSR fill (preprocess.hpp:186-274) over either cell layout.  The
 // pair count and largest degree size the fill; with spec they are not read
@@ -2750,7 +2780,13 @@ int graph_scans(cvg_context *ctx, const Cells &cells, int n, unsigned R, unsigne
                        as<double>(ctx->d_d2));
     CVG_HIP(ctx, hipGetLastError());
     if (spec) {
-        CVG_TRY(read_scalars(ctx));
+        // the sizes come back while the device already runs the curvelet
+        // stage's per-edge trig
+        CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox, ctx->d_scal.ptr, kScalarWords * sizeof(unsigned),
+                                    hipMemcpyDeviceToHost, st));
+        CVG_HIP(ctx, hipEventRecord(ctx->ev[5], st));
+        if (ctx->early_params && !ctx->edge_trig_queued) CVG_TRY(queue_edge_trig(ctx, *ctx->early_params));
+        CVG_HIP(ctx, hipEventSynchronize(ctx->ev[5]));
         if (ctx->mailbox[S_BAD] || ctx->mailbox[S_MISS]) return kSpecMiss;  // the exact path reports bad input
         CVG_TRY(take_counts());
     }
@@ -2811,6 +2847,7 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     ctx->graph_ready = false;
     ctx->curvelets_ready = false;
     ctx->counts_pending = false;
+    ctx->edge_trig_queued = false;  // a new graph: its per-edge trig is queued again
     if (n <= 0 || sz < 4 || d_edges == nullptr)  // preprocess.hpp:197
         return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "build_neighbor_csr_graph: need num_edges > 0, sz_edge_data >= 4");
     if (R > 4096) return fail(ctx, CVG_ERR_UNSUPPORTED, "neighbor_radius above 4096 cells");
@@ -3070,6 +3107,27 @@ int derive(cvg_context *ctx, const cvg_params &p, Derived &d)
     return CVG_OK;
 }
 
+// Per-edge trig of the fused stage (k_edge_trig), which needs only the
+// edges' orientations and the parameters: queued by build_curvelets, or
+// earlier, while the graph stage's last host read is in flight.
+int queue_edge_trig(cvg_context *ctx, const cvg_params &p)
+{
+    Derived d;
+    CVG_TRY(derive(ctx, p, d));
+    const int n = ctx->n;
+    if (static_cast<unsigned long long>(n) * (d.kt + 3) > 0xffffffffull)
+        return fail(ctx, CVG_ERR_UNSUPPORTED, "more than 2^32 (edge, angle) items in one run");
+    const int ntrig = cvg::TRIG_GRID + 2 * static_cast<int>(d.kt);
+    CVG_TRY(ensure(ctx, ctx->d_trig, static_cast<size_t>(ntrig) * n * sizeof(double)));
+    CVG_TRY(ensure(ctx, ctx->d_tdd, static_cast<size_t>(8 + 4 * d.kt) * n * sizeof(double)));
+    hipLaunchKernelGGL(k_edge_trig, dim3(blocks_for(static_cast<long long>(n) * (d.kt + 3))), dim3(kBlock), 0,
+                       ctx->stream, n, as<double>(ctx->d_th), d.dt, static_cast<int>(d.kt), p.st,
+                       as<double>(ctx->d_trig), as<double>(ctx->d_tdd), as<unsigned>(ctx->d_scal));
+    CVG_HIP(ctx, hipGetLastError());
+    ctx->edge_trig_queued = true;
+    return CVG_OK;
+}
+
 // The parameter tables to d_tables, unless the same bytes are already there
 // (repeated runs with the same parameters skip the copy; the no-alloc
 // scratch's regions are reused by other buffers, so there it always copies).
@@ -3240,8 +3298,19 @@ int record_rows(cvg_context *ctx, const cvg_params &p, RecordArgs R, int st_stri
     R.chains = out_chains;
     R.info = out_info;
     R.max_rows = ctx->caller_chains ? ctx->caller_rows : UINT_MAX;
-    if (pairs > 0)
-        hipLaunchKernelGGL(k_record, dim3(blocks_for(pairs, kRecordBlock)), dim3(kRecordBlock), 0, st, R);
+    if (pairs > 0) {
+        CVG_TRY(ensure(ctx, ctx->d_rowslot, np * sizeof(int)));
+        int *slots = as<int>(ctx->d_rowslot);
+        hipLaunchKernelGGL(k_row_slots, dim3(blocks_for(pairs)), dim3(kBlock), 0, st, R, slots);
+        // a resident grid strides over the rows (their count is on the device)
+        const auto rec = R.f32 ? k_record<true> : k_record<false>;
+        int occ = 0;
+        CVG_TRY(prepare_launch(ctx, reinterpret_cast<const void *>(rec), kBlock, 0, &occ));
+        const long long resident = static_cast<long long>(std::max(occ, 1)) * ctx->num_cu;
+        const int grid = static_cast<int>(std::min(resident, static_cast<long long>(blocks_for(pairs))));
+        hipLaunchKernelGGL(rec, dim3(grid), dim3(kBlock), 0, st, R, static_cast<const int *>(slots),
+                           static_cast<const unsigned *>(as<unsigned>(ctx->d_rowoff) + n));
+    }
     CVG_HIP(ctx, hipGetLastError());
     CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox + 32, as<unsigned>(ctx->d_rowoff) + n, sizeof(unsigned),
                                 hipMemcpyDeviceToHost, st));
@@ -3451,21 +3520,15 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     T.caps = reinterpret_cast<int *>(tb + o_caps);
 
     // per-edge trig, per-pair classes and sines, buckets
-    const int ntrig = cvg::TRIG_GRID + 2 * static_cast<int>(d.kt);
-    CVG_TRY(ensure(ctx, ctx->d_trig, static_cast<size_t>(ntrig) * n * sizeof(double)));
-    CVG_TRY(ensure(ctx, ctx->d_tdd, static_cast<size_t>(8 + 4 * d.kt) * n * sizeof(double)));
+    if (!ctx->edge_trig_queued) CVG_TRY(queue_edge_trig(ctx, p));
+    ctx->edge_trig_queued = false;
     CVG_TRY(ensure(ctx, ctx->d_cls, static_cast<size_t>(pairs) + 1));
     CVG_TRY(ensure(ctx, ctx->d_prec, static_cast<size_t>(std::max(pairs, 1)) * 8 * sizeof(double)));
     CVG_TRY(ensure(ctx, ctx->d_psin, static_cast<size_t>(std::max(pairs, 1)) * d.kt * 2 * sizeof(double)));
     CVG_TRY(ensure(ctx, ctx->d_lists, static_cast<size_t>(nb) * n * sizeof(int4)));
     unsigned *scal = as<unsigned>(ctx->d_scal);
-    if (static_cast<unsigned long long>(n) * (d.kt + 3) > 0xffffffffull ||
-        static_cast<unsigned long long>(pairs) * d.kt > 0xffffffffull)
-        return fail(ctx, CVG_ERR_UNSUPPORTED, "more than 2^32 (edge, angle) items in one run");
-    hipLaunchKernelGGL(k_edge_trig, dim3(blocks_for(static_cast<long long>(n) * (d.kt + 3))), dim3(kBlock), 0, st, n,
-                       as<double>(ctx->d_th), d.dt,
-                       static_cast<int>(d.kt), T.off_t, as<double>(ctx->d_trig), as<double>(ctx->d_tdd),
-                       scal + S_VALID, kScalarBufWords - S_VALID);
+    if (static_cast<unsigned long long>(pairs) * d.kt > 0xffffffffull)
+        return fail(ctx, CVG_ERR_UNSUPPORTED, "more than 2^32 (pair, angle) items in one run");
     if (pairs > 0) {
         hipLaunchKernelGGL(k_pair_prep, dim3(blocks_for(pairs)), dim3(kBlock), 0, st, pairs, n,
                            as<int>(ctx->d_pair_anchor), as<int>(ctx->d_ids), as<double>(ctx->d_x),
@@ -3664,7 +3727,11 @@ int run_common(cvg_context *ctx, int n, int sz, const double *d_edges, const cvg
     ctx->f32 = f32;
     CVG_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     ctx->use_cellrank = false;
-    CVG_TRY(build_graph(ctx, n, sz, d_edges, p->nrad, 3u, p->max_candidates, num_maps, eoff));  // main.cpp:48: 3
+    ctx->edge_trig_queued = false;
+    ctx->early_params = (!f32 && p->group_max_sz <= 64 && !ctx->force_generic) ? p : nullptr;
+    const int rc = build_graph(ctx, n, sz, d_edges, p->nrad, 3u, p->max_candidates, num_maps, eoff);  // main.cpp:48: 3
+    ctx->early_params = nullptr;
+    if (rc != CVG_OK) return rc;
     CVG_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     CVG_TRY(build_curvelets(ctx, *p));
     CVG_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
