@@ -445,3 +445,48 @@ def test_consecutive_maps_of_changing_shape(oracle):
             assert stats["valid_pairs"] == want["valid_pairs"], name
             assert np.array_equal(chains, want["chains"]), name
             np.testing.assert_allclose(info, want["info"], rtol=1e-9, atol=1e-12)
+
+
+def test_consecutive_maps_queued_and_to_host(oracle):
+    """A sequence of maps whose grids, pair counts and longest lists change,
+    through the calls that return before the counts are read (a queued run,
+    then its first query) and through host rows in one call, with a bad map
+    in between: every map gives the same results as the oracle, and the
+    context stays usable after the error."""
+    from curvelet_construction_on_gpu_amd import Context
+    base = synthetic("C1", 1001)
+    shifted = base.copy()
+    shifted[:, 0] += 500.0
+    dense = np.concatenate([base, base[:, :] + np.array([0.3, 0.2, 0.0, 0.0])])
+    small = base[:300].copy()
+    bad = base.copy()
+    bad[7, 1] = np.inf
+    p = CurveletParams(group_max_sz=6)
+    with Context(0) as c:
+        for k, (name, e) in enumerate((("base", base), ("base again", base), ("base third", base),
+                                       ("shifted", shifted), ("dense", dense), ("dense again", dense),
+                                       ("small", small), ("base", base), ("bad", bad), ("base", base),
+                                       ("dense", dense))):
+            print(f"no-read sequence: {name}", flush=True)
+            if name == "bad":
+                with pytest.raises(CurveletError) as ei:
+                    c.run(e, p, stats=False)
+                    c.curvelets()
+                assert ei.value.status == 2
+                continue
+            want = oracle.run(e, p)
+            if k % 2 == 0:
+                c.run(e, p, stats=False)
+                chains, info = c.curvelets()
+                g = c.graph()
+                assert np.array_equal(g.neighbor_offsets, want["csr"]["offsets"]), name
+                assert np.array_equal(g.neighbor_ids, want["csr"]["ids"]), name
+            else:
+                rows = len(want["chains"])
+                hc = np.zeros((rows + 5, 7), np.uint32)
+                hi = np.zeros((rows + 5, 10), np.float64)
+                got, _ = c.run_to_host(e, p, hc, hi)
+                assert got == rows, name
+                chains, info = hc[:rows], hi[:rows]
+            assert np.array_equal(chains, want["chains"]), name
+            np.testing.assert_allclose(info, want["info"], rtol=1e-9, atol=1e-12)
