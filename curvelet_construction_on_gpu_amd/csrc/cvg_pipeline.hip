@@ -150,6 +150,8 @@ struct cvg_context {
     // last single map's; spec_valid once there is one)
     bool spec_valid = false;
     bool spec_off = false;  // CVG_SPECULATE=0 at creation: every graph reads its sizes first
+    unsigned to_host_rows = 0;  // rows of the last cvg_run_to_host (its early copy's size)
+    unsigned long long rebuilds = 0;  // runs built again in the reference's within-cell order
     // a run's parameters while its graph is built (fused double maps): the
     // graph stage queues k_edge_trig behind its last host read
     const cvg_params *early_params = nullptr;
@@ -3239,6 +3241,7 @@ int finish_curvelets(cvg_context *ctx)
     ctx->counts_pending = false;
     if (ctx->mailbox[S_AMBIG] > 0 && !ctx->use_cellrank) {
         const cvg_params p = ctx->cur_params;
+        ++ctx->rebuilds;
         CVG_TRY(rebuild_graph_exact(ctx));
         CVG_TRY(build_curvelets(ctx, p));
         CVG_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
@@ -4208,13 +4211,36 @@ int cvg_run_to_host(cvg_context *ctx, int num_edges, int sz_edge_data, const dou
     const double *d = nullptr;
     CVG_TRY(upload(ctx, num_edges, sz_edge_data, edges, &d));
     CVG_TRY(run_common(ctx, num_edges, sz_edge_data, d, params, nullptr));
+    // As many rows as the last call had are copied as soon as the record
+    // ends, before the count comes back: a map with no more rows (a
+    // repeated or similar map) then needs no further round trip.  Only
+    // page-locked outputs (one DMA each); rows past the count may be
+    // overwritten.
+    size_t early = 0;
+    const size_t w = ctx->width;
+    if (ctx->to_host_rows > 0 && dma_direct(chains) && dma_direct(info)) {
+        // (within what the row buffers hold at this run's width)
+        early = std::min({static_cast<size_t>(ctx->to_host_rows), static_cast<size_t>(max_rows),
+                          ctx->d_chains.bytes / (w * sizeof(unsigned)), ctx->d_info.bytes / (kInfoWidth * sizeof(double))});
+        CVG_HIP(ctx, hipMemcpyAsync(chains, ctx->d_chains.ptr, early * w * sizeof(unsigned), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        CVG_HIP(ctx, hipMemcpyAsync(info, ctx->d_info.ptr, early * kInfoWidth * sizeof(double),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+    }
+    const unsigned long long rebuilds = ctx->rebuilds;
     CVG_HIP(ctx, hipEventSynchronize(ctx->ev[2]));
     CVG_TRY(finish_curvelets(ctx));
+    if (ctx->rebuilds != rebuilds) early = 0;  // built again after the early copy
     const unsigned rows = ctx->num_curvelets;
     if (num_rows) *num_rows = rows;
+    ctx->to_host_rows = rows;
     const size_t C = std::min(rows, max_rows);
-    CVG_TRY(download(ctx, {{chains, ctx->d_chains.ptr, C * ctx->width * sizeof(unsigned)},
-                           {info, ctx->d_info.ptr, C * kInfoWidth * sizeof(double)}}));
+    if (C > early)
+        CVG_TRY(download(ctx, {{chains + early * w, static_cast<const unsigned *>(ctx->d_chains.ptr) + early * w,
+                                (C - early) * w * sizeof(unsigned)},
+                               {info + early * kInfoWidth,
+                                static_cast<const double *>(ctx->d_info.ptr) + early * kInfoWidth,
+                                (C - early) * kInfoWidth * sizeof(double)}}));
     if (stats) fill_stats(ctx, stats);
     if (rows > max_rows)
         return fail(ctx, CVG_ERR_INVALID_ARGUMENT,

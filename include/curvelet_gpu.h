@@ -176,7 +176,10 @@ int cvg_run(cvg_context *ctx, int num_edges, int sz_edge_data, const double *edg
  * ones from cvg_host_alloc take one DMA each), *num_rows the row count.
  * Returns when the rows are in the outputs.  More rows than max_rows fails
  * with CVG_ERR_INVALID_ARGUMENT after writing the first max_rows (*num_rows
- * still says how many there are). */
+ * still says how many there are).  Rows past *num_rows (up to max_rows) may
+ * be overwritten: page-locked outputs receive as many rows as the context's
+ * previous call returned as soon as the record ends, before the row count
+ * is read back. */
 int cvg_run_to_host(cvg_context *ctx, int num_edges, int sz_edge_data, const double *edges, const cvg_params *params,
                     unsigned *chains, double *info, unsigned max_rows, unsigned *num_rows, cvg_run_stats *stats);
 /* run_curvelet<float> (main.cpp:110 with --float): the whole path on a float

@@ -490,3 +490,33 @@ def test_consecutive_maps_queued_and_to_host(oracle):
                 chains, info = hc[:rows], hi[:rows]
             assert np.array_equal(chains, want["chains"]), name
             np.testing.assert_allclose(info, want["info"], rtol=1e-9, atol=1e-12)
+
+
+def test_to_host_page_locked_rows_of_changing_maps(oracle):
+    """Page-locked outputs receive the last call's row count before this
+    call's count is read: maps with more rows (the rest follows), fewer rows,
+    rows built again in the reference's within-cell order, and a wider chain
+    must all land complete and exact."""
+    from curvelet_construction_on_gpu_amd import Context, pinned_empty
+    base = synthetic("C1", 1001)
+    dense = np.concatenate([base, base[:, :] + np.array([0.3, 0.2, 0.0, 0.0])])
+    small = base[:300].copy()
+    dup = base[:800].copy()
+    dup = np.concatenate([dup, dup[::7]])  # exact duplicates: the within-cell order is replayed
+    seq = [("base", base, 6), ("base again", base, 6), ("dense", dense, 6), ("small", small, 6),
+           ("duplicates", dup, 6), ("base", base, 6), ("base, G = 9", base, 9), ("dense", dense, 6)]
+    with Context(0) as c:
+        for name, e, G in seq:
+            print(f"page-locked rows: {name}", flush=True)
+            p = CurveletParams(group_max_sz=G)
+            want = oracle.run(e, p)
+            rows = len(want["chains"])
+            cap = 60000
+            hc = pinned_empty((cap, G + 1), np.uint32)
+            hi = pinned_empty((cap, 10), np.float64)
+            hc[:] = 0xFFFFFFFF
+            hi[:] = np.nan
+            got, _ = c.run_to_host(e, p, hc, hi)
+            assert got == rows, name
+            assert np.array_equal(hc[:rows], want["chains"]), name
+            np.testing.assert_allclose(hi[:rows], want["info"], rtol=1e-9, atol=1e-12)
