@@ -350,6 +350,21 @@ __device__ void block_max_sum(unsigned &mx, unsigned &cnt, unsigned *lds_max, un
 // Tile sums; with STATS also each tile's largest value and its count of values
 // >= thr (tile_max / tile_cnt), so a scan over counts also yields their
 // maximum and how many reached a cap, without contended atomics.
+// A thread's kScanItems consecutive inputs (zeros past n): two 16-byte loads
+// when the run lies inside the array and the array is 16-byte aligned.
+__device__ inline void load_items(const unsigned *in, long long base, long long n, unsigned (&v)[kScanItems])
+{
+    static_assert(kScanItems == 8, "two 16-byte loads per thread");
+    if (base + kScanItems <= n && (reinterpret_cast<size_t>(in) & 15) == 0) {
+        const uint4 *p = reinterpret_cast<const uint4 *>(in + base);
+        const uint4 a = p[0], b = p[1];
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+        for (int k = 0; k < kScanItems; ++k) v[k] = base + k < n ? in[base + k] : 0u;
+    }
+}
+
 template <bool STATS>
 __global__ __launch_bounds__(kBlock) void k_scan_tiles(const unsigned *__restrict__ in, long long n,
                                                        unsigned *__restrict__ tile_sums, unsigned thr,
@@ -359,15 +374,16 @@ __global__ __launch_bounds__(kBlock) void k_scan_tiles(const unsigned *__restric
     __shared__ unsigned lm[kBlock / 64], lc[kBlock / 64];
     const long long base = static_cast<long long>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
     unsigned s = 0, mx = 0, c = 0;
-    for (int k = 0; k < kScanItems; ++k)
-        if (base + k < n) {
-            const unsigned v = in[base + k];
-            s += v;
-            if constexpr (STATS) {
-                mx = max(mx, v);
-                c += v >= thr ? 1u : 0u;
-            }
+    unsigned v[kScanItems];
+    load_items(in, base, n, v);
+    for (int k = 0; k < kScanItems; ++k) {
+        s += v[k];
+        if constexpr (STATS) {
+            // items past n are zeros, which leave the maximum as it is
+            mx = max(mx, v[k]);
+            c += (base + k < n && v[k] >= thr) ? 1u : 0u;
         }
+    }
     unsigned tot;
     block_exclusive_scan(s, lw, &tot);
     if constexpr (STATS) block_max_sum(mx, c, lm, lc);
@@ -425,16 +441,23 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const unsigned *in, long 
     __shared__ unsigned lw[kBlock / 64];
     const long long base = static_cast<long long>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
     unsigned v[kScanItems];
+    load_items(in, base, n, v);
     unsigned s = 0;
-    for (int k = 0; k < kScanItems; ++k) {
-        v[k] = (base + k < n) ? in[base + k] : 0u;
-        s += v[k];
-    }
+    for (int k = 0; k < kScanItems; ++k) s += v[k];
     unsigned tot;
     unsigned run = block_exclusive_scan(s, lw, &tot) + tile_prefix[blockIdx.x];
+    unsigned o[kScanItems];
     for (int k = 0; k < kScanItems; ++k) {
-        if (base + k < n) out[base + k] = run;
+        o[k] = run;
         run += v[k];
+    }
+    if (base + kScanItems <= n && (reinterpret_cast<size_t>(out) & 15) == 0) {
+        uint4 *p = reinterpret_cast<uint4 *>(out + base);
+        p[0] = make_uint4(o[0], o[1], o[2], o[3]);
+        p[1] = make_uint4(o[4], o[5], o[6], o[7]);
+    } else {
+        for (int k = 0; k < kScanItems; ++k)
+            if (base + k < n) out[base + k] = o[k];
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = tile_prefix[ntiles];
 }
