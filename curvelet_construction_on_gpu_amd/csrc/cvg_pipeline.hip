@@ -117,7 +117,7 @@ struct cvg_context {
     // cvg_set_tuning, or CVG_FILL_LANES at creation; results never depend on it)
     int fill_lanes = 16;
     // grid of the light buckets' launches as a percentage of their resident
-    // capacity (25..100), or 0: automatic (cvg_set_tuning, or CVG_GRID_PCT)
+    // capacity (5..100), or 0: automatic (cvg_set_tuning, or CVG_GRID_PCT)
     int grid_pct = 0;
     // use the sorted-key cell layout even where the dense grid fits
     // (CVG_SPARSE_CELLS=1 at creation; results never depend on it)
@@ -1309,7 +1309,18 @@ __global__ __launch_bounds__(kBlock) void k_pair_prep(int pairs, int n, const in
 constexpr int kBucketBlock = 1024;  // few workgroups: one list-length atomic each per bucket
 constexpr int kHeavyCap = 32;       // buckets from this cap start ahead of the light bucket 0
 constexpr int kGateMaxNaps = 40;    // the start gate's bound: 40 sleeps of 127 x 64 clocks
-constexpr int kGridAutoEdges = 4 << 20;  // automatic light-bucket grids: 75% below this many edges, else 100%
+// Automatic light-bucket grids: kGridAutoMinPct percent of the resident grid
+// plus one percent per kGridAutoEdgesPerPct edges of the run, at most all of
+// it (measured best on single maps from 320x240 to 3840x2160: ~10% at 6k
+// edges, 20-25% at 30-50k, 33% at 90k, 50% at 150k, 50-75% at 200k, 100%
+// from ~350k).
+constexpr long long kGridAutoEdgesPerPct = 4000;
+constexpr int kGridAutoMinPct = 10;
+
+int auto_grid_pct(long long n)
+{
+    return static_cast<int>(std::min<long long>(100, kGridAutoMinPct + n / kGridAutoEdgesPerPct));
+}
 
 __global__ __launch_bounds__(kBucketBlock) void k_bucket(int n, const int *__restrict__ sorted,
                                                    const unsigned char *__restrict__ pflag,
@@ -3663,10 +3674,10 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
         // Launches run four at a time: a light bucket's persistent grid sized
         // for the whole GPU has workgroups that are dispatched late and still
         // own a full share of the list, which stretches the stage's tail.  A
-        // smaller grid fits one map's lists better; batches, whose launches
-        // run longer, keep the full one.
+        // grid in proportion to the run's edges fits its lists better; large
+        // maps and batches, whose launches run longer, keep the full one.
         if (caps[b] < kHeavyCap) {
-            const int pct = ctx->grid_pct > 0 ? ctx->grid_pct : (n < kGridAutoEdges ? 75 : 100);
+            const int pct = ctx->grid_pct > 0 ? ctx->grid_pct : auto_grid_pct(n);
             resident = std::max(1LL, resident * pct / 100);
         }
         const long long max_blocks = (static_cast<long long>(n) + l.apb - 1) / l.apb;
@@ -4038,7 +4049,7 @@ int cvg_create(int device, cvg_context **out_ctx)
         }
         if (const char *v = std::getenv("CVG_GRID_PCT")) {
             const int g = std::atoi(v);
-            ctx->grid_pct = (g >= 25 && g <= 100) ? g : 0;
+            ctx->grid_pct = (g >= 5 && g <= 100) ? g : 0;
         }
         if (const char *v = std::getenv("CVG_SPARSE_CELLS")) ctx->force_sparse = std::atoi(v) != 0;
         if (const char *v = std::getenv("CVG_GENERIC_KERNEL")) ctx->force_generic = std::atoi(v) != 0;
@@ -4396,10 +4407,10 @@ int cvg_set_tuning(cvg_context *ctx, const cvg_tuning *tuning)
     if (!tuning) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "cvg_set_tuning: null tuning");
     const int w = tuning->coop_waves, f = tuning->window_lanes, g = tuning->grid_pct;
     if (tuning->coop_min_key < 1 || !(w == 1 || w == 2 || w == 4) || !(f == 8 || f == 16 || f == 32 || f == 64) ||
-        !(g == 0 || (g >= 25 && g <= 100)))
+        !(g == 0 || (g >= 5 && g <= 100)))
         return fail(ctx, CVG_ERR_INVALID_ARGUMENT,
                     "cvg_set_tuning: need coop_min_key >= 1, coop_waves in {1, 2, 4}, window_lanes in {8, 16, 32, 64}, "
-                    "grid_pct 0 or 25..100");
+                    "grid_pct 0 or 5..100");
     ctx->coop_min_cap = tuning->coop_min_key;
     ctx->coop_waves = w;
     ctx->fill_lanes = f;
@@ -4440,10 +4451,10 @@ int cvg_autotune_batch(cvg_context *ctx, int num_maps, const int *num_edges, int
     const int keys[] = {4, 6, 8, 11, 16, 1 << 30};
     const int waves[] = {2, 4};
     const int lanes[] = {8, 16, 32};
-    const int grids[] = {50, 75, 100};
+    const int grids[] = {25, 50, 75, 100};
     for (int field = 0; field < 4; ++field) {
         const int *vals = field == 0 ? keys : (field == 1 ? waves : (field == 2 ? lanes : grids));
-        const int nv = field == 0 ? 6 : (field == 1 ? 2 : 3);
+        const int nv = field == 0 ? 6 : (field == 1 ? 2 : (field == 2 ? 3 : 4));
         cvg_tuning base = cur;
         for (int k = 0; k < nv; ++k) {
             cvg_tuning t = base;

@@ -265,8 +265,8 @@ typedef struct cvg_tuning {
     int coop_waves;    /* 1, 2 or 4 */
     int window_lanes;  /* lanes per anchor in the neighbour-window scans: 8, 16, 32 or 64 */
     int grid_pct;      /* grid of the light buckets' curvelet launches as a percentage of
-                          their resident capacity, 25..100; 0 = automatic (75 below 4M
-                          edges in the run, else 100) */
+                          their resident capacity, 5..100; 0 = automatic (10 plus one
+                          percent per 4,000 edges in the run, at most 100) */
 } cvg_tuning;
 typedef struct cvg_tuning_trial {
     cvg_tuning tuning;

@@ -139,7 +139,7 @@ def test_tuning_set_get_and_rejects(ctx):
         want = {"coop_min_key": 11, "coop_waves": 2, "window_lanes": 32, "grid_pct": 60}
         ctx.tuning = want
         assert ctx.tuning == want
-        for bad in ({"coop_min_key": 0}, {"coop_waves": 3}, {"window_lanes": 12}, {"grid_pct": 20},
+        for bad in ({"coop_min_key": 0}, {"coop_waves": 3}, {"window_lanes": 12}, {"grid_pct": 4},
                     {"grid_pct": 101}):
             with pytest.raises(CurveletError):
                 ctx.tuning = {**want, **bad}
@@ -167,7 +167,7 @@ def test_autotune_batch_keeps_results(ctx):
         assert min(ms for _, ms in trials) == pytest.approx(best_ms)
         assert ctx.tuning == best
         assert best["coop_waves"] in (1, 2, 4) and best["window_lanes"] in (8, 16, 32, 64)
-        assert best["grid_pct"] == 0 or 25 <= best["grid_pct"] <= 100
+        assert best["grid_pct"] == 0 or 5 <= best["grid_pct"] <= 100
         ctx.run_batch(maps, p)
         for i in range(len(maps)):
             got = ctx.batch_curvelets(i)
