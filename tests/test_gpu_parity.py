@@ -161,11 +161,12 @@ def test_full_size_map_matches_reference(ctx):
     np.testing.assert_allclose(info[rows["rows"]], rows["info"], rtol=INFO_RTOL, atol=INFO_ATOL)
 
 
-@pytest.mark.parametrize("min_cap,waves,grid", [(1, 4, 0), (1, 2, 100), (1, 1, 25), (1000, 4, 50), (8, 4, 100)])
+@pytest.mark.parametrize("min_cap,waves,grid", [(1, 4, 0), (1, 2, 100), (1, 1, 25), (1000, 4, 50), (8, 4, 100),
+                                                (8, 4, 5)])
 def test_launch_shapes_same_results(oracle_cr, monkeypatch, min_cap, waves, grid):
     """Every launch shape of the fused kernel (one anchor per wave, or one
-    anchor per workgroup of 1, 2 or 4 waves; light-bucket grids from a quarter
-    to all of their resident capacity) gives the same bits."""
+    anchor per workgroup of 1, 2 or 4 waves; light-bucket grids from 5% to all
+    of their resident capacity) gives the same bits."""
     from curvelet_construction_on_gpu_amd import Context
     monkeypatch.setenv("CVG_COOP_MIN_CAP", str(min_cap))
     monkeypatch.setenv("CVG_COOP_WAVES", str(waves))
