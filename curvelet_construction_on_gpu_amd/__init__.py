@@ -130,6 +130,7 @@ _SIGNATURES = {
     "cvg_get_stream": (C.c_int, [C.c_void_p, _P(C.c_void_p)]),
     "cvg_synchronize": (C.c_int, [C.c_void_p]),
     "cvg_recent_fused_ms": (C.c_int, [C.c_void_p, C.c_int, _P(C.c_float), _P(C.c_int)]),
+    "cvg_set_event_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "cvg_run_batch": (C.c_int, [C.c_void_p, C.c_int, _ip, C.c_int, _P(_dp), _P(_Params), _P(_RunStats)]),
     "cvg_run_batch_device": (C.c_int, [C.c_void_p, C.c_int, _ip, C.c_int, C.c_void_p, _P(_Params), _P(_RunStats)]),
     "cvg_batch_map_shape": (C.c_int, [C.c_void_p, C.c_int, _up, _up]),
@@ -414,11 +415,17 @@ class Context:
         return nc.value, vp.value
 
     def recent_fused_ms(self, count: int) -> list[float]:
-        """Durations (ms, HIP events) of the fused kernel in the last runs, oldest first (at most 64)."""
+        """Durations (ms, HIP events) of the fused kernel in the last timed runs
+        (runs with stats, or made while event timing is on), oldest first (at most 64)."""
         buf = (C.c_float * max(count, 0))()
         got = C.c_int()
         self.check(self._lib.cvg_recent_fused_ms(self._h, count, buf, C.byref(got)), "recent_fused_ms")
         return list(buf)[:got.value]
+
+    def set_event_timing(self, on: bool) -> None:
+        """Record the HIP timing events in runs without stats too (off at
+        creation: each timing event holds the stream for a few microseconds)."""
+        self.check(self._lib.cvg_set_event_timing(self._h, 1 if on else 0), "set_event_timing")
 
     def synchronize(self) -> None:
         self.check(self._lib.cvg_synchronize(self._h), "synchronize")

@@ -19,11 +19,17 @@
 //           k_pair_prep       per pair: direction-filter class, close/branch
 //                             flags, the bundle's cell-independent terms
 //           k_pair_trig       per (pair, grid angle): the transport's two sines
+//                             (at Kt = 7 with k_pair_prep's work per pair first;
+//                             queued with k_edge_trig behind the graph stage's
+//                             count read where the pair buffers hold the
+//                             guessed capacity)
 //           k_bucket          anchors binned by their largest run (LDS capacity)
 //           k_curvelets       one wave per anchor, bucket by bucket: bundles
 //                             (lane = cell) into the wave's LDS slice, chain
 //                             growth with ballot validity, dedup, staging rows
 //           scan              per-anchor accepted counts -> output row offsets
+//           k_row_slots       staging slot -> row, and the row count into the
+//                             scalars read back with the stream
 //           k_record          chain rows (1-based) and the 10 info columns
 //
 // The reference path this replaces is cpu_curvelet (preprocess.hpp,
@@ -100,11 +106,22 @@ struct cvg_context {
     // extra streams so the bucket launches of the fused kernel run concurrently
     hipStream_t aux[3] = {nullptr, nullptr, nullptr};
     hipEvent_t ev_join[3] = {nullptr, nullptr, nullptr};
-    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    // the fused kernel's start and end events of the last kFusedRing runs
+    // timing events: ev[0..2] run start, graph end, run end; ev[3..4] around
+    // the fused kernel's launches
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    // the fused kernel's start and end events of the last kFusedRing timed runs
     static constexpr int kFusedRing = 64;
     hipEvent_t fused_ev[kFusedRing][2] = {};
     long long fused_runs = 0;
+    // events without timing for ordering and host waits: the graph stage's
+    // count read, the fork to the auxiliary streams, the end of an untimed run
+    hipEvent_t ev_read = nullptr, ev_fork = nullptr, ev_end = nullptr;
+    // Runs with stats record the timing events (their stage times come from
+    // them); runs without stats only while event_timing is set
+    // (cvg_set_event_timing).  Each recorded timing event is a marker that
+    // holds the stream for several microseconds.
+    bool event_timing = false;
+    bool timed_run = false;  // the current run records the timing events
     std::string err;
     size_t lds_limit = 65536;
     // fused-kernel launch shape: buckets whose key is at least coop_min_cap
@@ -156,6 +173,7 @@ struct cvg_context {
     // graph stage queues k_edge_trig behind its last host read
     const cvg_params *early_params = nullptr;
     bool edge_trig_queued = false;  // k_edge_trig of the current run is queued
+    bool pair_setup_queued = false;  // and its per-pair records and sines
     GridDesc spec_g{};
     unsigned spec_R = 0;
     int spec_fill_cap = 1;
@@ -245,6 +263,20 @@ int fail(cvg_context *ctx, int code, const std::string &msg)
         int rc_ = (expr);          \
         if (rc_ != CVG_OK) return rc_; \
     } while (0)
+
+// Records timing event e on st in a timed run; nothing otherwise.
+int record_timing(cvg_context *ctx, hipEvent_t e, hipStream_t st)
+{
+    if (ctx->timed_run) CVG_HIP(ctx, hipEventRecord(e, st));
+    return CVG_OK;
+}
+
+// The event that marks the end of the current run's work on the context
+// stream: the timing event ev[2] in a timed run, ev_end otherwise.
+hipEvent_t end_event(const cvg_context *ctx)
+{
+    return ctx->timed_run ? ctx->ev[2] : ctx->ev_end;
+}
 
 int ensure(cvg_context *ctx, DevBuf &b, size_t bytes)
 {
@@ -546,7 +578,8 @@ struct SparseCells {
 // Scalars block layout (d_scal, uint32): [0..3] bbox (minx, maxx, miny, maxy as
 // int), [4] bad-input flag, [5] max degree, [6] truncated anchors,
 // [7] order-ambiguous anchors, [8] valid pairs, [9] anchors needing the
-// std::sort replay, [10..11] the 64-bit count of bundle intersections.  The
+// std::sort replay, [10..11] the 64-bit count of bundle intersections, [12]
+// CSR pairs, [13] a size guess that did not hold, [14] output rows.  The
 // fused kernel's bucket list lengths follow, one 128-byte line each (atomics
 // from many workgroups on one line serialize in L2).
 enum {
@@ -562,9 +595,10 @@ enum {
     S_TIES = 9,
     S_COUNT = 10,
     S_PAIRS = 12,
-    S_MISS = 13
+    S_MISS = 13,
+    S_ROWS = 14
 };
-constexpr int kScalarWords = S_MISS + 1;  // the words read back to the host
+constexpr int kScalarWords = S_ROWS + 1;  // the words read back to the host
 constexpr int kMaxBuckets = 14;
 constexpr int kBucketStride = 32;  // words
 constexpr int kBucketBase = 32;    // bucket b's list length at [kBucketBase + b * kBucketStride]
@@ -1274,31 +1308,40 @@ __global__ __launch_bounds__(kBlock) void k_edge_trig(int n, const double *__res
 //   pflag[q] = class | close << 2 | branch_a << 3
 //   prec[8q .. 8q+7] = DX, DY, DX2, DY2, DX cos(T0-dt), DY sin(T0-dt),
 //                      DX cos(T0+dt), DY sin(T0+dt)
-__global__ __launch_bounds__(kBlock) void k_pair_prep(int pairs, int n, const int *__restrict__ pair_anchor,
-                                                      const int *__restrict__ ids, const double *__restrict__ x,
-                                                      const double *__restrict__ y, const double *__restrict__ trig,
-                                                      double close_thr, unsigned char *__restrict__ pflag,
-                                                      double *__restrict__ prec)
+struct PrepArgs {
+    const double *x, *y, *trig;
+    double close_thr;
+    unsigned char *pflag;
+    double *prec;
+};
+
+// Pair q = (a, b)'s flags and record; returns whether the pair is close.
+__device__ __forceinline__ bool pair_prep_one(int q, int n, int a, int b, const PrepArgs &P)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= pairs) return;
-    const int a = pair_anchor[q];
-    const int b = ids[q];
-    const double ax = x[a], ay = y[a], bx = x[b], by = y[b];
-    const double cos_at = trig[cvg::TRIG_COS * n + a];
-    const double sin_at = trig[cvg::TRIG_SIN * n + a];
+    const double ax = P.x[a], ay = P.y[a], bx = P.x[b], by = P.y[b];
+    const double cos_at = P.trig[cvg::TRIG_COS * n + a];
+    const double sin_at = P.trig[cvg::TRIG_SIN * n + a];
     const int k = cvg::direction_class(ax, ay, cos_at, sin_at, bx, by, c_sincos_tab, c_atan_tab);
-    const cvg::PairRec p = cvg::pair_record(ax, ay, bx, by, trig[cvg::TRIG_SIN_P * n + b],
-                                            trig[cvg::TRIG_COS_P * n + b], trig[cvg::TRIG_SIN_M * n + b],
-                                            trig[cvg::TRIG_COS_M * n + b]);
-    const bool close = p.DX2 + p.DY2 < close_thr;
+    const cvg::PairRec p = cvg::pair_record(ax, ay, bx, by, P.trig[cvg::TRIG_SIN_P * n + b],
+                                            P.trig[cvg::TRIG_COS_P * n + b], P.trig[cvg::TRIG_SIN_M * n + b],
+                                            P.trig[cvg::TRIG_COS_M * n + b]);
+    const bool close = p.DX2 + p.DY2 < P.close_thr;
     const bool branch_a = (p.DX * cos_at + p.DY * sin_at) < 0;
-    pflag[q] = static_cast<unsigned char>(k | (close ? 4 : 0) | (branch_a ? 8 : 0));
-    double2 *o = reinterpret_cast<double2 *>(prec + static_cast<long long>(q) * 8);
+    P.pflag[q] = static_cast<unsigned char>(k | (close ? 4 : 0) | (branch_a ? 8 : 0));
+    double2 *o = reinterpret_cast<double2 *>(P.prec + static_cast<long long>(q) * 8);
     o[0] = make_double2(p.DX, p.DY);
     o[1] = make_double2(p.DX2, p.DY2);
     o[2] = make_double2(p.cT0m_DX, p.sT0m_DY);
     o[3] = make_double2(p.cT0p_DX, p.sT0p_DY);
+    return close;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pair_prep(int pairs, int n, const int *__restrict__ pair_anchor,
+                                                      const int *__restrict__ ids, PrepArgs P)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= pairs) return;
+    (void)pair_prep_one(q, n, pair_anchor[q], ids[q], P);
 }
 
 // Anchors in spatial (cell) order, binned by max(forward, backward) + neither
@@ -1407,29 +1450,39 @@ __global__ void k_gate(GateArgs G, const unsigned *__restrict__ scal, int max_na
 // both angles in tdd by the subtraction formula; the rare lane whose rounding
 // that does not settle evaluates the sine directly.  For Kt up to
 // kPairTrigMaxKt the block's results go through LDS (dynamic, kBlock x Kt
-// double2) so that its output span is written contiguously.
+// double2) so that its output span is written contiguously.  PREP: the
+// thread first forms its pair's flags and record (k_pair_prep's work) and
+// takes the close bit from there instead of reading pflag.  With scal (a
+// launch queued before the host has read the graph's counts, on buffers of
+// the guessed capacity) the pair count comes from the device, and a guess
+// that did not hold (or bad input) leaves the launch with nothing to do.
 constexpr int kPairTrigMaxKt = 16;
-template <int KT>
+template <int KT, bool PREP>
 __global__ __launch_bounds__(kBlock) void k_pair_trig(int pairs, int n, int kt_rt, const int *__restrict__ pair_anchor,
                                                       const int *__restrict__ ids,
                                                       const unsigned char *__restrict__ pflag,
-                                                      const double *__restrict__ th,
-                                                      const double *__restrict__ off_t, double dt,
-                                                      const double *__restrict__ tdd, double *__restrict__ psin)
+                                                      const double *__restrict__ th, double st_step, double dt,
+                                                      const double *__restrict__ tdd, double *__restrict__ psin,
+                                                      PrepArgs P, const unsigned *__restrict__ scal)
 {
     extern __shared__ double2 stage[];
+    if (scal) pairs = (scal[S_MISS] | scal[S_BAD]) ? 0 : static_cast<int>(scal[S_PAIRS]);
+    if (static_cast<int>(blockIdx.x * blockDim.x) >= pairs) return;  // the whole block
     const int kt = KT > 0 ? KT : kt_rt;
+    // delta_t2 of grid column j (cpu_curvelet.hpp:698), the host table's expression
+    const double half = (static_cast<double>(kt) - 1.0) / 2.0;
     const bool staged = kt <= kPairTrigMaxKt;
     const int q0 = blockIdx.x * blockDim.x;
     const int q = q0 + threadIdx.x;
     double2 *out = reinterpret_cast<double2 *>(psin) + static_cast<long long>(q0) * kt;
     double2 *row = (staged ? stage : out) + threadIdx.x * kt;
     if (q < pairs) {
-        if (pflag[q] & 4) {  // close, :682
+        const int a = pair_anchor[q];
+        const int b = ids[q];
+        const bool close = PREP ? pair_prep_one(q, n, a, b, P) : (pflag[q] & 4) != 0;
+        if (close) {  // :682
             for (int j = 0; j < kt; ++j) row[j] = make_double2(0.0, 0.0);
         } else {
-            const int a = pair_anchor[q];
-            const int b = ids[q];
             const double ta = th[a];
             const double nt = th[b];
             const double tm = nt - dt, tp = nt + dt;
@@ -1444,7 +1497,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_trig(int pairs, int n, int kt_r
                 for (int j = 0; j < (KT > 0 ? KT : 1); ++j) g[j] = gr[j];
 #pragma unroll
                 for (int j = 0; j < (KT > 0 ? KT : 1); ++j) {
-                    const double T2t = ta + off_t[j];
+                    const double T2t = ta + st_step * (static_cast<double>(j) - half);
                     const cvg::dd st = {g[j].x, g[j].y}, ct = {g[j].z, g[j].w};
                     double2 v;
                     v.x = cvg::sin_difference(T2t, tm, st, ct, sm, cm, c_sincos_tab);
@@ -1453,7 +1506,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_trig(int pairs, int n, int kt_r
                 }
             } else {
                 for (int j = 0; j < kt; ++j) {
-                    const double T2t = ta + off_t[j];
+                    const double T2t = ta + st_step * (static_cast<double>(j) - half);
                     const double4 g = gr[j];
                     const cvg::dd st = {g.x, g.y}, ct = {g.z, g.w};
                     double2 v;
@@ -2223,9 +2276,11 @@ __device__ inline void record_row(const RecordArgs &R, unsigned o, int q)
 // per output row records it; the row count (the last row offset) stays on
 // the device, and the rows beyond the caller's buffers are left out (the host
 // reports them).
-__global__ __launch_bounds__(kBlock) void k_row_slots(RecordArgs R, int *__restrict__ slot_of_row)
+__global__ __launch_bounds__(kBlock) void k_row_slots(RecordArgs R, int *__restrict__ slot_of_row,
+                                                      const unsigned *__restrict__ num_rows, unsigned *__restrict__ scal)
 {
     const int q = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q == 0) scal[S_ROWS] = *num_rows;  // read back with the other scalars
     if (q >= R.pairs) return;
     const int rk = R.st_rank[q];
     if (rk < 0) return;
@@ -2722,6 +2777,7 @@ int read_scalars(cvg_context *ctx)
 constexpr int kSpecMiss = 1000;  // internal: a guessed size did not hold (never returned to callers)
 
 int queue_edge_trig(cvg_context *ctx, const cvg_params &p);
+int queue_pair_setup(cvg_context *ctx, const cvg_params &p, unsigned long long pair_cap);
 
 // Neighbour counts, CSR offsets, then the gathering pass, the distance sort
 // and the CSR fill (preprocess.hpp:186-274) over either cell layout.  The
@@ -2820,11 +2876,23 @@ int graph_scans(cvg_context *ctx, const Cells &cells, int n, unsigned R, unsigne
         // stage's per-edge trig
         CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox, ctx->d_scal.ptr, kScalarWords * sizeof(unsigned),
                                     hipMemcpyDeviceToHost, st));
-        CVG_HIP(ctx, hipEventRecord(ctx->ev[5], st));
-        if (ctx->early_params && !ctx->edge_trig_queued) CVG_TRY(queue_edge_trig(ctx, *ctx->early_params));
-        CVG_HIP(ctx, hipEventSynchronize(ctx->ev[5]));
-        if (ctx->mailbox[S_BAD] || ctx->mailbox[S_MISS]) return kSpecMiss;  // the exact path reports bad input
-        CVG_TRY(take_counts());
+        CVG_HIP(ctx, hipEventRecord(ctx->ev_read, st));
+        if (ctx->early_params && !ctx->edge_trig_queued) {
+            CVG_TRY(queue_edge_trig(ctx, *ctx->early_params));
+            CVG_TRY(queue_pair_setup(ctx, *ctx->early_params, pair_cap));
+        }
+        CVG_HIP(ctx, hipEventSynchronize(ctx->ev_read));
+        if (ctx->mailbox[S_BAD] || ctx->mailbox[S_MISS]) {
+            // the queued per-pair launch found the miss and did nothing: the
+            // graph built again queues it again (k_edge_trig's rows still hold)
+            ctx->pair_setup_queued = false;
+            return kSpecMiss;  // the exact path reports bad input
+        }
+        const int rc = take_counts();
+        if (rc != CVG_OK) {
+            ctx->pair_setup_queued = false;
+            return rc;
+        }
     }
     return CVG_OK;
 }
@@ -2884,6 +2952,7 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     ctx->curvelets_ready = false;
     ctx->counts_pending = false;
     ctx->edge_trig_queued = false;  // a new graph: its per-edge trig is queued again
+    ctx->pair_setup_queued = false;
     if (n <= 0 || sz < 4 || d_edges == nullptr)  // preprocess.hpp:197
         return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "build_neighbor_csr_graph: need num_edges > 0, sz_edge_data >= 4");
     if (R > 4096) return fail(ctx, CVG_ERR_UNSUPPORTED, "neighbor_radius above 4096 cells");
@@ -3164,6 +3233,36 @@ int queue_edge_trig(cvg_context *ctx, const cvg_params &p)
     return CVG_OK;
 }
 
+// The per-pair records and sines (k_pair_trig with the record, Kt = 7),
+// queued behind k_edge_trig while the graph stage's count read is in flight:
+// the launch covers the CSR buffers' capacity pair_cap and takes the pair
+// count from the device.  Only where the pair buffers already hold that
+// capacity: nothing is allocated before the counts are known.
+int queue_pair_setup(cvg_context *ctx, const cvg_params &p, unsigned long long pair_cap)
+{
+    Derived d;
+    CVG_TRY(derive(ctx, p, d));
+    constexpr int kt = 7;
+    if (d.kt != kt || ctx->arena || pair_cap == 0 || pair_cap > static_cast<unsigned long long>(INT_MAX))
+        return CVG_OK;
+    if (ctx->d_cls.bytes < pair_cap + 1 || ctx->d_prec.bytes < pair_cap * 8 * sizeof(double) ||
+        ctx->d_psin.bytes < pair_cap * kt * 2 * sizeof(double))
+        return CVG_OK;
+    const int cap = static_cast<int>(pair_cap);
+    const int n = ctx->n;
+    const PrepArgs P{as<double>(ctx->d_x), as<double>(ctx->d_y), as<double>(ctx->d_trig), d.close_thr,
+                     as<unsigned char>(ctx->d_cls), as<double>(ctx->d_prec)};
+    const auto fn = k_pair_trig<kt, true>;
+    hipLaunchKernelGGL(fn, dim3(blocks_for(cap)), dim3(kBlock), kBlock * kt * sizeof(double2),
+                       ctx->stream, cap, n, kt, as<int>(ctx->d_pair_anchor), as<int>(ctx->d_ids),
+                       static_cast<const unsigned char *>(as<unsigned char>(ctx->d_cls)), as<double>(ctx->d_th), p.st,
+                       d.dt, as<double>(ctx->d_tdd), as<double>(ctx->d_psin), P,
+                       static_cast<const unsigned *>(as<unsigned>(ctx->d_scal)));
+    CVG_HIP(ctx, hipGetLastError());
+    ctx->pair_setup_queued = true;
+    return CVG_OK;
+}
+
 // The parameter tables to d_tables, unless the same bytes are already there
 // (repeated runs with the same parameters skip the copy; the no-alloc
 // scratch's regions are reused by other buffers, so there it always copies).
@@ -3278,11 +3377,11 @@ int finish_curvelets(cvg_context *ctx)
         ++ctx->rebuilds;
         CVG_TRY(rebuild_graph_exact(ctx));
         CVG_TRY(build_curvelets(ctx, p));
-        CVG_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+        CVG_TRY(record_timing(ctx, ctx->ev[2], ctx->stream));
         CVG_HIP(ctx, hipStreamSynchronize(ctx->stream));
         ctx->counts_pending = false;
     }
-    const unsigned C = ctx->mailbox[32];
+    const unsigned C = ctx->pairs > 0 ? ctx->mailbox[S_ROWS] : 0u;
     ctx->valid_pairs = ctx->mailbox[S_VALID];
     ctx->intersections = static_cast<unsigned long long>(ctx->mailbox[S_COUNT]) |
                          (static_cast<unsigned long long>(ctx->mailbox[S_COUNT + 1]) << 32);
@@ -3338,7 +3437,8 @@ int record_rows(cvg_context *ctx, const cvg_params &p, RecordArgs R, int st_stri
     if (pairs > 0) {
         CVG_TRY(ensure(ctx, ctx->d_rowslot, np * sizeof(int)));
         int *slots = as<int>(ctx->d_rowslot);
-        hipLaunchKernelGGL(k_row_slots, dim3(blocks_for(pairs)), dim3(kBlock), 0, st, R, slots);
+        hipLaunchKernelGGL(k_row_slots, dim3(blocks_for(pairs)), dim3(kBlock), 0, st, R, slots,
+                           static_cast<const unsigned *>(as<unsigned>(ctx->d_rowoff) + n), as<unsigned>(ctx->d_scal));
         // a resident grid strides over the rows (their count is on the device)
         const auto rec = R.f32 ? k_record<true> : k_record<false>;
         int occ = 0;
@@ -3349,8 +3449,7 @@ int record_rows(cvg_context *ctx, const cvg_params &p, RecordArgs R, int st_stri
                            static_cast<const unsigned *>(as<unsigned>(ctx->d_rowoff) + n));
     }
     CVG_HIP(ctx, hipGetLastError());
-    CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox + 32, as<unsigned>(ctx->d_rowoff) + n, sizeof(unsigned),
-                                hipMemcpyDeviceToHost, st));
+    // the counts (the row count in S_ROWS when there are pairs; none otherwise)
     CVG_HIP(ctx, hipMemcpyAsync(ctx->mailbox, ctx->d_scal.ptr, kScalarWords * sizeof(unsigned),
                                 hipMemcpyDeviceToHost, st));
     if (ctx->num_maps > 1) {
@@ -3490,13 +3589,13 @@ int build_curvelets_generic(cvg_context *ctx, const cvg_params &p)
     A.valid_pairs = scal + S_VALID;
     A.intersections = reinterpret_cast<unsigned long long *>(scal + S_COUNT);
     const int ring = static_cast<int>(ctx->fused_runs % cvg_context::kFusedRing);
-    CVG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
-    CVG_HIP(ctx, hipEventRecord(ctx->fused_ev[ring][0], st));
+    CVG_TRY(record_timing(ctx, ctx->ev[3], st));
+    CVG_TRY(record_timing(ctx, ctx->fused_ev[ring][0], st));
     hipLaunchKernelGGL(k_curvelets_generic<T>, dim3(blocks), dim3(64 * kGenWaves), 0, st, A);
     CVG_HIP(ctx, hipGetLastError());
-    CVG_HIP(ctx, hipEventRecord(ctx->ev[4], st));
-    CVG_HIP(ctx, hipEventRecord(ctx->fused_ev[ring][1], st));
-    ++ctx->fused_runs;
+    CVG_TRY(record_timing(ctx, ctx->ev[4], st));
+    CVG_TRY(record_timing(ctx, ctx->fused_ev[ring][1], st));
+    if (ctx->timed_run) ++ctx->fused_runs;
 
     RecordArgs R{};
     R.G = G;
@@ -3523,7 +3622,11 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     CVG_TRY(derive(ctx, p, d));
     // the fused kernel holds a chain in the lanes of a wave and up to four
     // 64-neighbour chunks per anchor; other maps take the general kernel
-    if (d.G > 64 || ctx->max_deg > 256 || ctx->force_generic) return build_curvelets_generic<double>(ctx, p);
+    if (d.G > 64 || ctx->max_deg > 256 || ctx->force_generic) {
+        ctx->edge_trig_queued = false;  // the general kernel computes its own trig
+        ctx->pair_setup_queued = false;
+        return build_curvelets_generic<double>(ctx, p);
+    }
     hipStream_t st = ctx->stream;
     const int n = ctx->n;
     const int pairs = ctx->pairs;
@@ -3559,6 +3662,8 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     // per-edge trig, per-pair classes and sines, buckets
     if (!ctx->edge_trig_queued) CVG_TRY(queue_edge_trig(ctx, p));
     ctx->edge_trig_queued = false;
+    const bool pair_setup_queued = ctx->pair_setup_queued;
+    ctx->pair_setup_queued = false;
     CVG_TRY(ensure(ctx, ctx->d_cls, static_cast<size_t>(pairs) + 1));
     CVG_TRY(ensure(ctx, ctx->d_prec, static_cast<size_t>(std::max(pairs, 1)) * 8 * sizeof(double)));
     CVG_TRY(ensure(ctx, ctx->d_psin, static_cast<size_t>(std::max(pairs, 1)) * d.kt * 2 * sizeof(double)));
@@ -3566,17 +3671,22 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     unsigned *scal = as<unsigned>(ctx->d_scal);
     if (static_cast<unsigned long long>(pairs) * d.kt > 0xffffffffull)
         return fail(ctx, CVG_ERR_UNSUPPORTED, "more than 2^32 (pair, angle) items in one run");
-    if (pairs > 0) {
-        hipLaunchKernelGGL(k_pair_prep, dim3(blocks_for(pairs)), dim3(kBlock), 0, st, pairs, n,
-                           as<int>(ctx->d_pair_anchor), as<int>(ctx->d_ids), as<double>(ctx->d_x),
-                           as<double>(ctx->d_y), as<double>(ctx->d_trig), d.close_thr,
-                           as<unsigned char>(ctx->d_cls), as<double>(ctx->d_prec));
+    if (pairs > 0 && !pair_setup_queued) {
+        const PrepArgs P{as<double>(ctx->d_x), as<double>(ctx->d_y), as<double>(ctx->d_trig), d.close_thr,
+                         as<unsigned char>(ctx->d_cls), as<double>(ctx->d_prec)};
+        // Kt = 7 (the default grid): one kernel forms the pairs' records and
+        // sines; other grids run the record first
+        const bool fused_prep = d.kt == 7;
+        if (!fused_prep)
+            hipLaunchKernelGGL(k_pair_prep, dim3(blocks_for(pairs)), dim3(kBlock), 0, st, pairs, n,
+                               as<int>(ctx->d_pair_anchor), as<int>(ctx->d_ids), P);
         const size_t trig_lds = d.kt <= static_cast<unsigned>(kPairTrigMaxKt) ? kBlock * d.kt * sizeof(double2) : 0;
-        hipLaunchKernelGGL(d.kt == 7 ? k_pair_trig<7> : k_pair_trig<0>, dim3(blocks_for(pairs)), dim3(kBlock),
-                           trig_lds, st,
+        const auto trig_fn = fused_prep ? k_pair_trig<7, true> : k_pair_trig<0, false>;
+        hipLaunchKernelGGL(trig_fn, dim3(blocks_for(pairs)), dim3(kBlock), trig_lds, st,
                            pairs, n, static_cast<int>(d.kt), as<int>(ctx->d_pair_anchor), as<int>(ctx->d_ids),
-                           as<unsigned char>(ctx->d_cls), as<double>(ctx->d_th), T.off_t, d.dt,
-                           as<double>(ctx->d_tdd), as<double>(ctx->d_psin));
+                           static_cast<const unsigned char *>(as<unsigned char>(ctx->d_cls)), as<double>(ctx->d_th),
+                           p.st, d.dt, as<double>(ctx->d_tdd), as<double>(ctx->d_psin), P,
+                           static_cast<const unsigned *>(nullptr));
     }
     hipLaunchKernelGGL(k_bucket, dim3(blocks_for(n, kBucketBlock)), dim3(kBucketBlock), 0, st, n, as<int>(ctx->d_sorted),
                        as<unsigned char>(ctx->d_cls), as<unsigned>(ctx->d_off), nb, T.caps, as<int4>(ctx->d_lists),
@@ -3694,9 +3804,11 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     // longest lists first, so their long-latency work starts at once and
     // overlaps the bulk
     const int ring = static_cast<int>(ctx->fused_runs % cvg_context::kFusedRing);
-    CVG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
-    CVG_HIP(ctx, hipEventRecord(ctx->fused_ev[ring][0], st));
-    for (int i = 0; i < 3; ++i) CVG_HIP(ctx, hipStreamWaitEvent(ctx->aux[i], ctx->ev[3], 0));
+    CVG_TRY(record_timing(ctx, ctx->ev[3], st));
+    CVG_TRY(record_timing(ctx, ctx->fused_ev[ring][0], st));
+    const hipEvent_t fork = ctx->timed_run ? ctx->ev[3] : ctx->ev_fork;
+    if (!ctx->timed_run) CVG_HIP(ctx, hipEventRecord(fork, st));
+    for (int i = 0; i < 3; ++i) CVG_HIP(ctx, hipStreamWaitEvent(ctx->aux[i], fork, 0));
     bool aux_used[3] = {false, false, false};
     int rr = 0;
     GateArgs gate{};
@@ -3730,9 +3842,9 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
         CVG_HIP(ctx, hipEventRecord(ctx->ev_join[i], ctx->aux[i]));
         CVG_HIP(ctx, hipStreamWaitEvent(st, ctx->ev_join[i], 0));
     }
-    CVG_HIP(ctx, hipEventRecord(ctx->ev[4], st));
-    CVG_HIP(ctx, hipEventRecord(ctx->fused_ev[ring][1], st));
-    ++ctx->fused_runs;
+    CVG_TRY(record_timing(ctx, ctx->ev[4], st));
+    CVG_TRY(record_timing(ctx, ctx->fused_ev[ring][1], st));
+    if (ctx->timed_run) ++ctx->fused_runs;
     // every anchor is in exactly one bucket, so every acc_count entry is written
     RecordArgs R{};
     R.G = static_cast<int>(d.G);
@@ -3758,20 +3870,23 @@ int check_ctx(cvg_context *ctx)
 
 void fill_stats(cvg_context *ctx, cvg_run_stats *stats);
 
+// timed: record the timing events although stats is null (the caller fills
+// stats itself later)
 int run_common(cvg_context *ctx, int n, int sz, const double *d_edges, const cvg_params *p, cvg_run_stats *stats,
-               int num_maps = 1, const long long *eoff = nullptr, bool f32 = false)
+               int num_maps = 1, const long long *eoff = nullptr, bool f32 = false, bool timed = false)
 {
     ctx->f32 = f32;
-    CVG_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    ctx->timed_run = timed || stats || ctx->event_timing;
+    CVG_TRY(record_timing(ctx, ctx->ev[0], ctx->stream));
     ctx->use_cellrank = false;
     ctx->edge_trig_queued = false;
     ctx->early_params = (!f32 && p->group_max_sz <= 64 && !ctx->force_generic) ? p : nullptr;
     const int rc = build_graph(ctx, n, sz, d_edges, p->nrad, 3u, p->max_candidates, num_maps, eoff);  // main.cpp:48: 3
     ctx->early_params = nullptr;
     if (rc != CVG_OK) return rc;
-    CVG_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    CVG_TRY(record_timing(ctx, ctx->ev[1], ctx->stream));
     CVG_TRY(build_curvelets(ctx, *p));
-    CVG_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+    CVG_TRY(record_timing(ctx, ctx->ev[2], ctx->stream));
     // without stats the call returns with the work queued: the counts come
     // back with the stream and are read at the first query of a result
     if (!stats) return CVG_OK;
@@ -4019,6 +4134,8 @@ int cvg_create(int device, cvg_context **out_ctx)
         if (rc != CVG_OK) break;
         for (auto &e : ctx->ev)
             if (hipEventCreate(&e) != hipSuccess) rc = CVG_ERR_DEVICE;
+        for (hipEvent_t *e : {&ctx->ev_read, &ctx->ev_fork, &ctx->ev_end})
+            if (rc == CVG_OK && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) rc = CVG_ERR_DEVICE;
         for (auto &pr : ctx->fused_ev)
             for (auto &e : pr)
                 if (rc == CVG_OK && hipEventCreate(&e) != hipSuccess) rc = CVG_ERR_DEVICE;
@@ -4070,6 +4187,8 @@ void cvg_destroy(cvg_context *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->for_each_pool([](DevBuf &b) { release(b); });
     for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {ctx->ev_read, ctx->ev_fork, ctx->ev_end})
         if (e) (void)hipEventDestroy(e);
     for (auto &pr : ctx->fused_ev)
         for (auto &e : pr)
@@ -4167,6 +4286,7 @@ int cvg_form_pairwise_bundles(cvg_context *ctx, const cvg_params *params, unsign
 {
     CVG_TRY(check_ctx(ctx));
     if (!params) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "null params");
+    ctx->timed_run = ctx->event_timing;
     CVG_TRY(build_curvelets(ctx, *params));
     CVG_TRY(finish_curvelets(ctx));
     if (valid_pairs) *valid_pairs = ctx->valid_pairs;
@@ -4178,6 +4298,7 @@ int cvg_build_curvelets_greedy(cvg_context *ctx, const cvg_params *params, unsig
     CVG_TRY(check_ctx(ctx));
     if (!params) return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "null params");
     if (!(ctx->curvelets_ready && same_params(ctx->cur_params, *params))) {
+        ctx->timed_run = ctx->event_timing;
         CVG_TRY(build_curvelets(ctx, *params));
     }
     CVG_TRY(finish_curvelets(ctx));
@@ -4244,7 +4365,10 @@ int cvg_run_to_host(cvg_context *ctx, int num_edges, int sz_edge_data, const dou
         return fail(ctx, CVG_ERR_INVALID_ARGUMENT, "cvg_run_to_host: bad arguments");
     const double *d = nullptr;
     CVG_TRY(upload(ctx, num_edges, sz_edge_data, edges, &d));
-    CVG_TRY(run_common(ctx, num_edges, sz_edge_data, d, params, nullptr));
+    CVG_TRY(run_common(ctx, num_edges, sz_edge_data, d, params, nullptr, 1, nullptr, false, stats != nullptr));
+    // the host waits for the counts (the end of the run's kernels) while the
+    // early copies below still run
+    if (!ctx->timed_run) CVG_HIP(ctx, hipEventRecord(ctx->ev_end, ctx->stream));
     // As many rows as the last call had are copied as soon as the record
     // ends, before the count comes back: a map with no more rows (a
     // repeated or similar map) then needs no further round trip.  Only
@@ -4262,7 +4386,7 @@ int cvg_run_to_host(cvg_context *ctx, int num_edges, int sz_edge_data, const dou
                                     hipMemcpyDeviceToHost, ctx->stream));
     }
     const unsigned long long rebuilds = ctx->rebuilds;
-    CVG_HIP(ctx, hipEventSynchronize(ctx->ev[2]));
+    CVG_HIP(ctx, hipEventSynchronize(end_event(ctx)));
     CVG_TRY(finish_curvelets(ctx));
     if (ctx->rebuilds != rebuilds) early = 0;  // built again after the early copy
     const unsigned rows = ctx->num_curvelets;
@@ -4561,6 +4685,13 @@ int cvg_recent_fused_ms(cvg_context *ctx, int max, float *ms, int *count)
         ms[i] = t;
     }
     *count = k;
+    return CVG_OK;
+}
+
+int cvg_set_event_timing(cvg_context *ctx, int on)
+{
+    CVG_TRY(check_ctx(ctx));
+    ctx->event_timing = on != 0;
     return CVG_OK;
 }
 

@@ -229,11 +229,18 @@ void cvg_host_free(void *ptr);
 int cvg_get_stream(cvg_context *ctx, void **hip_stream);
 int cvg_synchronize(cvg_context *ctx);
 /* Durations in ms (HIP events on the context stream around the fused kernel's
- * launches, k_curvelets) of the most recent runs, oldest first: up to
- * min(max, 64) values, *count = how many.  Waits for the stream.  No
+ * launches, k_curvelets) of the most recent timed runs, oldest first: up to
+ * min(max, 64) values, *count = how many.  Waits for the stream.  A run is
+ * timed when it returns stats, or while cvg_set_event_timing is on.  No
  * counterpart in the reference: it times the device work of runs made
  * without stats. */
 int cvg_recent_fused_ms(cvg_context *ctx, int max, float *ms, int *count);
+/* on != 0: runs made without stats record the HIP timing events as well (the
+ * stage and fused-kernel times that cvg_recent_fused_ms reads).  Off at
+ * creation: each timing event is a marker that holds the stream for ~6 us,
+ * five of them per run.  Runs that return stats always record them.  No
+ * counterpart in the reference (its timers are host-side). */
+int cvg_set_event_timing(cvg_context *ctx, int on);
 
 /* ---- batched form ---------------------------------------------------------
  * Several maps in one pass: every kernel of the path runs once over all of

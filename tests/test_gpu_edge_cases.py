@@ -321,14 +321,27 @@ def test_runs_without_stats_return_queued_and_match(ctx):
     db = da + a.size * 8
     st = ctx.run_device(da, len(a), a.shape[1], p)
     ca, ia = ctx.curvelets()
+    timed = ctx.recent_fused_ms(64)  # ends with the run above (runs with stats are timed)
     for _ in range(3):
         assert ctx.run_device(db, len(b), b.shape[1], p, stats=False) is None
         assert ctx.run_device(da, len(a), a.shape[1], p, stats=False) is None
     ca2, ia2 = ctx.curvelets()
     assert ca2.shape[0] == st["num_curvelets"]
     assert np.array_equal(ca, ca2) and np.array_equal(ia.view(np.uint64), ia2.view(np.uint64))
+    # runs without stats record no timing events unless asked to
+    assert ctx.recent_fused_ms(64) == timed
+    ctx.set_event_timing(True)
+    try:
+        for _ in range(3):
+            assert ctx.run_device(db, len(b), b.shape[1], p, stats=False) is None
+            assert ctx.run_device(da, len(a), a.shape[1], p, stats=False) is None
+        ca3, ia3 = ctx.curvelets()
+    finally:
+        ctx.set_event_timing(False)
+    assert np.array_equal(ca, ca3) and np.array_equal(ia.view(np.uint64), ia3.view(np.uint64))
     ms = ctx.recent_fused_ms(6)
     assert len(ms) == 6 and all(m > 0 for m in ms)
+    assert ctx.recent_fused_ms(64)[-7] == timed[-1]
 
 
 def test_downloads_into_device_memory(ctx):
