@@ -14,7 +14,11 @@
 //           scan              CSR offsets
 //           k_fill_csr        per anchor (a 16-lane segment): gather in the
 //                             reference's scan order, distance sort
-//                             (tie-exact std::sort replay), CSR
+//                             (tie-exact std::sort replay), CSR.  On a
+//                             guessed grid and buffer sizes (a map after
+//                             another) it counts the lists itself and finds
+//                             the offsets by a decoupled look-back, in place
+//                             of the counting pass and the scan
 //   curvelets k_edge_trig     per edge: sin/cos of theta, theta+-dt, theta+grid
 //           k_pair_prep       per pair: direction-filter class, close/branch
 //                             flags, the bundle's cell-independent terms
@@ -217,6 +221,14 @@ struct cvg_context {
     // diagnostics
     DevBuf d_dbg;
 
+    // look-back states of the fill's single-pass offsets (speculative graphs):
+    // cleared when allocated (while d_fillstat is fillstat_zero_ptr), then
+    // told apart by the launch's epoch
+    DevBuf d_fillstat;
+    const void *fillstat_zero_ptr = nullptr;
+    size_t fillstat_zero_bytes = 0;
+    unsigned fill_epoch = 0;
+
     // this device's copy of the static scratch array (cvg_run_device_noalloc);
     // while `arena` is set, ensure() carves buffers out of it instead of
     // allocating
@@ -238,7 +250,7 @@ struct cvg_context {
                           &d_tdd,      &d_cls,    &d_prec,   &d_psin,   &d_lists,  &d_tables,      &d_st_ids,
                           &d_st_meta,  &d_st_k,   &d_st_rank, &d_acc,   &d_rowoff, &d_rowslot, &d_gbundle, &d_chains,
                           &d_info,     &d_dbg,    &d_map,    &d_eoff,   &d_mapbox, &d_maprow,       &d_cellrank,
-                          &d_skey,     &d_sorttmp};
+                          &d_skey,     &d_sorttmp, &d_fillstat};
         for (DevBuf *b : bufs) f(*b);
     }
 };
@@ -1066,6 +1078,129 @@ __global__ __launch_bounds__(kBlock) void k_discover_count(int n, const double *
 }
 
 constexpr int kFillWaves = 4;  // waves per workgroup in k_fill_csr
+constexpr int kFillMaxAnchors = 32;  // anchors per k_fill_csr workgroup: (64 / G) x waves, G >= 8
+
+// Single-pass CSR offsets in k_fill_csr (decoupled look-back): each
+// workgroup publishes its anchors' degree sum, then its inclusive prefix, as
+// one 64-bit word: flag (1: own sums, 2: inclusive prefix) << 62 | epoch << 45
+// | largest degree << 31 | degree sum.  The epoch, one per launch, makes the
+// words of earlier launches read as not ready, so the array is cleared only
+// when it is allocated and when the epoch wraps.  The words are read and
+// written with relaxed device-scope atomics: one word carries all a reader
+// needs, so no fence (which would write back or invalidate the whole L2 of
+// the XCD) is required.  Sums saturate at 2^31 - 1, above any pair capacity,
+// and degrees at 2^14 - 1, above any LDS list capacity (both raise S_MISS).
+constexpr unsigned kFillEpochBits = 17;
+constexpr unsigned kFillSumMax = 0x7fffffffu;
+constexpr unsigned kFillDegMax = 0x3fffu;
+// polls of one predecessor's state before a workgroup gives up and raises
+// S_MISS (the host then builds the graph on the counting path)
+constexpr unsigned kLookbackSpins = 1u << 20;
+
+__device__ inline unsigned long long fill_word(unsigned flag, unsigned epoch, unsigned mx, unsigned long long sum)
+{
+    return static_cast<unsigned long long>(flag) << 62 | static_cast<unsigned long long>(epoch) << 45 |
+           static_cast<unsigned long long>(min(mx, kFillDegMax)) << 31 |
+           min(sum, static_cast<unsigned long long>(kFillSumMax));
+}
+
+__device__ inline unsigned long long wave_sum_u64(unsigned long long v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned lo = __shfl_xor(static_cast<unsigned>(v), o, 64);
+        const unsigned hi = __shfl_xor(static_cast<unsigned>(v >> 32), o, 64);
+        v += (static_cast<unsigned long long>(hi) << 32) | lo;
+    }
+    return v;
+}
+
+__device__ inline unsigned wave_max_u32(unsigned v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, static_cast<unsigned>(__shfl_xor(v, o, 64)));
+    return v;
+}
+
+// Workgroup wg's exclusive prefix over its predecessors (degree sum and
+// largest degree), by one whole wave, after lane 0 has published the
+// workgroup's own sums (flag 1): it reads the words of 64 predecessors at a
+// time (lane l: wg - 1 - l - 64 k), adding own sums back to the nearest
+// inclusive prefix, then publishes the inclusive prefix.  Every predecessor
+// was dispatched before wg, so every wait ends; a wait that does not end
+// within kLookbackSpins polls raises S_MISS instead of hanging.
+__device__ inline void fill_lookback(unsigned long long *st, unsigned epoch, int wg, unsigned agg, unsigned agg_max,
+                                     unsigned *scal, unsigned &excl, unsigned &excl_max)
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned tag_a = 1u << kFillEpochBits | epoch, tag_p = 2u << kFillEpochBits | epoch;
+    unsigned long long sum = 0;
+    unsigned mx = 0;
+    bool miss = false;
+    for (int base = wg - 1; base >= 0; base -= 64) {
+        const int j = base - lane;
+        unsigned tag = tag_p;  // before the first workgroup: an empty inclusive prefix
+        unsigned long long w = 0;
+        if (j >= 0) {
+            w = __hip_atomic_load(&st[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            unsigned spins = 0;
+            while (static_cast<unsigned>(w >> 45) != tag_a && static_cast<unsigned>(w >> 45) != tag_p) {
+                if (++spins > kLookbackSpins) {
+                    miss = true;
+                    w = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                w = __hip_atomic_load(&st[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            tag = miss ? tag_p : static_cast<unsigned>(w >> 45);
+        }
+        const unsigned long long pm = __ballot(tag == tag_p);
+        const int stop = pm != 0ULL ? __ffsll(static_cast<long long>(pm)) - 1 : 64;  // nearest inclusive prefix
+        const bool take = lane <= stop;
+        sum += wave_sum_u64(take ? (w & kFillSumMax) : 0ULL);
+        mx = max(mx, wave_max_u32(take ? static_cast<unsigned>(w >> 31) & kFillDegMax : 0u));
+        if (pm != 0ULL) break;
+    }
+    if (__ballot(miss) != 0ULL && lane == 0) atomicOr(&scal[S_MISS], 1u);
+    excl = static_cast<unsigned>(min(sum, static_cast<unsigned long long>(kFillSumMax)));
+    excl_max = mx;
+    if (lane == 0)
+        __hip_atomic_store(&st[wg], fill_word(2u, epoch, max(mx, agg_max), sum + agg), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Rank of list entry i (distance d, cell c) among the segment's dg entries
+// in LDS: by d2, ties by scan position, which is the insertion-sort order
+// std::sort yields up to 16 entries.  Also flags exact ties, two entries of
+// one cell, and exact ties within one cell.  Entries j < dmax lie inside the
+// segment's cap-sized area, so they are read unconditionally (branch-free,
+// unrollable) and masked.
+__device__ inline int list_rank(const double *ld, const int *lcell, int dg, int dmax, int i, bool active, double di,
+                                int celli, bool &tie, bool &shared, bool &ambiguous)
+{
+    int rank = 0;
+#pragma unroll 4
+    for (int j = 0; j < dmax; ++j) {
+        const double dj = ld[j];
+        const int cellj = lcell[j];
+        const bool in = j < dg;
+        rank += (in && (dj < di || (dj == di && j < i))) ? 1 : 0;
+        const bool eq = in && active && dj == di && j != i;
+        const bool same = in && active && cellj == celli && j != i;
+        tie = tie || eq;
+        shared = shared || same;
+        ambiguous = ambiguous || (eq && same);
+    }
+    return rank;
+}
+
+// LDS doubles per k_fill_csr segment of list capacity cap: d2 (8 B), id and
+// cell (4 B each), and with the counting variant the sorted order (4 B).
+__host__ __device__ inline int fill_seg_doubles(int cap, bool count)
+{
+    return count ? (cap * 5 + 1) / 2 : cap * 2;
+}
 
 // discover_neighbors_csr gathering pass + the distance sort (preprocess.hpp:141-183)
 // + the CSR fill (:240-262), one G-lane segment per anchor.  The segment walks
@@ -1076,35 +1211,61 @@ constexpr int kFillWaves = 4;  // waves per workgroup in k_fill_csr
 // position, which is the insertion-sort order std::sort yields up to 16
 // entries); lists with ties and more than 16 entries get their scan-order
 // copy in `cand` and are queued for the std::sort replay (k_tie_sort).
-template <int G, class Cells>
+// COUNT: no counting pass ran.  Each segment counts its in-radius candidates
+// over the whole window (the degree is the count capped at max_candidates,
+// preprocess.hpp:165), the workgroup finds its anchors' offsets with the
+// look-back above and writes off[] (and off[n]), and the last workgroup
+// writes the pair count, the largest degree and the anchors at
+// max_candidates to the scalars (what the offsets scan gives otherwise).
+template <int G, class Cells, bool COUNT>
 __global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const double *__restrict__ x,
                                                               const double *__restrict__ y, const int *__restrict__ cx,
                                                               const int *__restrict__ cy, Cells cells, int R,
                                                               double rad2, const int *__restrict__ sorted,
                                                               const int *__restrict__ scell,
                                                               const double2 *__restrict__ sxy,
-                                                              const unsigned *__restrict__ off, Cand *__restrict__ cand,
+                                                              unsigned *__restrict__ off, Cand *__restrict__ cand,
                                                               int *__restrict__ ids, double *__restrict__ d2,
                                                               int *__restrict__ pair_anchor, int *__restrict__ tie_list,
                                                               unsigned max_cand, int cap,
                                                               const int *__restrict__ map_of,
                                                               const long long *__restrict__ map_eoff,
                                                               unsigned *__restrict__ scal, bool f32,
-                                                              unsigned long long pair_cap)
+                                                              unsigned long long pair_cap,
+                                                              unsigned long long *__restrict__ lb, unsigned epoch)
 {
-    // a guess that did not hold (grid or sizes): the build is repeated
-    if (scal[S_MISS]) return;
-    // per segment: cap candidates (d2, id, cell) in LDS for the ranking
+    __shared__ unsigned s_deg[COUNT ? kFillMaxAnchors : 1];
+    __shared__ unsigned s_off[COUNT ? kFillMaxAnchors : 1];
+    __shared__ unsigned s_flag[1];
+    if constexpr (COUNT) {
+        // one reading per workgroup: its waves must agree (block barriers follow)
+        if (threadIdx.x == 0) s_flag[0] = scal[S_MISS];
+        __syncthreads();
+        if (s_flag[0]) {
+            // a guess that did not hold: publish an empty prefix so that no
+            // successor waits, and do nothing else
+            if (threadIdx.x == 0)
+                __hip_atomic_store(&lb[blockIdx.x], fill_word(2u, epoch, 0u, 0ULL), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+    } else {
+        // a guess that did not hold (grid or sizes): the build is repeated
+        if (scal[S_MISS]) return;
+    }
+    // per segment: cap candidates (d2, id, cell; COUNT: and the sorted order)
+    // in LDS for the ranking
     extern __shared__ double fill_lds[];
     const int lane = threadIdx.x & 63;
     const int sl = lane & (G - 1);
     const int seg = static_cast<int>(threadIdx.x) / G;
-    double *ld = fill_lds + static_cast<size_t>(seg) * cap * 2;
+    double *ld = fill_lds + static_cast<size_t>(seg) * fill_seg_doubles(cap, COUNT);
     int *lid = reinterpret_cast<int *>(ld + cap);
     int *lcell = lid + cap;
+    int *lord = lcell + cap;  // COUNT only: lord[rank] = scan position
     const int a = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x / G) + seg;
     const bool valid = a < n;
-    if (__ballot(valid) == 0ULL) return;  // wave-uniform
+    if (!COUNT && __ballot(valid) == 0ULL) return;  // wave-uniform (COUNT: every wave reaches the barriers)
     const unsigned long long segmask = segment_mask<G>(lane);
     const unsigned long long lt_mask = (1ULL << lane) - 1ULL;
     double ax = 0.0, ay = 0.0;
@@ -1115,15 +1276,21 @@ __global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const doubl
         ay = y[a];
         ix = cx[a];
         iy = cy[a];
-        o = off[a];
-        dg = static_cast<int>(off[a + 1] - o);
-        // buffers sized by a guess (the last map's): a list beyond them is
-        // skipped and S_MISS raised, and the host builds the graph again
-        if (dg > cap || static_cast<unsigned long long>(o) + static_cast<unsigned>(dg) > pair_cap) {
-            if (sl == 0) atomicOr(&scal[S_MISS], 1u);
-            dg = 0;
+        if (!COUNT) {
+            o = off[a];
+            dg = static_cast<int>(off[a + 1] - o);
+            // buffers sized by a guess (the last map's): a list beyond them is
+            // skipped and S_MISS raised, and the host builds the graph again
+            if (dg > cap || static_cast<unsigned long long>(o) + static_cast<unsigned>(dg) > pair_cap) {
+                if (sl == 0) atomicOr(&scal[S_MISS], 1u);
+                dg = 0;
+            }
         }
     }
+    // the cut position: the degree when it is known, max_candidates when the
+    // segment counts (a cut happens exactly when the count exceeds it)
+    const int lim = COUNT ? static_cast<int>(min(max_cand, static_cast<unsigned>(INT_MAX))) : dg;
+    const int keep = COUNT ? min(lim, cap) : dg;  // candidates stored in LDS
     int kept = 0;
     int last_cell = -1;  // cell of the last kept candidate
     bool ambiguous = false;
@@ -1147,7 +1314,7 @@ __global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const doubl
             }
             const unsigned long long m = __ballot(pass) & segmask;
             const int pos = kept + __popcll(m & lt_mask);
-            if (pass && pos < dg) {
+            if (pass && pos < keep) {
                 ld[pos] = d;
                 lid[pos] = b;
                 lcell[pos] = cell;
@@ -1155,22 +1322,115 @@ __global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const doubl
             // a max_candidates cut inside one cell makes the kept set depend on
             // the within-cell order: compare the first dropped candidate's cell
             // with the last kept one's
-            const unsigned long long bm = __ballot(pass && pos == dg) & segmask;
-            const unsigned long long pm = __ballot(pass && pos == dg - 1) & segmask;
+            const unsigned long long bm = __ballot(pass && pos == lim) & segmask;
+            const unsigned long long pm = __ballot(pass && pos == lim - 1) & segmask;
             const int bcell = __shfl(cell, bm != 0ULL ? __ffsll(static_cast<long long>(bm)) - 1 : lane);
             const int pcell = __shfl(cell, pm != 0ULL ? __ffsll(static_cast<long long>(pm)) - 1 : lane);
             const int hcell = __shfl(cell, m != 0ULL ? 63 - __clzll(static_cast<long long>(m)) : lane);
             if (bm != 0ULL) {
                 const int prev = pm != 0ULL ? pcell : last_cell;
-                if (dg > 0 && prev == bcell) ambiguous = true;
+                if (lim > 0 && prev == bcell) ambiguous = true;
                 done = true;
             } else if (m != 0ULL) {
-                if (kept + __popcll(m) <= dg) last_cell = hcell;
+                if (kept + __popcll(m) <= lim) last_cell = hcell;
             }
             kept += __popcll(m);
             // without a cut (dg < max_candidates) nothing passes after the dg-th
-            if (kept >= dg && static_cast<unsigned>(dg) < max_cand) done = true;
+            if (!COUNT && kept >= dg && static_cast<unsigned>(dg) < max_cand) done = true;
         }
+    }
+    if constexpr (COUNT) {
+        // The degrees; the workgroup's own sums published at once, the ranks
+        // computed while its predecessors finish, then the offsets
+        dg = valid ? min(kept, lim) : 0;
+        const int nseg = static_cast<int>(blockDim.x) / G;
+        if (sl == 0) s_deg[seg] = static_cast<unsigned>(dg);
+        __syncthreads();
+        unsigned agg = 0, agg_max = 0;
+        if (threadIdx.x < 64) {
+            unsigned agg_trunc = 0;
+            if (lane == 0) {
+                for (int s2 = 0; s2 < nseg; ++s2) {
+                    const unsigned v = s_deg[s2];
+                    s_off[s2] = agg;
+                    agg += v;
+                    agg_max = max(agg_max, v);
+                    agg_trunc += v >= max_cand ? 1u : 0u;  // preprocess.hpp:224, a >= test
+                }
+                if (blockIdx.x > 0)
+                    __hip_atomic_store(&lb[blockIdx.x], fill_word(1u, epoch, agg_max, agg), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                // anchors at max_candidates: rare, one atomic per workgroup that has any
+                if (agg_trunc > 0) atomicAdd(&scal[S_TRUNC], agg_trunc);
+            }
+            agg = bcast(static_cast<int>(agg), 0);
+            agg_max = bcast(static_cast<int>(agg_max), 0);
+        }
+        // a list longer than the LDS capacity was not kept whole: the guess
+        // did not hold
+        const bool fits = dg <= cap;
+        if (valid && !fits && sl == 0) atomicOr(&scal[S_MISS], 1u);
+        const int dr = fits ? dg : 0;
+        wave_sync();
+        bool tie = false, shared = false;
+        const int dmax = wave_max_over_segments<G>(dr);
+        for (int i0 = 0; i0 < dmax; i0 += G) {
+            const int i = i0 + sl;
+            const bool active = i < dr;
+            double di = 0.0;
+            int celli = -1;
+            if (active) {
+                di = ld[i];
+                celli = lcell[i];
+            }
+            const int rank = list_rank(ld, lcell, dr, dmax, i, active, di, celli, tie, shared, ambiguous);
+            if (active) lord[rank] = i;
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            unsigned excl, excl_max;
+            fill_lookback(lb, epoch, static_cast<int>(blockIdx.x), agg, agg_max, scal, excl, excl_max);
+            if (lane == 0) {
+                for (int s2 = 0; s2 < nseg; ++s2) {
+                    const unsigned long long v = static_cast<unsigned long long>(excl) + s_off[s2];
+                    s_off[s2] = static_cast<unsigned>(min(v, static_cast<unsigned long long>(kFillSumMax)));
+                }
+                if (blockIdx.x == gridDim.x - 1) {
+                    const unsigned long long t = static_cast<unsigned long long>(excl) + agg;
+                    const unsigned total = static_cast<unsigned>(min(t, static_cast<unsigned long long>(kFillSumMax)));
+                    off[n] = total;
+                    scal[S_PAIRS] = total;
+                    scal[S_MAXDEG] = max(excl_max, agg_max);
+                }
+            }
+        }
+        __syncthreads();
+        o = s_off[seg];
+        if (valid && sl == 0) off[a] = o;
+        // buffers sized by a guess (the last map's): a list beyond them is
+        // skipped and S_MISS raised, and the host builds the graph again
+        const bool room = static_cast<unsigned long long>(o) + static_cast<unsigned>(dg) <= pair_cap;
+        if (valid && !room && sl == 0) atomicOr(&scal[S_MISS], 1u);
+        const int dw = room ? dr : 0;
+        for (int r = sl; r < dw; r += G) {
+            const int i = lord[r];
+            ids[o + r] = lid[i];
+            d2[o + r] = ld[i];
+            pair_anchor[o + r] = a;
+        }
+        const bool any_tie = (__ballot(tie) & segmask) != 0ULL;
+        const bool any_shared = (__ballot(shared) & segmask) != 0ULL;
+        const bool any_amb = (__ballot(ambiguous) & segmask) != 0ULL;
+        if (any_tie && dw > 16) {
+            for (int i = sl; i < dw; i += G) cand[o + i] = Cand{ld[i], lid[i]};  // scan order
+            if (sl == 0) tie_list[atomicAdd(&scal[S_TIES], 1u)] = a;
+        }
+        // (the within-cell order cases of the counting path below)
+        if (valid && sl == 0 && (any_amb || (dw > 16 && any_tie && any_shared))) {
+            const long long map_n = map_of ? map_eoff[map_of[a] + 1] - map_eoff[map_of[a]] : n;
+            if (map_n > 16) atomicAdd(&scal[S_AMBIG], 1u);
+        }
+        return;
     }
     wave_sync();
     // rank sort by d2 with ties by scan position; detect ties, same-cell ties
@@ -1188,21 +1448,7 @@ __global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const doubl
             idi = lid[i];
             celli = lcell[i];
         }
-        // entries j < dmax lie inside the segment's cap-sized area, so they
-        // are read unconditionally (branch-free, unrollable) and masked
-        int rank = 0;
-#pragma unroll 4
-        for (int j = 0; j < dmax; ++j) {
-            const double dj = ld[j];
-            const int cellj = lcell[j];
-            const bool in = j < dg;
-            rank += (in && (dj < di || (dj == di && j < i))) ? 1 : 0;
-            const bool eq = in && active && dj == di && j != i;
-            const bool same = in && active && cellj == celli && j != i;
-            tie = tie || eq;
-            shared = shared || same;
-            ambiguous = ambiguous || (eq && same);
-        }
+        const int rank = list_rank(ld, lcell, dg, dmax, i, active, di, celli, tie, shared, ambiguous);
         if (active) {
             ids[o + rank] = idi;
             d2[o + rank] = di;
@@ -2792,9 +3038,13 @@ int graph_scans(cvg_context *ctx, const Cells &cells, int n, unsigned R, unsigne
     hipStream_t st = ctx->stream;
     unsigned *scal = as<unsigned>(ctx->d_scal);
     const size_t ni = static_cast<size_t>(n) * sizeof(int);
-    CVG_TRY(ensure(ctx, ctx->d_deg, ni));
     CVG_TRY(ensure(ctx, ctx->d_off, static_cast<size_t>(n + 1) * sizeof(unsigned)));
-    {
+    // on guessed buffers the fill counts the lists itself and finds the
+    // offsets in the same pass; otherwise a counting pass and the offsets
+    // scan size the buffers first
+    const bool count_in_fill = spec;
+    if (!count_in_fill) {
+        CVG_TRY(ensure(ctx, ctx->d_deg, ni));
         using CountFn = void (*)(int, const double *, const double *, const int *, const int *, Cells, int, double,
                                  unsigned, const int *, const double2 *, unsigned *, bool, const unsigned *);
         const int G = ctx->fill_lanes;
@@ -2806,10 +3056,10 @@ int graph_scans(cvg_context *ctx, const Cells &cells, int n, unsigned R, unsigne
                            as<double>(ctx->d_x), as<double>(ctx->d_y), as<int>(ctx->d_cx), as<int>(ctx->d_cy), cells,
                            static_cast<int>(R), rad2, max_cand, as<int>(ctx->d_sorted), as<double2>(ctx->d_sxy),
                            as<unsigned>(ctx->d_deg), ctx->f32, spec ? static_cast<const unsigned *>(scal + S_MISS) : nullptr);
+        CVG_HIP(ctx, hipGetLastError());
+        CVG_TRY(exclusive_scan(ctx, as<unsigned>(ctx->d_deg), n, as<unsigned>(ctx->d_off), ctx->d_tiles,
+                               scal + S_MAXDEG, scal + S_TRUNC, max_cand, scal + S_PAIRS));
     }
-    CVG_HIP(ctx, hipGetLastError());
-    CVG_TRY(exclusive_scan(ctx, as<unsigned>(ctx->d_deg), n, as<unsigned>(ctx->d_off), ctx->d_tiles,
-                           scal + S_MAXDEG, scal + S_TRUNC, max_cand, scal + S_PAIRS));
     auto take_counts = [&]() -> int {
         const unsigned pairs_u = ctx->mailbox[S_PAIRS];
         if (pairs_u > static_cast<unsigned>(INT_MAX))
@@ -2842,10 +3092,10 @@ int graph_scans(cvg_context *ctx, const Cells &cells, int n, unsigned R, unsigne
     // gather, sort, fill: one G-lane segment per anchor; each segment keeps
     // its candidates in LDS
     using FillFn = void (*)(int, const double *, const double *, const int *, const int *, Cells, int, double,
-                            const int *, const int *, const double2 *, const unsigned *, Cand *, int *, double *,
+                            const int *, const int *, const double2 *, unsigned *, Cand *, int *, double *,
                             int *, int *, unsigned, int, const int *, const long long *, unsigned *, bool,
-                            unsigned long long);
-    const size_t per_seg = static_cast<size_t>(fill_cap) * (2 * sizeof(double));
+                            unsigned long long, unsigned long long *, unsigned);
+    const size_t per_seg = static_cast<size_t>(fill_seg_doubles(fill_cap, count_in_fill)) * sizeof(double);
     int G = ctx->fill_lanes;
     // fewer anchors per wave when the lists are long
     while (G < 64 && per_seg * (64 / G) * kFillWaves > ctx->lds_limit) G *= 2;
@@ -2853,20 +3103,40 @@ int graph_scans(cvg_context *ctx, const Cells &cells, int n, unsigned R, unsigne
     while (fill_waves > 1 && per_seg * (64 / G) * fill_waves > ctx->lds_limit) fill_waves /= 2;
     if (per_seg * (64 / G) * fill_waves > ctx->lds_limit)
         return fail(ctx, CVG_ERR_UNSUPPORTED, "an anchor has too many neighbours for one workgroup's LDS");
-    const FillFn fill = G == 8    ? k_fill_csr<8, Cells>
-                        : G == 16 ? k_fill_csr<16, Cells>
-                        : G == 32 ? k_fill_csr<32, Cells>
-                                  : k_fill_csr<64, Cells>;
+    const FillFn fill = count_in_fill ? (G == 8    ? k_fill_csr<8, Cells, true>
+                                         : G == 16 ? k_fill_csr<16, Cells, true>
+                                         : G == 32 ? k_fill_csr<32, Cells, true>
+                                                   : k_fill_csr<64, Cells, true>)
+                                      : (G == 8    ? k_fill_csr<8, Cells, false>
+                                         : G == 16 ? k_fill_csr<16, Cells, false>
+                                         : G == 32 ? k_fill_csr<32, Cells, false>
+                                                   : k_fill_csr<64, Cells, false>);
     const int anchors_per_block = (64 / G) * fill_waves;
     const unsigned fill_shmem = static_cast<unsigned>(per_seg * anchors_per_block);
+    const int fill_blocks = (n + anchors_per_block - 1) / anchors_per_block;
+    unsigned long long *lb = nullptr;
+    unsigned epoch = 0;
+    if (count_in_fill) {
+        const size_t lb_bytes = static_cast<size_t>(fill_blocks) * sizeof(unsigned long long);
+        CVG_TRY(ensure(ctx, ctx->d_fillstat, lb_bytes));
+        if (++ctx->fill_epoch >= (1u << kFillEpochBits)) ctx->fill_epoch = 1;
+        if (ctx->d_fillstat.ptr != ctx->fillstat_zero_ptr || ctx->d_fillstat.bytes != ctx->fillstat_zero_bytes ||
+            ctx->fill_epoch == 1) {
+            CVG_HIP(ctx, hipMemsetAsync(ctx->d_fillstat.ptr, 0, ctx->d_fillstat.bytes, st));
+            ctx->fillstat_zero_ptr = ctx->d_fillstat.ptr;
+            ctx->fillstat_zero_bytes = ctx->d_fillstat.bytes;
+        }
+        lb = as<unsigned long long>(ctx->d_fillstat);
+        epoch = ctx->fill_epoch;
+    }
     CVG_TRY(prepare_launch(ctx, reinterpret_cast<const void *>(fill), 64 * fill_waves, fill_shmem, nullptr));
-    hipLaunchKernelGGL(fill, dim3((n + anchors_per_block - 1) / anchors_per_block), dim3(64 * fill_waves), fill_shmem,
+    hipLaunchKernelGGL(fill, dim3(fill_blocks), dim3(64 * fill_waves), fill_shmem,
                        st, n, as<double>(ctx->d_x), as<double>(ctx->d_y), as<int>(ctx->d_cx), as<int>(ctx->d_cy),
                        cells, static_cast<int>(R), rad2, as<int>(ctx->d_sorted), as<int>(ctx->d_scell),
                        as<double2>(ctx->d_sxy), as<unsigned>(ctx->d_off), as<Cand>(ctx->d_cand),
                        as<int>(ctx->d_ids), as<double>(ctx->d_d2), as<int>(ctx->d_pair_anchor),
                        as<int>(ctx->d_tmp_ids), max_cand, fill_cap, num_maps > 1 ? as<int>(ctx->d_map) : nullptr,
-                       as<long long>(ctx->d_eoff), scal, ctx->f32, pair_cap);
+                       as<long long>(ctx->d_eoff), scal, ctx->f32, pair_cap, lb, epoch);
     hipLaunchKernelGGL(k_tie_sort, dim3(std::min(blocks_for(n), 64)), dim3(kBlock), 0, st, as<int>(ctx->d_tmp_ids),
                        scal, as<unsigned>(ctx->d_off), as<Cand>(ctx->d_cand), as<int>(ctx->d_ids),
                        as<double>(ctx->d_d2));

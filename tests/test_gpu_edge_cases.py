@@ -12,7 +12,7 @@ mirror of the reference API.
 import numpy as np
 import pytest
 
-from curvelet_construction_on_gpu_amd import (CurveletError, CurveletGPU, CurveletParams, NeighborGraph,
+from curvelet_construction_on_gpu_amd import (Context, CurveletError, CurveletGPU, CurveletParams, NeighborGraph,
                                               build_neighbor_csr_graph, dt_radians)
 from tests._inputs import synthetic
 
@@ -133,6 +133,25 @@ def test_synthetic_cut_inside_cells(ctx, oracle):
     _, g, want = check_same(ctx, oracle, e, p)
     assert g.truncated_anchors == want["csr"]["truncated"] > 0
     assert g.order_ambiguous_anchors > 0
+
+
+@pytest.mark.parametrize("case", ["lattice cut", "cut inside cells", "C2", "duplicates"])
+def test_counting_fill_on_repeated_maps(oracle, case):
+    """A map run again on its context takes the guessed grid and buffer sizes,
+    where the CSR fill counts the lists and finds the offsets itself (no
+    counting pass): the same bits, pair count, largest degree and
+    max_candidates count as the oracle, on both runs."""
+    d = synthetic("C1", 1001)[:800].copy()
+    e, p = {"lattice cut": (lattice(30, 30, seed=4), CurveletParams(max_candidates=5)),
+            "cut inside cells": (synthetic("C1", 1001), CurveletParams(max_candidates=6)),
+            "C2": (synthetic("C2", 1002), CurveletParams(group_max_sz=7)),
+            "duplicates": (np.concatenate([d, d[::7]]), CurveletParams(group_max_sz=7))}[case]
+    with Context(0) as c:
+        for k in range(3):
+            print(f"{case}: run {k}", flush=True)
+            _, g, want = check_same(c, oracle, e, p)
+            assert g.truncated_anchors == want["csr"]["truncated"], k
+            assert g.max_num_of_neighbors == int(np.diff(want["csr"]["offsets"]).max()), k
 
 
 def test_duplicates_exact(ctx, oracle):
