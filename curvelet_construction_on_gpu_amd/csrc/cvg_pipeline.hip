@@ -82,6 +82,11 @@ static __device__ __align__(256) unsigned char scratch[static_cast<size_t>(CVG_N
 namespace {
 
 constexpr int kBlock = 256;
+// k_fill_csr's look-back: polls of one predecessor's state before a
+// workgroup gives up and raises S_MISS (the host then builds the graph on the
+// counting path); a context created with CVG_LOOKBACK_SPINS set uses that
+// limit instead (tests force the fallback with 0)
+constexpr unsigned kLookbackSpins = 1u << 20;
 constexpr int kInfoWidth = 10;
 constexpr long long kMaxGridCells = 1LL << 28;  // larger spans take the sorted-key cells
 constexpr double kMaxCoord = 2147475456.0;        // 2^31 - 2^13
@@ -225,6 +230,7 @@ struct cvg_context {
     // cleared when allocated (while d_fillstat is fillstat_zero_ptr), then
     // told apart by the launch's epoch
     DevBuf d_fillstat;
+    unsigned lookback_spins = kLookbackSpins;  // CVG_LOOKBACK_SPINS at creation
     const void *fillstat_zero_ptr = nullptr;
     size_t fillstat_zero_bytes = 0;
     unsigned fill_epoch = 0;
@@ -1093,9 +1099,6 @@ constexpr int kFillMaxAnchors = 32;  // anchors per k_fill_csr workgroup: (64 / 
 constexpr unsigned kFillEpochBits = 17;
 constexpr unsigned kFillSumMax = 0x7fffffffu;
 constexpr unsigned kFillDegMax = 0x3fffu;
-// polls of one predecessor's state before a workgroup gives up and raises
-// S_MISS (the host then builds the graph on the counting path)
-constexpr unsigned kLookbackSpins = 1u << 20;
 
 __device__ inline unsigned long long fill_word(unsigned flag, unsigned epoch, unsigned mx, unsigned long long sum)
 {
@@ -1128,9 +1131,9 @@ __device__ inline unsigned wave_max_u32(unsigned v)
 // time (lane l: wg - 1 - l - 64 k), adding own sums back to the nearest
 // inclusive prefix, then publishes the inclusive prefix.  Every predecessor
 // was dispatched before wg, so every wait ends; a wait that does not end
-// within kLookbackSpins polls raises S_MISS instead of hanging.
+// within max_spins polls raises S_MISS instead of hanging.
 __device__ inline void fill_lookback(unsigned long long *st, unsigned epoch, int wg, unsigned agg, unsigned agg_max,
-                                     unsigned *scal, unsigned &excl, unsigned &excl_max)
+                                     unsigned max_spins, unsigned *scal, unsigned &excl, unsigned &excl_max)
 {
     const int lane = threadIdx.x & 63;
     const unsigned tag_a = 1u << kFillEpochBits | epoch, tag_p = 2u << kFillEpochBits | epoch;
@@ -1145,7 +1148,7 @@ __device__ inline void fill_lookback(unsigned long long *st, unsigned epoch, int
             w = __hip_atomic_load(&st[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             unsigned spins = 0;
             while (static_cast<unsigned>(w >> 45) != tag_a && static_cast<unsigned>(w >> 45) != tag_p) {
-                if (++spins > kLookbackSpins) {
+                if (++spins > max_spins) {
                     miss = true;
                     w = 0;
                     break;
@@ -1232,7 +1235,8 @@ __global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const doubl
                                                               const long long *__restrict__ map_eoff,
                                                               unsigned *__restrict__ scal, bool f32,
                                                               unsigned long long pair_cap,
-                                                              unsigned long long *__restrict__ lb, unsigned epoch)
+                                                              unsigned long long *__restrict__ lb, unsigned epoch,
+                                                              unsigned max_spins)
 {
     __shared__ unsigned s_deg[COUNT ? kFillMaxAnchors : 1];
     __shared__ unsigned s_off[COUNT ? kFillMaxAnchors : 1];
@@ -1389,7 +1393,7 @@ __global__ __launch_bounds__(64 * kFillWaves) void k_fill_csr(int n, const doubl
         __syncthreads();
         if (threadIdx.x < 64) {
             unsigned excl, excl_max;
-            fill_lookback(lb, epoch, static_cast<int>(blockIdx.x), agg, agg_max, scal, excl, excl_max);
+            fill_lookback(lb, epoch, static_cast<int>(blockIdx.x), agg, agg_max, max_spins, scal, excl, excl_max);
             if (lane == 0) {
                 for (int s2 = 0; s2 < nseg; ++s2) {
                     const unsigned long long v = static_cast<unsigned long long>(excl) + s_off[s2];
@@ -3094,7 +3098,7 @@ int graph_scans(cvg_context *ctx, const Cells &cells, int n, unsigned R, unsigne
     using FillFn = void (*)(int, const double *, const double *, const int *, const int *, Cells, int, double,
                             const int *, const int *, const double2 *, unsigned *, Cand *, int *, double *,
                             int *, int *, unsigned, int, const int *, const long long *, unsigned *, bool,
-                            unsigned long long, unsigned long long *, unsigned);
+                            unsigned long long, unsigned long long *, unsigned, unsigned);
     const size_t per_seg = static_cast<size_t>(fill_seg_doubles(fill_cap, count_in_fill)) * sizeof(double);
     int G = ctx->fill_lanes;
     // fewer anchors per wave when the lists are long
@@ -3136,7 +3140,7 @@ int graph_scans(cvg_context *ctx, const Cells &cells, int n, unsigned R, unsigne
                        as<double2>(ctx->d_sxy), as<unsigned>(ctx->d_off), as<Cand>(ctx->d_cand),
                        as<int>(ctx->d_ids), as<double>(ctx->d_d2), as<int>(ctx->d_pair_anchor),
                        as<int>(ctx->d_tmp_ids), max_cand, fill_cap, num_maps > 1 ? as<int>(ctx->d_map) : nullptr,
-                       as<long long>(ctx->d_eoff), scal, ctx->f32, pair_cap, lb, epoch);
+                       as<long long>(ctx->d_eoff), scal, ctx->f32, pair_cap, lb, epoch, ctx->lookback_spins);
     hipLaunchKernelGGL(k_tie_sort, dim3(std::min(blocks_for(n), 64)), dim3(kBlock), 0, st, as<int>(ctx->d_tmp_ids),
                        scal, as<unsigned>(ctx->d_off), as<Cand>(ctx->d_cand), as<int>(ctx->d_ids),
                        as<double>(ctx->d_d2));
@@ -4441,6 +4445,8 @@ int cvg_create(int device, cvg_context **out_ctx)
         if (const char *v = std::getenv("CVG_SPARSE_CELLS")) ctx->force_sparse = std::atoi(v) != 0;
         if (const char *v = std::getenv("CVG_GENERIC_KERNEL")) ctx->force_generic = std::atoi(v) != 0;
         if (const char *v = std::getenv("CVG_SPECULATE")) ctx->spec_off = std::atoi(v) == 0;
+        if (const char *v = std::getenv("CVG_LOOKBACK_SPINS"))
+            ctx->lookback_spins = static_cast<unsigned>(std::min<long long>(std::max(0LL, std::atoll(v)), kLookbackSpins));
     } while (false);
     if (rc != CVG_OK) {
         cvg_destroy(ctx);

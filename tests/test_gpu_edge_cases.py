@@ -154,6 +154,20 @@ def test_counting_fill_on_repeated_maps(oracle, case):
             assert g.max_num_of_neighbors == int(np.diff(want["csr"]["offsets"]).max()), k
 
 
+def test_counting_fill_gives_up_to_the_counting_path(monkeypatch, oracle):
+    """The fill's look-back gives up after a bounded number of polls; with the
+    limit forced to none, workgroups whose predecessors are not ready yet
+    raise S_MISS and the graph is built again on the counting path: the same
+    bits as the oracle on every run, and no wait."""
+    monkeypatch.setenv("CVG_LOOKBACK_SPINS", "0")
+    e, p = synthetic("C2", 1002), CurveletParams(group_max_sz=7)
+    with Context(0) as c:
+        for k in range(3):
+            print(f"look-back limit 0: run {k}", flush=True)
+            _, g, want = check_same(c, oracle, e, p)
+            assert g.truncated_anchors == want["csr"]["truncated"], k
+
+
 def test_duplicates_exact(ctx, oracle):
     e = synthetic("C1", 1001)[:800].copy()
     e = np.concatenate([e, e[::7]])  # exact duplicates: same cell, equal distances
