@@ -155,10 +155,11 @@ def test_counting_fill_on_repeated_maps(oracle, case):
 
 
 def test_counting_fill_gives_up_to_the_counting_path(monkeypatch, oracle):
-    """The fill's look-back gives up after a bounded number of polls; with the
-    limit forced to none, workgroups whose predecessors are not ready yet
-    raise S_MISS and the graph is built again on the counting path: the same
-    bits as the oracle on every run, and no wait."""
+    """The look-backs give up after a bounded number of polls; with the limit
+    forced to none, fill workgroups whose predecessors are not ready yet raise
+    S_MISS and the graph is built again on the counting path, and row-offset
+    scan tiles sum their predecessors' inputs directly: the same bits as the
+    oracle on every run, and no wait."""
     monkeypatch.setenv("CVG_LOOKBACK_SPINS", "0")
     e, p = synthetic("C2", 1002), CurveletParams(group_max_sz=7)
     with Context(0) as c:
