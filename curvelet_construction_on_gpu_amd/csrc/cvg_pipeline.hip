@@ -1761,12 +1761,20 @@ __global__ __launch_bounds__(kBucketBlock) void k_bucket(int n, const int *__res
         a = sorted[p];
         o0 = off[a];
         o1 = off[a + 1];
+        // the anchor's run of flags, four per aligned 32-bit load (the flag
+        // buffer is padded to whole words)
         unsigned c[3] = {0u, 0u, 0u};
-        for (unsigned q = o0; q < o1; ++q) {
-            const unsigned k = pflag[q] & 3u;
-            c[0] += k == 0u ? 1u : 0u;
-            c[1] += k == 1u ? 1u : 0u;
-            c[2] += k == 2u ? 1u : 0u;
+        for (unsigned q4 = o0 & ~3u; q4 < o1; q4 += 4) {
+            const unsigned w = *reinterpret_cast<const unsigned *>(pflag + q4);
+#pragma unroll
+            for (unsigned k = 0; k < 4; ++k) {
+                const unsigned q = q4 + k;
+                const unsigned f = (w >> (8 * k)) & 3u;
+                const bool in = q >= o0 && q < o1;
+                c[0] += in && f == 0u ? 1u : 0u;
+                c[1] += in && f == 1u ? 1u : 0u;
+                c[2] += in && f == 2u ? 1u : 0u;
+            }
         }
         const unsigned m = max(c[1], c[2]) + c[0];
         b = 0;
@@ -3156,6 +3164,12 @@ constexpr int kSpecMiss = 1000;  // internal: a guessed size did not hold (never
 
 int queue_edge_trig(cvg_context *ctx, const cvg_params &p);
 int queue_pair_setup(cvg_context *ctx, const cvg_params &p, unsigned long long pair_cap);
+// Bytes of the pair-flag buffer for pairs pairs: whole 32-bit words, since
+// k_bucket reads each anchor's run of flags a word at a time.
+inline size_t cls_bytes(unsigned long long pairs)
+{
+    return static_cast<size_t>((pairs + 4) & ~3ULL);
+}
 
 // Neighbour counts, CSR offsets, then the gathering pass, the distance sort
 // and the CSR fill (preprocess.hpp:186-274) over either cell layout.  The
@@ -3647,7 +3661,7 @@ int queue_pair_setup(cvg_context *ctx, const cvg_params &p, unsigned long long p
     constexpr int kt = 7;
     if (d.kt != kt || ctx->arena || pair_cap == 0 || pair_cap > static_cast<unsigned long long>(INT_MAX))
         return CVG_OK;
-    if (ctx->d_cls.bytes < pair_cap + 1 || ctx->d_prec.bytes < pair_cap * 8 * sizeof(double) ||
+    if (ctx->d_cls.bytes < cls_bytes(pair_cap) || ctx->d_prec.bytes < pair_cap * 8 * sizeof(double) ||
         ctx->d_psin.bytes < pair_cap * kt * 2 * sizeof(double))
         return CVG_OK;
     const int cap = static_cast<int>(pair_cap);
@@ -4066,7 +4080,7 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
     ctx->edge_trig_queued = false;
     const bool pair_setup_queued = ctx->pair_setup_queued;
     ctx->pair_setup_queued = false;
-    CVG_TRY(ensure(ctx, ctx->d_cls, static_cast<size_t>(pairs) + 1));
+    CVG_TRY(ensure(ctx, ctx->d_cls, cls_bytes(pairs)));
     CVG_TRY(ensure(ctx, ctx->d_prec, static_cast<size_t>(std::max(pairs, 1)) * 8 * sizeof(double)));
     CVG_TRY(ensure(ctx, ctx->d_psin, static_cast<size_t>(std::max(pairs, 1)) * d.kt * 2 * sizeof(double)));
     CVG_TRY(ensure(ctx, ctx->d_lists, static_cast<size_t>(nb) * n * sizeof(int4)));
