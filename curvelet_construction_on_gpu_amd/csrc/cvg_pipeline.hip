@@ -231,6 +231,7 @@ struct cvg_context {
     // told apart by the launch's epoch
     DevBuf d_fillstat;
     unsigned lookback_spins = kLookbackSpins;  // CVG_LOOKBACK_SPINS at creation
+    int gate_naps = 40;  // the start gate's bound in naps (CVG_GATE_NAPS at creation)
     // the same for the single-pass scans (k_scan_1pass)
     DevBuf d_scanstat;
     const void *scanstat_zero_ptr = nullptr;
@@ -1729,7 +1730,6 @@ __global__ __launch_bounds__(kBlock) void k_pair_prep(int pairs, int n, const in
 // Each list entry carries (anchor, CSR base, degree).
 constexpr int kBucketBlock = 1024;  // few workgroups: one list-length atomic each per bucket
 constexpr int kHeavyCap = 32;       // buckets from this cap start ahead of the light bucket 0
-constexpr int kGateMaxNaps = 40;    // the start gate's bound: 40 sleeps of 127 x 64 clocks
 // Automatic light-bucket grids: kGridAutoMinPct percent of the resident grid
 // plus one percent per kGridAutoEdgesPerPct edges of the run, at most all of
 // it (measured best on single maps from 320x240 to 3840x2160: ~10% at 6k
@@ -4245,7 +4245,7 @@ int build_curvelets(cvg_context *ctx, const cvg_params &p)
         }
         if (b == 0 && gate.nheavy > 0) {
             hipLaunchKernelGGL(k_gate, dim3(1), dim3(64), 0, st, gate, static_cast<const unsigned *>(scal),
-                               kGateMaxNaps);
+                               ctx->gate_naps);
             CVG_HIP(ctx, hipGetLastError());
         }
         A.deg_cap = l.dcap;
@@ -4587,6 +4587,7 @@ int cvg_create(int device, cvg_context **out_ctx)
         if (const char *v = std::getenv("CVG_SPARSE_CELLS")) ctx->force_sparse = std::atoi(v) != 0;
         if (const char *v = std::getenv("CVG_GENERIC_KERNEL")) ctx->force_generic = std::atoi(v) != 0;
         if (const char *v = std::getenv("CVG_SPECULATE")) ctx->spec_off = std::atoi(v) == 0;
+        if (const char *v = std::getenv("CVG_GATE_NAPS")) ctx->gate_naps = std::min(std::max(0, std::atoi(v)), 400);
         if (const char *v = std::getenv("CVG_LOOKBACK_SPINS"))
             ctx->lookback_spins = static_cast<unsigned>(std::min<long long>(std::max(0LL, std::atoll(v)), kLookbackSpins));
     } while (false);
