@@ -770,10 +770,15 @@ constexpr int kScalarBufWords = kStartedWord + 32;
 // S_BAD.
 // Each workgroup leaves its cell box and bad flag in part[5 * block]
 // (k_bbox_finish reduces them).
+// With cnt (a map on the guessed grid g): each edge whose window (cell +- R)
+// lies inside g is also counted into its cell (k_cell_count's work); an
+// edge outside it makes the bounding box miss the grid, so the counts are
+// never used and the host rebuilds them.
 __global__ __launch_bounds__(kBlock) void k_prep_edges(int n, int sz, const double *__restrict__ e,
                                                        double *__restrict__ x, double *__restrict__ y,
                                                        double *__restrict__ th, int *__restrict__ cx,
-                                                       int *__restrict__ cy, int *__restrict__ part)
+                                                       int *__restrict__ cy, int *__restrict__ part, GridDesc g,
+                                                       unsigned *__restrict__ cnt, int R)
 {
     __shared__ int smin_x, smax_x, smin_y, smax_y, sbad;
     if (threadIdx.x == 0) {
@@ -798,6 +803,8 @@ __global__ __launch_bounds__(kBlock) void k_prep_edges(int n, int sz, const doub
         th[i] = tv;
         cx[i] = ix;
         cy[i] = iy;
+        if (cnt && ok && ix >= g.x0 + R && ix <= g.x0 + g.w - 1 - R && iy >= g.y0 + R && iy <= g.y0 + g.h - 1 - R)
+            atomicAdd(&cnt[(iy - g.y0) * g.w + (ix - g.x0)], 1u);
     }
     // wave min/max, then one LDS atomic per wave
     int mnx = i < n ? cx[i] : INT_MAX, mxx = i < n ? cx[i] : INT_MIN;
@@ -3339,11 +3346,14 @@ int graph_dense(cvg_context *ctx, int n, GridDesc g, unsigned R, unsigned max_ca
     // counts of a grid no larger than the last one in the same context
     // buffer start zeroed (not tracked for the no-alloc scratch, whose
     // regions other buffers reuse)
-    const bool zeroed = !ctx->arena && ctx->d_grid.ptr == ctx->grid_zero_ptr && cells <= ctx->grid_zero_cells;
-    ctx->grid_zero_ptr = nullptr;
-    if (!zeroed) CVG_HIP(ctx, hipMemsetAsync(cnt, 0, static_cast<size_t>(cells) * sizeof(unsigned), st));
-    hipLaunchKernelGGL(k_cell_count, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
-                       as<int>(ctx->d_cy), g, cnt, static_cast<int>(R), spec ? scal + S_MISS : nullptr);
+    if (!spec) {  // (on the guessed grid k_prep_edges counted the cells)
+        const bool zeroed =
+            !ctx->arena && ctx->d_grid.ptr == ctx->grid_zero_ptr && cells <= ctx->grid_zero_cells;
+        ctx->grid_zero_ptr = nullptr;
+        if (!zeroed) CVG_HIP(ctx, hipMemsetAsync(cnt, 0, static_cast<size_t>(cells) * sizeof(unsigned), st));
+        hipLaunchKernelGGL(k_cell_count, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
+                           as<int>(ctx->d_cy), g, cnt, static_cast<int>(R), static_cast<unsigned *>(nullptr));
+    }
     CVG_TRY(exclusive_scan_1pass(ctx, cnt, cells, start, ctx->d_tiles));
     hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, as<int>(ctx->d_cx),
                        as<int>(ctx->d_cy), g, start, cnt, as<int>(ctx->d_tmp_ids),
@@ -3404,9 +3414,19 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
     const bool spec = num_maps == 1 && !ctx->force_sparse && !ctx->arena && !ctx->spec_off && ctx->spec_valid &&
                       ctx->spec_R == R;
     auto prep = [&](bool spec_on) -> int {
+        // on the guessed grid the cells are counted here (no k_cell_count)
+        unsigned *cnt = nullptr;
+        if (spec_on) {
+            const long long cells = static_cast<long long>(ctx->spec_g.w) * ctx->spec_g.h;
+            CVG_TRY(ensure(ctx, ctx->d_grid, static_cast<size_t>(2 * cells + 1) * sizeof(unsigned)));
+            cnt = as<unsigned>(ctx->d_grid);
+            const bool zeroed = ctx->d_grid.ptr == ctx->grid_zero_ptr && cells <= ctx->grid_zero_cells;
+            ctx->grid_zero_ptr = nullptr;
+            if (!zeroed) CVG_HIP(ctx, hipMemsetAsync(cnt, 0, static_cast<size_t>(cells) * sizeof(unsigned), st));
+        }
         hipLaunchKernelGGL(k_prep_edges, dim3(prep_blocks), dim3(kBlock), 0, st, n, sz, d_edges,
                            as<double>(ctx->d_x), as<double>(ctx->d_y), as<double>(ctx->d_th), as<int>(ctx->d_cx),
-                           as<int>(ctx->d_cy), as<int>(ctx->d_part));
+                           as<int>(ctx->d_cy), as<int>(ctx->d_part), ctx->spec_g, cnt, static_cast<int>(R));
         hipLaunchKernelGGL(k_bbox_finish, dim3(1), dim3(1024), 0, st, prep_blocks, as<int>(ctx->d_part), scal,
                            spec_on ? 1 : 0, ctx->spec_g, static_cast<int>(R));
         CVG_HIP(ctx, hipGetLastError());
@@ -3424,6 +3444,7 @@ int build_graph(cvg_context *ctx, int n, int sz, const double *d_edges, double r
         }
         if (rc != kSpecMiss) return rc;
         ctx->spec_valid = false;
+        ctx->grid_zero_ptr = nullptr;  // the guessed grid's counts may be partial
         CVG_TRY(prep(false));  // the guess may have moved some cells: the map's own cells again
     }
     CVG_TRY(read_scalars(ctx));
