@@ -2203,7 +2203,7 @@ __global__ __launch_bounds__(256) void k_curvelets(FusedArgs A)
 
     unsigned vpw = 0;     // valid or close pairs computed by this wave
     unsigned isect = 0;   // intersections tested by this wave
-    for (int it = 0; L < len_list; ++it) {
+    while (L < len_list) {
         const int buf = 0;
         double sT2t[CPL], cT2t[CPL];
         int nfl[NCH], nid[NCH];
