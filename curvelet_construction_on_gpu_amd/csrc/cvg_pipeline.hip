@@ -3230,6 +3230,8 @@ int graph_scans(cvg_context *ctx, const Cells &cells, int n, unsigned R, unsigne
         fill_cap = ctx->spec_fill_cap;
         pair_cap = std::min({ctx->d_cand.bytes / sizeof(Cand), ctx->d_ids.bytes / sizeof(int),
                              ctx->d_d2.bytes / sizeof(double), ctx->d_pair_anchor.bytes / sizeof(int)});
+        // below the fill's saturated offset, so that any saturated sum misses
+        pair_cap = std::min<unsigned long long>(pair_cap, kFillSumMax - 1);
     } else {
         CVG_TRY(read_scalars(ctx));
         CVG_TRY(take_counts());
