@@ -238,7 +238,7 @@ int cvg_recent_fused_ms(cvg_context *ctx, int max, float *ms, int *count);
 /* on != 0: runs made without stats record the HIP timing events as well (the
  * stage and fused-kernel times that cvg_recent_fused_ms reads).  Off at
  * creation: each timing event is a marker that holds the stream for ~6 us,
- * five of them per run.  Runs that return stats always record them.  No
+ * seven of them per run.  Runs that return stats always record them.  No
  * counterpart in the reference (its timers are host-side). */
 int cvg_set_event_timing(cvg_context *ctx, int on);
 
