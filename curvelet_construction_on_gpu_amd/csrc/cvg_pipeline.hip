@@ -3684,6 +3684,10 @@ int queue_pair_setup(cvg_context *ctx, const cvg_params &p, unsigned long long p
     constexpr int kt = 7;
     if (d.kt != kt || ctx->arena || pair_cap == 0 || pair_cap > static_cast<unsigned long long>(INT_MAX))
         return CVG_OK;
+    // after a much larger map the capacity would launch mostly idle
+    // workgroups: then the launch waits for the count (ctx->pairs still holds
+    // the last graph's)
+    if (pair_cap > 2ULL * static_cast<unsigned long long>(std::max(ctx->pairs, 0)) + 65536ULL) return CVG_OK;
     if (ctx->d_cls.bytes < cls_bytes(pair_cap) || ctx->d_prec.bytes < pair_cap * 8 * sizeof(double) ||
         ctx->d_psin.bytes < pair_cap * kt * 2 * sizeof(double))
         return CVG_OK;
